@@ -1,0 +1,228 @@
+"""ctypes binding of libhfgpu.so (include/hfgpu.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# every function include/hfgpu.h declares
+EXPORTED_SYMBOLS = [
+    "hf_default_options", "hf_create", "hf_destroy", "hf_last_error", "hf_set_basis", "hf_set_molecule",
+    "hf_nbf", "hf_nshell", "hf_electrons", "hf_nuclear_repulsion", "hf_packed_index", "hf_packed_len",
+    "hf_eri_packed", "hf_eri_packed_device", "hf_build_jk", "hf_build_jk_device", "hf_one_electron",
+    "hf_scf_defaults", "hf_scf", "hf_kernel_stats",
+]
+
+
+class HFError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("hfgpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("schwarz_threshold", ctypes.c_double),
+                ("prim_threshold", ctypes.c_double), ("exact_transfer", ctypes.c_int),
+                ("precision", ctypes.c_int), ("fp32_threshold", ctypes.c_double), ("profile", ctypes.c_int)]
+
+
+class BasisSoa(ctypes.Structure):
+    _fields_ = [("nshell", ctypes.c_int), ("nbf", ctypes.c_int),
+                ("L", ctypes.POINTER(ctypes.c_int)), ("nprim", ctypes.POINTER(ctypes.c_int)),
+                ("prim_offset", ctypes.POINTER(ctypes.c_int)), ("bf_offset", ctypes.POINTER(ctypes.c_int)),
+                ("atom", ctypes.POINTER(ctypes.c_int)), ("xyz", ctypes.POINTER(ctypes.c_double)),
+                ("alpha", ctypes.POINTER(ctypes.c_double)), ("coef", ctypes.POINTER(ctypes.c_double)),
+                ("natoms", ctypes.c_int), ("Z", ctypes.POINTER(ctypes.c_int)),
+                ("atom_xyz", ctypes.POINTER(ctypes.c_double))]
+
+
+class ScfParams(ctypes.Structure):
+    _fields_ = [("unrestricted", ctypes.c_int), ("max_iterations", ctypes.c_int), ("alpha", ctypes.c_double),
+                ("init_guess", ctypes.c_double), ("use_diis", ctypes.c_int), ("max_diis_iterations", ctypes.c_int),
+                ("normal_iter_after_diis", ctypes.c_int), ("add_asymmetry", ctypes.c_int),
+                ("asymmetry", ctypes.c_double), ("energy_tolerance", ctypes.c_double),
+                ("density_tolerance", ctypes.c_double), ("diis_tolerance", ctypes.c_double),
+                ("terminate", ctypes.POINTER(ctypes.c_int))]
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("homo", ctypes.c_double), ("s2", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("jk_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("fp32_builds", ctypes.c_int)]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("la", ctypes.c_int), ("lb", ctypes.c_int), ("lc", ctypes.c_int), ("ld", ctypes.c_int),
+                ("quartets", ctypes.c_int64), ("prim_quartets", ctypes.c_int64), ("flops", ctypes.c_double),
+                ("ms", ctypes.c_double)]
+
+    def name(self):
+        s = "spdfg"
+        return "(%s%s|%s%s)" % (s[self.la], s[self.lb], s[self.lc], s[self.ld])
+
+
+def library_path():
+    return os.path.join(_HERE, "libhfgpu.so")
+
+
+def lib():
+    """Load libhfgpu.so; raises if it has not been built (no fallback path exists)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError("libhfgpu.so not built: run `python -m hartreefock_amd.build` (%s)" % path)
+    L = ctypes.CDLL(path)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.hf_default_options.argtypes = [ctypes.POINTER(Options)]
+    L.hf_default_options.restype = None
+    L.hf_create.argtypes = [ctypes.POINTER(Options), ctypes.POINTER(vp)]
+    L.hf_destroy.argtypes = [vp]
+    L.hf_last_error.argtypes = [vp]
+    L.hf_last_error.restype = ctypes.c_char_p
+    L.hf_set_basis.argtypes = [vp, ctypes.POINTER(BasisSoa)]
+    L.hf_set_molecule.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ip, dp]
+    L.hf_nbf.argtypes = [vp]
+    L.hf_nshell.argtypes = [vp]
+    L.hf_electrons.argtypes = [vp, ip, ip]
+    L.hf_nuclear_repulsion.argtypes = [vp]
+    L.hf_nuclear_repulsion.restype = ctypes.c_double
+    L.hf_packed_index.argtypes = [ctypes.c_uint32] * 4
+    L.hf_packed_index.restype = ctypes.c_uint64
+    L.hf_packed_len.argtypes = [ctypes.c_int]
+    L.hf_packed_len.restype = ctypes.c_uint64
+    L.hf_eri_packed.argtypes = [vp, dp, ctypes.c_uint64]
+    L.hf_eri_packed_device.argtypes = [vp, vp, ctypes.c_uint64, vp]
+    L.hf_build_jk.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+    L.hf_build_jk_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+    L.hf_one_electron.argtypes = [vp, dp, dp, dp]
+    L.hf_scf_defaults.argtypes = [ctypes.POINTER(ScfParams)]
+    L.hf_scf_defaults.restype = None
+    L.hf_scf.argtypes = [vp, ctypes.POINTER(ScfParams), ctypes.POINTER(ScfResult), dp]
+    L.hf_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int]
+    _LIB = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def packed_index(i, j, k, l):
+    return lib().hf_packed_index(i, j, k, l)
+
+
+def packed_len(n):
+    return lib().hf_packed_len(n)
+
+
+class Engine:
+    """One hf_ctx: a molecule on one GPU."""
+
+    def __init__(self, **opts):
+        L = lib()
+        o = Options()
+        L.hf_default_options(ctypes.byref(o))
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise TypeError("unknown option %s" % k)
+            setattr(o, k, v)
+        self.options = o
+        h = ctypes.c_void_p()
+        rc = L.hf_create(ctypes.byref(o), ctypes.byref(h))
+        if rc != 0:
+            raise HFError(rc, "hf_create failed (is a GPU visible?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HFError(rc, lib().hf_last_error(self.h).decode())
+
+    def set_molecule(self, basis_file, atoms):
+        z = np.ascontiguousarray([a[0] for a in atoms], dtype=np.int32)
+        xyz = np.ascontiguousarray([c for a in atoms for c in a[1]], dtype=np.float64)
+        self._check(lib().hf_set_molecule(self.h, basis_file.encode(), len(atoms), _ip(z), _dp(xyz)))
+        return self
+
+    @property
+    def nbf(self):
+        return lib().hf_nbf(self.h)
+
+    @property
+    def nshell(self):
+        return lib().hf_nshell(self.h)
+
+    def electrons(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().hf_electrons(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def nuclear_repulsion(self):
+        return lib().hf_nuclear_repulsion(self.h)
+
+    def eri_packed(self):
+        n = self.nbf
+        ln = lib().hf_packed_len(n)
+        out = np.zeros(ln, np.float64)
+        self._check(lib().hf_eri_packed(self.h, _dp(out), ln))
+        return out
+
+    def build_jk(self, D):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        n = self.nbf
+        nd = 1 if D.ndim == 2 else D.shape[0]
+        D3 = D.reshape(nd, n, n)
+        J = np.zeros((n, n))
+        K = np.zeros((nd, n, n))
+        self._check(lib().hf_build_jk(self.h, nd, _dp(D3), _dp(J), _dp(K)))
+        return J, (K[0] if D.ndim == 2 else K)
+
+    def build_jk_device(self, d_D, d_J, d_K, ndens=1, stream=None):
+        """Device pointers (ints) in, device pointers out; stream is a hipStream_t as int (None: the engine's)."""
+        self._check(lib().hf_build_jk_device(self.h, ndens, ctypes.c_void_p(d_D), ctypes.c_void_p(d_J),
+                                             ctypes.c_void_p(d_K), ctypes.c_void_p(stream or 0)))
+
+    def one_electron(self):
+        n = self.nbf
+        S, T, V = (np.zeros((n, n)) for _ in range(3))
+        self._check(lib().hf_one_electron(self.h, _dp(S), _dp(T), _dp(V)))
+        return S, T, V
+
+    def scf(self, unrestricted=False, want_density=False, check=True, **kw):
+        p = ScfParams()
+        lib().hf_scf_defaults(ctypes.byref(p))
+        p.unrestricted = 1 if unrestricted else 0
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise TypeError("unknown SCF parameter %s" % k)
+            setattr(p, k, v)
+        r = ScfResult()
+        n = self.nbf
+        D = np.zeros((2 if unrestricted else 1, n, n)) if want_density else None
+        rc = lib().hf_scf(self.h, ctypes.byref(p), ctypes.byref(r), _dp(D) if D is not None else None)
+        if check:
+            self._check(rc)
+        return (r, D) if want_density else r
+
+    def kernel_stats(self):
+        arr = (KernelStat * 64)()
+        k = lib().hf_kernel_stats(self.h, arr, 64)
+        return [arr[i] for i in range(k)]

@@ -1,0 +1,489 @@
+// libhfgpu context: molecule upload, shell-pair data, Schwarz screening, task lists,
+// and the ERI / J-K entry points of the C ABI (include/hfgpu.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "generated/dispatch.inc"
+#include "hf_internal.h"
+
+using namespace hfgpu;
+
+namespace {
+
+__global__ void fill_kernel(double* __restrict__ p, size_t n, double v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// out = in + in^T for `count` n x n matrices
+__global__ void symmetrize_kernel(const double* __restrict__ in, double* __restrict__ out, int n, int count) {
+    const size_t nn = (size_t)n * n;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn * count;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t m = idx / nn, r = idx % nn;
+        const size_t i = r / n, j = r % n;
+        out[idx] = in[m * nn + i * n + j] + in[m * nn + j * n + i];
+    }
+}
+
+int grid_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (int)std::min<size_t>(std::max<size_t>(b, 1), 8192);
+}
+
+int pair_class_index(int la, int lb) {
+    for (int i = 0; i < kNumPairClasses; ++i)
+        if (kPairClassL[i][0] == la && kPairClassL[i][1] == lb) return i;
+    return -1;
+}
+
+int upload_pairs(hf_ctx* ctx, PairClassHost& pc) {
+    const size_t np = pc.sa.size();
+    HF_HIP_CHECK(ctx, pc.d_sa.alloc(np));
+    HF_HIP_CHECK(ctx, pc.d_sb.alloc(np));
+    HF_HIP_CHECK(ctx, pc.d_prim0.alloc(np));
+    HF_HIP_CHECK(ctx, pc.d_nprim.alloc(np));
+    HF_HIP_CHECK(ctx, pc.d_Q.alloc(np));
+    if (!np) return HF_OK;
+    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_sa.p, pc.sa.data(), np * sizeof(int), hipMemcpyHostToDevice));
+    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_sb.p, pc.sb.data(), np * sizeof(int), hipMemcpyHostToDevice));
+    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_prim0.p, pc.prim0.data(), np * sizeof(int), hipMemcpyHostToDevice));
+    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_nprim.p, pc.nprim.data(), np * sizeof(int), hipMemcpyHostToDevice));
+    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_Q.p, pc.Q.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    return HF_OK;
+}
+
+PairClassDev dev_view(const PairClassHost& pc) {
+    PairClassDev d;
+    d.sa = pc.d_sa.p;
+    d.sb = pc.d_sb.p;
+    d.prim0 = pc.d_prim0.p;
+    d.nprim = pc.d_nprim.p;
+    d.Q = pc.d_Q.p;
+    d.npairs = pc.npairs();
+    return d;
+}
+
+// Launch every quartet class of a task set in `mode`.
+int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, hipStream_t st, bool record) {
+    ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
+    if (record) ctx->stats.clear();
+    const bool prof = ctx->opt.profile != 0;
+    if (prof && ctx->ev.size() < 2 * (size_t)kNumClasses) {
+        for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+        ctx->ev.assign(2 * kNumClasses, nullptr);
+        for (auto& e : ctx->ev) HF_HIP_CHECK(ctx, hipEventCreate(&e));
+    }
+    std::vector<int> launched;
+    for (const QuartetClassTasks& qc : ts.classes) {
+        if (qc.ntasks <= 0) continue;
+        const ClassInfo& ci = kClasses[qc.cls];
+        TaskList tl;
+        tl.bra = dev_view(ctx->pcls[qc.bra]);
+        tl.ket = dev_view(ctx->pcls[qc.ket]);
+        tl.prefix = ts.d_prefix.p + qc.prefix_off;
+        tl.ntasks = qc.ntasks;
+        tl.same_class = qc.bra == qc.ket;
+        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls], st));
+        hipError_t e = ci.fn(mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, st);
+        if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));
+        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls + 1], st));
+        launched.push_back(qc.cls);
+        if (record) {
+            const int pf = ctx->opt.exact_transfer && ci.exact_differs ? ci.prim_flops_exact : ci.prim_flops_ref;
+            const int L = ci.la + ci.lb + ci.lc + ci.ld;
+            // Boys (Taylor 14 + recursion 4/m + scaling 1/m) and the pair-geometry terms (~40) per primitive quartet
+            const double boys_flops = 14 + 5.0 * L + 1 + 40;
+            const int na = (ci.la + 1) * (ci.la + 2) / 2, nb = (ci.lb + 1) * (ci.lb + 2) / 2;
+            const int nc = (ci.lc + 1) * (ci.lc + 2) / 2, nd = (ci.ld + 1) * (ci.ld + 2) / 2;
+            const double nout = (double)na * nb * nc * nd;
+            double sink_flops = 0;
+            if (mode == MODE_JK) sink_flops = nout * 2.0 * (2 + 4 * sink.ndens);   // 2 J + 4 K contractions per density
+            KernelStat ks;
+            ks.cls = qc.cls;
+            ks.quartets = qc.ntasks;
+            ks.prim_quartets = qc.prim_quartets;
+            ks.flops = (double)qc.prim_quartets * (pf + boys_flops) + (double)qc.ntasks * (ci.hrr_flops + sink_flops);
+            ks.ms = 0;
+            ctx->stats.push_back(ks);
+        }
+    }
+    if (prof && record) {
+        HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        for (size_t i = 0; i < launched.size(); ++i) {
+            float ms = 0;
+            HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * launched[i]], ctx->ev[2 * launched[i] + 1]));
+            ctx->stats[i].ms = ms;
+        }
+    }
+    return HF_OK;
+}
+
+// Shell pairs, primitive pairs, Schwarz bounds, sorted pair lists.
+int setup_molecule(hf_ctx* ctx) {
+    const hfhost::Molecule& mol = ctx->mol;
+    const int ns = (int)mol.shells.size();
+    if (mol.maxL > kLmax) return ctx->fail(HF_E_UNSUPPORTED, "basis has L > 2 shells; this build supports s, p, d");
+    std::vector<double> sxyz(3 * ns);
+    std::vector<int> sbf(ns);
+    for (int s = 0; s < ns; ++s) {
+        for (int k = 0; k < 3; ++k) sxyz[3 * s + k] = mol.shells[s].xyz[k];
+        sbf[s] = mol.shells[s].bf;
+    }
+    HF_HIP_CHECK(ctx, ctx->d_shell_xyz.alloc(sxyz.size()));
+    HF_HIP_CHECK(ctx, ctx->d_shell_bf.alloc(sbf.size()));
+    HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_shell_xyz.p, sxyz.data(), sxyz.size() * sizeof(double), hipMemcpyHostToDevice));
+    HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_shell_bf.p, sbf.data(), sbf.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!ctx->d_boys.p) {
+        std::vector<double> tab;
+        hfhost::boys_table(tab);
+        HF_HIP_CHECK(ctx, ctx->d_boys.alloc(tab.size()));
+        HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_boys.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // pairs (La >= Lb; for equal L, first shell index >= second)
+    std::vector<PrimPair> pp;
+    ctx->pcls.clear();
+    ctx->pcls.resize(kNumPairClasses);
+    for (int sa = 0; sa < ns; ++sa)
+        for (int sb = 0; sb < ns; ++sb) {
+            const hfhost::Shell& A = mol.shells[sa];
+            const hfhost::Shell& B = mol.shells[sb];
+            if (A.L < B.L || (A.L == B.L && sb > sa)) continue;
+            const int ci = pair_class_index(A.L, B.L);
+            const double ABx = A.xyz[0] - B.xyz[0], ABy = A.xyz[1] - B.xyz[1], ABz = A.xyz[2] - B.xyz[2];
+            const double R2 = ABx * ABx + ABy * ABy + ABz * ABz;
+            const int start = (int)pp.size();
+            for (size_t i = 0; i < A.alpha.size(); ++i)
+                for (size_t j = 0; j < B.alpha.size(); ++j) {
+                    const double a = A.alpha[i], b = B.alpha[j], z = a + b;
+                    const double K = A.coef[i] * B.coef[j] * std::exp(-a * b / z * R2);
+                    if (std::fabs(K) < ctx->opt.prim_threshold) continue;
+                    PrimPair q;
+                    q.Px = (a * A.xyz[0] + b * B.xyz[0]) / z;
+                    q.Py = (a * A.xyz[1] + b * B.xyz[1]) / z;
+                    q.Pz = (a * A.xyz[2] + b * B.xyz[2]) / z;
+                    q.zeta = z;
+                    q.Kz = K / z;
+                    q.inv_zeta = 1.0 / z;
+                    q.pad0 = q.pad1 = 0;
+                    pp.push_back(q);
+                }
+            const int cnt = (int)pp.size() - start;
+            if (cnt == 0) continue;
+            PairClassHost& pc = ctx->pcls[ci];
+            pc.sa.push_back(sa);
+            pc.sb.push_back(sb);
+            pc.prim0.push_back(start);
+            pc.nprim.push_back(cnt);
+            pc.Q.push_back(0.0);
+        }
+    HF_HIP_CHECK(ctx, ctx->d_pp.alloc(std::max<size_t>(pp.size(), 1)));
+    if (!pp.empty())
+        HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_pp.p, pp.data(), pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice));
+    for (auto& pc : ctx->pcls) {
+        int rc = upload_pairs(ctx, pc);
+        if (rc) return rc;
+    }
+    // Schwarz bounds from the diagonal quartets (ab|ab), always with exact integrals
+    {
+        DevBuf<double> dQ;
+        TaskSet diag;
+        size_t maxp = 1;
+        for (auto& pc : ctx->pcls) maxp = std::max<size_t>(maxp, pc.sa.size());
+        HF_HIP_CHECK(ctx, dQ.alloc(maxp));
+        ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
+        for (int c = 0; c < kNumClasses; ++c) {
+            const ClassInfo& ci = kClasses[c];
+            if (ci.la != ci.lc || ci.lb != ci.ld) continue;
+            const int pc_i = pair_class_index(ci.la, ci.lb);
+            PairClassHost& pc = ctx->pcls[pc_i];
+            if (pc.sa.empty()) continue;
+            TaskList tl;
+            tl.bra = tl.ket = dev_view(pc);
+            tl.prefix = nullptr;
+            tl.ntasks = pc.npairs();
+            tl.same_class = 1;
+            SinkArgs sink{};
+            sink.Qout = dQ.p;
+            hipError_t e = ci.fn(MODE_DIAG, 1, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ctx->stream);
+            if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("Schwarz kernel: ") + hipGetErrorString(e));
+            HF_HIP_CHECK(ctx, hipMemcpyAsync(pc.Q.data(), dQ.p, pc.Q.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                             ctx->stream));
+            HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        }
+    }
+    // sort each class by Q, descending
+    for (auto& pc : ctx->pcls) {
+        const size_t np = pc.sa.size();
+        std::vector<size_t> order(np);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return pc.Q[x] > pc.Q[y]; });
+        PairClassHost s;
+        for (size_t i : order) {
+            s.sa.push_back(pc.sa[i]);
+            s.sb.push_back(pc.sb[i]);
+            s.prim0.push_back(pc.prim0[i]);
+            s.nprim.push_back(pc.nprim[i]);
+            s.Q.push_back(pc.Q[i]);
+        }
+        pc.sa.swap(s.sa);
+        pc.sb.swap(s.sb);
+        pc.prim0.swap(s.prim0);
+        pc.nprim.swap(s.nprim);
+        pc.Q.swap(s.Q);
+        int rc = upload_pairs(ctx, pc);
+        if (rc) return rc;
+    }
+    ctx->tasks_all.threshold = -1;
+    ctx->tasks_screened.threshold = -1;
+    ctx->have_mol = true;
+    return HF_OK;
+}
+
+int ensure_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
+    if (ts.threshold == thr && !ts.classes.empty()) return HF_OK;
+    return hf_build_tasks(ctx, thr, ts);
+}
+
+}  // namespace
+
+int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
+    ts.classes.clear();
+    std::vector<int64_t> prefix_all;
+    for (int c = 0; c < kNumClasses; ++c) {
+        const ClassInfo& ci = kClasses[c];
+        QuartetClassTasks qc;
+        qc.cls = c;
+        qc.bra = pair_class_index(ci.la, ci.lb);
+        qc.ket = pair_class_index(ci.lc, ci.ld);
+        const PairClassHost& B = ctx->pcls[qc.bra];
+        const PairClassHost& K = ctx->pcls[qc.ket];
+        const bool same = qc.bra == qc.ket;
+        std::vector<int64_t> knp(K.npairs() + 1, 0);     // prefix of ket primitive-pair counts
+        for (int q = 0; q < K.npairs(); ++q) knp[q + 1] = knp[q] + K.nprim[q];
+        qc.prefix_off = prefix_all.size();
+        int64_t acc = 0, pq = 0;
+        prefix_all.push_back(0);
+        for (int p = 0; p < B.npairs(); ++p) {
+            int64_t cnt = K.npairs();
+            if (thr > 0) {
+                if (B.Q[p] <= 0) cnt = 0;
+                else {
+                    const double need = thr / B.Q[p];
+                    // ket Q descending: count of q with Q_q >= need
+                    cnt = std::upper_bound(K.Q.begin(), K.Q.end(), need, [](double v, double x) { return v > x; }) -
+                          K.Q.begin();
+                }
+            }
+            if (same) cnt = std::min<int64_t>(cnt, p + 1);
+            acc += cnt;
+            pq += knp[cnt] * B.nprim[p];
+            prefix_all.push_back(acc);
+        }
+        qc.ntasks = acc;
+        qc.prim_quartets = pq;
+        ts.classes.push_back(qc);
+    }
+    HF_HIP_CHECK(ctx, ts.d_prefix.alloc(prefix_all.size()));
+    HF_HIP_CHECK(ctx, hipMemcpy(ts.d_prefix.p, prefix_all.data(), prefix_all.size() * sizeof(int64_t),
+                                hipMemcpyHostToDevice));
+    ts.threshold = thr;
+    return HF_OK;
+}
+
+int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st) {
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
+    const int n = ctx->mol.nbf;
+    const size_t nn = (size_t)n * n;
+    int rc = ensure_tasks(ctx, ctx->opt.schwarz_threshold, ctx->tasks_screened);
+    if (rc) return rc;
+    HF_HIP_CHECK(ctx, ctx->d_Jp.alloc(nn));
+    HF_HIP_CHECK(ctx, ctx->d_Kp.alloc(2 * nn));
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(nn)), dim3(256), 0, st, ctx->d_Jp.p, nn, 0.0);
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(ndens * nn)), dim3(256), 0, st, ctx->d_Kp.p, ndens * nn, 0.0);
+    SinkArgs sink{};
+    sink.n = n;
+    sink.ndens = ndens;
+    sink.D = d_D;
+    sink.J = ctx->d_Jp.p;
+    sink.K = ctx->d_Kp.p;
+    rc = run_classes(ctx, MODE_JK, ctx->tasks_screened, sink, st, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(nn)), dim3(256), 0, st, ctx->d_Jp.p, d_J, n, 1);
+    hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(ndens * nn)), dim3(256), 0, st, ctx->d_Kp.p, d_K, n, ndens);
+    HF_HIP_CHECK(ctx, hipGetLastError());
+    return HF_OK;
+}
+
+extern "C" {
+
+void hf_default_options(hf_options* o) {
+    o->device = 0;
+    o->schwarz_threshold = 1e-12;
+    o->prim_threshold = 1e-20;
+    o->exact_transfer = 0;
+    o->precision = HF_PREC_FP64;
+    o->fp32_threshold = 1e-6;
+    o->profile = 0;
+}
+
+int hf_create(const hf_options* opt, hf_ctx** out) {
+    if (!out) return HF_E_ARG;
+    *out = nullptr;
+    hf_ctx* ctx = new (std::nothrow) hf_ctx();
+    if (!ctx) return HF_E_OOM;
+    if (opt) ctx->opt = *opt; else hf_default_options(&ctx->opt);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        delete ctx;
+        return HF_E_HIP;
+    }
+    if (ctx->opt.device < 0 || ctx->opt.device >= ndev || hipSetDevice(ctx->opt.device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return HF_E_HIP;
+    }
+    if (rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return HF_E_HIP;
+    }
+    rocblas_set_stream(ctx->blas, ctx->stream);
+    *out = ctx;
+    return HF_OK;
+}
+
+int hf_destroy(hf_ctx* ctx) {
+    if (!ctx) return HF_OK;
+    (void)hipSetDevice(ctx->opt.device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+    if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HF_OK;
+}
+
+const char* hf_last_error(const hf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int hf_set_molecule(hf_ctx* ctx, const char* basis_file, int natoms, const int* Z, const double* xyz) {
+    if (!ctx || !basis_file || natoms <= 0 || !Z || !xyz) return ctx ? ctx->fail(HF_E_ARG, "bad arguments") : HF_E_ARG;
+    ctx->have_mol = false;
+    std::string err;
+    if (!hfhost::load_molecule(basis_file, natoms, Z, xyz, ctx->mol, err)) return ctx->fail(HF_E_IO, err);
+    return setup_molecule(ctx);
+}
+
+int hf_set_basis(hf_ctx* ctx, const hf_basis_soa* b) {
+    if (!ctx || !b) return ctx ? ctx->fail(HF_E_ARG, "null basis") : HF_E_ARG;
+    ctx->have_mol = false;
+    std::string err;
+    if (!hfhost::molecule_from_soa(b, ctx->mol, err)) return ctx->fail(HF_E_ARG, err);
+    return setup_molecule(ctx);
+}
+
+int hf_nbf(const hf_ctx* ctx) { return ctx && ctx->have_mol ? ctx->mol.nbf : 0; }
+int hf_nshell(const hf_ctx* ctx) { return ctx && ctx->have_mol ? (int)ctx->mol.shells.size() : 0; }
+
+int hf_electrons(const hf_ctx* ctx, int* alpha, int* beta) {
+    if (!ctx || !ctx->have_mol) return HF_E_STATE;
+    const int ne = ctx->mol.electrons();
+    if (alpha) *alpha = ne / 2;
+    if (beta) *beta = ne - ne / 2;
+    return ne;
+}
+
+double hf_nuclear_repulsion(const hf_ctx* ctx) {
+    return ctx && ctx->have_mol ? hfhost::nuclear_repulsion(ctx->mol) : 0.0;
+}
+
+uint64_t hf_packed_index(uint32_t i, uint32_t j, uint32_t k, uint32_t l) {
+    auto two = [](uint64_t a, uint64_t b) { return a < b ? b * (b + 1) / 2 + a : a * (a + 1) / 2 + b; };
+    return two(two(i, j), two(k, l));
+}
+
+uint64_t hf_packed_len(int n) { return hf_packed_index(n, n, n, n) + 1; }
+
+int hf_eri_packed_device(hf_ctx* ctx, double* d_out, uint64_t len, void* stream) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    const int n = ctx->mol.nbf;
+    const uint64_t need = hf_packed_index(n - 1, n - 1, n - 1, n - 1) + 1;
+    if (!d_out || len < need) return ctx->fail(HF_E_ARG, "packed buffer too small");
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    int rc = ensure_tasks(ctx, 0.0, ctx->tasks_all);
+    if (rc) return rc;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(len)), dim3(256), 0, st, d_out, (size_t)len, 0.0);
+    SinkArgs sink{};
+    sink.n = n;
+    sink.packed = d_out;
+    sink.packed_len = len;
+    return run_classes(ctx, MODE_PACKED, ctx->tasks_all, sink, st, true);
+}
+
+int hf_eri_packed(hf_ctx* ctx, double* out, uint64_t len) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!out) return ctx->fail(HF_E_ARG, "null output");
+    DevBuf<double> d;
+    HF_HIP_CHECK(ctx, d.alloc(len));
+    int rc = hf_eri_packed_device(ctx, d.p, len, nullptr);
+    if (rc) return rc;
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(out, d.p, len * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return HF_OK;
+}
+
+int hf_build_jk_device(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, void* stream) {
+    if (!ctx) return HF_E_ARG;
+    if (!d_D || !d_J || !d_K) return ctx->fail(HF_E_ARG, "null device buffer");
+    return hf_jk_device_impl(ctx, ndens, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int hf_build_jk(hf_ctx* ctx, int ndens, const double* D, double* J, double* K) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!D || !J || !K || ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "bad arguments");
+    const size_t nn = (size_t)ctx->mol.nbf * ctx->mol.nbf;
+    HF_HIP_CHECK(ctx, ctx->d_D.alloc(2 * nn));
+    HF_HIP_CHECK(ctx, ctx->d_J.alloc(nn));
+    HF_HIP_CHECK(ctx, ctx->d_K.alloc(2 * nn));
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_D.p, D, ndens * nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = hf_jk_device_impl(ctx, ndens, ctx->d_D.p, ctx->d_J.p, ctx->d_K.p, ctx->stream);
+    if (rc) return rc;
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(J, ctx->d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(K, ctx->d_K.p, ndens * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return HF_OK;
+}
+
+int hf_one_electron(hf_ctx* ctx, double* S, double* T, double* V) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!S || !T || !V) return ctx->fail(HF_E_ARG, "null output");
+    hfhost::one_electron(ctx->mol, S, T, V);
+    return HF_OK;
+}
+
+int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max) {
+    if (!ctx) return 0;
+    int k = 0;
+    for (const KernelStat& s : ctx->stats) {
+        if (k >= max) break;
+        const ClassInfo& ci = kClasses[s.cls];
+        out[k].la = ci.la;
+        out[k].lb = ci.lb;
+        out[k].lc = ci.lc;
+        out[k].ld = ci.ld;
+        out[k].quartets = s.quartets;
+        out[k].prim_quartets = s.prim_quartets;
+        out[k].flops = s.flops;
+        out[k].ms = s.ms;
+        ++k;
+    }
+    return k;
+}
+
+}  // extern "C"

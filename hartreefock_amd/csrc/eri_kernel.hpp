@@ -1,0 +1,285 @@
+// Shell-quartet ERI kernel, one template instance per angular class and sink.
+//
+// One thread evaluates one contracted shell quartet (ab|cd):
+//   for every primitive pair of the bra x every primitive pair of the ket:
+//     Boys F_0..F_L (table + Taylor, FP64), HGP vertical recursion and electron
+//     transfer (generated straight-line code), accumulated into the contracted
+//     [e0|f0] block held in registers;
+//   then the horizontal recursions on bra and ket (generated) give the Cartesian
+//   block (ab|cd), which goes to one of three sinks:
+//     MODE_JK      digestion into J and K with FP64 atomics (8-fold symmetry),
+//     MODE_PACKED  the reference's packed unique-ERI store,
+//     MODE_DIAG    the Schwarz bound sqrt(max |(ab|ab)|) of a shell pair.
+#pragma once
+
+#include "hf_device.h"
+#include "generated/eri_classes.inc"
+
+namespace hfgpu {
+
+constexpr double kTwoPi25 = 34.98683665524972497;   // 2 pi^{5/2}
+constexpr double kSqrtPiHalf = 0.88622692545275801365;  // sqrt(pi)/2
+
+// F_m(T) * pref for m = 0..L.
+template <int L>
+__device__ __forceinline__ void boys_scaled(double T, double pref, const double* __restrict__ tab, double* F) {
+    if (T < BOYS_TMAX) {
+        const int k = __double2int_rn(T * BOYS_INV_DT);
+        const double x = k * BOYS_DT - T;          // F_m(T) = sum_j F_{m+j}(T_k) x^j / j!
+        const double* row = tab + k * BOYS_NCOL + L;
+        double f = row[7];
+        f = row[6] + x * f * (1.0 / 7.0);
+        f = row[5] + x * f * (1.0 / 6.0);
+        f = row[4] + x * f * (1.0 / 5.0);
+        f = row[3] + x * f * (1.0 / 4.0);
+        f = row[2] + x * f * (1.0 / 3.0);
+        f = row[1] + x * f * (1.0 / 2.0);
+        f = row[0] + x * f;
+        const double e = exp(-T);
+        F[L] = f;
+#pragma unroll
+        for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) * (1.0 / (2 * m + 1));
+    } else {
+        const double e = exp(-T);
+        const double inv2T = 0.5 / T;
+        F[0] = kSqrtPiHalf * rsqrt(T);
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * inv2T;
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) F[m] *= pref;
+}
+
+__device__ __forceinline__ uint64_t two_index(uint64_t i, uint64_t j) {
+    return i < j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j;
+}
+
+template <class C>
+__device__ __forceinline__ void sink_jk(const double* __restrict__ v, int a0, int b0, int c0, int d0, double w,
+                                        const SinkArgs& s) {
+    constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+    const int n = s.n;
+    const size_t nn = (size_t)n * n;
+    // Coulomb: J'_ab += 2w sum_cd (ab|cd) D_cd ;  J'_cd += 2w sum_ab (ab|cd) D_ab  (D summed over densities)
+    {
+        double Dcd[NC * ND], Dab[NA * NB];
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                double x = 0;
+                for (int k = 0; k < s.ndens; ++k) x += s.D[k * nn + (size_t)(c0 + i) * n + d0 + j];
+                Dcd[i * ND + j] = x;
+            }
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                double x = 0;
+                for (int k = 0; k < s.ndens; ++k) x += s.D[k * nn + (size_t)(a0 + i) * n + b0 + j];
+                Dab[i * NB + j] = x;
+            }
+        const double w2 = 2.0 * w;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                double acc = 0;
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dcd[ic * ND + id];
+                unsafeAtomicAdd(&s.J[(size_t)(a0 + ia) * n + b0 + ib], w2 * acc);
+            }
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0;
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int ib = 0; ib < NB; ++ib) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dab[ia * NB + ib];
+                unsafeAtomicAdd(&s.J[(size_t)(c0 + ic) * n + d0 + id], w2 * acc);
+            }
+    }
+    // Exchange, per density: K'_ac += w (ab|cd) D_bd, K'_ad += w D_bc, K'_bc += w D_ad, K'_bd += w D_ac
+    for (int k = 0; k < s.ndens; ++k) {
+        const double* __restrict__ D = s.D + k * nn;
+        double* __restrict__ K = s.K + k * nn;
+        double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+#pragma unroll
+            for (int j = 0; j < ND; ++j) Dbd[i * ND + j] = D[(size_t)(b0 + i) * n + d0 + j];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) Dbc[i * NC + j] = D[(size_t)(b0 + i) * n + c0 + j];
+        }
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+#pragma unroll
+            for (int j = 0; j < ND; ++j) Dad[i * ND + j] = D[(size_t)(a0 + i) * n + d0 + j];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) Dac[i * NC + j] = D[(size_t)(a0 + i) * n + c0 + j];
+        }
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) {
+                double acc = 0;
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dbd[ib * ND + id];
+                unsafeAtomicAdd(&K[(size_t)(a0 + ia) * n + c0 + ic], w * acc);
+            }
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0;
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dbc[ib * NC + ic];
+                unsafeAtomicAdd(&K[(size_t)(a0 + ia) * n + d0 + id], w * acc);
+            }
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) {
+                double acc = 0;
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dad[ia * ND + id];
+                unsafeAtomicAdd(&K[(size_t)(b0 + ib) * n + c0 + ic], w * acc);
+            }
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0;
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dac[ia * NC + ic];
+                unsafeAtomicAdd(&K[(size_t)(b0 + ib) * n + d0 + id], w * acc);
+            }
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void sink_packed(const double* __restrict__ v, int A, int B, int Cs, int Ds, int a0, int b0,
+                                            int c0, int d0, bool same_pair, const SinkArgs& s) {
+    constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+            const int a = a0 + ia, b = b0 + ib;
+            if (A == B && a < b) continue;
+            const uint64_t ab = two_index(a, b);
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) {
+                    const int c = c0 + ic, d = d0 + id;
+                    if (Cs == Ds && c < d) continue;
+                    const uint64_t cd = two_index(c, d);
+                    if (same_pair && ab < cd) continue;
+                    const uint64_t idx = two_index(ab, cd);
+                    if (idx < s.packed_len) s.packed[idx] = v[((ia * NB + ib) * NC + ic) * ND + id];
+                }
+        }
+}
+
+template <int LA, int LB, int LC, int LD, int EXACT, int MODE>
+__global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev sh, const PrimPair* __restrict__ pp,
+                                                        const double* __restrict__ boys, SinkArgs sink) {
+    using C = EriClass<LA, LB, LC, LD, EXACT>;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tl.ntasks) return;
+    int p, q;
+    if (MODE == MODE_DIAG) {
+        p = q = (int)t;
+    } else {
+        int lo = 0, hi = tl.bra.npairs;          // prefix[lo] <= t < prefix[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (tl.prefix[mid] <= t) lo = mid; else hi = mid;
+        }
+        p = lo;
+        q = (int)(t - tl.prefix[lo]);
+    }
+    const int A = tl.bra.sa[p], B = tl.bra.sb[p];
+    const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
+    const double Ax = sh.xyz[3 * A], Ay = sh.xyz[3 * A + 1], Az = sh.xyz[3 * A + 2];
+    const double Bx = sh.xyz[3 * B], By = sh.xyz[3 * B + 1], Bz = sh.xyz[3 * B + 2];
+    const double Cx = sh.xyz[3 * Cs], Cy = sh.xyz[3 * Cs + 1], Cz = sh.xyz[3 * Cs + 2];
+    const double Dx = sh.xyz[3 * Ds], Dy = sh.xyz[3 * Ds + 1], Dz = sh.xyz[3 * Ds + 2];
+
+    double acc[C::NACC];
+#pragma unroll
+    for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
+
+    const int pb0 = tl.bra.prim0[p], npb = tl.bra.nprim[p];
+    const int pk0 = tl.ket.prim0[q], npk = tl.ket.nprim[q];
+    for (int i = 0; i < npb; ++i) {
+        const PrimPair P = pp[pb0 + i];
+        const double PAx = P.Px - Ax, PAy = P.Py - Ay, PAz = P.Pz - Az;
+        const double inv2z = 0.5 * P.inv_zeta;
+        for (int j = 0; j < npk; ++j) {
+            const PrimPair Q = pp[pk0 + j];
+            const double inv_s = 1.0 / (P.zeta + Q.zeta);
+            const double rho = P.zeta * Q.zeta * inv_s;
+            const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            const double pref = kTwoPi25 * P.Kz * Q.Kz * sqrt(inv_s);
+            double F[C::L + 1];
+            boys_scaled<C::L>(T, pref, boys, F);
+            const double rz = Q.zeta * inv_s;                       // rho / zeta
+            const double zoe = P.zeta * Q.inv_zeta;                 // zeta / eta
+            const double inv2e = 0.5 * Q.inv_zeta;
+            const double dlx = zoe * PAx + (Q.Px - Cx);             // delta = (zeta/eta) PA + QC
+            const double dly = zoe * PAy + (Q.Py - Cy);
+            const double dlz = zoe * PAz + (Q.Pz - Cz);
+            C::prim(F, PAx, PAy, PAz, -rz * PQx, -rz * PQy, -rz * PQz, inv2z, rz, dlx, dly, dlz, zoe, inv2e, acc);
+        }
+    }
+    double out[C::NOUT];
+    C::hrr(acc, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy, Cz - Dz, out);
+
+    const int a0 = sh.bf[A], b0 = sh.bf[B], c0 = sh.bf[Cs], d0 = sh.bf[Ds];
+    if (MODE == MODE_JK) {
+        double w = 1.0;
+        if (A == B) w *= 0.5;
+        if (Cs == Ds) w *= 0.5;
+        if (tl.same_class && p == q) w *= 0.5;
+        sink_jk<C>(out, a0, b0, c0, d0, w, sink);
+    } else if (MODE == MODE_PACKED) {
+        sink_packed<C>(out, A, B, Cs, Ds, a0, b0, c0, d0, tl.same_class && p == q, sink);
+    } else {
+        double mx = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < C::NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < C::NB; ++ib) {
+                const double x = fabs(out[((ia * C::NB + ib) * C::NC + ia) * C::ND + ib]);
+                mx = x > mx ? x : mx;
+            }
+        sink.Qout[p] = sqrt(mx);
+    }
+}
+
+template <int LA, int LB, int LC, int LD, int EXACT, int MODE>
+inline hipError_t launch_eri(const TaskList& tl, const ShellsDev& sh, const PrimPair* pp, const double* boys,
+                             const SinkArgs& sink, hipStream_t st) {
+    if (tl.ntasks <= 0) return hipSuccess;
+    const int64_t blocks = (tl.ntasks + 255) / 256;
+    hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, EXACT, MODE>), dim3((unsigned)blocks), dim3(256), 0, st, tl,
+                       sh, pp, boys, sink);
+    return hipGetLastError();
+}
+
+}  // namespace hfgpu

@@ -1,0 +1,370 @@
+"""Generate straight-line HIP code for the Head-Gordon-Pople ERI scheme, one
+struct per angular class (LA LB | LC LD) with LA>=LB, LC>=LD, LA+LB>=LC+LD.
+
+The scheme is the one of the reference (HartreeFock/GaussianTwoElectrons.cpp):
+
+  vertical recursion on centre A   [e+1_i|0]^(m) = PA_i [e]^(m) + WP_i [e]^(m+1)
+                                          + e_i/(2 zeta) ([e-1_i]^(m) - rho/zeta [e-1_i]^(m+1))   (:98-133)
+  electron transfer A -> C         (e, f+1_i) = delta_i (e,f) - zeta/eta (e+1_i, f)
+                                          + e_i/(2 eta) (e-1_i, f) + f_i/(2 eta) (e, f-1_i)       (:137-192)
+  contraction over primitives, then
+  horizontal recursion on the bra  (a, b+1_i| = (a+1_i, b| + AB_i (a, b|                           (:198-273)
+  horizontal recursion on the ket  |c, d+1_i) = |c+1_i, d) + CD_i |c, d)                           (:278-360)
+
+with the recursion direction i chosen as the largest Cartesian component of the
+target (ties x > y > z), as there.  Unlike the reference, which fills whole
+index ranges, only the intermediates the final block depends on are emitted.
+
+Transfer convention.  The reference fills bra rows e at ket level |f| only for
+|e| >= |f|-1 (:152); rows below stay zero.  For (dd|dd) the rows |e|=2 at |f|=4
+are needed, so its two-centre-bra (dd|dd) integrals differ from the exact ones.
+`exact=False` reproduces the reference (those entries are the constant 0);
+`exact=True` computes them.
+
+Each generated struct provides
+  prim(F, PA, WP, inv2z, rz, dl, zoe, inv2e, acc)   one primitive quartet, adds into acc
+  hrr(acc, AB, CD, out)                              contracted block -> (ab|cd) with the
+                                                     per-component normalisation applied
+and the exact operation counts PRIM_FLOPS / HRR_FLOPS (one add or one multiply
+= 1 flop; a fused multiply-add counts 2).
+
+Usage: python gen_eri.py OUT_DIR [LMAX]
+
+Writes OUT_DIR/eri_classes.inc (the structs), OUT_DIR/k_<class>_<mode>_<variant>.hip (one
+kernel instance each), OUT_DIR/d_<class>.hip (the per-class launcher) and OUT_DIR/dispatch.inc
+(the host-side class table with the flop counts).
+"""
+import math
+import sys
+
+
+def comps(L):
+    """Cartesian components of shell L in canonical order (xx,xy,xz,yy,yz,zz ...),
+    reference HartreeFock/QuantumNumbers.h:60-80."""
+    out = []
+    for l in range(L, -1, -1):
+        for m in range(L - l, -1, -1):
+            out.append((l, m, L - l - m))
+    return out
+
+
+def direction(c):
+    mx = max(c)
+    return 0 if c[0] == mx else (1 if c[1] == mx else 2)
+
+
+def add(c, i, k=1):
+    t = list(c)
+    t[i] += k
+    return tuple(t)
+
+
+def dfact(n):
+    r = 1
+    while n > 1:
+        r *= n
+        n -= 2
+    return r
+
+
+def comp_norm(c):
+    return 1.0 / math.sqrt(dfact(2 * c[0] - 1) * dfact(2 * c[1] - 1) * dfact(2 * c[2] - 1))
+
+
+def cname(c):
+    return "%d%d%d" % c
+
+
+XYZ = "xyz"
+
+
+class ClassGen:
+    def __init__(self, la, lb, lc, ld, exact):
+        self.la, self.lb, self.lc, self.ld = la, lb, lc, ld
+        self.exact = exact
+        self.L = la + lb + lc + ld
+        self.E = [c for L in range(la, la + lb + 1) for c in comps(L)]
+        self.F = [c for L in range(lc, lc + ld + 1) for c in comps(L)]
+        self.eidx = {c: i for i, c in enumerate(self.E)}
+        self.fidx = {c: i for i, c in enumerate(self.F)}
+        self.flops_prim = 0
+        self.flops_hrr = 0
+
+    # ---------------- primitive part: VRR + ETR + accumulate ----------------
+    def etr_zero(self, e, f):
+        return (not self.exact) and sum(f) > 0 and sum(e) < sum(f) - 1
+
+    def gen_prim(self):
+        lines = []
+        vrr_done = {}
+        etr_done = {}
+        fl = [0]
+
+        def vrr(e, m):
+            key = (e, m)
+            if key in vrr_done:
+                return vrr_done[key]
+            if sum(e) == 0:
+                vrr_done[key] = "F[%d]" % m
+                return vrr_done[key]
+            i = direction(e)
+            e1 = add(e, i, -1)
+            a = vrr(e1, m)
+            b = vrr(e1, m + 1)
+            expr = "PA%s * %s + WP%s * %s" % (XYZ[i], a, XYZ[i], b)
+            fl[0] += 3
+            if e1[i] > 0:
+                e2 = add(e1, i, -1)
+                c = vrr(e2, m)
+                d = vrr(e2, m + 1)
+                expr += " + %d.0 * inv2z * (%s - rz * %s)" % (e1[i], c, d)
+                fl[0] += 5
+            name = "v%s_%d" % (cname(e), m)
+            lines.append("        const double %s = %s;" % (name, expr))
+            vrr_done[key] = name
+            return name
+
+        def etr(e, f):
+            key = (e, f)
+            if key in etr_done:
+                return etr_done[key]
+            if sum(f) == 0:
+                etr_done[key] = vrr(e, 0)
+                return etr_done[key]
+            if self.etr_zero(e, f):
+                etr_done[key] = None
+                return None
+            i = direction(f)
+            f1 = add(f, i, -1)
+            terms = []
+            t = etr(e, f1)
+            if t is not None:
+                terms.append("dl%s * %s" % (XYZ[i], t))
+            t = etr(add(e, i), f1)
+            if t is not None:
+                terms.append("- zoe * %s" % t)
+            if e[i] > 0:
+                t = etr(add(e, i, -1), f1)
+                if t is not None:
+                    terms.append("+ %d.0 * inv2e * %s" % (e[i], t))
+            if f1[i] > 0:
+                t = etr(e, add(f1, i, -1))
+                if t is not None:
+                    terms.append("+ %d.0 * inv2e * %s" % (f1[i], t))
+            if not terms:
+                etr_done[key] = None
+                return None
+            expr = " ".join(terms)
+            if expr.startswith("- "):
+                expr = "-" + expr[2:]
+            elif expr.startswith("+ "):
+                expr = expr[2:]
+            fl[0] += 2 * len(terms) - 1 + sum(1 for x in terms if "inv2e" in x)
+            name = "t%s_%s" % (cname(e), cname(f))
+            lines.append("        const double %s = %s;" % (name, expr))
+            etr_done[key] = name
+            return name
+
+        for f in self.F:
+            for e in self.E:
+                v = etr(e, f)
+                if v is not None:
+                    lines.append("        acc[%d] += %s;" % (self.eidx[e] * len(self.F) + self.fidx[f], v))
+                    fl[0] += 1
+        self.flops_prim = fl[0]
+        return lines
+
+    # ---------------- contracted part: HRR bra, HRR ket, normalisation ----------------
+    def gen_hrr(self):
+        lines = []
+        NF = len(self.F)
+        A, B, C, D = comps(self.la), comps(self.lb), comps(self.lc), comps(self.ld)
+        fl = 0
+        # bra: (a, b | f) for every f in F
+        bra = {}
+
+        def hb(a, b, f):
+            nonlocal fl
+            key = (a, b, f)
+            if key in bra:
+                return bra[key]
+            if sum(b) == 0:
+                bra[key] = "acc[%d]" % (self.eidx[a] * NF + self.fidx[f])
+                return bra[key]
+            i = direction(b)
+            b1 = add(b, i, -1)
+            x = hb(add(a, i), b1, f)
+            y = hb(a, b1, f)
+            name = "h%s_%s_%s" % (cname(a), cname(b), cname(f))
+            lines.append("        const double %s = %s + AB%s * %s;" % (name, x, XYZ[i], y))
+            fl += 2
+            bra[key] = name
+            return name
+
+        ket = {}
+
+        def hk(a, b, c, d):
+            nonlocal fl
+            key = (a, b, c, d)
+            if key in ket:
+                return ket[key]
+            if sum(d) == 0:
+                ket[key] = hb(a, b, c)
+                return ket[key]
+            i = direction(d)
+            d1 = add(d, i, -1)
+            x = hk(a, b, add(c, i), d1)
+            y = hk(a, b, c, d1)
+            name = "k%s_%s_%s_%s" % (cname(a), cname(b), cname(c), cname(d))
+            lines.append("        const double %s = %s + CD%s * %s;" % (name, x, XYZ[i], y))
+            fl += 2
+            ket[key] = name
+            return name
+
+        nb, nc, nd = len(B), len(C), len(D)
+        for ia, a in enumerate(A):
+            for ib, b in enumerate(B):
+                for ic, c in enumerate(C):
+                    for id_, d in enumerate(D):
+                        v = hk(a, b, c, d)
+                        s = comp_norm(a) * comp_norm(b) * comp_norm(c) * comp_norm(d)
+                        idx = ((ia * nb + ib) * nc + ic) * nd + id_
+                        if s == 1.0:
+                            lines.append("        out[%d] = %s;" % (idx, v))
+                        else:
+                            lines.append("        out[%d] = %.17g * %s;" % (idx, s, v))
+                            fl += 1
+        self.flops_hrr = fl
+        return lines
+
+    def emit(self):
+        name = "EriClass<%d, %d, %d, %d, %d>" % (self.la, self.lb, self.lc, self.ld, 1 if self.exact else 0)
+        prim = self.gen_prim()
+        hrr = self.gen_hrr()
+        na, nb, nc, nd = (len(comps(x)) for x in (self.la, self.lb, self.lc, self.ld))
+        out = []
+        out.append("template <> struct %s {" % name)
+        out.append("    static constexpr int L = %d, NA = %d, NB = %d, NC = %d, ND = %d;" % (self.L, na, nb, nc, nd))
+        out.append("    static constexpr int NE = %d, NF = %d, NACC = %d, NOUT = %d;" %
+                   (len(self.E), len(self.F), len(self.E) * len(self.F), na * nb * nc * nd))
+        out.append("    static constexpr int PRIM_FLOPS = %d, HRR_FLOPS = %d;" % (self.flops_prim, self.flops_hrr))
+        out.append("    template <typename R>")
+        out.append("    __device__ __forceinline__ static void prim(const R* __restrict__ F, R PAx, R PAy, R PAz, "
+                   "R WPx, R WPy, R WPz, R inv2z, R rz, R dlx, R dly, R dlz, R zoe, R inv2e, R* __restrict__ acc) {")
+        for l in prim:
+            out.append(l.replace("const double ", "const R "))
+        out.append("        (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz; (void)inv2z; (void)rz;")
+        out.append("        (void)dlx; (void)dly; (void)dlz; (void)zoe; (void)inv2e;")
+        out.append("    }")
+        out.append("    template <typename R>")
+        out.append("    __device__ __forceinline__ static void hrr(const R* __restrict__ acc, R ABx, R ABy, R ABz, "
+                   "R CDx, R CDy, R CDz, R* __restrict__ out) {")
+        for l in hrr:
+            out.append(l.replace("const double ", "const R "))
+        out.append("        (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;")
+        out.append("    }")
+        out.append("};")
+        return "\n".join(out)
+
+
+def classes(lmax):
+    pairs = [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
+    pairs.sort(key=lambda p: (p[0] + p[1], p[0]))
+    out = []
+    for i, (la, lb) in enumerate(pairs):
+        for j in range(i + 1):
+            lc, ld = pairs[j]
+            out.append((la, lb, lc, ld))
+    return pairs, out
+
+
+def main():
+    outdir = sys.argv[1]
+    lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    import os
+    os.makedirs(outdir, exist_ok=True)
+    pairs, cls = classes(lmax)
+    parts = ["// Generated by gen_eri.py - do not edit.", "#pragma once", ""]
+    parts.append("template <int LA, int LB, int LC, int LD, int EXACT> struct EriClass;")
+    parts.append("")
+    info = []
+    for (la, lb, lc, ld) in cls:
+        ref = ClassGen(la, lb, lc, ld, False)
+        ex = ClassGen(la, lb, lc, ld, True)
+        parts.append(ref.emit())
+        parts.append(ex.emit())
+        parts.append("")
+        differs = ref.flops_prim != ex.flops_prim
+        info.append((la, lb, lc, ld, differs, ref.flops_prim, ex.flops_prim, ref.flops_hrr))
+    write_if_changed(os.path.join(outdir, "eri_classes.inc"), "\n".join(parts))
+    decl = []
+    table = []
+    for (la, lb, lc, ld, differs, fr, fe, fh) in info:
+        tag = "%d%d%d%d" % (la, lb, lc, ld)
+        diag = (la, lb) == (lc, ld)
+        sig = "(const TaskList& tl, const ShellsDev& sh, const PrimPair* pp, const double* boys, " \
+              "const SinkArgs& s, hipStream_t st)"
+        # one translation unit per (mode, variant) so the large classes compile in parallel
+        variants = []
+        for mode in ("MODE_JK", "MODE_PACKED") + (("MODE_DIAG",) if diag else ()):
+            for ex in ((0, 1) if differs else (0,)):
+                fn = "run_%s_%s_%d" % (tag, mode.split("_")[1].lower(), ex)
+                variants.append((mode, ex, fn))
+                body = ["// Generated by gen_eri.py - do not edit.", '#include "../eri_kernel.hpp"',
+                        "namespace hfgpu {",
+                        "hipError_t %s%s {" % (fn, sig),
+                        "    return launch_eri<%d, %d, %d, %d, %d, %s>(tl, sh, pp, boys, s, st);" %
+                        (la, lb, lc, ld, ex, mode),
+                        "}", "}  // namespace hfgpu"]
+                write_if_changed(os.path.join(outdir, "k_%s.hip" % fn[4:]), "\n".join(body) + "\n")
+        body = ["// Generated by gen_eri.py - do not edit.", '#include "../hf_device.h"', "namespace hfgpu {"]
+        for (_, _, fn) in variants:
+            body.append("hipError_t %s%s;" % (fn, sig))
+        body.append("hipError_t launch_class_%s(int mode, int exact, const TaskList& tl, const ShellsDev& sh, "
+                    "const PrimPair* pp, const double* boys, const SinkArgs& s, hipStream_t st) {" % tag)
+        for (mode, ex, fn) in variants:
+            cond = "mode == %s" % mode
+            if differs:
+                cond += " && %sexact" % ("" if ex else "!")
+            body.append("    if (%s) return %s(tl, sh, pp, boys, s, st);" % (cond, fn))
+        body.append("    (void)exact;")
+        body.append("    return hipErrorInvalidValue;")
+        body.append("}")
+        body.append("}  // namespace hfgpu")
+        write_if_changed(os.path.join(outdir, "d_%s.hip" % tag), "\n".join(body) + "\n")
+        decl.append("hipError_t launch_class_%s(int, int, const TaskList&, const ShellsDev&, const PrimPair*, "
+                    "const double*, const SinkArgs&, hipStream_t);" % tag)
+        table.append("    {%d, %d, %d, %d, %d, %d, %d, %d, launch_class_%s}," % (la, lb, lc, ld, 1 if differs else 0,
+                                                                             fr, fe, fh, tag))
+    disp = ["// Generated by gen_eri.py - do not edit.", "#pragma once", '#include "../hf_device.h"',
+            "namespace hfgpu {"]
+    disp += decl
+    disp.append("typedef hipError_t (*LaunchFn)(int, int, const TaskList&, const ShellsDev&, const PrimPair*, "
+                "const double*, const SinkArgs&, hipStream_t);")
+    disp.append("struct ClassInfo { int la, lb, lc, ld, exact_differs, prim_flops_ref, prim_flops_exact, hrr_flops; "
+                "LaunchFn fn; };")
+    disp.append("constexpr int kLmax = %d;" % lmax)
+    disp.append("constexpr int kNumPairClasses = %d;" % len(pairs))
+    disp.append("static const int kPairClassL[%d][2] = {%s};" % (len(pairs), ", ".join("{%d, %d}" % p for p in pairs)))
+    disp.append("static const ClassInfo kClasses[] = {")
+    disp += table
+    disp.append("};")
+    disp.append("constexpr int kNumClasses = %d;" % len(info))
+    disp.append("}  // namespace hfgpu")
+    write_if_changed(os.path.join(outdir, "dispatch.inc"), "\n".join(disp) + "\n")
+
+
+def write_if_changed(path, text):
+    """Keep timestamps (and so incremental builds) stable when nothing changed."""
+    try:
+        with open(path) as fh:
+            if fh.read() == text:
+                return
+    except OSError:
+        pass
+    with open(path, "w") as fh:
+        fh.write(text)
+
+
+if __name__ == "__main__":
+    main()

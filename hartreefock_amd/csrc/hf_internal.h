@@ -1,0 +1,148 @@
+// Host-side internals of libhfgpu: molecule model, context state, helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/hfgpu.h"
+#include "hf_device.h"
+
+namespace hfhost {
+
+// One shell after splitting SP shells: L, centre, primitives with coef = c * radial norm.
+struct Shell {
+    int L = 0, atom = 0, bf = 0;
+    double xyz[3] = {0, 0, 0};
+    std::vector<double> alpha, coef;
+};
+
+struct Molecule {
+    std::vector<int> Z;
+    std::vector<double> xyz;      // 3 * natoms
+    std::vector<Shell> shells;
+    int nbf = 0;
+    int maxL = 0;
+    int electrons() const {
+        int e = 0;
+        for (int z : Z) e += z;
+        return e;
+    }
+};
+
+// Parse an NWChem basis file and build the molecule's shell list (basis.cpp).
+bool load_molecule(const char* basis_file, int natoms, const int* Z, const double* xyz, Molecule& mol,
+                   std::string& err);
+bool molecule_from_soa(const hf_basis_soa* b, Molecule& mol, std::string& err);
+double radial_norm(double alpha, int L);
+double nuclear_repulsion(const Molecule& mol);
+
+// Boys functions F_0..F_mmax(T) on the host (boys.cpp).
+void boys_host(int mmax, double T, double* F);
+// Table F_m(k*dt) for the device kernels.
+void boys_table(std::vector<double>& tab);
+
+// One-electron matrices (one_electron.cpp).
+void one_electron(const Molecule& mol, double* S, double* T, double* V);
+
+inline int ncart(int L) { return (L + 1) * (L + 2) / 2; }
+
+}  // namespace hfhost
+
+// Device buffer helper (owned).
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e == hipSuccess) n = count; else p = nullptr;
+        return e;
+    }
+};
+
+struct QuartetClassTasks {
+    int cls = -1;             // index into kClasses
+    int bra = -1, ket = -1;   // pair-class indices
+    int64_t ntasks = 0;
+    int64_t prim_quartets = 0;
+    size_t prefix_off = 0;    // offset of this class's prefix array in the shared device buffer
+};
+
+struct TaskSet {
+    double threshold = -1;
+    std::vector<QuartetClassTasks> classes;
+    DevBuf<int64_t> d_prefix;
+};
+
+struct PairClassHost {
+    std::vector<int> sa, sb, prim0, nprim;
+    std::vector<double> Q;
+    DevBuf<int> d_sa, d_sb, d_prim0, d_nprim;
+    DevBuf<double> d_Q;
+    int npairs() const { return (int)sa.size(); }
+};
+
+struct KernelStat {
+    int cls;
+    int64_t quartets, prim_quartets;
+    double flops, ms;
+};
+
+struct hf_ctx {
+    hf_options opt;
+    std::string err;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    bool have_mol = false;
+    hfhost::Molecule mol;
+    // device molecule data
+    DevBuf<double> d_shell_xyz;
+    DevBuf<int> d_shell_bf;
+    DevBuf<hfgpu::PrimPair> d_pp;
+    DevBuf<double> d_boys;
+    std::vector<PairClassHost> pcls;
+    TaskSet tasks_screened, tasks_all;
+    // J/K scratch
+    DevBuf<double> d_Jp, d_Kp, d_D, d_J, d_K;
+    std::vector<KernelStat> stats;
+    std::vector<hipEvent_t> ev;
+    double last_jk_ms = 0;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+};
+
+// Implemented in context.hip
+int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st);
+int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts);
+
+#define HF_HIP_CHECK(ctx, call)                                                                         \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return (ctx)->fail(HF_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)

@@ -1,0 +1,164 @@
+/*
+ * hfgpu.h — C ABI of the MI355X (gfx950) two-electron integral / J-K Fock-build
+ * engine and the device-resident SCF built on it.
+ *
+ * The reference (aromanro/HartreeFock) has no FFI: its hot path is reached through
+ * in-process C++ classes.  This header is the thin C layer those classes sit on;
+ * each entry point names the reference member it replaces.  Conventions:
+ *   - every function returns HF_OK (0) or a negative HF_E_* code; hf_last_error()
+ *     gives the message; no C++ exception crosses this boundary;
+ *   - matrices are n x n doubles, row-major (all exchanged matrices are symmetric);
+ *   - coordinates are bohr, as in the reference's Molecule;
+ *   - basis functions are ordered atom -> shell (file order) -> Cartesian component
+ *     (xx,xy,xz,yy,yz,zz), the order of HartreeFock/Molecule.cpp:41-72;
+ *   - one context per host thread; a context owns all device memory it uses.
+ */
+#ifndef HFGPU_H
+#define HFGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HF_OK 0
+#define HF_E_ARG (-1)
+#define HF_E_OOM (-2)
+#define HF_E_HIP (-3)
+#define HF_E_NOT_CONVERGED (-4)
+#define HF_E_CANCELLED (-5)
+#define HF_E_IO (-6)
+#define HF_E_STATE (-7)
+#define HF_E_UNSUPPORTED (-8)
+
+#define HF_PREC_FP64 0
+#define HF_PREC_MIXED 1
+
+typedef struct hf_ctx hf_ctx;
+
+typedef struct hf_options {
+    int device;                /* HIP device ordinal */
+    double schwarz_threshold;  /* direct J/K: skip quartets with Q_ab*Q_cd below this (0 = none) */
+    double prim_threshold;     /* drop primitive pairs with |c_a c_b N_a N_b e^{-ab/p AB^2}| below this */
+    int exact_transfer;        /* 0: the reference's ERI convention (see hf_eri_packed); 1: exact ERIs */
+    int precision;             /* HF_PREC_FP64 or HF_PREC_MIXED */
+    double fp32_threshold;     /* mixed: quartets whose bound is below this run in FP32 */
+    int profile;               /* 1: time every class kernel with HIP events (hf_kernel_stats) */
+} hf_options;
+
+void hf_default_options(hf_options* opt);
+
+/* Context lifetime.  hf_create replaces constructing a HartreeFockAlgorithm (its
+ * IntegralsRepository member, HartreeFock/HartreeFockAlgorithm.h:51). */
+int hf_create(const hf_options* opt, hf_ctx** out);
+int hf_destroy(hf_ctx* ctx);
+const char* hf_last_error(const hf_ctx* ctx);
+
+/* Shells with SP shells already split into S and P (same exponents).  coef holds the
+ * contraction coefficient times the radial normalisation (2a/pi)^{3/4} (4a)^{L/2};
+ * the per-component factor 1/sqrt((2l-1)!!(2m-1)!!(2n-1)!!) is applied by the library
+ * (together these are the reference's per-primitive factor, HartreeFock/GaussianOrbital.cpp:25-30). */
+typedef struct hf_basis_soa {
+    int nshell;
+    int nbf;
+    const int* L;              /* nshell */
+    const int* nprim;          /* nshell */
+    const int* prim_offset;    /* nshell: first primitive in alpha/coef */
+    const int* bf_offset;      /* nshell: first basis function */
+    const int* atom;           /* nshell: atom index */
+    const double* xyz;         /* 3*nshell, bohr */
+    const double* alpha;       /* exponents */
+    const double* coef;        /* coefficient x radial normalisation */
+    int natoms;
+    const int* Z;              /* natoms: nuclear charges */
+    const double* atom_xyz;    /* 3*natoms, bohr */
+} hf_basis_soa;
+
+/* Replaces IntegralsRepository::Reset(Molecule*) (HartreeFock/IntegralsRepository.cpp:19-27):
+ * installs a molecule, uploads shells, primitive pairs and the Schwarz table. */
+int hf_set_basis(hf_ctx* ctx, const hf_basis_soa* basis);
+/* Same, parsing an NWChem basis file as Chemistry::Basis::Load (HartreeFock/Basis.cpp:15-158)
+ * and placing atoms as Molecule::LoadXYZ (HartreeFock/Molecule.cpp:143-199). */
+int hf_set_molecule(hf_ctx* ctx, const char* basis_file, int natoms, const int* Z, const double* xyz);
+int hf_nbf(const hf_ctx* ctx);
+int hf_nshell(const hf_ctx* ctx);
+/* Electrons as Molecule::Init (HartreeFock/Molecule.cpp:127-140): alpha = N/2, beta = N - alpha. */
+int hf_electrons(const hf_ctx* ctx, int* alpha, int* beta);
+double hf_nuclear_repulsion(const hf_ctx* ctx);   /* Molecule::NuclearRepulsionEnergy */
+
+/* Packed unique ERIs at index GetElectronElectronIndex(i,j,k,l)
+ * (HartreeFock/IntegralsRepository.h:151-162).  Replaces
+ * IntegralsRepository::CalculateElectronElectronIntegrals (HartreeFock/IntegralsRepository.cpp:566-584)
+ * followed by reads through getElectronElectron(int,int,int,int) (:138-141).
+ * With exact_transfer = 0 the (dd|dd) values follow the reference's electron-transfer
+ * range (HartreeFock/GaussianTwoElectrons.cpp:152), which leaves the rows |e|=2 at
+ * |f|=4 at zero; exact_transfer = 1 gives the exact integrals. */
+uint64_t hf_packed_index(uint32_t i, uint32_t j, uint32_t k, uint32_t l);
+uint64_t hf_packed_len(int nbf);           /* GetElectronElectronIndex(n,n,n,n)+1, as the reference sizes it */
+int hf_eri_packed(hf_ctx* ctx, double* out, uint64_t len);                 /* host buffer */
+int hf_eri_packed_device(hf_ctx* ctx, double* d_out, uint64_t len, void* stream);
+
+/* Direct J/K build, replacing the N^4 loops of RestrictedHartreeFock::InitFockMatrix
+ * (HartreeFock/RestrictedHartreeFock.cpp:189-202) and UnrestrictedHartreeFock::InitFockMatrices
+ * (HartreeFock/UnrestrictedHartreeFock.cpp:228-238):
+ *   J  = J[D_0 + ... + D_{ndens-1}],  J_ij = sum_kl (ij|kl) D_kl
+ *   K_d = K[D_d],                      K_ij = sum_kl (il|kj) D_kl
+ * ndens is 1 (RHF) or 2 (UHF: alpha and beta in one pass).  D is ndens*n*n, J n*n,
+ * K ndens*n*n.  Quartets with Q_ab*Q_cd < schwarz_threshold are skipped. */
+int hf_build_jk(hf_ctx* ctx, int ndens, const double* D, double* J, double* K);             /* host */
+int hf_build_jk_device(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K,
+                       void* stream);                                                       /* device */
+
+/* One-electron matrices S, T, V (HartreeFock/QuantumMatrix.cpp:34-155). Host buffers, n*n. */
+int hf_one_electron(hf_ctx* ctx, double* S, double* T, double* V);
+
+/* SCF, replacing RestrictedHartreeFock / UnrestrictedHartreeFock Init() + Calculate()
+ * (HartreeFock/HartreeFockAlgorithm.cpp:21-192).  Field meanings and defaults follow the
+ * reference library (HartreeFock/HartreeFockAlgorithm.cpp:10-11, UnrestrictedHartreeFock.cpp:9). */
+typedef struct hf_scf_params {
+    int unrestricted;              /* 0 RHF, 1 UHF */
+    int max_iterations;            /* 3000 */
+    double alpha;                  /* 0.75: density mixing when DIIS is not used */
+    double init_guess;             /* 0.75: iteration-0 F = init_guess*S_ij*(h_ii+h_jj)/2; 0 -> F = h */
+    int use_diis;                  /* 1 */
+    int max_diis_iterations;       /* 1000 */
+    int normal_iter_after_diis;    /* 500 */
+    int add_asymmetry;             /* UHF: 1 */
+    double asymmetry;              /* UHF: 0.1 */
+    double energy_tolerance;       /* 0: the reference's (1e-11 with DIIS, 1e-13 without) */
+    double density_tolerance;      /* 0: the reference's 1e-10 (Frobenius norm of the change) */
+    double diis_tolerance;         /* 0: the reference's 1e-10 */
+    volatile int* terminate;       /* optional cancellation flag, polled once per iteration */
+} hf_scf_params;
+
+typedef struct hf_scf_result {
+    double energy;                 /* total energy, Eh */
+    double homo;                   /* HOMO energy, Eh */
+    double s2;                     /* <S^2> for UHF (not in the reference; 0 for RHF) */
+    int iterations;
+    int converged;
+    double jk_ms;                  /* time in J/K builds */
+    double total_ms;               /* wall time of hf_scf */
+    int fp32_builds;               /* mixed precision: J/K builds that used FP32 quartets */
+} hf_scf_result;
+
+void hf_scf_defaults(hf_scf_params* p);
+/* D_out (optional): n*n (RHF total density) or 2*n*n (UHF alpha then beta). */
+int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out);
+
+/* Per-class kernel statistics of the last J/K build or packed-ERI pass. */
+typedef struct hf_kernel_stat {
+    int la, lb, lc, ld;            /* angular class (la lb | lc ld) */
+    int64_t quartets;              /* contracted shell quartets evaluated */
+    int64_t prim_quartets;         /* primitive quartets evaluated */
+    double flops;                  /* algorithmic FP64 flops (generated-code operation count) */
+    double ms;                     /* kernel time from HIP events (profile = 1), else 0 */
+} hf_kernel_stat;
+int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max);   /* returns count */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+"""Pin the CPU oracle before trusting it: golden vectors from an independent
+McMurchie-Davidson code, the reference's own known answers, textbook values,
+closed forms and invariants.  CPU only."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import basis
+from hartreefock_amd import geometry as g
+from oracle import refcpu
+import md_eri
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def load_eris(name):
+    out = []
+    for line in open(os.path.join(GOLD, name)):
+        if line.startswith("#"):
+            continue
+        i, j, k, l, v = line.split()
+        out.append((int(i) - 1, int(j) - 1, int(k) - 1, int(l) - 1, float(v)))
+    return out
+
+
+def load_matrix(name, n):
+    M = np.zeros((n, n))
+    for line in open(os.path.join(GOLD, name)):
+        if line.startswith("#"):
+            continue
+        i, j, v = line.split()
+        M[int(i) - 1, int(j) - 1] = M[int(j) - 1, int(i) - 1] = float(v)
+    return M
+
+
+# ---------------------------------------------------------------- Boys
+def test_boys_matches_mpmath():
+    """Oracle Boys (incomplete gamma + downward recursion, maxM = 9 as for d bases)
+    against 40-digit mpmath; relative 1e-13.  The reference clamps T < 1e-10 to 1e-10
+    for the top order only (HartreeFock/BoysFunction.cpp:13-14); the downward recursion
+    uses the true T, so orders below the top are exact and the top order (never read
+    by the vertical recursion, which needs m <= 4 Lmax) equals F_9(1e-10)."""
+    rows = [l.split() for l in open(os.path.join(GOLD, "boys_mpmath.txt")) if not l.startswith("#")]
+    table = {}
+    for T, m, v in rows:
+        table.setdefault(float(T), {})[int(m)] = float(v)
+    worst = 0.0
+    for T, vals in table.items():
+        f = refcpu.boys(9, T)
+        for m in range(10):
+            ref = vals[m]
+            if m == 9 and T < 1e-10:
+                ref = md_eri.boys_mp(9, 1e-10)
+            worst = max(worst, abs(f[m] - ref) / ref)
+    assert worst < 1e-13, worst
+
+
+def test_boys_closed_forms():
+    f = refcpu.boys(9, 0.0)
+    for m in range(9):                       # F_m(0) = 1/(2m+1) below the clamped top order
+        assert abs(f[m] - 1.0 / (2 * m + 1)) < 1e-15
+    for T in (0.3, 2.0, 11.0):
+        f0 = refcpu.boys(9, T)[0]
+        assert abs(f0 - 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))) < 1e-14
+
+
+# ---------------------------------------------------------------- known answers
+def test_nuclear_repulsion_known_answers():
+    """HartreeFock/Test.cpp:623 and :781."""
+    m = refcpu.Molecule(basis("sto3g.txt"), g.water_crawford())
+    assert abs(m.nuclear_repulsion() - 8.002367061810450) < 1e-11
+    m = refcpu.Molecule(basis("sto3g.txt"), g.methane_crawford())
+    assert abs(m.nuclear_repulsion() - 13.497304462036480) < 1e-11
+
+
+def test_crawford_water_scf():
+    """Crawford Project #3 H2O/STO-3G SCF energy -74.942079928192 (the external target the
+    reference's harness compares against, HartreeFock/Test.cpp:32-34)."""
+    m = refcpu.Molecule(basis("sto3g.txt"), g.water_crawford())
+    r, _ = m.scf(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+    assert r.converged
+    assert abs(r.energy - (-74.942079928192)) < 1e-9
+
+
+def test_h2_szabo_ostlund():
+    """Szabo & Ostlund STO-3G H2 at R = 1.4 bohr: S12 = 0.6593, T11 = 0.7600, T12 = 0.2365,
+    E = -1.1167 Eh.  (With DIIS the reference's solver meets an exactly singular DIIS
+    system for this 2-function case, so DIIS is off here.)"""
+    m = refcpu.Molecule(basis("sto3g.txt"), [(1, (0, 0, 0)), (1, (0, 0, 1.4))])
+    S, T, V = m.one_electron()
+    assert abs(S[0, 1] - 0.6593) < 1e-4 and abs(T[0, 0] - 0.7600) < 1e-4 and abs(T[0, 1] - 0.2365) < 1e-4
+    assert abs(V[0, 0] - (-1.8804)) < 1e-4 and abs(V[0, 1] - (-1.1948)) < 1e-4
+    assert abs(m.eri(0, 0, 0, 0) - 0.7746) < 1e-4 and abs(m.eri(0, 0, 1, 1) - 0.5697) < 1e-4
+    assert abs(m.eri(1, 0, 0, 0) - 0.4441) < 1e-4 and abs(m.eri(1, 0, 1, 0) - 0.2970) < 1e-4
+    r, _ = m.scf(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0, use_diis=0)
+    assert abs(r.energy - (-1.1167)) < 1e-4
+
+
+def test_h_atom_uhf():
+    """H atom, STO-3G, UHF: E = <1s|h|1s> = -0.4666 Eh (Szabo & Ostlund), <S^2> = 3/4."""
+    m = refcpu.Molecule(basis("sto3g.txt"), [(1, (0, 0, 0))])
+    r, _ = m.scf(unrestricted=True, normal_iter_after_diis=0)
+    assert abs(r.energy - (-0.4666)) < 1e-4
+    assert abs(r.s2 - 0.75) < 1e-12
+
+
+# ---------------------------------------------------------------- golden vectors
+def test_eri_h2o_sto3g_all_unique_vs_md():
+    m = refcpu.Molecule(basis("sto3g.txt"), g.water_cfg1())
+    packed = m.eri_packed()
+    gold = load_eris("eri_h2o_sto3g.txt")
+    assert len(gold) == 406
+    err = max(abs(packed[refcpu.packed_index(i, j, k, l)] - v) for i, j, k, l, v in gold)
+    assert err < 1e-13, err
+
+
+def test_eri_h2o_631gs_sample_vs_md():
+    m = refcpu.Molecule(basis("6-31g_st_.1.nw"), g.water_cfg1())
+    for i, j, k, l, v in load_eris("eri_h2o_631gs_sample.txt"):
+        assert abs(m.eri(i, j, k, l) - v) < 1e-13
+
+
+@pytest.mark.parametrize("bname,tag", [("sto3g.txt", "sto3g"), ("6-31g_st_.1.nw", "631gs")])
+def test_one_electron_vs_md(bname, tag):
+    m = refcpu.Molecule(basis(bname), g.water_cfg1())
+    S, T, V = m.one_electron()
+    n = m.nbf
+    for name, M in (("s", S), ("t", T), ("v", V)):
+        G = load_matrix("%s_h2o_%s.txt" % (name, tag), n)
+        assert np.abs(M - G).max() < 1e-12, name
+
+
+def test_dddd_transfer_convention():
+    """The reference fills electron-transfer rows |e| >= |f|-1 only
+    (HartreeFock/GaussianTwoElectrons.cpp:152).  (dd|dd) needs |e|=2 at |f|=4, whose
+    coefficient in the bra HRR is proportional to AB^2: with a one-centre bra the
+    reference convention is exact, with a two-centre bra it is not.  Exact mode
+    reproduces the independent MD values everywhere."""
+    atoms = [(6, (0.0, 0.0, 0.0)), (6, (0.3, 1.2, 2.5))]
+    m = refcpu.Molecule(basis("6-31g_st_.1.nw"), atoms)
+    _, _, at, _ = m.bf_info()
+    gold = load_eris("eri_c2_631gs_dddd.txt")
+    two_centre_diff = 0.0
+    for i, j, k, l, v in gold:
+        refcpu.set_exact_transfer(True)
+        ex = m.eri(i, j, k, l)
+        refcpu.set_exact_transfer(False)
+        rv = m.eri(i, j, k, l)
+        assert abs(ex - v) < 1e-13
+        if at[i] == at[j] and at[k] == at[l]:
+            assert abs(rv - v) < 1e-13             # both pairs one-centre: no omitted term survives
+        else:
+            two_centre_diff = max(two_centre_diff, abs(rv - v))
+    assert two_centre_diff > 1e-4
+
+
+# ---------------------------------------------------------------- invariants
+def test_eri_permutational_symmetry_and_schwarz():
+    m = refcpu.Molecule(basis("6-31g_st_.1.nw"), g.water_cfg1())
+    rng = np.random.default_rng(3)
+    n = m.nbf
+    for _ in range(40):
+        i, j, k, l = (int(x) for x in rng.integers(0, n, size=4))
+        v = m.eri(i, j, k, l)
+        for p in ((j, i, k, l), (i, j, l, k), (k, l, i, j), (l, k, j, i)):
+            assert abs(m.eri(*p) - v) < 1e-14
+        assert abs(v) <= math.sqrt(m.eri(i, j, i, j) * m.eri(k, l, k, l)) + 1e-14
+        assert m.eri(i, j, i, j) >= -1e-15
+
+
+def test_eri_translation_invariance():
+    atoms = g.water_cfg1()
+    shifted = [(z, (x + 0.37, y - 1.1, w + 2.0)) for z, (x, y, w) in atoms]
+    a = refcpu.Molecule(basis("6-31g_st_.1.nw"), atoms).eri_packed()
+    b = refcpu.Molecule(basis("6-31g_st_.1.nw"), shifted).eri_packed()
+    assert np.abs(a - b).max() < 1e-12
+
+
+def test_jk_from_packed_matches_dense_contraction():
+    m = refcpu.Molecule(basis("sto3g.txt"), g.water_cfg1())
+    n = m.nbf
+    packed = m.eri_packed()
+    full = np.zeros((n, n, n, n))
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for l in range(n):
+                    full[i, j, k, l] = packed[refcpu.packed_index(i, j, k, l)]
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(n, n))
+    D = X + X.T
+    J, K = refcpu.jk_from_packed(n, packed, D)
+    assert np.abs(J - np.einsum("ijkl,kl->ij", full, D)).max() < 1e-12
+    assert np.abs(K - np.einsum("ilkj,kl->ij", full, D)).max() < 1e-12
+
+
+def test_scf_energy_is_variational_minimum_and_trace():
+    """RHF density: Tr(D S) = N_e; converged energy below the core-guess energy."""
+    m = refcpu.Molecule(basis("sto3g.txt"), g.water_cfg1())
+    r, D = m.scf(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+    S, T, V = m.one_electron()
+    assert abs(np.trace(D[0] @ S) - 10.0) < 1e-8
+    assert r.converged and r.energy < -74.9
+
+
+# ---------------------------------------------------------------- workload sizes
+def test_config_basis_sizes():
+    """Basis sizes of the benchmark workloads (n = 7 / 194 / 800 / 462)."""
+    assert refcpu.Molecule(basis("sto3g.txt"), g.water_cfg1()).nbf == 7
+    assert refcpu.Molecule(basis("6-31g_st_.1.nw"), g.decane_cfg2()).nbf == 194
+    assert refcpu.Molecule(basis("6-31g_st__st_.nw"), g.water32_cfg3()).nbf == 800
+    assert refcpu.Molecule(basis("6-31+g_st_.1.nw"), g.eicosyl_cfg5()).nbf == 462
+    e = refcpu.Molecule(basis("6-31+g_st_.1.nw"), g.eicosyl_cfg5()).electrons()
+    assert e == (80, 81)
+
+
+def test_serving_mix_sizes():
+    mols = g.serving_mix(60, seed=1)
+    sizes = [refcpu.Molecule(basis("6-31g.1.nw"), atoms).nbf for _, atoms in mols]
+    assert min(sizes) >= 13 and max(sizes) <= 212
+
+
+def test_geometry_generators():
+    atoms = g.alkane_zigzag(10, seed=0, jitter=0.0)
+    c = np.array([p for z, p in atoms if z == 6]) * g.BOHR_ANGSTROM
+    d = np.linalg.norm(np.diff(c, axis=0), axis=1)
+    assert np.allclose(d, 1.54, atol=1e-12)
+    w = g.water_cluster(32, seed=2024, box=10.0)
+    o = np.array([p for z, p in w if z == 8]) * g.BOHR_ANGSTROM
+    dist = np.linalg.norm(o[:, None] - o[None], axis=-1) + np.eye(len(o)) * 99
+    assert dist.min() >= 2.6 - 1e-9
