@@ -12,7 +12,7 @@ EXPORTED_SYMBOLS = [
     "hf_default_options", "hf_create", "hf_destroy", "hf_last_error", "hf_set_basis", "hf_set_molecule",
     "hf_nbf", "hf_nshell", "hf_electrons", "hf_nuclear_repulsion", "hf_packed_index", "hf_packed_len",
     "hf_eri_packed", "hf_eri_packed_device", "hf_build_jk", "hf_build_jk_device", "hf_one_electron",
-    "hf_scf_defaults", "hf_scf", "hf_kernel_stats",
+    "hf_scf_defaults", "hf_scf", "hf_kernel_stats", "hf_kernel_stats_reset",
 ]
 
 
@@ -55,8 +55,8 @@ class ScfResult(ctypes.Structure):
 
 class KernelStat(ctypes.Structure):
     _fields_ = [("la", ctypes.c_int), ("lb", ctypes.c_int), ("lc", ctypes.c_int), ("ld", ctypes.c_int),
-                ("quartets", ctypes.c_int64), ("prim_quartets", ctypes.c_int64), ("flops", ctypes.c_double),
-                ("ms", ctypes.c_double)]
+                ("launches", ctypes.c_int64), ("quartets", ctypes.c_int64), ("prim_quartets", ctypes.c_int64),
+                ("eris", ctypes.c_double), ("flops", ctypes.c_double), ("ms", ctypes.c_double)]
 
     def name(self):
         s = "spdfg"
@@ -103,6 +103,7 @@ def lib():
     L.hf_scf_defaults.restype = None
     L.hf_scf.argtypes = [vp, ctypes.POINTER(ScfParams), ctypes.POINTER(ScfResult), dp]
     L.hf_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int]
+    L.hf_kernel_stats_reset.argtypes = [vp]
     _LIB = L
     return L
 
@@ -226,3 +227,6 @@ class Engine:
         arr = (KernelStat * 64)()
         k = lib().hf_kernel_stats(self.h, arr, 64)
         return [arr[i] for i in range(k)]
+
+    def reset_kernel_stats(self):
+        self._check(lib().hf_kernel_stats_reset(self.h))

@@ -88,6 +88,18 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
             raise RuntimeError("link failed: %s\n%s" % (p.stdout, p.stderr))
         os.replace(LIB + ".tmp", LIB)
     if oracle:
+        # test harness: generated class code compiled for the host (CPU checks of the kernel math)
+        hsrc = os.path.join(ROOT, "tests", "cpu_kernel", "quartet_host.hip")
+        hlib = os.path.join(ROOT, "tests", "cpu_kernel", "libquartet_host.so")
+        hdeps = kdeps + [hsrc, os.path.join(GEN, "host_eval.inc"), os.path.join(CSRC, "boys.cpp")]
+        if force or _newer(hdeps, hlib):
+            cmd = [HIPCC] + COMMON[1:] + ["-O1", "-x", "hip", "--offload-arch=" + ARCH, "-shared", "-o", hlib + ".tmp",
+                                          hsrc, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                                          os.path.join(CSRC, "boys.cpp")]
+            p = subprocess.run(cmd, capture_output=True, text=True)
+            if p.returncode != 0:
+                raise RuntimeError("host harness build failed:\n%s%s" % (p.stdout, p.stderr))
+            os.replace(hlib + ".tmp", hlib)
         p = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)
         if p.returncode != 0:
             raise RuntimeError("oracle build failed:\n%s%s" % (p.stdout, p.stderr))

@@ -65,11 +65,14 @@ PairClassDev dev_view(const PairClassHost& pc) {
     return d;
 }
 
-// Launch every quartet class of a task set in `mode`.
+// Launch every quartet class of a task set in `mode`; with record, add to the per-class statistics.
 int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, hipStream_t st, bool record) {
     ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
-    if (record) ctx->stats.clear();
-    const bool prof = ctx->opt.profile != 0;
+    if (ctx->stats.size() != (size_t)kNumClasses) {
+        ctx->stats.assign(kNumClasses, KernelStat{});
+        for (int c = 0; c < kNumClasses; ++c) ctx->stats[c].cls = c;
+    }
+    const bool prof = ctx->opt.profile != 0 && record;
     if (prof && ctx->ev.size() < 2 * (size_t)kNumClasses) {
         for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
         ctx->ev.assign(2 * kNumClasses, nullptr);
@@ -93,28 +96,28 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
         if (record) {
             const int pf = ctx->opt.exact_transfer && ci.exact_differs ? ci.prim_flops_exact : ci.prim_flops_ref;
             const int L = ci.la + ci.lb + ci.lc + ci.ld;
-            // Boys (Taylor 14 + recursion 4/m + scaling 1/m) and the pair-geometry terms (~40) per primitive quartet
-            const double boys_flops = 14 + 5.0 * L + 1 + 40;
+            // per primitive quartet: Boys (8-term Taylor 22, recursion 4/order, scaling 1/order)
+            // and the quartet geometry/prefactor terms (36)
+            const double boys_flops = 22 + 5.0 * L + 1 + 36;
             const int na = (ci.la + 1) * (ci.la + 2) / 2, nb = (ci.lb + 1) * (ci.lb + 2) / 2;
             const int nc = (ci.lc + 1) * (ci.lc + 2) / 2, nd = (ci.ld + 1) * (ci.ld + 2) / 2;
             const double nout = (double)na * nb * nc * nd;
             double sink_flops = 0;
             if (mode == MODE_JK) sink_flops = nout * 2.0 * (2 + 4 * sink.ndens);   // 2 J + 4 K contractions per density
-            KernelStat ks;
-            ks.cls = qc.cls;
-            ks.quartets = qc.ntasks;
-            ks.prim_quartets = qc.prim_quartets;
-            ks.flops = (double)qc.prim_quartets * (pf + boys_flops) + (double)qc.ntasks * (ci.hrr_flops + sink_flops);
-            ks.ms = 0;
-            ctx->stats.push_back(ks);
+            KernelStat& ks = ctx->stats[qc.cls];
+            ks.launches += 1;
+            ks.quartets += qc.ntasks;
+            ks.prim_quartets += qc.prim_quartets;
+            ks.eris += (double)qc.ntasks * nout;
+            ks.flops += (double)qc.prim_quartets * (pf + boys_flops) + (double)qc.ntasks * (ci.hrr_flops + sink_flops);
         }
     }
-    if (prof && record) {
+    if (prof) {
         HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
-        for (size_t i = 0; i < launched.size(); ++i) {
+        for (int c : launched) {
             float ms = 0;
-            HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * launched[i]], ctx->ev[2 * launched[i] + 1]));
-            ctx->stats[i].ms = ms;
+            HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * c], ctx->ev[2 * c + 1]));
+            ctx->stats[c].ms += ms;
         }
     }
     return HF_OK;
@@ -468,22 +471,35 @@ int hf_one_electron(hf_ctx* ctx, double* S, double* T, double* V) {
 }
 
 int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max) {
-    if (!ctx) return 0;
+    if (!ctx || !out) return 0;
     int k = 0;
     for (const KernelStat& s : ctx->stats) {
         if (k >= max) break;
+        if (s.launches == 0) continue;
         const ClassInfo& ci = kClasses[s.cls];
         out[k].la = ci.la;
         out[k].lb = ci.lb;
         out[k].lc = ci.lc;
         out[k].ld = ci.ld;
+        out[k].launches = s.launches;
         out[k].quartets = s.quartets;
         out[k].prim_quartets = s.prim_quartets;
+        out[k].eris = s.eris;
         out[k].flops = s.flops;
         out[k].ms = s.ms;
         ++k;
     }
     return k;
+}
+
+int hf_kernel_stats_reset(hf_ctx* ctx) {
+    if (!ctx) return HF_E_ARG;
+    for (KernelStat& s : ctx->stats) {
+        const int c = s.cls;
+        s = KernelStat{};
+        s.cls = c;
+    }
+    return HF_OK;
 }
 
 }  // extern "C"

@@ -22,9 +22,9 @@ constexpr double kSqrtPiHalf = 0.88622692545275801365;  // sqrt(pi)/2
 
 // F_m(T) * pref for m = 0..L.
 template <int L>
-__device__ __forceinline__ void boys_scaled(double T, double pref, const double* __restrict__ tab, double* F) {
+__host__ __device__ __forceinline__ void boys_scaled(double T, double pref, const double* __restrict__ tab, double* F) {
     if (T < BOYS_TMAX) {
-        const int k = __double2int_rn(T * BOYS_INV_DT);
+        const int k = (int)rint(T * BOYS_INV_DT);
         const double x = k * BOYS_DT - T;          // F_m(T) = sum_j F_{m+j}(T_k) x^j / j!
         const double* row = tab + k * BOYS_NCOL + L;
         double f = row[7];
@@ -42,7 +42,7 @@ __device__ __forceinline__ void boys_scaled(double T, double pref, const double*
     } else {
         const double e = exp(-T);
         const double inv2T = 0.5 / T;
-        F[0] = kSqrtPiHalf * rsqrt(T);
+        F[0] = kSqrtPiHalf / sqrt(T);
 #pragma unroll
         for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * inv2T;
     }
@@ -194,6 +194,73 @@ __device__ __forceinline__ void sink_packed(const double* __restrict__ v, int A,
         }
 }
 
+// One contracted shell quartet: primitive loops (Boys, VRR, transfer, contraction) then HRR and the
+// component normalisation.  A..D are the four centres; out receives NA*NB*NC*ND values, index
+// ((ia*NB + ib)*NC + ic)*ND + id.
+template <class C>
+__host__ __device__ __forceinline__ void eval_quartet(const double* __restrict__ A, const double* __restrict__ B,
+                                                     const double* __restrict__ Cc, const double* __restrict__ Dd,
+                                                     const PrimPair* __restrict__ bra, int npb,
+                                                     const PrimPair* __restrict__ ket, int npk,
+                                                     const double* __restrict__ boys, double* __restrict__ out) {
+    double acc[C::NACC];
+#pragma unroll
+    for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
+    for (int i = 0; i < npb; ++i) {
+        const PrimPair P = bra[i];
+        const double PAx = P.Px - A[0], PAy = P.Py - A[1], PAz = P.Pz - A[2];
+        const double inv2z = 0.5 * P.inv_zeta;
+        for (int j = 0; j < npk; ++j) {
+            const PrimPair Q = ket[j];
+            const double inv_s = 1.0 / (P.zeta + Q.zeta);
+            const double rho = P.zeta * Q.zeta * inv_s;
+            const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
+            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            const double pref = kTwoPi25 * P.Kz * Q.Kz * sqrt(inv_s);
+            double F[C::L + 1];
+            boys_scaled<C::L>(T, pref, boys, F);
+            const double rz = Q.zeta * inv_s;                       // rho / zeta
+            const double zoe = P.zeta * Q.inv_zeta;                 // zeta / eta
+            const double inv2e = 0.5 * Q.inv_zeta;
+            const double dlx = zoe * PAx + (Q.Px - Cc[0]);          // delta = (zeta/eta) PA + QC
+            const double dly = zoe * PAy + (Q.Py - Cc[1]);
+            const double dlz = zoe * PAz + (Q.Pz - Cc[2]);
+            C::prim(F, PAx, PAy, PAz, -rz * PQx, -rz * PQy, -rz * PQz, inv2z, rz, dlx, dly, dlz, zoe, inv2e, acc);
+        }
+    }
+    C::hrr(acc, A[0] - B[0], A[1] - B[1], A[2] - B[2], Cc[0] - Dd[0], Cc[1] - Dd[1], Cc[2] - Dd[2], out);
+}
+
+// A class whose reference convention truncates the transfer step (C::TRUNCATED) has values that
+// depend on which pair is the bra.  The reference chooses per component
+// (HartreeFock/IntegralsRepository.cpp:344-370, 436-475): after ordering each pair by basis-function
+// ID, the bra is the pair with the larger (first ID, second ID).  Pairs are stored with the higher
+// shell index first, so shell indices decide unless both pairs start with the same shell; then the
+// block is evaluated both ways and each component takes its own choice.  Bra and ket classes are
+// equal for every truncated class (L <= 2: only (dd|dd)).
+template <class C>
+__host__ __device__ __forceinline__ void eval_quartet_reference_order(
+    int A, int Cs, const double* __restrict__ xa, const double* __restrict__ xb, const double* __restrict__ xc,
+    const double* __restrict__ xd, const PrimPair* __restrict__ bra, int npb, const PrimPair* __restrict__ ket, int npk,
+    int a0, int b0, int c0, int d0, const double* __restrict__ boys, double* __restrict__ out) {
+    static_assert(C::NA == C::NC && C::NB == C::ND, "truncated classes have equal bra and ket classes");
+    constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+    const bool needP = A >= Cs, needQ = A <= Cs;
+    if (needP) eval_quartet<C>(xa, xb, xc, xd, bra, npb, ket, npk, boys, out);
+    if (!needQ) return;
+    double t[C::NOUT];
+    eval_quartet<C>(xc, xd, xa, xb, ket, npk, bra, npb, boys, t);     // t[c][d][a][b]
+    for (int ia = 0; ia < NA; ++ia)
+        for (int ib = 0; ib < NB; ++ib)
+            for (int ic = 0; ic < NC; ++ic)
+                for (int id = 0; id < ND; ++id) {
+                    const int a = a0 + ia, b = b0 + ib, c = c0 + ic, d = d0 + id;
+                    const int i = a > b ? a : b, j = a > b ? b : a, k = c > d ? c : d, l = c > d ? d : c;
+                    const bool braP = needP && !(i < k || (i == k && j < l));
+                    if (!braP) out[((ia * NB + ib) * NC + ic) * ND + id] = t[((ic * ND + id) * NA + ia) * NB + ib];
+                }
+}
+
 template <int LA, int LB, int LC, int LD, int EXACT, int MODE>
 __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev sh, const PrimPair* __restrict__ pp,
                                                         const double* __restrict__ boys, SinkArgs sink) {
@@ -214,43 +281,17 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
     }
     const int A = tl.bra.sa[p], B = tl.bra.sb[p];
     const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
-    const double Ax = sh.xyz[3 * A], Ay = sh.xyz[3 * A + 1], Az = sh.xyz[3 * A + 2];
-    const double Bx = sh.xyz[3 * B], By = sh.xyz[3 * B + 1], Bz = sh.xyz[3 * B + 2];
-    const double Cx = sh.xyz[3 * Cs], Cy = sh.xyz[3 * Cs + 1], Cz = sh.xyz[3 * Cs + 2];
-    const double Dx = sh.xyz[3 * Ds], Dy = sh.xyz[3 * Ds + 1], Dz = sh.xyz[3 * Ds + 2];
-
-    double acc[C::NACC];
-#pragma unroll
-    for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
-
-    const int pb0 = tl.bra.prim0[p], npb = tl.bra.nprim[p];
-    const int pk0 = tl.ket.prim0[q], npk = tl.ket.nprim[q];
-    for (int i = 0; i < npb; ++i) {
-        const PrimPair P = pp[pb0 + i];
-        const double PAx = P.Px - Ax, PAy = P.Py - Ay, PAz = P.Pz - Az;
-        const double inv2z = 0.5 * P.inv_zeta;
-        for (int j = 0; j < npk; ++j) {
-            const PrimPair Q = pp[pk0 + j];
-            const double inv_s = 1.0 / (P.zeta + Q.zeta);
-            const double rho = P.zeta * Q.zeta * inv_s;
-            const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
-            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = kTwoPi25 * P.Kz * Q.Kz * sqrt(inv_s);
-            double F[C::L + 1];
-            boys_scaled<C::L>(T, pref, boys, F);
-            const double rz = Q.zeta * inv_s;                       // rho / zeta
-            const double zoe = P.zeta * Q.inv_zeta;                 // zeta / eta
-            const double inv2e = 0.5 * Q.inv_zeta;
-            const double dlx = zoe * PAx + (Q.Px - Cx);             // delta = (zeta/eta) PA + QC
-            const double dly = zoe * PAy + (Q.Py - Cy);
-            const double dlz = zoe * PAz + (Q.Pz - Cz);
-            C::prim(F, PAx, PAy, PAz, -rz * PQx, -rz * PQy, -rz * PQz, inv2z, rz, dlx, dly, dlz, zoe, inv2e, acc);
-        }
-    }
-    double out[C::NOUT];
-    C::hrr(acc, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy, Cz - Dz, out);
-
     const int a0 = sh.bf[A], b0 = sh.bf[B], c0 = sh.bf[Cs], d0 = sh.bf[Ds];
+    double out[C::NOUT];
+    if constexpr (C::TRUNCATED) {
+        eval_quartet_reference_order<C>(A, Cs, sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds,
+                                        pp + tl.bra.prim0[p], tl.bra.nprim[p], pp + tl.ket.prim0[q],
+                                        tl.ket.nprim[q], a0, b0, c0, d0, boys, out);
+    } else {
+        eval_quartet<C>(sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds, pp + tl.bra.prim0[p],
+                        tl.bra.nprim[p], pp + tl.ket.prim0[q], tl.ket.nprim[q], boys, out);
+    }
+
     if (MODE == MODE_JK) {
         double w = 1.0;
         if (A == B) w *= 0.5;

@@ -90,6 +90,30 @@ class ClassGen:
         self.flops_prim = 0
         self.flops_hrr = 0
 
+    def needs_zero_rows(self):
+        """True when some intermediate the final block depends on lies in the rows the
+        reference leaves at zero (|e| < |f| - 1)."""
+        if self.exact:
+            return False
+        seen = set()
+        stack = [(e, f) for e in self.E for f in self.F]
+        while stack:
+            e, f = stack.pop()
+            if (e, f) in seen or sum(f) == 0:
+                continue
+            seen.add((e, f))
+            if sum(e) < sum(f) - 1:
+                return True
+            i = direction(f)
+            f1 = add(f, i, -1)
+            stack.append((e, f1))
+            stack.append((add(e, i), f1))
+            if e[i] > 0:
+                stack.append((add(e, i, -1), f1))
+            if f1[i] > 0:
+                stack.append((e, add(f1, i, -1)))
+        return False
+
     # ---------------- primitive part: VRR + ETR + accumulate ----------------
     def etr_zero(self, e, f):
         return (not self.exact) and sum(f) > 0 and sum(e) < sum(f) - 1
@@ -237,7 +261,7 @@ class ClassGen:
         self.flops_hrr = fl
         return lines
 
-    def emit(self):
+    def emit(self, truncated=False):
         name = "EriClass<%d, %d, %d, %d, %d>" % (self.la, self.lb, self.lc, self.ld, 1 if self.exact else 0)
         prim = self.gen_prim()
         hrr = self.gen_hrr()
@@ -248,8 +272,11 @@ class ClassGen:
         out.append("    static constexpr int NE = %d, NF = %d, NACC = %d, NOUT = %d;" %
                    (len(self.E), len(self.F), len(self.E) * len(self.F), na * nb * nc * nd))
         out.append("    static constexpr int PRIM_FLOPS = %d, HRR_FLOPS = %d;" % (self.flops_prim, self.flops_hrr))
+        # TRUNCATED: the reference convention drops transfer rows for this class, so the value depends
+        # on which pair is the bra; kernels then pick the bra per component as the reference does.
+        out.append("    static constexpr bool TRUNCATED = %s;" % ("true" if truncated else "false"))
         out.append("    template <typename R>")
-        out.append("    __device__ __forceinline__ static void prim(const R* __restrict__ F, R PAx, R PAy, R PAz, "
+        out.append("    __host__ __device__ __forceinline__ static void prim(const R* __restrict__ F, R PAx, R PAy, R PAz, "
                    "R WPx, R WPy, R WPz, R inv2z, R rz, R dlx, R dly, R dlz, R zoe, R inv2e, R* __restrict__ acc) {")
         for l in prim:
             out.append(l.replace("const double ", "const R "))
@@ -257,7 +284,7 @@ class ClassGen:
         out.append("        (void)dlx; (void)dly; (void)dlz; (void)zoe; (void)inv2e;")
         out.append("    }")
         out.append("    template <typename R>")
-        out.append("    __device__ __forceinline__ static void hrr(const R* __restrict__ acc, R ABx, R ABy, R ABz, "
+        out.append("    __host__ __device__ __forceinline__ static void hrr(const R* __restrict__ acc, R ABx, R ABy, R ABz, "
                    "R CDx, R CDy, R CDz, R* __restrict__ out) {")
         for l in hrr:
             out.append(l.replace("const double ", "const R "))
@@ -291,10 +318,11 @@ def main():
     for (la, lb, lc, ld) in cls:
         ref = ClassGen(la, lb, lc, ld, False)
         ex = ClassGen(la, lb, lc, ld, True)
-        parts.append(ref.emit())
-        parts.append(ex.emit())
+        ex_text = ex.emit()
+        differs = any(ref.etr_zero(e, f) for e in ref.E for f in ref.F) or ref.needs_zero_rows()
+        parts.append(ref.emit(truncated=differs))
+        parts.append(ex_text)
         parts.append("")
-        differs = ref.flops_prim != ex.flops_prim
         info.append((la, lb, lc, ld, differs, ref.flops_prim, ex.flops_prim, ref.flops_hrr))
     write_if_changed(os.path.join(outdir, "eri_classes.inc"), "\n".join(parts))
     decl = []
@@ -352,6 +380,28 @@ def main():
     disp.append("constexpr int kNumClasses = %d;" % len(info))
     disp.append("}  // namespace hfgpu")
     write_if_changed(os.path.join(outdir, "dispatch.inc"), "\n".join(disp) + "\n")
+    # host-side evaluation switch (CPU check of the generated code; used by tests only)
+    hv = ["// Generated by gen_eri.py - do not edit.", "#pragma once", '#include "../eri_kernel.hpp"',
+          "namespace hfgpu {",
+          "inline int eval_quartet_host(int la, int lb, int lc, int ld, int exact, const double* A, const double* B, "
+          "const double* C, const double* D, const PrimPair* bra, int npb, const PrimPair* ket, int npk, "
+          "const double* boys, double* out, int sA, int sC, int a0, int b0, int c0, int d0) {"]
+    for (la, lb, lc, ld, differs, fr, fe, fh) in info:
+        for ex in ((0, 1) if differs else (0,)):
+            cond = "la == %d && lb == %d && lc == %d && ld == %d" % (la, lb, lc, ld)
+            if differs:
+                cond += " && %sexact" % ("" if ex else "!")
+            cls = "EriClass<%d, %d, %d, %d, %d>" % (la, lb, lc, ld, ex)
+            if differs and not ex:   # reference convention: per-component bra choice, as the kernels do
+                hv.append("    if (%s) { eval_quartet_reference_order<%s>(sA, sC, A, B, C, D, bra, npb, ket, npk, "
+                          "a0, b0, c0, d0, boys, out); return 0; }" % (cond, cls))
+            else:
+                hv.append("    if (%s) { eval_quartet<%s>(A, B, C, D, bra, npb, ket, npk, boys, out); return 0; }"
+                          % (cond, cls))
+    hv.append("    return -1;")
+    hv.append("}")
+    hv.append("}  // namespace hfgpu")
+    write_if_changed(os.path.join(outdir, "host_eval.inc"), "\n".join(hv) + "\n")
 
 
 def write_if_changed(path, text):

@@ -105,9 +105,9 @@ struct PairClassHost {
 };
 
 struct KernelStat {
-    int cls;
-    int64_t quartets, prim_quartets;
-    double flops, ms;
+    int cls = 0;
+    int64_t launches = 0, quartets = 0, prim_quartets = 0;
+    double eris = 0, flops = 0, ms = 0;
 };
 
 struct hf_ctx {

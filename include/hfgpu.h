@@ -148,15 +148,20 @@ void hf_scf_defaults(hf_scf_params* p);
 /* D_out (optional): n*n (RHF total density) or 2*n*n (UHF alpha then beta). */
 int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out);
 
-/* Per-class kernel statistics of the last J/K build or packed-ERI pass. */
+/* Per-class ERI kernel statistics, summed over every launch since the last reset
+ * (J/K builds, packed-ERI passes).  No counterpart in the reference (it has no timers);
+ * used for ERI/s and roofline reporting. */
 typedef struct hf_kernel_stat {
     int la, lb, lc, ld;            /* angular class (la lb | lc ld) */
+    int64_t launches;
     int64_t quartets;              /* contracted shell quartets evaluated */
     int64_t prim_quartets;         /* primitive quartets evaluated */
-    double flops;                  /* algorithmic FP64 flops (generated-code operation count) */
-    double ms;                     /* kernel time from HIP events (profile = 1), else 0 */
+    double eris;                   /* Cartesian ERIs evaluated (quartets x components) */
+    double flops;                  /* algorithmic flops (generated-code operation count) */
+    double ms;                     /* kernel time from HIP events on the launch stream (profile = 1), else 0 */
 } hf_kernel_stat;
 int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max);   /* returns count */
+int hf_kernel_stats_reset(hf_ctx* ctx);
 
 #ifdef __cplusplus
 }

@@ -841,6 +841,53 @@ int ref_eri_packed_rows(ref_mol* m, double* out, uint64_t len, int i0, int i1) {
     return 0;
 }
 
+/* Every stride-th (i, j, k) triple of the loop above (offset selects which), with its
+ * whole l loop and the same cache lifetime: a bounded, unbiased sample of the ERI
+ * phase for timing.  *count receives the number of ERIs evaluated. */
+int ref_eri_packed_sample(ref_mol* m, double* out, uint64_t len, uint64_t stride, uint64_t offset, uint64_t* count) {
+    const int n = (int)m->bfs.size();
+    if (len < ref_packed_len(n) || stride == 0) return -1;
+    uint64_t t = 0, cnt = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const uint64_t ij = TwoIndex(i, j);
+            for (int k = 0; k < n; ++k, ++t) {
+                if (t % stride != offset) continue;
+                for (int l = 0; l <= k; ++l) {
+                    const uint64_t kl = TwoIndex(k, l);
+                    if (ij <= kl) {
+                        out[TwoIndex(ij, kl)] = ContractedERI(m, m->bfs[i], m->bfs[j], m->bfs[k], m->bfs[l]);
+                        ++cnt;
+                    }
+                    if (l == k || m->bfs[l]->shellID != m->bfs[l + 1]->shellID) m->prim.clear();
+                }
+            }
+        }
+    m->prim.clear();
+    m->boys.clear();
+    if (count) *count = cnt;
+    return 0;
+}
+
+/* The RHF two-electron matrix exactly as HartreeFock/RestrictedHartreeFock.cpp:189-202:
+ * G_ij = sum_kl D_kl [(ij|kl) - 1/2 (il|kj)], one N^4 loop (timed as the CPU baseline);
+ * rows i in [i0, i1) only. */
+int ref_rhf_g(int n, const double* eri, const double* D, double* G, int i0, int i1) {
+    if (i0 < 0 || i1 > n || i0 > i1) return -1;
+    for (int i = i0; i < i1; ++i)
+        for (int j = 0; j < n; ++j) {
+            double g = 0;
+            for (int k = 0; k < n; ++k)
+                for (int l = 0; l < n; ++l) {
+                    const double coulomb = eri[ref_packed_index(i, j, k, l)];
+                    const double exchange = eri[ref_packed_index(i, l, k, j)];
+                    g += D[(size_t)k * n + l] * (coulomb - 0.5 * exchange);
+                }
+            G[(size_t)i * n + j] = g;
+        }
+    return 0;
+}
+
 int ref_eri_packed(ref_mol* m, double* out, uint64_t len) {
     return ref_eri_packed_rows(m, out, len, 0, (int)m->bfs.size());
 }

@@ -56,6 +56,12 @@ int ref_eri_packed(ref_mol* m, double* out, uint64_t len);
 /* Same, restricted to basis functions i in [i0, i1): the bounded CPU-baseline sample. */
 int ref_eri_packed_rows(ref_mol* m, double* out, uint64_t len, int i0, int i1);
 
+/* Every stride-th (i,j,k) triple of the packed loop with its whole l loop (timing sample). */
+int ref_eri_packed_sample(ref_mol* m, double* out, uint64_t len, uint64_t stride, uint64_t offset,
+                          uint64_t* count);
+/* RHF G = sum_kl D_kl [(ij|kl) - 1/2 (il|kj)], the reference's N^4 loop, rows [i0, i1). */
+int ref_rhf_g(int n, const double* eri, const double* D, double* G, int i0, int i1);
+
 /* One-electron matrices S, T, V (HartreeFock/QuantumMatrix.cpp:34-155). */
 int ref_one_electron(ref_mol* m, double* S, double* T, double* V);
 

@@ -53,6 +53,9 @@ def lib():
         L.ref_packed_len.restype = ctypes.c_uint64
         L.ref_eri_packed.argtypes = [ctypes.c_void_p, dp, ctypes.c_uint64]
         L.ref_eri_packed_rows.argtypes = [ctypes.c_void_p, dp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+        L.ref_eri_packed_sample.argtypes = [ctypes.c_void_p, dp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                            ctypes.POINTER(ctypes.c_uint64)]
+        L.ref_rhf_g.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.c_int]
         L.ref_one_electron.argtypes = [ctypes.c_void_p, dp, dp, dp]
         L.ref_jk_from_packed.argtypes = [ctypes.c_int, dp, dp, dp, dp]
         L.ref_scf_defaults.argtypes = [ctypes.POINTER(ScfParams)]
@@ -122,6 +125,17 @@ class Molecule:
             raise RuntimeError("ref_eri_packed failed")
         return out
 
+    def eri_packed_sample(self, stride, offset=0, out=None):
+        """Every stride-th (i,j,k) triple of the packed loop; returns (array, ERIs evaluated)."""
+        ln = lib().ref_packed_len(self.nbf)
+        if out is None:
+            out = np.zeros(ln, np.float64)
+        cnt = ctypes.c_uint64()
+        rc = lib().ref_eri_packed_sample(self.h, _dp(out), ln, stride, offset, ctypes.byref(cnt))
+        if rc != 0:
+            raise RuntimeError("ref_eri_packed_sample failed")
+        return out, cnt.value
+
     def one_electron(self):
         n = self.nbf
         S, T, V = (np.zeros((n, n)) for _ in range(3))
@@ -158,6 +172,14 @@ def packed_index(i, j, k, l):
 
 def packed_len(n):
     return lib().ref_packed_len(n)
+
+
+def rhf_g(n, eri, D, rows=None):
+    G = np.zeros((n, n))
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    i0, i1 = rows if rows is not None else (0, n)
+    lib().ref_rhf_g(n, _dp(eri), _dp(D), _dp(G), i0, i1)
+    return G
 
 
 def jk_from_packed(n, eri, D):

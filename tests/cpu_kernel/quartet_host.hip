@@ -1,0 +1,24 @@
+// Test harness: the generated ERI class code and eval_quartet() compiled for the HOST, so
+// the CPU test suite can check every angular class against the oracle without a GPU.
+// Built by hartreefock_amd/build.py into tests/cpu_kernel/libquartet_host.so; the product
+// library never links it.
+#include "generated/host_eval.inc"
+#include "hf_internal.h"
+
+// sA, sC: shell indices of the first shell of bra and ket pair; a0..d0: first basis function of each
+// shell (used only by classes whose reference convention depends on the bra choice).
+extern "C" int hfq_eval(int la, int lb, int lc, int ld, int exact, const double* A, const double* B,
+                        const double* C, const double* D, const hfgpu::PrimPair* bra, int npb,
+                        const hfgpu::PrimPair* ket, int npk, const double* boys, double* out, int sA, int sC,
+                        int a0, int b0, int c0, int d0) {
+    return hfgpu::eval_quartet_host(la, lb, lc, ld, exact, A, B, C, D, bra, npb, ket, npk, boys, out, sA, sC, a0, b0,
+                                    c0, d0);
+}
+
+extern "C" void hfq_boys_table(double* out) {
+    std::vector<double> t;
+    hfhost::boys_table(t);
+    for (size_t i = 0; i < t.size(); ++i) out[i] = t[i];
+}
+
+extern "C" int hfq_boys_table_size() { return hfgpu::BOYS_NROW * hfgpu::BOYS_NCOL; }

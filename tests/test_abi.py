@@ -40,3 +40,14 @@ def test_options_defaults():
     p = hf.ScfParams()
     hf.lib().hf_scf_defaults(ctypes.byref(p))
     assert (p.alpha, p.init_guess, p.use_diis, p.normal_iter_after_diis) == (0.75, 0.75, 1, 500)
+
+
+def test_cpp_dropin_header_compiles(tmp_path):
+    """The C++ drop-in header (reference class names over the C ABI) builds and links."""
+    import subprocess
+    exe = tmp_path / "dropin_example"
+    cmd = ["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "dropin_example.cpp"),
+           "-L" + os.path.join(ROOT, "hartreefock_amd"), "-lhfgpu", "-o", str(exe)]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr

@@ -1,0 +1,145 @@
+"""CPU check of the generated ERI class code (the same eval_quartet() the GPU kernels run,
+compiled for the host by the test harness tests/cpu_kernel/quartet_host.hip) against the
+oracle, for every angular class, on molecules whose pairs are two-centre."""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import basis
+from hartreefock_amd import geometry as g
+from oracle import refcpu
+import md_eri
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_H = None
+
+
+def harness():
+    global _H
+    if _H is None:
+        path = os.path.join(HERE, "cpu_kernel", "libquartet_host.so")
+        if not os.path.exists(path):
+            pytest.skip("host harness not built (python -m hartreefock_amd.build)")
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.hfq_eval.argtypes = [ctypes.c_int] * 5 + [dp] * 4 + [dp, ctypes.c_int, dp, ctypes.c_int, dp, dp] + \
+            [ctypes.c_int] * 6
+        L.hfq_boys_table.argtypes = [dp]
+        _H = L
+    return _H
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def shells_of(bfile, atoms):
+    """Shells after splitting SP, in the reference's basis-function order."""
+    sh = md_eri.parse_basis(bfile)
+    out = []
+    bf = 0
+    for ai, (z, pos) in enumerate(atoms):
+        for s in sh[z]:
+            for li, letter in enumerate(s["type"]):
+                L = "SPD".index(letter)
+                coef = [c[li] * (2 * a / math.pi) ** 0.75 * (4 * a) ** (L / 2.0) for a, c in zip(s["exps"], s["coefs"])]
+                out.append(dict(L=L, xyz=np.array(pos, float), alpha=list(s["exps"]), coef=coef, bf=bf, atom=ai,
+                                index=len(out)))
+                bf += (L + 1) * (L + 2) // 2
+    return out
+
+
+def prim_pairs(A, B):
+    """64-byte PrimPair records, as the library builds them (csrc/context.hip)."""
+    rec = []
+    R2 = float(((A["xyz"] - B["xyz"]) ** 2).sum())
+    for a, ca in zip(A["alpha"], A["coef"]):
+        for b, cb in zip(B["alpha"], B["coef"]):
+            z = a + b
+            P = (a * A["xyz"] + b * B["xyz"]) / z
+            K = ca * cb * math.exp(-a * b / z * R2)
+            rec.append([P[0], P[1], P[2], z, K / z, 1.0 / z, 0.0, 0.0])
+    return np.ascontiguousarray(rec, dtype=np.float64)
+
+
+def block(A, B, C, D, exact, boys):
+    nA, nB, nC, nD = (((s["L"] + 1) * (s["L"] + 2)) // 2 for s in (A, B, C, D))
+    out = np.zeros(nA * nB * nC * nD)
+    bp, kp = prim_pairs(A, B), prim_pairs(C, D)
+    rc = harness().hfq_eval(A["L"], B["L"], C["L"], D["L"], exact, _dp(A["xyz"]), _dp(B["xyz"]), _dp(C["xyz"]),
+                            _dp(D["xyz"]), _dp(bp), len(bp), _dp(kp), len(kp), _dp(boys), _dp(out),
+                            A["index"], C["index"], A["bf"], B["bf"], C["bf"], D["bf"])
+    assert rc == 0
+    return out.reshape(nA, nB, nC, nD)
+
+
+def boys_table():
+    t = np.zeros(361 * 16)
+    harness().hfq_boys_table(_dp(t))
+    return t
+
+
+CASES = [("ethane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(2, seed=5)),
+         ("c2_631gs", "6-31g_st_.1.nw", [(6, (0.0, 0.0, 0.0)), (6, (0.3, 1.2, 2.5))])]
+
+
+@pytest.mark.parametrize("name,bfile,atoms", CASES, ids=[c[0] for c in CASES])
+def test_every_class_matches_oracle_exact(name, bfile, atoms):
+    """Exact convention, all canonical classes (La>=Lb, Lc>=Ld, bra L-sum >= ket L-sum), a seeded
+    sample of shell quartets per class, every component against the oracle (exact mode)."""
+    shells = shells_of(basis(bfile), atoms)
+    mol = refcpu.Molecule(basis(bfile), atoms)
+    boys = boys_table()
+    rng = np.random.default_rng(3)
+    by_l = {}
+    for i, s in enumerate(shells):
+        by_l.setdefault(s["L"], []).append(i)
+    pairs = [(a, b) for a in range(3) for b in range(a + 1)]
+    worst = 0.0
+    refcpu.set_exact_transfer(True)
+    try:
+        for (la, lb) in pairs:
+            for (lc, ld) in pairs:
+                if (lc + ld, lc) > (la + lb, la):
+                    continue
+                if not (by_l.get(la) and by_l.get(lb) and by_l.get(lc) and by_l.get(ld)):
+                    continue
+                for _ in range(3):
+                    A, B, C, D = (shells[int(rng.choice(by_l[x]))] for x in (la, lb, lc, ld))
+                    blk = block(A, B, C, D, 1, boys)
+                    for ia in range(blk.shape[0]):
+                        for ib in range(blk.shape[1]):
+                            for ic in range(blk.shape[2]):
+                                for id_ in range(blk.shape[3]):
+                                    ref = mol.eri(A["bf"] + ia, B["bf"] + ib, C["bf"] + ic, D["bf"] + id_)
+                                    worst = max(worst, abs(blk[ia, ib, ic, id_] - ref))
+    finally:
+        refcpu.set_exact_transfer(False)
+    assert worst < 1e-12, worst
+
+
+@pytest.mark.parametrize("name,bfile,atoms", CASES, ids=[c[0] for c in CASES])
+def test_dddd_reference_convention_matches_oracle(name, bfile, atoms):
+    """Reference convention for (dd|dd): every shell quartet of d shells (pairs stored with the
+    higher shell index first, as the library does), every component, against the oracle in its
+    default (reference) mode, which canonicalises each component quartet as the reference does."""
+    shells = shells_of(basis(bfile), atoms)
+    mol = refcpu.Molecule(basis(bfile), atoms)
+    boys = boys_table()
+    d = [s for s in shells if s["L"] == 2]
+    pairs = [(x, y) for x in d for y in d if x["index"] >= y["index"]]
+    worst = 0.0
+    for pi, (A, B) in enumerate(pairs):
+        for (C, D) in pairs[: pi + 1]:
+            for (P, Q) in (((A, B), (C, D)), ((C, D), (A, B))):    # both launch orientations
+                blk = block(P[0], P[1], Q[0], Q[1], 0, boys)
+                for ia in range(6):
+                    for ib in range(6):
+                        for ic in range(6):
+                            for id_ in range(6):
+                                ref = mol.eri(P[0]["bf"] + ia, P[1]["bf"] + ib, Q[0]["bf"] + ic, Q[1]["bf"] + id_)
+                                worst = max(worst, abs(blk[ia, ib, ic, id_] - ref))
+    assert worst < 1e-12, worst
