@@ -76,7 +76,7 @@ def cpu_baseline(bfile, atoms, iterations, budget_s):
     return {"value": 1.0 / total, "unit": "molecules/s", "cores": 1, "kind": "port",
             "sample": ("oracle on %d of %d unique ERIs (every %d-th (i,j,k) triple, %.1fs -> ERI phase %.0fs) + "
                        "%d of %d rows of one N^4 Fock loop (%.1fs -> %.1fs/iteration) x %d iterations = %.0fs/molecule"
-                       % (cnt, unique, stride, t_eri, eri_phase, rows, n, t_g, fock, iterations, iterations, total))}
+                       % (cnt, unique, stride, t_eri, eri_phase, rows, n, t_g, fock, iterations, total))}
 
 
 def main():
@@ -94,7 +94,8 @@ def main():
         sys.exit(2)
     import hartreefock_amd as hf
     label, bfile, atoms = workload(a.workload)
-    eng = hf.Engine(schwarz_threshold=1e-12, profile=1).set_molecule(bfile, atoms)
+    # exact integrals: the reference's (dd|dd) transfer convention makes this SCF diverge
+    eng = hf.Engine(schwarz_threshold=1e-12, profile=1, exact_transfer=1).set_molecule(bfile, atoms)
     for _ in range(a.warmup):
         eng.scf(**SCF_KW)
     eng.reset_kernel_stats()
@@ -138,6 +139,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded geometry)",
         "config": {"workload": label, "nbf": eng.nbf, "schwarz_threshold": 1e-12, "energy_tolerance": 1e-8,
+                   "eri_convention": "exact",
                    "scf": "alpha=0.5, core guess, DIIS(6/5), normalIterAfterDIIS=0", "parallelism": "single GPU"},
         "energy_Eh": energies[-1],
         "iterations": iters[-1],

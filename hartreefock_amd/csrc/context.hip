@@ -65,6 +65,26 @@ PairClassDev dev_view(const PairClassHost& pc) {
     return d;
 }
 
+void add_stats(hf_ctx* ctx, int mode, const QuartetClassTasks& qc, const SinkArgs& sink) {
+    const ClassInfo& ci = kClasses[qc.cls];
+    const int pf = ctx->opt.exact_transfer && ci.exact_differs ? ci.prim_flops_exact : ci.prim_flops_ref;
+    const int L = ci.la + ci.lb + ci.lc + ci.ld;
+    // per primitive quartet: Boys (8-term Taylor 22, recursion 4/order, scaling 1/order)
+    // and the quartet geometry/prefactor terms (36)
+    const double boys_flops = 22 + 5.0 * L + 1 + 36;
+    const int na = (ci.la + 1) * (ci.la + 2) / 2, nb = (ci.lb + 1) * (ci.lb + 2) / 2;
+    const int nc = (ci.lc + 1) * (ci.lc + 2) / 2, nd = (ci.ld + 1) * (ci.ld + 2) / 2;
+    const double nout = (double)na * nb * nc * nd;
+    double sink_flops = 0;
+    if (mode == MODE_JK) sink_flops = nout * 2.0 * (2 + 4 * sink.ndens);   // 2 J + 4 K contractions per density
+    KernelStat& ks = ctx->stats[qc.cls];
+    ks.launches += 1;
+    ks.quartets += qc.ntasks;
+    ks.prim_quartets += qc.prim_quartets;
+    ks.eris += (double)qc.ntasks * nout;
+    ks.flops += (double)qc.prim_quartets * (pf + boys_flops) + (double)qc.ntasks * (ci.hrr_flops + sink_flops);
+}
+
 // Launch every quartet class of a task set in `mode`; with record, add to the per-class statistics.
 int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, hipStream_t st, bool record) {
     ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
@@ -78,9 +98,58 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
         ctx->ev.assign(2 * kNumClasses, nullptr);
         for (auto& e : ctx->ev) HF_HIP_CHECK(ctx, hipEventCreate(&e));
     }
+    // fork: every side stream waits for the work already queued on st
+    constexpr int kSide = 4;
+    if (ctx->side.empty()) {
+        ctx->side.assign(kSide, nullptr);
+        for (auto& s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        ctx->side_ev.assign(kSide + 1, nullptr);
+        for (auto& e : ctx->side_ev) HF_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[0], st));
+    for (hipStream_t s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamWaitEvent(s2, ctx->side_ev[0], 0));
+    // largest estimated work first, each to the least-loaded side stream
+    // Classes evaluated in the reference's truncated-transfer convention carry two full output
+    // blocks in scratch (~39 KB per lane for (dd|dd)); they run alone on st after the join, since
+    // concurrent dispatches could not all get that much scratch.
+    const bool trunc_mode = ctx->opt.exact_transfer == 0 && mode != MODE_DIAG;
+    std::vector<const QuartetClassTasks*> order, serial;
+    for (const QuartetClassTasks& qc : ts.classes)
+        if (qc.ntasks > 0) (trunc_mode && kClasses[qc.cls].exact_differs ? serial : order).push_back(&qc);
+    auto cost = [](const QuartetClassTasks* q) {
+        const ClassInfo& ci = kClasses[q->cls];
+        return (double)q->prim_quartets * (ci.prim_flops_ref + 60) + (double)q->ntasks * ci.hrr_flops;
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](const QuartetClassTasks* a, const QuartetClassTasks* b) { return cost(a) > cost(b); });
+    std::vector<double> load(kSide, 0.0);
     std::vector<int> launched;
-    for (const QuartetClassTasks& qc : ts.classes) {
-        if (qc.ntasks <= 0) continue;
+    for (const QuartetClassTasks* qcp : order) {
+        const QuartetClassTasks& qc = *qcp;
+        const ClassInfo& ci = kClasses[qc.cls];
+        const int si = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[si] += cost(qcp);
+        hipStream_t ks = ctx->side[si];
+        TaskList tl;
+        tl.bra = dev_view(ctx->pcls[qc.bra]);
+        tl.ket = dev_view(ctx->pcls[qc.ket]);
+        tl.prefix = ts.d_prefix.p + qc.prefix_off;
+        tl.ntasks = qc.ntasks;
+        tl.same_class = qc.bra == qc.ket;
+        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls], ks));
+        hipError_t e = ci.fn(mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ks);
+        if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));
+        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls + 1], ks));
+        launched.push_back(qc.cls);
+        if (record) add_stats(ctx, mode, qc, sink);
+    }
+    // join: st continues after every side stream's work
+    for (int k = 0; k < kSide; ++k) {
+        HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[k + 1], ctx->side[k]));
+        HF_HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->side_ev[k + 1], 0));
+    }
+    for (const QuartetClassTasks* qcp : serial) {
+        const QuartetClassTasks& qc = *qcp;
         const ClassInfo& ci = kClasses[qc.cls];
         TaskList tl;
         tl.bra = dev_view(ctx->pcls[qc.bra]);
@@ -93,24 +162,7 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
         if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));
         if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls + 1], st));
         launched.push_back(qc.cls);
-        if (record) {
-            const int pf = ctx->opt.exact_transfer && ci.exact_differs ? ci.prim_flops_exact : ci.prim_flops_ref;
-            const int L = ci.la + ci.lb + ci.lc + ci.ld;
-            // per primitive quartet: Boys (8-term Taylor 22, recursion 4/order, scaling 1/order)
-            // and the quartet geometry/prefactor terms (36)
-            const double boys_flops = 22 + 5.0 * L + 1 + 36;
-            const int na = (ci.la + 1) * (ci.la + 2) / 2, nb = (ci.lb + 1) * (ci.lb + 2) / 2;
-            const int nc = (ci.lc + 1) * (ci.lc + 2) / 2, nd = (ci.ld + 1) * (ci.ld + 2) / 2;
-            const double nout = (double)na * nb * nc * nd;
-            double sink_flops = 0;
-            if (mode == MODE_JK) sink_flops = nout * 2.0 * (2 + 4 * sink.ndens);   // 2 J + 4 K contractions per density
-            KernelStat& ks = ctx->stats[qc.cls];
-            ks.launches += 1;
-            ks.quartets += qc.ntasks;
-            ks.prim_quartets += qc.prim_quartets;
-            ks.eris += (double)qc.ntasks * nout;
-            ks.flops += (double)qc.prim_quartets * (pf + boys_flops) + (double)qc.ntasks * (ci.hrr_flops + sink_flops);
-        }
+        if (record) add_stats(ctx, mode, qc, sink);
     }
     if (prof) {
         HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
@@ -363,6 +415,8 @@ int hf_destroy(hf_ctx* ctx) {
     (void)hipSetDevice(ctx->opt.device);
     (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->side_ev) (void)hipEventDestroy(e);
+    for (hipStream_t s2 : ctx->side) (void)hipStreamDestroy(s2);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

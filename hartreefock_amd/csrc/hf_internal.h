@@ -128,6 +128,10 @@ struct hf_ctx {
     DevBuf<double> d_Jp, d_Kp, d_D, d_J, d_K;
     std::vector<KernelStat> stats;
     std::vector<hipEvent_t> ev;
+    // side streams: the class kernels of one pass run concurrently, forked from and joined
+    // back to the caller's stream with events
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_ev;   // [0] fork, [1..] join per side stream
     double last_jk_ms = 0;
 
     int fail(int code, const std::string& msg) {

@@ -15,6 +15,8 @@
 #include <rocsolver/rocsolver.h>
 
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -457,13 +459,23 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     };
 
     // ---- Calculate (HartreeFock/HartreeFockAlgorithm.cpp:105-192) ----
+    // HFGPU_TRACE=1 prints one line per iteration to stderr (the reference's per-step TRACE,
+    // HartreeFock/RestrictedHartreeFock.cpp:149).
+    const char* tenv = std::getenv("HFGPU_TRACE");
+    const bool trace = tenv && tenv[0] == '1';
     double cur = 0, prev = std::numeric_limits<double>::infinity(), rms = 0;
+    auto trace_line = [&](int it, double rmsv) {
+        if (trace)
+            std::fprintf(stderr, "hf_scf %s iter %4d  E % .12f  dE % .3e  |dD| %.3e  diis %.3e\n", uhf ? "UHF" : "RHF",
+                         it, energy, energy - prev, rmsv, lastErr);
+    };
     int iter = 0;
     bool converged = false, cancelled = false;
     auto stop = [&]() { return p->terminate && *p->terminate; };
     for (; iter < p->max_iterations; ++iter) {
         if (int rc = step(iter, rms)) return rc;
         cur = energy;
+        trace_line(iter, rms);
         if (std::fabs(prev - cur) <= (useDIIS ? etolD : etolN) && rms < dtol && lastErr < itol) {
             converged = true;
             break;
@@ -486,6 +498,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         for (; iter < p->max_iterations; ++iter) {
             if (int rc = step(iter, rms)) return rc;
             cur = energy;
+            trace_line(iter, rms);
             if (std::fabs(prev - cur) <= etolN && rms < dtol) { converged = true; break; }
             if (stop()) { cancelled = true; break; }
             prev = cur;
