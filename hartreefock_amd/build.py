@@ -53,7 +53,8 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
     os.makedirs(OBJ, exist_ok=True)
     subprocess.run([sys.executable, os.path.join(CSRC, "gen_eri.py"), GEN, str(LMAX)], check=True)
     kdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "eri_kernel.hpp"),
-             os.path.join(GEN, "eri_classes.inc")]
+             os.path.join(CSRC, "coop_kernel.hpp"), os.path.join(GEN, "eri_classes.inc")]
+    kdeps += sorted(glob.glob(os.path.join(GEN, "coop_*.inc")))
     hdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "hf_internal.h"),
              os.path.join(GEN, "dispatch.inc"), os.path.join(ROOT, "include", "hfgpu.h")]
     for stale in glob.glob(os.path.join(OBJ, "k_????.hip.o")):
@@ -63,9 +64,9 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
         units.append((src, True, kdeps))
     for src in sorted(glob.glob(os.path.join(GEN, "d_*.hip"))):
         units.append((src, True, [os.path.join(CSRC, "hf_device.h")]))
-    for name in ("context.hip", "scf.hip"):
+    for name in ("context.hip", "scf.hip", "one_electron.hip"):
         units.append((os.path.join(CSRC, name), True, hdeps))
-    for name in ("basis.cpp", "boys.cpp", "one_electron.cpp"):
+    for name in ("basis.cpp", "boys.cpp"):
         units.append((os.path.join(CSRC, name), False, hdeps))
     objs = []
     # biggest kernels first so the parallel build ends sooner

@@ -109,13 +109,11 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
     HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[0], st));
     for (hipStream_t s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamWaitEvent(s2, ctx->side_ev[0], 0));
     // largest estimated work first, each to the least-loaded side stream
-    // Classes evaluated in the reference's truncated-transfer convention carry two full output
-    // blocks in scratch (~39 KB per lane for (dd|dd)); they run alone on st after the join, since
-    // concurrent dispatches could not all get that much scratch.
-    const bool trunc_mode = ctx->opt.exact_transfer == 0 && mode != MODE_DIAG;
+    // Every class runs concurrently: the high-L classes use the one-wavefront-per-quartet LDS
+    // kernel and no class needs more than a few hundred bytes of scratch per lane.
     std::vector<const QuartetClassTasks*> order, serial;
     for (const QuartetClassTasks& qc : ts.classes)
-        if (qc.ntasks > 0) (trunc_mode && kClasses[qc.cls].exact_differs ? serial : order).push_back(&qc);
+        if (qc.ntasks > 0) order.push_back(&qc);
     auto cost = [](const QuartetClassTasks* q) {
         const ClassInfo& ci = kClasses[q->cls];
         return (double)q->prim_quartets * (ci.prim_flops_ref + 60) + (double)q->ntasks * ci.hrr_flops;
@@ -292,6 +290,8 @@ int setup_molecule(hf_ctx* ctx) {
     }
     ctx->tasks_all.threshold = -1;
     ctx->tasks_screened.threshold = -1;
+    ctx->d_oe_L.release();
+    ctx->d_oe_prim.release();
     ctx->have_mol = true;
     return HF_OK;
 }
@@ -520,7 +520,14 @@ int hf_one_electron(hf_ctx* ctx, double* S, double* T, double* V) {
     if (!ctx) return HF_E_ARG;
     if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
     if (!S || !T || !V) return ctx->fail(HF_E_ARG, "null output");
-    hfhost::one_electron(ctx->mol, S, T, V);
+    const size_t nn = (size_t)ctx->mol.nbf * ctx->mol.nbf;
+    DevBuf<double> d;
+    HF_HIP_CHECK(ctx, d.alloc(3 * nn));
+    if (int rc = hf_one_electron_device(ctx, d.p, d.p + nn, d.p + 2 * nn, ctx->stream)) return rc;
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(S, d.p, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(T, d.p + nn, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(V, d.p + 2 * nn, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return HF_OK;
 }
 

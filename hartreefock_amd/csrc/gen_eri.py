@@ -327,6 +327,7 @@ def main():
     write_if_changed(os.path.join(outdir, "eri_classes.inc"), "\n".join(parts))
     decl = []
     table = []
+    coop_flops = {}
     for (la, lb, lc, ld, differs, fr, fe, fh) in info:
         tag = "%d%d%d%d" % (la, lb, lc, ld)
         diag = (la, lb) == (lc, ld)
@@ -334,15 +335,27 @@ def main():
               "const SinkArgs& s, hipStream_t st)"
         # one translation unit per (mode, variant) so the large classes compile in parallel
         variants = []
+        coop = la + lb + lc + ld >= COOP_MIN_L
+        if coop:
+            for ex in ((0, 1) if differs else (0,)):
+                cg = CoopGen(la, lb, lc, ld, bool(ex)).build()
+                cg.truncated = differs and not ex
+                write_if_changed(os.path.join(outdir, "coop_%s_%d.inc" % (tag, ex)), cg.emit("%s_%d" % (tag, ex)))
+                coop_flops[(la, lb, lc, ld, ex)] = (cg.flops_prim, cg.flops_hrr)
         for mode in ("MODE_JK", "MODE_PACKED") + (("MODE_DIAG",) if diag else ()):
             for ex in ((0, 1) if differs else (0,)):
                 fn = "run_%s_%s_%d" % (tag, mode.split("_")[1].lower(), ex)
                 variants.append((mode, ex, fn))
-                body = ["// Generated by gen_eri.py - do not edit.", '#include "../eri_kernel.hpp"',
+                if coop:
+                    inc = '#include "coop_%s_%d.inc"' % (tag, ex)
+                    call = "launch_coop<CoopClass<%d, %d, %d, %d, %d>, %s>" % (la, lb, lc, ld, ex, mode)
+                else:
+                    inc = '#include "../eri_kernel.hpp"'
+                    call = "launch_eri<%d, %d, %d, %d, %d, %s>" % (la, lb, lc, ld, ex, mode)
+                body = ["// Generated by gen_eri.py - do not edit.", inc,
                         "namespace hfgpu {",
                         "hipError_t %s%s {" % (fn, sig),
-                        "    return launch_eri<%d, %d, %d, %d, %d, %s>(tl, sh, pp, boys, s, st);" %
-                        (la, lb, lc, ld, ex, mode),
+                        "    return %s(tl, sh, pp, boys, s, st);" % call,
                         "}", "}  // namespace hfgpu"]
                 write_if_changed(os.path.join(outdir, "k_%s.hip" % fn[4:]), "\n".join(body) + "\n")
         body = ["// Generated by gen_eri.py - do not edit.", '#include "../hf_device.h"', "namespace hfgpu {"]
@@ -414,6 +427,243 @@ def write_if_changed(path, text):
         pass
     with open(path, "w") as fh:
         fh.write(text)
+
+
+
+# ---------------------------------------------------------------------------------------
+# Cooperative (one wavefront per quartet) form for the high-L classes.
+#
+# The same recursions as ClassGen, expressed as a dependency-levelled list of ops
+#     val[dst] = sum_{r<4} coef[c_r] * val[s_r]
+# over an LDS array `val`; the 64 lanes of a wave execute the ops of one level in parallel.
+# Coefficient ids index a small per-primitive-quartet array (see COEF below).
+# ---------------------------------------------------------------------------------------
+
+COOP_MIN_L = 6      # classes with LA+LB+LC+LD >= this run one wavefront per quartet
+
+COEF = {"zero": 0, "one": 1, "PA": 2, "WP": 5, "DL": 8, "mzoe": 11, "kinv2z": 12, "mrzkinv2z": 20,
+        "kinv2e": 28, "AB": 36, "CD": 39}
+NCOEF = 42
+
+
+class CoopGen:
+    def __init__(self, la, lb, lc, ld, exact):
+        self.g = ClassGen(la, lb, lc, ld, exact)
+        self.la, self.lb, self.lc, self.ld = la, lb, lc, ld
+        self.exact = exact
+
+    def build(self):
+        g = self.g
+        L = g.L
+        # ---- phase 1: per primitive quartet ----
+        nodes = {}          # key -> (level, terms) ; terms = [(coef_id, src_key)]
+        base = {}
+
+        def vrr(e, m):
+            key = ("v", e, m)
+            if key in nodes or key in base:
+                return key
+            if sum(e) == 0:
+                base[key] = m
+                return key
+            i = direction(e)
+            e1 = add(e, i, -1)
+            terms = [(COEF["PA"] + i, vrr(e1, m)), (COEF["WP"] + i, vrr(e1, m + 1))]
+            if e1[i] > 0:
+                e2 = add(e1, i, -1)
+                terms.append((COEF["kinv2z"] + e1[i] - 1, vrr(e2, m)))
+                terms.append((COEF["mrzkinv2z"] + e1[i] - 1, vrr(e2, m + 1)))
+            nodes[key] = terms
+            return key
+
+        def etr(e, f):
+            if sum(f) == 0:
+                return vrr(e, 0)
+            key = ("t", e, f)
+            if key in nodes:
+                return key
+            if g.etr_zero(e, f):
+                return None
+            i = direction(f)
+            f1 = add(f, i, -1)
+            terms = []
+            t = etr(e, f1)
+            if t is not None:
+                terms.append((COEF["DL"] + i, t))
+            t = etr(add(e, i), f1)
+            if t is not None:
+                terms.append((COEF["mzoe"], t))
+            if e[i] > 0:
+                t = etr(add(e, i, -1), f1)
+                if t is not None:
+                    terms.append((COEF["kinv2e"] + e[i] - 1, t))
+            if f1[i] > 0:
+                t = etr(e, add(f1, i, -1))
+                if t is not None:
+                    terms.append((COEF["kinv2e"] + f1[i] - 1, t))
+            if not terms:
+                return None
+            nodes[key] = terms
+            return key
+
+        NF = len(g.F)
+        acc_src = {}
+        for f in g.F:
+            for e in g.E:
+                k = etr(e, f)
+                if k is not None:
+                    acc_src[g.eidx[e] * NF + g.fidx[f]] = k
+        level = {}
+
+        def lev(k):
+            if k in base:
+                return 0
+            if k in level:
+                return level[k]
+            lv = 1 + max(lev(s) for _, s in nodes[k])
+            level[k] = lv
+            return lv
+
+        for k in nodes:
+            lev(k)
+        # LDS slots: 0 = constant zero; [1, 1+NACC) the contracted block (persistent across
+        # primitive quartets); after it the phase-1 region (F_m, VRR, transfer values) and, reusing
+        # the same space once the primitive loop is over, the phase-2 region (HRR values).
+        NFc = len(g.F)
+        nacc = len(g.E) * NFc
+        acc0 = 1
+        slot = {}
+        nxt = acc0 + nacc
+        for k, m in sorted(base.items(), key=lambda kv: kv[1]):
+            slot[k] = nxt
+            nxt += 1
+        self.F0 = slot[("v", (0, 0, 0), 0)]
+        order = sorted(nodes, key=lambda k: (level[k], str(k)))
+        for k in order:
+            slot[k] = nxt
+            nxt += 1
+        phase1_end = nxt
+        # ---- phase 2: HRR after contraction ----
+        A, B, C, D = comps(self.la), comps(self.lb), comps(self.lc), comps(self.ld)
+        h_nodes = {}
+        h_base = {}
+
+        def hb(a, b, f):
+            key = ("hb", a, b, f)
+            if key in h_nodes or key in h_base:
+                return key
+            if sum(b) == 0:
+                h_base[key] = g.eidx[a] * NF + g.fidx[f]
+                return key
+            i = direction(b)
+            b1 = add(b, i, -1)
+            h_nodes[key] = [(COEF["one"], hb(add(a, i), b1, f)), (COEF["AB"] + i, hb(a, b1, f))]
+            return key
+
+        def hk(a, b, c, d):
+            if sum(d) == 0:
+                return hb(a, b, c)
+            key = ("hk", a, b, c, d)
+            if key in h_nodes:
+                return key
+            i = direction(d)
+            d1 = add(d, i, -1)
+            h_nodes[key] = [(COEF["one"], hk(a, b, add(c, i), d1)), (COEF["CD"] + i, hk(a, b, c, d1))]
+            return key
+
+        outs = []
+        for a in A:
+            for b in B:
+                for c in C:
+                    for d in D:
+                        outs.append((hk(a, b, c, d), comp_norm(a) * comp_norm(b) * comp_norm(c) * comp_norm(d)))
+        hlevel = {}
+
+        def hlev(k):
+            if k in h_base:
+                return 0
+            if k in hlevel:
+                return hlevel[k]
+            lv = 1 + max(hlev(s) for _, s in h_nodes[k])
+            hlevel[k] = lv
+            return lv
+
+        for k in h_nodes:
+            hlev(k)
+        for k, ai in h_base.items():
+            slot[k] = acc0 + ai
+        h2 = acc0 + nacc
+        horder = sorted(h_nodes, key=lambda k: (hlevel[k], str(k)))
+        for k in horder:
+            slot[k] = h2
+            h2 += 1
+        nslot = max(phase1_end, h2)
+        # ---- op lists by level ----
+        def ops_of(keys, nodes_, lvl):
+            levels = {}
+            for k in keys:
+                levels.setdefault(lvl[k], []).append(k)
+            out, bounds = [], [0]
+            for lv in sorted(levels):
+                for k in levels[lv]:
+                    terms = nodes_[k] + [(COEF["zero"], None)] * (4 - len(nodes_[k]))
+                    out.append((slot[k], [slot[s] if s is not None else 0 for _, s in terms],
+                                [c for c, _ in terms]))
+                bounds.append(len(out))
+            return out, bounds
+
+        p_ops, p_bounds = ops_of(order, nodes, level)
+        # accumulation level: acc[i] += val[src]
+        acc_ops = [(acc0 + i, [acc0 + i, slot[k], 0, 0], [COEF["one"], COEF["one"], 0, 0])
+                   for i, k in sorted(acc_src.items())]
+        p_ops += acc_ops
+        p_bounds.append(len(p_ops))
+        h_ops, h_bounds = ops_of(horder, h_nodes, hlevel)
+        self.nslot = nslot
+        self.acc0, self.nacc = acc0, nacc
+        self.p_ops, self.p_bounds = p_ops, p_bounds
+        self.h_ops, self.h_bounds = h_ops, h_bounds
+        self.out_slot = [slot[k] for k, _ in outs]
+        self.out_norm = [nv for _, nv in outs]
+        self.flops_prim = sum(2 * sum(1 for c in cs if c != 0) - 1 for _, _, cs in p_ops)
+        self.flops_hrr = sum(3 for _ in h_ops) + sum(1 for v in self.out_norm if v != 1.0)
+        return self
+
+    def emit(self, tag):
+        def packed(ops):
+            rows = []
+            for d, s, c in ops:
+                rows.append("{%d, {%d, %d, %d, %d}, {%d, %d, %d, %d}}" % (d, s[0], s[1], s[2], s[3], c[0], c[1], c[2], c[3]))
+            return ",\n    ".join(rows) if rows else "{0, {0, 0, 0, 0}, {0, 0, 0, 0}}"
+        la, lb, lc, ld = self.la, self.lb, self.lc, self.ld
+        n = "CoopClass<%d, %d, %d, %d, %d>" % (la, lb, lc, ld, 1 if self.exact else 0)
+        out = ["// Generated by gen_eri.py - do not edit.", "#pragma once", '#include "../coop_kernel.hpp"',
+               "namespace hfgpu {"]
+        out.append("static __device__ const CoopOp coop_prim_%s[] = {\n    %s};" % (tag, packed(self.p_ops)))
+        out.append("static __device__ const CoopOp coop_hrr_%s[] = {\n    %s};" % (tag, packed(self.h_ops)))
+        out.append("static __device__ const unsigned short coop_out_%s[] = {%s};" %
+                   (tag, ", ".join(str(x) for x in self.out_slot)))
+        out.append("static __device__ const double coop_norm_%s[] = {%s};" %
+                   (tag, ", ".join("%.17g" % x for x in self.out_norm)))
+        out.append("template <> struct %s {" % n)
+        na, nb, nc, nd = (len(comps(x)) for x in (la, lb, lc, ld))
+        out.append("    static constexpr int L = %d, NA = %d, NB = %d, NC = %d, ND = %d, NOUT = %d;" %
+                   (la + lb + lc + ld, na, nb, nc, nd, na * nb * nc * nd))
+        out.append("    static constexpr int NSLOT = %d, F0 = %d, ACC0 = %d, NACC = %d;" %
+                   (self.nslot, self.F0, self.acc0, self.nacc))
+        # TRUNCATED: reference transfer convention; the kernel evaluates both bra choices and picks
+        # per component as the reference does
+        out.append("    static constexpr bool TRUNCATED = %s;" % ("true" if getattr(self, "truncated", False) else "false"))
+        out.append("    static constexpr int NPL = %d, NHL = %d;" % (len(self.p_bounds) - 1, len(self.h_bounds) - 1))
+        out.append("    static constexpr int PL[%d] = {%s};" % (len(self.p_bounds), ", ".join(map(str, self.p_bounds))))
+        out.append("    static constexpr int HL[%d] = {%s};" % (len(self.h_bounds), ", ".join(map(str, self.h_bounds))))
+        out.append("    __device__ static const CoopOp* prim_ops() { return coop_prim_%s; }" % tag)
+        out.append("    __device__ static const CoopOp* hrr_ops() { return coop_hrr_%s; }" % tag)
+        out.append("    __device__ static const unsigned short* out_slot() { return coop_out_%s; }" % tag)
+        out.append("    __device__ static const double* out_norm() { return coop_norm_%s; }" % tag)
+        out.append("};")
+        out.append("}  // namespace hfgpu")
+        return "\n".join(out) + "\n"
 
 
 if __name__ == "__main__":

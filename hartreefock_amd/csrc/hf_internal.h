@@ -46,9 +46,6 @@ void boys_host(int mmax, double T, double* F);
 // Table F_m(k*dt) for the device kernels.
 void boys_table(std::vector<double>& tab);
 
-// One-electron matrices (one_electron.cpp).
-void one_electron(const Molecule& mol, double* S, double* T, double* V);
-
 inline int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
 }  // namespace hfhost
@@ -126,6 +123,10 @@ struct hf_ctx {
     TaskSet tasks_screened, tasks_all;
     // J/K scratch
     DevBuf<double> d_Jp, d_Kp, d_D, d_J, d_K;
+    // one-electron kernel inputs: per-shell L/bf/prim0/nprim, and alpha|coef|Z|atom xyz
+    DevBuf<int> d_oe_L;
+    DevBuf<double> d_oe_prim;
+    size_t oe_nprim = 0;
     std::vector<KernelStat> stats;
     std::vector<hipEvent_t> ev;
     // side streams: the class kernels of one pass run concurrently, forked from and joined
@@ -143,6 +144,8 @@ struct hf_ctx {
 // Implemented in context.hip
 int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st);
 int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts);
+// Implemented in one_electron.hip: S, T, V (n x n, device) on stream st.
+int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st);
 
 #define HF_HIP_CHECK(ctx, call)                                                                         \
     do {                                                                                                \

@@ -36,6 +36,24 @@ __global__ void fock_kernel(const double* __restrict__ h, const double* __restri
     }
 }
 
+__global__ void add_kernel(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c,
+                           size_t nn) {
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x)
+        c[idx] = a[idx] + b[idx];
+}
+
+// F = g S_ij (h_ii + h_jj)/2 (g > 0) or h; Fa = F plus `asym` on (0,1) and (1,0).
+__global__ void guess_kernel(const double* __restrict__ S, const double* __restrict__ h, double g, int n, double asym,
+                             double* __restrict__ Fa, double* __restrict__ Fb) {
+    const size_t nn = (size_t)n * n;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = idx / n, j = idx % n;
+        const double f = g > 0 ? g * S[idx] * (h[i * n + i] + h[j * n + j]) * 0.5 : h[idx];
+        Fb[idx] = f;
+        Fa[idx] = (idx == 1 || idx == (size_t)n) ? f + asym : f;
+    }
+}
+
 __global__ void antisym_kernel(const double* __restrict__ Y, double* __restrict__ E, int n) {
     const size_t nn = (size_t)n * n;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
@@ -213,9 +231,24 @@ int gemm(hf_ctx* ctx, bool ta, bool tb, int m, int n, int k, double alpha, const
 }
 
 // Symmetric eigenproblem, in place: A -> eigenvectors (column-major), w ascending.
+// HFGPU_EIGEN selects the rocSOLVER routine: syevd (divide and conquer, default), syevj (Jacobi),
+// syevdj, syev (QR iteration).  All FP64.
 int syevd(hf_ctx* ctx, ScfState& st, double* A, double* w) {
-    BL(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.work.p,
-                        st.info.p));
+    static const char* env = std::getenv("HFGPU_EIGEN");
+    static const int alg = !env ? 0 : (!std::strcmp(env, "syevj") ? 1 : !std::strcmp(env, "syevdj") ? 2
+                                                                     : !std::strcmp(env, "syev") ? 3 : 0);
+    if (alg == 1) {
+        BL(rocsolver_dsyevj(ctx->blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, st.n, A,
+                            st.n, 0.0, st.red.p + 8, 100, st.info.p + 1, w, st.info.p));
+    } else if (alg == 2) {
+        BL(rocsolver_dsyevdj(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.info.p));
+    } else if (alg == 3) {
+        BL(rocsolver_dsyev(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.work.p,
+                           st.info.p));
+    } else {
+        BL(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.work.p,
+                            st.info.p));
+    }
     return HF_OK;
 }
 
@@ -298,9 +331,6 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     st.nn = (size_t)n * n;
     const size_t nn = st.nn;
     // ---- Init (HartreeFock/HartreeFockAlgorithm.cpp:21-101) ----
-    std::vector<double> S(nn), T(nn), V(nn), h(nn);
-    hfhost::one_electron(ctx->mol, S.data(), T.data(), V.data());
-    for (size_t i = 0; i < nn; ++i) h[i] = T[i] + V[i];
     const double Enuc = hfhost::nuclear_repulsion(ctx->mol);
     const int nsp = uhf ? 2 : 1;
     HF_HIP_CHECK(ctx, st.S.alloc(nn));
@@ -319,9 +349,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     HF_HIP_CHECK(ctx, st.err.alloc(nn));
     HF_HIP_CHECK(ctx, st.work.alloc(n));
     HF_HIP_CHECK(ctx, st.red.alloc(16));
-    HF_HIP_CHECK(ctx, st.info.alloc(1));
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(st.S.p, S.data(), nn * sizeof(double), hipMemcpyHostToDevice, s));
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(st.h.p, h.data(), nn * sizeof(double), hipMemcpyHostToDevice, s));
+    HF_HIP_CHECK(ctx, st.info.alloc(2));
+    // S, T, V on the device; h = T + V
+    if (int rc = hf_one_electron_device(ctx, st.S.p, st.T1.p, st.T2.p, s)) return rc;
+    hipLaunchKernelGGL(add_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.T2.p, st.h.p, nn);
     // S^{-1/2} = U s^{-1/2} U^T, symmetrised (:48-59)
     HF_HIP_CHECK(ctx, hipMemcpyAsync(st.T1.p, st.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (int rc = syevd(ctx, st, st.T1.p, st.eps.p)) return rc;
@@ -344,23 +375,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         occ[0] = a;
         occ[1] = b;
     }
-    // iteration-0 Fock matrices (InitFockMatrix / InitFockMatrices, iter == 0)
-    {
-        std::vector<double> F0(nn);
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n; ++j)
-                F0[(size_t)i * n + j] = p->init_guess > 0
-                    ? p->init_guess * S[(size_t)i * n + j] * (h[(size_t)i * n + i] + h[(size_t)j * n + j]) * 0.5
-                    : h[(size_t)i * n + j];
-        std::vector<double> F0a(F0);
-        if (uhf && p->add_asymmetry && n > 1) {   // UHF iteration-0 asymmetry (UnrestrictedHartreeFock.cpp:217-221)
-            F0a[1] += p->asymmetry;
-            F0a[n] = F0a[1];
-        }
-        HF_HIP_CHECK(ctx, hipMemcpyAsync(st.F0.p, F0a.data(), nn * sizeof(double), hipMemcpyHostToDevice, s));
-        HF_HIP_CHECK(ctx, hipMemcpyAsync(st.F0.p + nn, F0.data(), nn * sizeof(double), hipMemcpyHostToDevice, s));
-        HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
-    }
+    // iteration-0 Fock matrices (InitFockMatrix / InitFockMatrices, iter == 0), with the UHF
+    // asymmetry on F_alpha(0,1) (UnrestrictedHartreeFock.cpp:217-221)
+    hipLaunchKernelGGL(guess_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.S.p, st.h.p, p->init_guess, n,
+                       uhf && p->add_asymmetry && n > 1 ? p->asymmetry : 0.0, st.F0.p, st.F0.p + nn);
     HF_HIP_CHECK(ctx, hipMemsetAsync(st.D.p, 0, 2 * nn * sizeof(double), s));
     Diis diis[2];
     for (int k = 0; k < nsp; ++k) {

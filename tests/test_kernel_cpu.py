@@ -143,3 +143,117 @@ def test_dddd_reference_convention_matches_oracle(name, bfile, atoms):
                                 ref = mol.eri(P[0]["bf"] + ia, P[1]["bf"] + ib, Q[0]["bf"] + ic, Q[1]["bf"] + id_)
                                 worst = max(worst, abs(blk[ia, ib, ic, id_] - ref))
     assert worst < 1e-12, worst
+
+
+def _coop_eval(cg, A, B, C, D, boys):
+    """Interpret a cooperative class's op tables (what the wavefront kernel executes) in numpy."""
+    import sys
+    csrc = os.path.join(os.path.dirname(HERE), "hartreefock_amd", "csrc")
+    if csrc not in sys.path:
+        sys.path.insert(0, csrc)
+    val = np.zeros(cg.nslot)
+    L = cg.g.L
+    for P in prim_pairs(A, B):
+        for Q in prim_pairs(C, D):
+            inv_s = 1.0 / (P[3] + Q[3])
+            rho = P[3] * Q[3] * inv_s
+            PQ = P[:3] - Q[:3]
+            T = rho * float(PQ @ PQ)
+            pref = 2 * math.pi ** 2.5 * P[4] * Q[4] * math.sqrt(inv_s)
+            F = [pref * md_eri.boys_mp(m, T) for m in range(L + 1)]
+            for m in range(L + 1):
+                val[cg.F0 + m] = F[m]
+            PA = P[:3] - A["xyz"]
+            rz = Q[3] * inv_s
+            zoe = P[3] * Q[5]
+            cf = np.zeros(48)
+            cf[1] = 1.0
+            cf[2:5] = PA
+            cf[5:8] = -rz * PQ
+            cf[8:11] = zoe * PA + (Q[:3] - C["xyz"])
+            cf[11] = -zoe
+            for k in range(1, 9):
+                cf[11 + k] = k * 0.5 * P[5]
+                cf[19 + k] = -rz * k * 0.5 * P[5]
+                cf[27 + k] = k * 0.5 * Q[5]
+            for dst, src, cs in cg.p_ops:
+                val[dst] = sum(cf[c] * val[s] for s, c in zip(src, cs))
+    cf = np.zeros(48)
+    cf[1] = 1.0
+    cf[36:39] = A["xyz"] - B["xyz"]
+    cf[39:42] = C["xyz"] - D["xyz"]
+    for dst, src, cs in cg.h_ops:
+        val[dst] = sum(cf[c] * val[s] for s, c in zip(src, cs))
+    return np.array([val[s] * nv for s, nv in zip(cg.out_slot, cg.out_norm)])
+
+
+def test_cooperative_op_tables_match_oracle():
+    """The op lists the one-wavefront-per-quartet kernels execute, interpreted level by level in
+    numpy (Boys from mpmath), against the oracle (exact convention) for every cooperative class."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "hartreefock_amd", "csrc"))
+    import gen_eri
+    atoms = [(6, (0.0, 0.0, 0.0)), (6, (0.3, 1.2, 2.5))]
+    bfile = basis("6-31g_st_.1.nw")
+    shells = shells_of(bfile, atoms)
+    mol = refcpu.Molecule(bfile, atoms)
+    by = {}
+    for s in shells:
+        by.setdefault((s["L"], s["atom"]), []).append(s)
+    refcpu.set_exact_transfer(True)
+    worst = 0.0
+    try:
+        for cls in [(2, 1, 2, 1), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)]:
+            cg = gen_eri.CoopGen(*cls, True).build()
+            A = by[(cls[0], 1)][0]
+            B = by[(cls[1], 0)][-1]
+            C = by[(cls[2], 0)][0]
+            D = by[(cls[3], 1)][-1]
+            blk = _coop_eval(cg, A, B, C, D, None)
+            nb_, nc_, nd_ = ((x + 1) * (x + 2) // 2 for x in cls[1:])
+            for idx, v in enumerate(blk):
+                id_ = idx % nd_
+                ic = (idx // nd_) % nc_
+                ib = (idx // (nd_ * nc_)) % nb_
+                ia = idx // (nd_ * nc_ * nb_)
+                ref = mol.eri(A["bf"] + ia, B["bf"] + ib, C["bf"] + ic, D["bf"] + id_)
+                worst = max(worst, abs(v - ref))
+    finally:
+        refcpu.set_exact_transfer(False)
+    assert worst < 1e-12, worst
+
+
+def test_cooperative_truncated_dddd_matches_oracle_reference_mode():
+    """The reference-convention (dd|dd) op tables with the kernel's per-component bra choice
+    (both launch orientations, one- and two-centre pairs) against the oracle's default mode."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "hartreefock_amd", "csrc"))
+    import gen_eri
+    atoms = [(6, (0.0, 0.0, 0.0)), (6, (0.3, 1.2, 2.5))]
+    bfile = basis("6-31g_st_.1.nw")
+    shells = shells_of(bfile, atoms)
+    mol = refcpu.Molecule(bfile, atoms)
+    cg = gen_eri.CoopGen(2, 2, 2, 2, False).build()
+    d = [s for s in shells if s["L"] == 2]
+    pairs = [(x, y) for x in d for y in d if x["index"] >= y["index"]]
+    worst = 0.0
+    for (A, B) in pairs:
+        for (C, D) in pairs:
+            needP, needQ = A["index"] >= C["index"], A["index"] <= C["index"]
+            res = np.zeros(1296)
+            if needP:
+                res[:] = _coop_eval(cg, A, B, C, D, None)
+            if needQ:
+                t = _coop_eval(cg, C, D, A, B, None)
+                for idx in range(1296):
+                    id_, ic, ib, ia = idx % 6, (idx // 6) % 6, (idx // 36) % 6, idx // 216
+                    a, b, c, dd = A["bf"] + ia, B["bf"] + ib, C["bf"] + ic, D["bf"] + id_
+                    i, j, k, l = max(a, b), min(a, b), max(c, dd), min(c, dd)
+                    braP = needP and not (i < k or (i == k and j < l))
+                    if not braP:
+                        res[idx] = t[((ic * 6 + id_) * 6 + ia) * 6 + ib]
+            for idx in range(0, 1296, 7):
+                id_, ic, ib, ia = idx % 6, (idx // 6) % 6, (idx // 36) % 6, idx // 216
+                ref = mol.eri(A["bf"] + ia, B["bf"] + ib, C["bf"] + ic, D["bf"] + id_)
+                worst = max(worst, abs(res[idx] - ref))
+    assert worst < 1e-12, worst
