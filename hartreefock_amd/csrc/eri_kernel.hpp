@@ -7,10 +7,13 @@
 //     [e0|f0] block held in registers;
 //   then the horizontal recursions on bra and ket (generated) give the Cartesian
 //   block (ab|cd), which goes to one of three sinks:
-//     MODE_JK      digestion into J and K with FP64 atomics (8-fold symmetry),
+//     MODE_JK      digestion into J and K (8-fold symmetry): exchange rows of the block's bra
+//                  pairs staged in LDS, Coulomb ket block with FP64 global atomics,
 //     MODE_PACKED  the reference's packed unique-ERI store,
 //     MODE_DIAG    the Schwarz bound sqrt(max |(ab|ab)|) of a shell pair.
 #pragma once
+
+#include <algorithm>
 
 #include "hf_device.h"
 #include "generated/eri_classes.inc"
@@ -169,6 +172,132 @@ __device__ __forceinline__ void sink_jk(const double* __restrict__ v, int a0, in
     }
 }
 
+// Digestion of one quartet whose bra pair has an LDS slot.  The slot holds, per density k, the
+// exchange rows of the bra's functions: NA rows a0.. then NB rows b0.., n columns each; the four
+// exchange blocks K'_ac, K'_ad, K'_bc, K'_bd all land in those rows.  The ket-side Coulomb block
+// J'_cd goes to global memory; the bra-side Coulomb block J'_ab is returned in jab for a
+// reduction across the wavefront.
+template <class C>
+__device__ __forceinline__ void sink_jk_rows(const double* __restrict__ v, double* __restrict__ rows, int a0, int b0,
+                                             int c0, int d0, double w, const SinkArgs& s, double* __restrict__ jab) {
+    constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+    const int n = s.n;
+    const size_t nn = (size_t)n * n;
+    {
+        double Dcd[NC * ND], Dab[NA * NB];
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                double x = 0;
+                for (int k = 0; k < s.ndens; ++k) x += s.D[k * nn + (size_t)(c0 + i) * n + d0 + j];
+                Dcd[i * ND + j] = x;
+            }
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                double x = 0;
+                for (int k = 0; k < s.ndens; ++k) x += s.D[k * nn + (size_t)(a0 + i) * n + b0 + j];
+                Dab[i * NB + j] = x;
+            }
+        const double w2 = 2.0 * w;
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+                double acc = 0;
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dcd[ic * ND + id];
+                jab[ia * NB + ib] = w2 * acc;
+            }
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0;
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int ib = 0; ib < NB; ++ib) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dab[ia * NB + ib];
+                unsafeAtomicAdd(&s.J[(size_t)(c0 + ic) * n + d0 + id], w2 * acc);
+            }
+    }
+    for (int k = 0; k < s.ndens; ++k) {
+        const double* __restrict__ D = s.D + k * nn;
+        double* ra = rows + (size_t)k * (NA + NB) * n;
+        double* rb = ra + NA * n;
+        double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+#pragma unroll
+            for (int j = 0; j < ND; ++j) Dbd[i * ND + j] = D[(size_t)(b0 + i) * n + d0 + j];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) Dbc[i * NC + j] = D[(size_t)(b0 + i) * n + c0 + j];
+        }
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+#pragma unroll
+            for (int j = 0; j < ND; ++j) Dad[i * ND + j] = D[(size_t)(a0 + i) * n + d0 + j];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) Dac[i * NC + j] = D[(size_t)(a0 + i) * n + c0 + j];
+        }
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia) {
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) {
+                double acc = 0;
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dbd[ib * ND + id];
+                atomicAdd(&ra[ia * n + c0 + ic], w * acc);
+            }
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0;
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dbc[ib * NC + ic];
+                atomicAdd(&ra[ia * n + d0 + id], w * acc);
+            }
+        }
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic) {
+                double acc = 0;
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dad[ia * ND + id];
+                atomicAdd(&rb[ib * n + c0 + ic], w * acc);
+            }
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0;
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                    for (int ic = 0; ic < NC; ++ic) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dac[ia * NC + ic];
+                atomicAdd(&rb[ib * n + d0 + id], w * acc);
+            }
+        }
+    }
+}
+
+// Bra pair owning task t: the largest p in [lo, hi) with prefix[p] <= t (prefix[hi] > t).
+__device__ __forceinline__ int bra_of_task(const int64_t* __restrict__ prefix, int64_t t, int lo, int hi) {
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 template <class C>
 __device__ __forceinline__ void sink_packed(const double* __restrict__ v, int A, int B, int Cs, int Ds, int a0, int b0,
                                             int c0, int d0, bool same_pair, const SinkArgs& s) {
@@ -261,28 +390,10 @@ __host__ __device__ __forceinline__ void eval_quartet_reference_order(
                 }
 }
 
-template <int LA, int LB, int LC, int LD, int EXACT, int MODE>
-__global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev sh, const PrimPair* __restrict__ pp,
-                                                        const double* __restrict__ boys, SinkArgs sink) {
-    using C = EriClass<LA, LB, LC, LD, EXACT>;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tl.ntasks) return;
-    int p, q;
-    if (MODE == MODE_DIAG) {
-        p = q = (int)t;
-    } else {
-        int lo = 0, hi = tl.bra.npairs;          // prefix[lo] <= t < prefix[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (tl.prefix[mid] <= t) lo = mid; else hi = mid;
-        }
-        p = lo;
-        q = (int)(t - tl.prefix[lo]);
-    }
-    const int A = tl.bra.sa[p], B = tl.bra.sb[p];
-    const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
-    const int a0 = sh.bf[A], b0 = sh.bf[B], c0 = sh.bf[Cs], d0 = sh.bf[Ds];
-    double out[C::NOUT];
+template <class C>
+__device__ __forceinline__ void quartet_values(const TaskList& tl, const ShellsDev& sh, const PrimPair* __restrict__ pp,
+                                               const double* __restrict__ boys, int p, int q, int A, int B, int Cs,
+                                               int Ds, int a0, int b0, int c0, int d0, double* out) {
     if constexpr (C::TRUNCATED) {
         eval_quartet_reference_order<C>(A, Cs, sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds,
                                         pp + tl.bra.prim0[p], tl.bra.nprim[p], pp + tl.ket.prim0[q],
@@ -291,25 +402,126 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
         eval_quartet<C>(sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds, pp + tl.bra.prim0[p],
                         tl.bra.nprim[p], pp + tl.ket.prim0[q], tl.ket.nprim[q], boys, out);
     }
+}
 
-    if (MODE == MODE_JK) {
+// J/K digestion for a block of consecutive tasks.  Tasks are ordered by bra pair, so a block covers
+// a short run of bra pairs [pb, pe].  The first sink.lds_slots of them get an LDS slot holding
+// their exchange rows; every K' contribution of their quartets is an LDS atomic, and at the end
+// the block adds the slots to K' with column-coalesced global atomics (one per touched element
+// instead of one per quartet and component).  J'_ab is summed across the wavefront when the
+// wavefront shares one slot.  Quartets of further bra pairs use the plain global-atomic sink.
+template <class C>
+__device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev& sh, const PrimPair* __restrict__ pp,
+                                             const double* __restrict__ boys, const SinkArgs& sink) {
+    extern __shared__ double rows[];
+    constexpr int NA = C::NA, NB = C::NB, NAB = C::NA * C::NB;
+    const int n = sink.n;
+    const size_t nn = (size_t)n * n;
+    const int rows_per_slot = sink.ndens * (NA + NB);
+    const int slot_len = rows_per_slot * n;
+    const int64_t tb = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t tlast = (tb + blockDim.x < tl.ntasks ? tb + blockDim.x : tl.ntasks) - 1;
+    const int pb = bra_of_task(tl.prefix, tb, 0, tl.bra.npairs);
+    const int pe = bra_of_task(tl.prefix, tlast, pb, tl.bra.npairs);
+    const int nslot = pe - pb + 1 < sink.lds_slots ? pe - pb + 1 : sink.lds_slots;
+    for (int i = threadIdx.x; i < nslot * slot_len; i += blockDim.x) rows[i] = 0.0;
+    __syncthreads();
+
+    const int64_t t = tb + threadIdx.x;
+    const bool active = t <= tlast;
+    int key = -1, a0 = 0, b0 = 0;
+    double jab[NAB];
+#pragma unroll
+    for (int i = 0; i < NAB; ++i) jab[i] = 0.0;
+    if (active) {
+        const int p = bra_of_task(tl.prefix, t, pb, pe + 1);
+        const int q = (int)(t - tl.prefix[p]);
+        const int A = tl.bra.sa[p], B = tl.bra.sb[p];
+        const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
+        a0 = sh.bf[A];
+        b0 = sh.bf[B];
+        const int c0 = sh.bf[Cs], d0 = sh.bf[Ds];
+        double out[C::NOUT];
+        quartet_values<C>(tl, sh, pp, boys, p, q, A, B, Cs, Ds, a0, b0, c0, d0, out);
         double w = 1.0;
         if (A == B) w *= 0.5;
         if (Cs == Ds) w *= 0.5;
         if (tl.same_class && p == q) w *= 0.5;
-        sink_jk<C>(out, a0, b0, c0, d0, w, sink);
-    } else if (MODE == MODE_PACKED) {
-        sink_packed<C>(out, A, B, Cs, Ds, a0, b0, c0, d0, tl.same_class && p == q, sink);
+        if (p - pb < nslot) {
+            key = p - pb;
+            sink_jk_rows<C>(out, rows + (size_t)key * slot_len, a0, b0, c0, d0, w, sink, jab);
+        } else {
+            sink_jk<C>(out, a0, b0, c0, d0, w, sink);
+        }
+    }
+    // J'_ab: lanes without a slot hold zeros; lane 0 is active whenever any lane is
+    const int key0 = __shfl(key, 0);
+    if (__all(key == key0 || key < 0)) {
+#pragma unroll
+        for (int i = 0; i < NAB; ++i) {
+            double x = jab[i];
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
+            jab[i] = x;
+        }
+        if (key0 >= 0 && (threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                for (int ib = 0; ib < NB; ++ib)
+                    unsafeAtomicAdd(&sink.J[(size_t)(a0 + ia) * n + b0 + ib], jab[ia * NB + ib]);
+        }
+    } else if (key >= 0) {
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) unsafeAtomicAdd(&sink.J[(size_t)(a0 + ia) * n + b0 + ib], jab[ia * NB + ib]);
+    }
+    __syncthreads();
+    // flush: row by row, consecutive threads on consecutive columns
+    for (int sr = 0; sr < nslot * rows_per_slot; ++sr) {
+        const int s = sr / rows_per_slot, kr = sr - s * rows_per_slot;
+        const int k = kr / (NA + NB), r = kr - k * (NA + NB);
+        const int p = pb + s;
+        const int grow = r < NA ? sh.bf[tl.bra.sa[p]] + r : sh.bf[tl.bra.sb[p]] + (r - NA);
+        const double* src = rows + (size_t)sr * n;
+        double* dst = sink.K + k * nn + (size_t)grow * n;
+        for (int col = threadIdx.x; col < n; col += blockDim.x) {
+            const double x = src[col];
+            if (x != 0.0) unsafeAtomicAdd(&dst[col], x);
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD, int EXACT, int MODE>
+__global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev sh, const PrimPair* __restrict__ pp,
+                                                        const double* __restrict__ boys, SinkArgs sink) {
+    using C = EriClass<LA, LB, LC, LD, EXACT>;
+    if constexpr (MODE == MODE_JK) {
+        eri_jk_block<C>(tl, sh, pp, boys, sink);
     } else {
-        double mx = 0.0;
+        const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= tl.ntasks) return;
+        const int p = MODE == MODE_DIAG ? (int)t : bra_of_task(tl.prefix, t, 0, tl.bra.npairs);
+        const int q = MODE == MODE_DIAG ? (int)t : (int)(t - tl.prefix[p]);
+        const int A = tl.bra.sa[p], B = tl.bra.sb[p];
+        const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
+        const int a0 = sh.bf[A], b0 = sh.bf[B], c0 = sh.bf[Cs], d0 = sh.bf[Ds];
+        double out[C::NOUT];
+        quartet_values<C>(tl, sh, pp, boys, p, q, A, B, Cs, Ds, a0, b0, c0, d0, out);
+        if constexpr (MODE == MODE_PACKED) {
+            sink_packed<C>(out, A, B, Cs, Ds, a0, b0, c0, d0, tl.same_class && p == q, sink);
+        } else {
+            double mx = 0.0;
 #pragma unroll
-        for (int ia = 0; ia < C::NA; ++ia)
+            for (int ia = 0; ia < C::NA; ++ia)
 #pragma unroll
-            for (int ib = 0; ib < C::NB; ++ib) {
-                const double x = fabs(out[((ia * C::NB + ib) * C::NC + ia) * C::ND + ib]);
-                mx = x > mx ? x : mx;
-            }
-        sink.Qout[p] = sqrt(mx);
+                for (int ib = 0; ib < C::NB; ++ib) {
+                    const double x = fabs(out[((ia * C::NB + ib) * C::NC + ia) * C::ND + ib]);
+                    mx = x > mx ? x : mx;
+                }
+            sink.Qout[p] = sqrt(mx);
+        }
     }
 }
 
@@ -318,8 +530,18 @@ inline hipError_t launch_eri(const TaskList& tl, const ShellsDev& sh, const Prim
                              const SinkArgs& sink, hipStream_t st) {
     if (tl.ntasks <= 0) return hipSuccess;
     const int64_t blocks = (tl.ntasks + 255) / 256;
-    hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, EXACT, MODE>), dim3((unsigned)blocks), dim3(256), 0, st, tl,
-                       sh, pp, boys, sink);
+    SinkArgs s = sink;
+    size_t lds = 0;
+    if (MODE == MODE_JK) {
+        using C = EriClass<LA, LB, LC, LD, 0>;
+        // up to 4 bra pairs per block, within 40 KB of LDS (a block of 256 tasks rarely spans more)
+        const size_t slot_bytes = (size_t)sink.ndens * (C::NA + C::NB) * sink.n * sizeof(double);
+        const size_t cap = 40 * 1024;
+        s.lds_slots = (int)std::min<size_t>(4, cap / slot_bytes);
+        lds = s.lds_slots * slot_bytes;
+    }
+    hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, EXACT, MODE>), dim3((unsigned)blocks), dim3(256), lds, st, tl,
+                       sh, pp, boys, s);
     return hipGetLastError();
 }
 

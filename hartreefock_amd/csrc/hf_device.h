@@ -62,6 +62,7 @@ struct SinkArgs {
     double* packed;          // MODE_PACKED output (reference 8-fold packed order)
     uint64_t packed_len;
     double* Qout;            // MODE_DIAG output, one per bra pair
+    int lds_slots;           // MODE_JK: bra pairs per block whose exchange rows are staged in LDS
 };
 
 }  // namespace hfgpu
