@@ -8,10 +8,13 @@
 //   UnrestrictedHartreeFock::Step / DIISStep / InitFockMatrices / CalculateEnergy
 //     (HartreeFock/UnrestrictedHartreeFock.cpp:62-293),
 //   DIIS (HartreeFock/DIIS.h:9-96): 6 kept, extrapolation from 5, B c = [0..0 1].
-// All matrices live in HBM; the dense steps use rocBLAS dgemm and rocSOLVER dsyevd
-// (FP64).  Only eigenvalues, a few scalars and the DIIS inner products come back to
-// the host each iteration.  Matrices are symmetric except the MO coefficients C,
-// which are stored column-major (column j = orbital j) as rocSOLVER returns them.
+// All matrices live in HBM; the dense steps use rocBLAS dgemm (FP64).  The reference
+// diagonalises the orthogonalised Fock matrix every step; here the new density is by
+// default the same aufbau projector obtained by SP2 purification (dgemm only), with
+// rocSOLVER dsyevd as the fallback and for the final orbital energies.  Only a few
+// scalars and the DIIS inner products come back to the host each iteration.  Matrices
+// are symmetric except the MO coefficients C, which are stored column-major (column
+// j = orbital j) as rocSOLVER returns them.
 #include <rocsolver/rocsolver.h>
 
 #include <chrono>
@@ -140,6 +143,108 @@ int grid_for(size_t n) {
     return (int)std::min<size_t>(std::max<size_t>(b, 1), 4096);
 }
 
+// ---- Density matrix by second-order spectral projection (SP2; Niklasson, PRB 66, 155115 (2002)) ----
+// For the orthogonalised Fock matrix F' the aufbau density is the spectral projector onto its
+// `nocc` lowest eigenvectors.  Starting from X0 = (hi I - F')/(hi - lo), with [lo, hi] enclosing the
+// spectrum (Gershgorin discs), every step replaces X by X^2 or 2X - X^2, whichever keeps trace(X)
+// closer to nocc; the eigenvalues are driven to 0 or 1 with the eigenvectors unchanged.  Each step
+// is one dgemm plus two small kernels, and no eigendecomposition is needed.  Stopping rule
+// (Kruchinina, Rudberg, Rubensson, JCTC 12, 5788 (2016)): once the idempotency error
+// e_k = trace(X - X^2) is small, stop as soon as it no longer decreases (e_k >= e_{k-2}).
+// State (device, 8 doubles): lo, hi, done, e_{k-1}, e_{k-2}, steps, use_square, apply.
+enum { SP2_LO, SP2_HI, SP2_DONE, SP2_E1, SP2_E2, SP2_STEPS, SP2_SQUARE, SP2_APPLY };
+
+__device__ double block_sum_256(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(256) sp2_bounds_kernel(const double* __restrict__ F, int n, double* __restrict__ st) {
+    __shared__ double slo[256], shi[256];
+    double lo = 1e300, hi = -1e300;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {     // column j (F symmetric), coalesced over j
+        double r = 0;
+        for (int i = 0; i < n; ++i)
+            if (i != j) r += fabs(F[(size_t)i * n + j]);
+        const double c = F[(size_t)j * n + j];
+        lo = fmin(lo, c - r);
+        hi = fmax(hi, c + r);
+    }
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + s]);
+            shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        lo = slo[0];
+        hi = shi[0];
+        if (hi - lo < 1e-12) hi = lo + 1.0;
+        st[SP2_LO] = lo;
+        st[SP2_HI] = hi;
+        st[SP2_DONE] = 0;
+        st[SP2_E1] = 1e300;
+        st[SP2_E2] = 1e300;
+        st[SP2_STEPS] = 0;
+        st[SP2_SQUARE] = 0;
+        st[SP2_APPLY] = 0;
+    }
+}
+
+__global__ void sp2_init_kernel(const double* __restrict__ F, double* __restrict__ X, int n,
+                                const double* __restrict__ st) {
+    const size_t nn = (size_t)n * n;
+    const double lo = st[SP2_LO], hi = st[SP2_HI], inv = 1.0 / (hi - lo);
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = idx / n, j = idx % n;
+        X[idx] = ((i == j ? hi : 0.0) - F[idx]) * inv;
+    }
+}
+
+__global__ void __launch_bounds__(256) sp2_decide_kernel(const double* __restrict__ X, const double* __restrict__ X2,
+                                                         int n, double nocc, double* __restrict__ st) {
+    __shared__ double sh[256];
+    if (st[SP2_DONE] != 0) {
+        if (threadIdx.x == 0) st[SP2_APPLY] = 0;
+        return;
+    }
+    double t1 = 0, t2 = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        t1 += X[(size_t)i * n + i];
+        t2 += X2[(size_t)i * n + i];
+    }
+    t1 = block_sum_256(t1, sh);
+    t2 = block_sum_256(t2, sh);
+    if (threadIdx.x == 0) {
+        const double e = t1 - t2;
+        st[SP2_SQUARE] = fabs(t2 - nocc) <= fabs(2.0 * t1 - t2 - nocc) ? 1.0 : 0.0;
+        st[SP2_APPLY] = 1.0;
+        st[SP2_STEPS] += 1.0;
+        if (e < 1e-4 && e >= st[SP2_E2]) st[SP2_DONE] = 1.0;
+        st[SP2_E2] = st[SP2_E1];
+        st[SP2_E1] = e;
+    }
+}
+
+__global__ void sp2_update_kernel(double* __restrict__ X, const double* __restrict__ X2, size_t nn,
+                                  const double* __restrict__ st) {
+    if (st[SP2_APPLY] == 0) return;
+    const bool sq = st[SP2_SQUARE] != 0;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x)
+        X[idx] = sq ? X2[idx] : 2.0 * X[idx] - X2[idx];
+}
+
 // Column-pivoted Householder QR solve of the (m x m) DIIS system (row-major A).
 std::vector<double> qr_solve(int m, std::vector<double> A, std::vector<double> b) {
     std::vector<int> perm(m);
@@ -211,7 +316,7 @@ struct Diis {
 struct ScfState {
     int n = 0;
     size_t nn = 0;
-    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, C, eps, T1, T2, err, work, red;
+    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, C, eps, T1, T2, T3, err, work, red, sp2;
     DevBuf<rocblas_int> info;
 };
 
@@ -249,6 +354,34 @@ int syevd(hf_ctx* ctx, ScfState& st, double* A, double* w) {
         BL(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.work.p,
                             st.info.p));
     }
+    return HF_OK;
+}
+
+// Aufbau projector of the orthogonalised Fock matrix Fp (not modified) onto its nocc lowest
+// eigenvectors, by SP2 steps, into P; W is workspace.  ok = false when the stopping rule was not
+// met within kMaxSteps (a vanishing HOMO-LUMO gap leaves fractional eigenvalues).
+int sp2_projector(hf_ctx* ctx, ScfState& st, const double* Fp, int nocc, double* P, double* W, bool& ok) {
+    hipStream_t s = ctx->stream;
+    const int n = st.n;
+    constexpr int kMaxSteps = 100, kFirstCheck = 16, kEvery = 4;
+    hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(256), 0, s, Fp, n, st.sp2.p);
+    hipLaunchKernelGGL(sp2_init_kernel, dim3(grid_for(st.nn)), dim3(256), 0, s, Fp, P, n, st.sp2.p);
+    ok = false;
+    for (int k = 1; k <= kMaxSteps; ++k) {
+        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, P, P, 0.0, W)) return rc;
+        hipLaunchKernelGGL(sp2_decide_kernel, dim3(1), dim3(256), 0, s, P, W, n, (double)nocc, st.sp2.p);
+        hipLaunchKernelGGL(sp2_update_kernel, dim3(grid_for(st.nn)), dim3(256), 0, s, P, W, st.nn, st.sp2.p);
+        if ((k >= kFirstCheck && (k - kFirstCheck) % kEvery == 0) || k == kMaxSteps) {
+            double done = 0;
+            HF_HIP_CHECK(ctx, hipMemcpyAsync(&done, st.sp2.p + SP2_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
+            HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+            if (done != 0) {
+                ok = true;
+                break;
+            }
+        }
+    }
+    HF_HIP_CHECK(ctx, hipGetLastError());
     return HF_OK;
 }
 
@@ -315,6 +448,7 @@ void hf_scf_defaults(hf_scf_params* p) {
     p->normal_iter_after_diis = 500;
     p->add_asymmetry = 1;
     p->asymmetry = 0.1;
+    p->density_method = 0;
 }
 
 int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out) {
@@ -346,6 +480,8 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     HF_HIP_CHECK(ctx, st.eps.alloc(2 * n));
     HF_HIP_CHECK(ctx, st.T1.alloc(nn));
     HF_HIP_CHECK(ctx, st.T2.alloc(nn));
+    HF_HIP_CHECK(ctx, st.T3.alloc(nn));
+    HF_HIP_CHECK(ctx, st.sp2.alloc(8));
     HF_HIP_CHECK(ctx, st.err.alloc(nn));
     HF_HIP_CHECK(ctx, st.work.alloc(n));
     HF_HIP_CHECK(ctx, st.red.alloc(16));
@@ -391,6 +527,8 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     const double dtol = p->density_tolerance > 0 ? p->density_tolerance : 1e-10;
     const double itol = p->diis_tolerance > 0 ? p->diis_tolerance : 1e-10;
     double energy = 0, homo = 0, lastErr = 0, jk_ms = 0;
+    int diag_steps = 0;
+    bool last_diag[2] = {false, false};
     bool useDIIS = p->use_diis != 0;
     std::vector<double> eps_h(2 * (size_t)n);
 
@@ -429,44 +567,50 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             }
         } else
             lastErr = 0;
-        // orthogonalise, diagonalise, back-transform, new density
+        // orthogonalise; new density from the SP2 projector of F' or, failing that, from the
+        // eigenvectors of F' (back-transformed)
         for (int k = 0; k < nsp; ++k) {
+            const double scale = uhf ? 1.0 : 2.0;
+            double* Dk = st.Dn.p + k * nn;
             if (int rc = gemm(ctx, false, false, n, n, n, 1.0, F[k], st.X.p, 0.0, st.T1.p)) return rc;
             if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
-            if (int rc = syevd(ctx, st, st.T2.p, st.eps.p + k * n)) return rc;
-            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T2.p, 0.0, st.C.p + k * nn)) return rc;
-            if (int rc = gemm(ctx, false, true, n, n, occ[k], uhf ? 1.0 : 2.0, st.C.p + k * nn, st.C.p + k * nn, 0.0,
-                              st.Dn.p + k * nn))
-                return rc;
+            bool ok = false;
+            if (occ[k] == 0) {
+                HF_HIP_CHECK(ctx, hipMemsetAsync(Dk, 0, nn * sizeof(double), s));
+                ok = true;
+            } else if (p->density_method == 0) {
+                if (int rc = sp2_projector(ctx, st, st.T2.p, occ[k], st.T1.p, st.T3.p, ok)) return rc;
+                if (ok) {
+                    if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.T1.p, st.X.p, 0.0, st.T3.p)) return rc;
+                    if (int rc = gemm(ctx, false, false, n, n, n, scale, st.X.p, st.T3.p, 0.0, Dk)) return rc;
+                }
+            }
+            last_diag[k] = !ok;
+            if (!ok) {
+                ++diag_steps;
+                if (int rc = syevd(ctx, st, st.T2.p, st.eps.p + k * n)) return rc;
+                if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T2.p, 0.0, st.C.p + k * nn)) return rc;
+                if (int rc = gemm(ctx, false, true, n, n, occ[k], scale, st.C.p + k * nn, st.C.p + k * nn, 0.0, Dk))
+                    return rc;
+            }
         }
-        // energy pieces and density change
+        // energy E = (D.h + D.F)/2 (sum over spins; D.F is twice (RHF) or once (UHF) the sum of the
+        // occupied orbital energies) and the density change
         DotJobs jobs{};
         jobs.m = 0;
         for (int k = 0; k < nsp; ++k) {
             jobs.x[jobs.m] = st.Dn.p + k * nn; jobs.y[jobs.m] = st.h.p; jobs.diff[jobs.m++] = 0;
+            jobs.x[jobs.m] = st.Dn.p + k * nn; jobs.y[jobs.m] = F[k]; jobs.diff[jobs.m++] = 0;
             jobs.x[jobs.m] = st.Dn.p + k * nn; jobs.y[jobs.m] = st.D.p + k * nn; jobs.diff[jobs.m++] = 1;
         }
         hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, nn, st.red.p);
         double red[8];
         HF_HIP_CHECK(ctx, hipMemcpyAsync(red, st.red.p, jobs.m * sizeof(double), hipMemcpyDeviceToHost, s));
-        HF_HIP_CHECK(ctx, hipMemcpyAsync(eps_h.data(), st.eps.p, nsp * n * sizeof(double), hipMemcpyDeviceToHost, s));
         HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
-        double eh = 0, rms2 = 0;
+        double e = 0, rms2 = 0;
         for (int k = 0; k < nsp; ++k) {
-            eh += red[2 * k];
-            rms2 += red[2 * k + 1];
-        }
-        double e = 0;
-        if (!uhf) {
-            e = 0.5 * eh;
-            for (int v = 0; v < occ[0]; ++v) e += eps_h[v];
-            homo = eps_h[occ[0] - 1];
-        } else {
-            e = eh;
-            for (int v = 0; v < occ[0]; ++v) e += eps_h[v];
-            for (int v = 0; v < occ[1]; ++v) e += eps_h[n + v];
-            homo = std::max(occ[0] ? eps_h[occ[0] - 1] : 0.0, occ[1] ? eps_h[n + occ[1] - 1] : 0.0);
-            e *= 0.5;
+            e += 0.5 * (red[3 * k] + red[3 * k + 1]);
+            rms2 += red[3 * k + 2];
         }
         energy = e + Enuc;
         rms = std::sqrt(rms2);
@@ -522,11 +666,26 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             prev = cur;
         }
     }
+    // orbital energies (HOMO) and coefficients of the last Fock matrices
+    for (int k = 0; k < nsp; ++k) {
+        if (last_diag[k]) continue;
+        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.Fa.p + k * nn, st.X.p, 0.0, st.T1.p)) return rc;
+        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
+        if (int rc = syevd(ctx, st, st.T2.p, st.eps.p + k * n)) return rc;
+        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T2.p, 0.0, st.C.p + k * nn)) return rc;
+    }
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(eps_h.data(), st.eps.p, nsp * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (!uhf)
+        homo = eps_h[occ[0] - 1];
+    else
+        homo = std::max(occ[0] ? eps_h[occ[0] - 1] : 0.0, occ[1] ? eps_h[n + occ[1] - 1] : 0.0);
     r->energy = cur;
     r->homo = homo;
     r->iterations = iter + (converged ? 1 : 0);
     r->converged = converged ? 1 : 0;
     r->jk_ms = jk_ms;
+    r->diag_steps = diag_steps;
     if (uhf) {
         // <S^2> = ((Na-Nb)/2)^2 + (Na+Nb)/2 - sum_ij ((C_a,occ)^T S C_b,occ)_ij^2
         if (occ[0] > 0 && occ[1] > 0) {

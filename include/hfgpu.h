@@ -131,6 +131,9 @@ typedef struct hf_scf_params {
     double density_tolerance;      /* 0: the reference's 1e-10 (Frobenius norm of the change) */
     double diis_tolerance;         /* 0: the reference's 1e-10 */
     volatile int* terminate;       /* optional cancellation flag, polled once per iteration */
+    int density_method;            /* 0: spectral-projection purification (SP2), falling back to
+                                      diagonalisation for a step it cannot resolve; 1: diagonalise
+                                      every step (the reference's route).  Same fixed point. */
 } hf_scf_params;
 
 typedef struct hf_scf_result {
@@ -142,6 +145,7 @@ typedef struct hf_scf_result {
     double jk_ms;                  /* time in J/K builds */
     double total_ms;               /* wall time of hf_scf */
     int fp32_builds;               /* mixed precision: J/K builds that used FP32 quartets */
+    int diag_steps;                /* density steps that used diagonalisation */
 } hf_scf_result;
 
 void hf_scf_defaults(hf_scf_params* p);
