@@ -51,6 +51,10 @@ int upload_pairs(hf_ctx* ctx, PairClassHost& pc) {
     HF_HIP_CHECK(ctx, hipMemcpy(pc.d_prim0.p, pc.prim0.data(), np * sizeof(int), hipMemcpyHostToDevice));
     HF_HIP_CHECK(ctx, hipMemcpy(pc.d_nprim.p, pc.nprim.data(), np * sizeof(int), hipMemcpyHostToDevice));
     HF_HIP_CHECK(ctx, hipMemcpy(pc.d_Q.p, pc.Q.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    if (!pc.group.empty()) {
+        HF_HIP_CHECK(ctx, pc.d_group.alloc(pc.group.size()));
+        HF_HIP_CHECK(ctx, hipMemcpy(pc.d_group.p, pc.group.data(), pc.group.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     return HF_OK;
 }
 
@@ -83,6 +87,21 @@ void add_stats(hf_ctx* ctx, int mode, const QuartetClassTasks& qc, const SinkArg
     ks.prim_quartets += qc.prim_quartets;
     ks.eris += (double)qc.ntasks * nout;
     ks.flops += (double)qc.prim_quartets * (pf + boys_flops) + (double)qc.ntasks * (ci.hrr_flops + sink_flops);
+}
+
+TaskList task_list(hf_ctx* ctx, const TaskSet& ts, const QuartetClassTasks& qc) {
+    TaskList tl{};
+    tl.bra = dev_view(ctx->pcls[qc.bra]);
+    tl.ket = dev_view(ctx->pcls[qc.ket]);
+    tl.prefix = ts.d_prefix.p;
+    tl.koff = ts.d_koff.p;
+    tl.kgroup = ctx->pcls[qc.ket].d_group.p;
+    tl.seg = ts.d_seg.p + qc.seg_off;
+    tl.nseg = qc.nseg;
+    tl.ntasks = qc.ntasks;
+    tl.nblocks = qc.nblocks;
+    tl.same_class = qc.bra == qc.ket;
+    return tl;
 }
 
 // Launch every quartet class of a task set in `mode`; with record, add to the per-class statistics.
@@ -128,12 +147,7 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
         const int si = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         load[si] += cost(qcp);
         hipStream_t ks = ctx->side[si];
-        TaskList tl;
-        tl.bra = dev_view(ctx->pcls[qc.bra]);
-        tl.ket = dev_view(ctx->pcls[qc.ket]);
-        tl.prefix = ts.d_prefix.p + qc.prefix_off;
-        tl.ntasks = qc.ntasks;
-        tl.same_class = qc.bra == qc.ket;
+        const TaskList tl = task_list(ctx, ts, qc);
         if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls], ks));
         hipError_t e = ci.fn(mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ks);
         if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));
@@ -149,12 +163,7 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
     for (const QuartetClassTasks* qcp : serial) {
         const QuartetClassTasks& qc = *qcp;
         const ClassInfo& ci = kClasses[qc.cls];
-        TaskList tl;
-        tl.bra = dev_view(ctx->pcls[qc.bra]);
-        tl.ket = dev_view(ctx->pcls[qc.ket]);
-        tl.prefix = ts.d_prefix.p + qc.prefix_off;
-        tl.ntasks = qc.ntasks;
-        tl.same_class = qc.bra == qc.ket;
+        const TaskList tl = task_list(ctx, ts, qc);
         if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls], st));
         hipError_t e = ci.fn(mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, st);
         if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));
@@ -252,9 +261,8 @@ int setup_molecule(hf_ctx* ctx) {
             const int pc_i = pair_class_index(ci.la, ci.lb);
             PairClassHost& pc = ctx->pcls[pc_i];
             if (pc.sa.empty()) continue;
-            TaskList tl;
+            TaskList tl{};
             tl.bra = tl.ket = dev_view(pc);
-            tl.prefix = nullptr;
             tl.ntasks = pc.npairs();
             tl.same_class = 1;
             SinkArgs sink{};
@@ -266,12 +274,15 @@ int setup_molecule(hf_ctx* ctx) {
             HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         }
     }
-    // sort each class by Q, descending
+    // sort each class into contraction groups (primitive-pair count, descending), each by Q, descending
     for (auto& pc : ctx->pcls) {
         const size_t np = pc.sa.size();
         std::vector<size_t> order(np);
         std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return pc.Q[x] > pc.Q[y]; });
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+            if (pc.nprim[x] != pc.nprim[y]) return pc.nprim[x] > pc.nprim[y];
+            return pc.Q[x] > pc.Q[y];
+        });
         PairClassHost s;
         for (size_t i : order) {
             s.sa.push_back(pc.sa[i]);
@@ -285,6 +296,10 @@ int setup_molecule(hf_ctx* ctx) {
         pc.prim0.swap(s.prim0);
         pc.nprim.swap(s.nprim);
         pc.Q.swap(s.Q);
+        pc.group.clear();
+        for (size_t i = 0; i < np; ++i)
+            if (i == 0 || pc.nprim[i] != pc.nprim[i - 1]) pc.group.push_back((int)i);
+        pc.group.push_back((int)np);
         int rc = upload_pairs(ctx, pc);
         if (rc) return rc;
     }
@@ -306,6 +321,8 @@ int ensure_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
 int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
     ts.classes.clear();
     std::vector<int64_t> prefix_all;
+    std::vector<int> koff_all;
+    std::vector<TaskSeg> seg_all;
     for (int c = 0; c < kNumClasses; ++c) {
         const ClassInfo& ci = kClasses[c];
         QuartetClassTasks qc;
@@ -315,34 +332,72 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
         const PairClassHost& B = ctx->pcls[qc.bra];
         const PairClassHost& K = ctx->pcls[qc.ket];
         const bool same = qc.bra == qc.ket;
-        std::vector<int64_t> knp(K.npairs() + 1, 0);     // prefix of ket primitive-pair counts
-        for (int q = 0; q < K.npairs(); ++q) knp[q + 1] = knp[q] + K.nprim[q];
-        qc.prefix_off = prefix_all.size();
-        int64_t acc = 0, pq = 0;
-        prefix_all.push_back(0);
-        for (int p = 0; p < B.npairs(); ++p) {
-            int64_t cnt = K.npairs();
-            if (thr > 0) {
-                if (B.Q[p] <= 0) cnt = 0;
-                else {
-                    const double need = thr / B.Q[p];
-                    // ket Q descending: count of q with Q_q >= need
-                    cnt = std::upper_bound(K.Q.begin(), K.Q.end(), need, [](double v, double x) { return v > x; }) -
-                          K.Q.begin();
+        qc.seg_off = seg_all.size();
+        const int nbg = B.group.empty() ? 0 : (int)B.group.size() - 1;
+        const int nkg = K.group.empty() ? 0 : (int)K.group.size() - 1;
+        for (int bg = 0; bg < nbg; ++bg) {
+            const int b0 = B.group[bg], b1 = B.group[bg + 1];
+            TaskSeg sg{};
+            sg.bra0 = b0;
+            sg.nbra = b1 - b0;
+            sg.nkg = nkg;
+            sg.prefix_off = (int64_t)prefix_all.size();
+            sg.koff_off = (int64_t)koff_all.size();
+            int64_t acc = 0;
+            prefix_all.push_back(0);
+            for (int p = b0; p < b1; ++p) {
+                int own = 0;
+                koff_all.push_back(0);
+                for (int kg = 0; kg < nkg; ++kg) {
+                    const int k0 = K.group[kg], k1 = K.group[kg + 1];
+                    int cnt = 0;
+                    if (!(same && kg > bg)) {       // one class: each unordered pair of pairs once (q <= p)
+                        cnt = k1 - k0;
+                        if (thr > 0) {
+                            if (B.Q[p] <= 0) cnt = 0;
+                            else {
+                                const double need = thr / B.Q[p];
+                                // Q descending inside the group: count of q with Q_q >= need
+                                cnt = (int)(std::upper_bound(K.Q.begin() + k0, K.Q.begin() + k1, need,
+                                                             [](double v, double x) { return v > x; }) -
+                                            (K.Q.begin() + k0));
+                            }
+                        }
+                        if (same && kg == bg) cnt = std::min(cnt, p - b0 + 1);
+                    }
+                    own += cnt;
+                    koff_all.push_back(own);
+                    qc.prim_quartets += (int64_t)cnt * B.nprim[p] * K.nprim[k0];
                 }
+                acc += own;
+                prefix_all.push_back(acc);
             }
-            if (same) cnt = std::min<int64_t>(cnt, p + 1);
-            acc += cnt;
-            pq += knp[cnt] * B.nprim[p];
-            prefix_all.push_back(acc);
+            if (acc == 0) {
+                prefix_all.resize(sg.prefix_off);
+                koff_all.resize(sg.koff_off);
+                continue;
+            }
+            sg.ntasks = acc;
+            sg.task0 = qc.ntasks;
+            sg.block0 = qc.nblocks;
+            qc.ntasks += acc;
+            qc.nblocks += (acc + 255) / 256;
+            seg_all.push_back(sg);
         }
-        qc.ntasks = acc;
-        qc.prim_quartets = pq;
+        qc.nseg = (int)(seg_all.size() - qc.seg_off);
         ts.classes.push_back(qc);
     }
-    HF_HIP_CHECK(ctx, ts.d_prefix.alloc(prefix_all.size()));
-    HF_HIP_CHECK(ctx, hipMemcpy(ts.d_prefix.p, prefix_all.data(), prefix_all.size() * sizeof(int64_t),
-                                hipMemcpyHostToDevice));
+    HF_HIP_CHECK(ctx, ts.d_prefix.alloc(std::max<size_t>(prefix_all.size(), 1)));
+    HF_HIP_CHECK(ctx, ts.d_koff.alloc(std::max<size_t>(koff_all.size(), 1)));
+    HF_HIP_CHECK(ctx, ts.d_seg.alloc(std::max<size_t>(seg_all.size(), 1)));
+    if (!prefix_all.empty())
+        HF_HIP_CHECK(ctx, hipMemcpy(ts.d_prefix.p, prefix_all.data(), prefix_all.size() * sizeof(int64_t),
+                                    hipMemcpyHostToDevice));
+    if (!koff_all.empty())
+        HF_HIP_CHECK(ctx, hipMemcpy(ts.d_koff.p, koff_all.data(), koff_all.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!seg_all.empty())
+        HF_HIP_CHECK(ctx, hipMemcpy(ts.d_seg.p, seg_all.data(), seg_all.size() * sizeof(TaskSeg),
+                                    hipMemcpyHostToDevice));
     ts.threshold = thr;
     return HF_OK;
 }

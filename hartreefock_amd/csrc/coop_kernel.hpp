@@ -110,13 +110,12 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
     if (MODE == MODE_DIAG) {
         p = q = (int)t;
     } else {
-        int lo = 0, hi = tl.bra.npairs;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (tl.prefix[mid] <= t) lo = mid; else hi = mid;
-        }
-        p = lo;
-        q = (int)(t - tl.prefix[lo]);
+        const TaskSeg sg = tl.seg[segment_of(tl.seg, tl.nseg, t, false)];
+        const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
+        const int pl = bra_of_task(pre, t - sg.task0, 0, sg.nbra);
+        p = sg.bra0 + pl;
+        q = ket_of_rank(tl.koff + sg.koff_off + (int64_t)pl * (sg.nkg + 1), tl.kgroup, sg.nkg,
+                        (int)(t - sg.task0 - pre[pl]));
     }
     const int A = tl.bra.sa[p], B = tl.bra.sb[p];
     const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];

@@ -289,15 +289,6 @@ __device__ __forceinline__ void sink_jk_rows(const double* __restrict__ v, doubl
     }
 }
 
-// Bra pair owning task t: the largest p in [lo, hi) with prefix[p] <= t (prefix[hi] > t).
-__device__ __forceinline__ int bra_of_task(const int64_t* __restrict__ prefix, int64_t t, int lo, int hi) {
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= t) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 template <class C>
 __device__ __forceinline__ void sink_packed(const double* __restrict__ v, int A, int B, int Cs, int Ds, int a0, int b0,
                                             int c0, int d0, bool same_pair, const SinkArgs& s) {
@@ -419,10 +410,12 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
     const size_t nn = (size_t)n * n;
     const int rows_per_slot = sink.ndens * (NA + NB);
     const int slot_len = rows_per_slot * n;
-    const int64_t tb = (int64_t)blockIdx.x * blockDim.x;
-    const int64_t tlast = (tb + blockDim.x < tl.ntasks ? tb + blockDim.x : tl.ntasks) - 1;
-    const int pb = bra_of_task(tl.prefix, tb, 0, tl.bra.npairs);
-    const int pe = bra_of_task(tl.prefix, tlast, pb, tl.bra.npairs);
+    const TaskSeg sg = tl.seg[segment_of(tl.seg, tl.nseg, blockIdx.x, true)];
+    const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
+    const int64_t tb = ((int64_t)blockIdx.x - sg.block0) * blockDim.x;     // segment-local tasks
+    const int64_t tlast = (tb + blockDim.x < sg.ntasks ? tb + blockDim.x : sg.ntasks) - 1;
+    const int pb = sg.bra0 + bra_of_task(pre, tb, 0, sg.nbra);
+    const int pe = sg.bra0 + bra_of_task(pre, tlast, pb - sg.bra0, sg.nbra);
     const int nslot = pe - pb + 1 < sink.lds_slots ? pe - pb + 1 : sink.lds_slots;
     for (int i = threadIdx.x; i < nslot * slot_len; i += blockDim.x) rows[i] = 0.0;
     __syncthreads();
@@ -434,8 +427,10 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
 #pragma unroll
     for (int i = 0; i < NAB; ++i) jab[i] = 0.0;
     if (active) {
-        const int p = bra_of_task(tl.prefix, t, pb, pe + 1);
-        const int q = (int)(t - tl.prefix[p]);
+        const int pl = bra_of_task(pre, t, pb - sg.bra0, pe - sg.bra0 + 1);
+        const int p = sg.bra0 + pl;
+        const int q = ket_of_rank(tl.koff + sg.koff_off + (int64_t)pl * (sg.nkg + 1), tl.kgroup, sg.nkg,
+                                  (int)(t - pre[pl]));
         const int A = tl.bra.sa[p], B = tl.bra.sb[p];
         const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
         a0 = sh.bf[A];
@@ -500,10 +495,20 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
     if constexpr (MODE == MODE_JK) {
         eri_jk_block<C>(tl, sh, pp, boys, sink);
     } else {
-        const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (t >= tl.ntasks) return;
-        const int p = MODE == MODE_DIAG ? (int)t : bra_of_task(tl.prefix, t, 0, tl.bra.npairs);
-        const int q = MODE == MODE_DIAG ? (int)t : (int)(t - tl.prefix[p]);
+        int p, q;
+        if (MODE == MODE_DIAG) {
+            const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            if (t >= tl.ntasks) return;
+            p = q = (int)t;
+        } else {
+            const TaskSeg sg = tl.seg[segment_of(tl.seg, tl.nseg, blockIdx.x, true)];
+            const int64_t t = ((int64_t)blockIdx.x - sg.block0) * blockDim.x + threadIdx.x;
+            if (t >= sg.ntasks) return;
+            const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
+            const int pl = bra_of_task(pre, t, 0, sg.nbra);
+            p = sg.bra0 + pl;
+            q = ket_of_rank(tl.koff + sg.koff_off + (int64_t)pl * (sg.nkg + 1), tl.kgroup, sg.nkg, (int)(t - pre[pl]));
+        }
         const int A = tl.bra.sa[p], B = tl.bra.sb[p];
         const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
         const int a0 = sh.bf[A], b0 = sh.bf[B], c0 = sh.bf[Cs], d0 = sh.bf[Ds];
@@ -529,7 +534,7 @@ template <int LA, int LB, int LC, int LD, int EXACT, int MODE>
 inline hipError_t launch_eri(const TaskList& tl, const ShellsDev& sh, const PrimPair* pp, const double* boys,
                              const SinkArgs& sink, hipStream_t st) {
     if (tl.ntasks <= 0) return hipSuccess;
-    const int64_t blocks = (tl.ntasks + 255) / 256;
+    const int64_t blocks = MODE == MODE_DIAG ? (tl.ntasks + 255) / 256 : tl.nblocks;
     SinkArgs s = sink;
     size_t lds = 0;
     if (MODE == MODE_JK) {

@@ -42,16 +42,64 @@ struct ShellsDev {
 
 enum EriMode { MODE_JK = 0, MODE_PACKED = 1, MODE_DIAG = 2 };
 
-// The quartet tasks of one launch: bra pair p (in class `bra`) x ket pair q
-// (in class `ket`).  Tasks are enumerated by a prefix array over bra pairs:
-// bra p owns tasks [prefix[p], prefix[p+1]) and its ket pairs are q = 0..count-1
-// of the (descending-Q) ket list.  In MODE_DIAG task t is the quartet (t|t).
+// Within a pair class, pairs are grouped by contraction length (number of primitive pairs,
+// descending) and sorted by Schwarz bound (descending) inside each group.  A segment is one bra
+// group of a quartet class, so every bra in it has the same primitive-pair count.  Its tasks are
+// enumerated by a prefix array over the bra group: bra p = bra0 + i owns tasks
+// [prefix[i], prefix[i+1]), and those run over the ket groups in order, taking in each ket
+// group g the screened head (Q descending) of length koff[i][g+1] - koff[i][g].  Consecutive
+// tasks of one bra therefore share the bra and, except at a group change, the ket contraction
+// length: wavefronts see little divergence in the primitive loops and the bra-side J/K
+// contributions of a block stay on one or two bra pairs.
+struct TaskSeg {
+    int bra0, nbra, nkg, pad;
+    int64_t prefix_off;      // first of the segment's nbra + 1 prefix entries
+    int64_t koff_off;        // first of the segment's nbra * (nkg + 1) ket-group offsets
+    int64_t ntasks;
+    int64_t task0;           // first task of the segment within the launch
+    int64_t block0;          // first 256-task block (segments padded to whole blocks)
+};
+
+// The quartet tasks of one launch: the segments of one quartet class.  In MODE_DIAG task t is
+// the quartet (t|t) of the bra class and there are no segments.
 struct TaskList {
     PairClassDev bra, ket;
-    const int64_t* prefix;   // nbra + 1 entries (unused in MODE_DIAG)
-    int64_t ntasks;
+    const int64_t* prefix;   // prefix arrays of every segment
+    const int* koff;         // per-bra ket-group offsets of every segment
+    const int* kgroup;       // first pair of each ket group (ket class), nkg + 1 entries
+    const TaskSeg* seg;
+    int nseg;
+    int64_t ntasks;          // sum over segments (MODE_DIAG: bra pairs)
+    int64_t nblocks;         // sum over segments of whole 256-task blocks
     int same_class;          // bra and ket are one class: apply the p==q degeneracy
 };
+
+// Segment containing launch unit x: the largest s with key(s) <= x, key = block0 (by_block) or task0.
+__device__ __forceinline__ int segment_of(const TaskSeg* __restrict__ seg, int nseg, int64_t x, bool by_block) {
+    int lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((by_block ? seg[mid].block0 : seg[mid].task0) <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Ket pair for rank r (0 <= r < koff[nkg]) in the ket list of one bra, whose group offsets are ko.
+__device__ __forceinline__ int ket_of_rank(const int* __restrict__ ko, const int* __restrict__ kgroup, int nkg, int r) {
+    int g = 0;
+    while (g + 1 < nkg && ko[g + 1] <= r) ++g;
+    return kgroup[g] + (r - ko[g]);
+}
+
+// Bra pair (index within the segment's bra group) owning task t: the largest i in [lo, hi) with
+// prefix[i] <= t (prefix[hi] > t).
+__device__ __forceinline__ int bra_of_task(const int64_t* __restrict__ prefix, int64_t t, int lo, int hi) {
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 
 struct SinkArgs {
     int n;                   // basis functions

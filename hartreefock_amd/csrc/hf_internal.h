@@ -83,20 +83,25 @@ struct QuartetClassTasks {
     int cls = -1;             // index into kClasses
     int bra = -1, ket = -1;   // pair-class indices
     int64_t ntasks = 0;
+    int64_t nblocks = 0;      // whole 256-task blocks over the segments
     int64_t prim_quartets = 0;
-    size_t prefix_off = 0;    // offset of this class's prefix array in the shared device buffer
+    size_t seg_off = 0;       // first segment in the shared device segment buffer
+    int nseg = 0;
 };
 
 struct TaskSet {
     double threshold = -1;
     std::vector<QuartetClassTasks> classes;
     DevBuf<int64_t> d_prefix;
+    DevBuf<int> d_koff;
+    DevBuf<hfgpu::TaskSeg> d_seg;
 };
 
 struct PairClassHost {
     std::vector<int> sa, sb, prim0, nprim;
     std::vector<double> Q;
-    DevBuf<int> d_sa, d_sb, d_prim0, d_nprim;
+    std::vector<int> group;   // contraction groups: pairs [group[g], group[g+1]) share nprim
+    DevBuf<int> d_sa, d_sb, d_prim0, d_nprim, d_group;
     DevBuf<double> d_Q;
     int npairs() const { return (int)sa.size(); }
 };

@@ -1,7 +1,8 @@
 // One-electron matrices S, T, V on the device (the reference builds them on the host in
 // HartreeFock/QuantumMatrix.cpp:34-155 from Obara-Saika recursions).  McMurchie-Davidson
-// Hermite expansion, one thread per shell pair (a >= b), every primitive pair and every
-// nucleus inside the thread — deterministic (no atomics); O(N^2) work.
+// Hermite expansion, one thread per (shell pair a >= b, nucleus C): every primitive pair inside
+// the thread; the nucleus-0 thread also writes S and T; V contributions are added with FP64
+// atomics into a zeroed V.  O(N^2 * atoms) work spread over that many threads.
 //   S_ab = E^x_0 E^y_0 E^z_0 (pi/p)^{3/2}
 //   T    = sum over directions of  b(2j+1) S_ij - 2b^2 S_i,j+2 - j(j-1)/2 S_i,j-2  (times the other two S)
 //   V_ab = -sum_C Z_C 2pi/p sum_tuv E^x_t E^y_u E^z_v R_tuv(p, P - C)
@@ -91,10 +92,16 @@ struct OneEArgs {
     double *S, *T, *V;
 };
 
+__global__ void zero_kernel(double* __restrict__ x, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = 0.0;
+}
+
 __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t npairs = (int64_t)a.nshell * (a.nshell + 1) / 2;
+    const int c = blockIdx.y;           // nucleus
     if (t >= npairs) return;
+    const bool first = c == 0;
     // t -> (sa >= sb)
     int sa = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
     while ((int64_t)sa * (sa + 1) / 2 > t) --sa;
@@ -113,26 +120,28 @@ __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a) {
             const double ea = a.alpha[a.sprim0[sa] + ia], eb = a.alpha[a.sprim0[sb] + ib], p = ea + eb;
             const double w = a.coef[a.sprim0[sa] + ia] * a.coef[a.sprim0[sb] + ib];
             double E[3][kI][kJ][kT];
-            for (int d = 0; d < 3; ++d) hermite_e(ea, eb, AB[d], La, Lb + 2, E[d]);
+            for (int d = 0; d < 3; ++d) hermite_e(ea, eb, AB[d], La, first ? Lb + 2 : Lb, E[d]);
             const double sq = sqrt(M_PI / p);
             const double P[3] = {(ea * A[0] + eb * B[0]) / p, (ea * A[1] + eb * B[1]) / p, (ea * A[2] + eb * B[2]) / p};
-            for (int i = 0; i < na; ++i)
-                for (int j = 0; j < nb; ++j) {
-                    int li[3], lj[3];
-                    comp_of(La, i, li[0], li[1], li[2]);
-                    comp_of(Lb, j, lj[0], lj[1], lj[2]);
-                    double s1[3], t1[3];
-                    for (int d = 0; d < 3; ++d) {
-                        const double s0 = E[d][li[d]][lj[d]][0] * sq;
-                        const double sp2 = E[d][li[d]][lj[d] + 2][0] * sq;
-                        const double sm2 = lj[d] >= 2 ? E[d][li[d]][lj[d] - 2][0] * sq : 0.0;
-                        s1[d] = s0;
-                        t1[d] = eb * (2 * lj[d] + 1) * s0 - 2 * eb * eb * sp2 - 0.5 * lj[d] * (lj[d] - 1) * sm2;
+            if (first) {
+                for (int i = 0; i < na; ++i)
+                    for (int j = 0; j < nb; ++j) {
+                        int li[3], lj[3];
+                        comp_of(La, i, li[0], li[1], li[2]);
+                        comp_of(Lb, j, lj[0], lj[1], lj[2]);
+                        double s1[3], t1[3];
+                        for (int d = 0; d < 3; ++d) {
+                            const double s0 = E[d][li[d]][lj[d]][0] * sq;
+                            const double sp2 = E[d][li[d]][lj[d] + 2][0] * sq;
+                            const double sm2 = lj[d] >= 2 ? E[d][li[d]][lj[d] - 2][0] * sq : 0.0;
+                            s1[d] = s0;
+                            t1[d] = eb * (2 * lj[d] + 1) * s0 - 2 * eb * eb * sp2 - 0.5 * lj[d] * (lj[d] - 1) * sm2;
+                        }
+                        s[i * nb + j] += w * s1[0] * s1[1] * s1[2];
+                        k[i * nb + j] += w * (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]);
                     }
-                    s[i * nb + j] += w * s1[0] * s1[1] * s1[2];
-                    k[i * nb + j] += w * (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]);
-                }
-            for (int c = 0; c < a.natoms; ++c) {
+            }
+            {
                 const double PC[3] = {P[0] - a.axyz[3 * c], P[1] - a.axyz[3 * c + 1], P[2] - a.axyz[3 * c + 2]};
                 double F[kL];
                 boys_rt(p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), L, a.boys, F);
@@ -184,10 +193,13 @@ __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a) {
             comp_of(Lb, j, lj[0], lj[1], lj[2]);
             const double f = 1.0 / sqrt(dfact(2 * li[0] - 1) * dfact(2 * li[1] - 1) * dfact(2 * li[2] - 1) *
                                         dfact(2 * lj[0] - 1) * dfact(2 * lj[1] - 1) * dfact(2 * lj[2] - 1));
-            const size_t r = (size_t)(a0 + i), c = (size_t)(b0 + j);
-            a.S[r * n + c] = a.S[c * n + r] = f * s[i * nb + j];
-            a.T[r * n + c] = a.T[c * n + r] = f * k[i * nb + j];
-            a.V[r * n + c] = a.V[c * n + r] = f * v[i * nb + j];
+            const size_t r = (size_t)(a0 + i), cc = (size_t)(b0 + j);
+            if (first) {
+                a.S[r * n + cc] = a.S[cc * n + r] = f * s[i * nb + j];
+                a.T[r * n + cc] = a.T[cc * n + r] = f * k[i * nb + j];
+            }
+            unsafeAtomicAdd(&a.V[r * n + cc], f * v[i * nb + j]);
+            if (sa != sb) unsafeAtomicAdd(&a.V[cc * n + r], f * v[i * nb + j]);
         }
 }
 
@@ -242,7 +254,11 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
     a.T = dT;
     a.V = dV;
     const int64_t npairs = (int64_t)ns * (ns + 1) / 2;
-    hipLaunchKernelGGL(one_electron_kernel, dim3((unsigned)((npairs + 127) / 128)), dim3(128), 0, st, a);
+    const size_t nn = (size_t)mol.nbf * mol.nbf;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, st, dV, nn);
+    if (a.natoms > 0)
+        hipLaunchKernelGGL(one_electron_kernel, dim3((unsigned)((npairs + 127) / 128), (unsigned)a.natoms), dim3(128), 0,
+                           st, a);
     HF_HIP_CHECK(ctx, hipGetLastError());
     return HF_OK;
 }
