@@ -44,13 +44,15 @@ class ScfParams(ctypes.Structure):
                 ("normal_iter_after_diis", ctypes.c_int), ("add_asymmetry", ctypes.c_int),
                 ("asymmetry", ctypes.c_double), ("energy_tolerance", ctypes.c_double),
                 ("density_tolerance", ctypes.c_double), ("diis_tolerance", ctypes.c_double),
-                ("terminate", ctypes.POINTER(ctypes.c_int)), ("density_method", ctypes.c_int)]
+                ("terminate", ctypes.POINTER(ctypes.c_int)), ("density_method", ctypes.c_int),
+                ("incremental_fock", ctypes.c_int), ("full_rebuild_every", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
     _fields_ = [("energy", ctypes.c_double), ("homo", ctypes.c_double), ("s2", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("jk_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double), ("fp32_builds", ctypes.c_int), ("diag_steps", ctypes.c_int)]
+                ("total_ms", ctypes.c_double), ("fp32_builds", ctypes.c_int), ("diag_steps", ctypes.c_int),
+                ("full_builds", ctypes.c_int), ("incremental_builds", ctypes.c_int), ("quartets", ctypes.c_double)]
 
 
 class KernelStat(ctypes.Structure):

@@ -17,13 +17,16 @@ __global__ void fill_kernel(double* __restrict__ p, size_t n, double v) {
 }
 
 // out = in + in^T for `count` n x n matrices
-__global__ void symmetrize_kernel(const double* __restrict__ in, double* __restrict__ out, int n, int count) {
+// out = in + in^T (per matrix), or out += in + in^T
+__global__ void symmetrize_kernel(const double* __restrict__ in, double* __restrict__ out, int n, int count,
+                                  int accumulate) {
     const size_t nn = (size_t)n * n;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn * count;
          idx += (size_t)gridDim.x * blockDim.x) {
         const size_t m = idx / nn, r = idx % nn;
         const size_t i = r / n, j = r % n;
-        out[idx] = in[m * nn + i * n + j] + in[m * nn + j * n + i];
+        const double v = in[m * nn + i * n + j] + in[m * nn + j * n + i];
+        out[idx] = accumulate ? out[idx] + v : v;
     }
 }
 
@@ -304,7 +307,7 @@ int setup_molecule(hf_ctx* ctx) {
         if (rc) return rc;
     }
     ctx->tasks_all.threshold = -1;
-    ctx->tasks_screened.threshold = -1;
+    ctx->tasks_by_level.clear();
     ctx->d_oe_L.release();
     ctx->d_oe_prim.release();
     ctx->have_mol = true;
@@ -314,6 +317,15 @@ int setup_molecule(hf_ctx* ctx) {
 int ensure_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
     if (ts.threshold == thr && !ts.classes.empty()) return HF_OK;
     return hf_build_tasks(ctx, thr, ts);
+}
+
+// Screened task set for threshold thr, rounded down to the grid 10^(k/4) (never screens more than
+// asked); built once per level and molecule.
+int screened_tasks(hf_ctx* ctx, double thr, TaskSet*& out) {
+    const int level = (int)std::floor(4.0 * std::log10(thr) + 1e-9);
+    TaskSet& ts = ctx->tasks_by_level[level];
+    out = &ts;
+    return ensure_tasks(ctx, std::pow(10.0, level / 4.0), ts);
 }
 
 }  // namespace
@@ -402,12 +414,14 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
     return HF_OK;
 }
 
-int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st) {
+int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st, double thr,
+                      bool accumulate) {
     if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
     if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
     const int n = ctx->mol.nbf;
     const size_t nn = (size_t)n * n;
-    int rc = ensure_tasks(ctx, ctx->opt.schwarz_threshold, ctx->tasks_screened);
+    TaskSet* ts = nullptr;
+    int rc = screened_tasks(ctx, thr > 0 ? thr : ctx->opt.schwarz_threshold, ts);
     if (rc) return rc;
     HF_HIP_CHECK(ctx, ctx->d_Jp.alloc(nn));
     HF_HIP_CHECK(ctx, ctx->d_Kp.alloc(2 * nn));
@@ -419,10 +433,12 @@ int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, do
     sink.D = d_D;
     sink.J = ctx->d_Jp.p;
     sink.K = ctx->d_Kp.p;
-    rc = run_classes(ctx, MODE_JK, ctx->tasks_screened, sink, st, true);
+    rc = run_classes(ctx, MODE_JK, *ts, sink, st, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(nn)), dim3(256), 0, st, ctx->d_Jp.p, d_J, n, 1);
-    hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(ndens * nn)), dim3(256), 0, st, ctx->d_Kp.p, d_K, n, ndens);
+    hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(nn)), dim3(256), 0, st, ctx->d_Jp.p, d_J, n, 1,
+                       accumulate ? 1 : 0);
+    hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(ndens * nn)), dim3(256), 0, st, ctx->d_Kp.p, d_K, n, ndens,
+                       accumulate ? 1 : 0);
     HF_HIP_CHECK(ctx, hipGetLastError());
     return HF_OK;
 }
@@ -551,7 +567,7 @@ int hf_eri_packed(hf_ctx* ctx, double* out, uint64_t len) {
 int hf_build_jk_device(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, void* stream) {
     if (!ctx) return HF_E_ARG;
     if (!d_D || !d_J || !d_K) return ctx->fail(HF_E_ARG, "null device buffer");
-    return hf_jk_device_impl(ctx, ndens, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream);
+    return hf_jk_device_impl(ctx, ndens, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream, 0.0, false);
 }
 
 int hf_build_jk(hf_ctx* ctx, int ndens, const double* D, double* J, double* K) {
@@ -563,7 +579,7 @@ int hf_build_jk(hf_ctx* ctx, int ndens, const double* D, double* J, double* K) {
     HF_HIP_CHECK(ctx, ctx->d_J.alloc(nn));
     HF_HIP_CHECK(ctx, ctx->d_K.alloc(2 * nn));
     HF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_D.p, D, ndens * nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    int rc = hf_jk_device_impl(ctx, ndens, ctx->d_D.p, ctx->d_J.p, ctx->d_K.p, ctx->stream);
+    int rc = hf_jk_device_impl(ctx, ndens, ctx->d_D.p, ctx->d_J.p, ctx->d_K.p, ctx->stream, 0.0, false);
     if (rc) return rc;
     HF_HIP_CHECK(ctx, hipMemcpyAsync(J, ctx->d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HF_HIP_CHECK(ctx, hipMemcpyAsync(K, ctx->d_K.p, ndens * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

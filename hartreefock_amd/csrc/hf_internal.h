@@ -125,7 +125,8 @@ struct hf_ctx {
     DevBuf<hfgpu::PrimPair> d_pp;
     DevBuf<double> d_boys;
     std::vector<PairClassHost> pcls;
-    TaskSet tasks_screened, tasks_all;
+    TaskSet tasks_all;                     // unscreened (packed ERIs)
+    std::map<int, TaskSet> tasks_by_level; // screened, by threshold level k (10^(k/4))
     // J/K scratch
     DevBuf<double> d_Jp, d_Kp, d_D, d_J, d_K;
     // one-electron kernel inputs: per-shell L/bf/prim0/nprim, and alpha|coef|Z|atom xyz
@@ -147,7 +148,10 @@ struct hf_ctx {
 };
 
 // Implemented in context.hip
-int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st);
+// J/K of the densities d_D (ndens x n x n) on stream st, screened at Schwarz product thr (0: the
+// context's threshold); accumulate adds to d_J/d_K instead of overwriting them.
+int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st, double thr,
+                      bool accumulate);
 int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts);
 // Implemented in one_electron.hip: S, T, V (n x n, device) on stream st.
 int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st);

@@ -138,6 +138,25 @@ __global__ void __launch_bounds__(1024) dots_kernel(DotJobs jobs, size_t nn, dou
     }
 }
 
+// out = D - Dp; *mx = max |D - Dp|
+__global__ void __launch_bounds__(1024) delta_kernel(const double* __restrict__ D, const double* __restrict__ Dp,
+                                                     double* __restrict__ out, size_t m, double* __restrict__ mx) {
+    __shared__ double part[1024];
+    double v = 0;
+    for (size_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const double d = D[i] - Dp[i];
+        out[i] = d;
+        v = fmax(v, fabs(d));
+    }
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *mx = part[0];
+}
+
 int grid_for(size_t n) {
     size_t b = (n + 255) / 256;
     return (int)std::min<size_t>(std::max<size_t>(b, 1), 4096);
@@ -316,7 +335,7 @@ struct Diis {
 struct ScfState {
     int n = 0;
     size_t nn = 0;
-    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, C, eps, T1, T2, T3, err, work, red, sp2;
+    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, Dprev, dD, C, eps, T1, T2, T3, err, work, red, sp2;
     DevBuf<rocblas_int> info;
 };
 
@@ -449,6 +468,8 @@ void hf_scf_defaults(hf_scf_params* p) {
     p->add_asymmetry = 1;
     p->asymmetry = 0.1;
     p->density_method = 0;
+    p->incremental_fock = 0;
+    p->full_rebuild_every = 8;
 }
 
 int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out) {
@@ -476,6 +497,8 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     HF_HIP_CHECK(ctx, st.K.alloc(2 * nn));
     HF_HIP_CHECK(ctx, st.D.alloc(2 * nn));
     HF_HIP_CHECK(ctx, st.Dn.alloc(2 * nn));
+    HF_HIP_CHECK(ctx, st.Dprev.alloc(2 * nn));
+    HF_HIP_CHECK(ctx, st.dD.alloc(2 * nn));
     HF_HIP_CHECK(ctx, st.C.alloc(2 * nn));
     HF_HIP_CHECK(ctx, st.eps.alloc(2 * n));
     HF_HIP_CHECK(ctx, st.T1.alloc(nn));
@@ -529,6 +552,17 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     double energy = 0, homo = 0, lastErr = 0, jk_ms = 0;
     int diag_steps = 0;
     bool last_diag[2] = {false, false};
+    // incremental Fock build state: J, K hold the J/K of Dprev
+    int full_builds = 0, incr_builds = 0, since_full = 0;
+    bool force_full = false, last_incremental = false;
+    double quartets = 0, last_dmax = 0, last_quartets = 0;
+    const bool incremental = p->incremental_fock != 0;
+    const int rebuild_every = p->full_rebuild_every > 0 ? p->full_rebuild_every : 8;
+    auto quartet_count = [&]() {
+        double q = 0;
+        for (const auto& ks : ctx->stats) q += (double)ks.quartets;
+        return q;
+    };
     bool useDIIS = p->use_diis != 0;
     std::vector<double> eps_h(2 * (size_t)n);
 
@@ -540,8 +574,34 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 HF_HIP_CHECK(ctx, hipMemcpyAsync(F[k], st.F0.p + k * nn, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
         } else {
             const auto t0 = std::chrono::steady_clock::now();
-            if (int rc = hf_jk_device_impl(ctx, nsp, st.D.p, st.J.p, st.K.p, s)) return rc;
+            const double q0 = quartet_count();
+            const bool full = !incremental || force_full || full_builds == 0 || since_full >= rebuild_every;
+            if (full) {
+                if (int rc = hf_jk_device_impl(ctx, nsp, st.D.p, st.J.p, st.K.p, s, 0.0, false)) return rc;
+                ++full_builds;
+                since_full = 0;
+                force_full = false;
+            } else {
+                // J/K are linear in D: add the J/K of D - Dprev, screened at thr / max|D - Dprev|
+                hipLaunchKernelGGL(delta_kernel, dim3(1), dim3(1024), 0, s, st.D.p, st.Dprev.p, st.dD.p, nsp * nn,
+                                   st.red.p + 15);
+                double dmax = 0;
+                HF_HIP_CHECK(ctx, hipMemcpyAsync(&dmax, st.red.p + 15, sizeof(double), hipMemcpyDeviceToHost, s));
+                HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+                last_dmax = dmax;
+                if (dmax > 0) {
+                    const double thr = ctx->opt.schwarz_threshold / std::min(1.0, dmax);
+                    if (int rc = hf_jk_device_impl(ctx, nsp, st.dD.p, st.J.p, st.K.p, s, thr, true)) return rc;
+                }
+                ++incr_builds;
+                ++since_full;
+            }
+            last_incremental = !full;
+            if (incremental)
+                HF_HIP_CHECK(ctx, hipMemcpyAsync(st.Dprev.p, st.D.p, nsp * nn * sizeof(double), hipMemcpyDeviceToDevice, s));
             HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+            last_quartets = quartet_count() - q0;
+            quartets += last_quartets;
             jk_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             for (int k = 0; k < nsp; ++k)
                 hipLaunchKernelGGL(fock_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.h.p, st.J.p, st.K.p + k * nn,
@@ -628,8 +688,9 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     double cur = 0, prev = std::numeric_limits<double>::infinity(), rms = 0;
     auto trace_line = [&](int it, double rmsv) {
         if (trace)
-            std::fprintf(stderr, "hf_scf %s iter %4d  E % .12f  dE % .3e  |dD| %.3e  diis %.3e\n", uhf ? "UHF" : "RHF",
-                         it, energy, energy - prev, rmsv, lastErr);
+            std::fprintf(stderr, "hf_scf %s iter %4d  E % .12f  dE % .3e  |dD| %.3e  diis %.3e  jk %s max|dD| %.2e quartets %.3e\n",
+                         uhf ? "UHF" : "RHF", it, energy, energy - prev, rmsv, lastErr,
+                         last_incremental ? "incr" : "full", last_dmax, last_quartets);
     };
     int iter = 0;
     bool converged = false, cancelled = false;
@@ -639,8 +700,11 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         cur = energy;
         trace_line(iter, rms);
         if (std::fabs(prev - cur) <= (useDIIS ? etolD : etolN) && rms < dtol && lastErr < itol) {
-            converged = true;
-            break;
+            if (!last_incremental) {
+                converged = true;
+                break;
+            }
+            force_full = true;     // confirm on a step with a full J/K build
         }
         if (stop()) { cancelled = true; break; }
         prev = cur;
@@ -661,7 +725,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             if (int rc = step(iter, rms)) return rc;
             cur = energy;
             trace_line(iter, rms);
-            if (std::fabs(prev - cur) <= etolN && rms < dtol) { converged = true; break; }
+            if (std::fabs(prev - cur) <= etolN && rms < dtol) {
+                if (!last_incremental) { converged = true; break; }
+                force_full = true;
+            }
             if (stop()) { cancelled = true; break; }
             prev = cur;
         }
@@ -686,6 +753,9 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     r->converged = converged ? 1 : 0;
     r->jk_ms = jk_ms;
     r->diag_steps = diag_steps;
+    r->full_builds = full_builds;
+    r->incremental_builds = incr_builds;
+    r->quartets = quartets;
     if (uhf) {
         // <S^2> = ((Na-Nb)/2)^2 + (Na+Nb)/2 - sum_ij ((C_a,occ)^T S C_b,occ)_ij^2
         if (occ[0] > 0 && occ[1] > 0) {

@@ -134,6 +134,10 @@ typedef struct hf_scf_params {
     int density_method;            /* 0: spectral-projection purification (SP2), falling back to
                                       diagonalisation for a step it cannot resolve; 1: diagonalise
                                       every step (the reference's route).  Same fixed point. */
+    int incremental_fock;          /* 1: between full builds, J/K of the density change only, screened
+                                      at threshold / max|dD|; convergence is only accepted on a step
+                                      with a full build.  0 (default): full J/K every step. */
+    int full_rebuild_every;        /* incremental steps between full builds (default 8) */
 } hf_scf_params;
 
 typedef struct hf_scf_result {
@@ -146,6 +150,9 @@ typedef struct hf_scf_result {
     double total_ms;               /* wall time of hf_scf */
     int fp32_builds;               /* mixed precision: J/K builds that used FP32 quartets */
     int diag_steps;                /* density steps that used diagonalisation */
+    int full_builds;               /* J/K builds of the full density */
+    int incremental_builds;        /* J/K builds of a density change */
+    double quartets;               /* shell quartets evaluated over all J/K builds */
 } hf_scf_result;
 
 void hf_scf_defaults(hf_scf_params* p);
