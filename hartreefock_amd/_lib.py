@@ -45,7 +45,8 @@ class ScfParams(ctypes.Structure):
                 ("asymmetry", ctypes.c_double), ("energy_tolerance", ctypes.c_double),
                 ("density_tolerance", ctypes.c_double), ("diis_tolerance", ctypes.c_double),
                 ("terminate", ctypes.POINTER(ctypes.c_int)), ("density_method", ctypes.c_int),
-                ("incremental_fock", ctypes.c_int), ("full_rebuild_every", ctypes.c_int)]
+                ("incremental_fock", ctypes.c_int), ("full_rebuild_every", ctypes.c_int),
+                ("final_fp64_iterations", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -58,11 +59,12 @@ class ScfResult(ctypes.Structure):
 class KernelStat(ctypes.Structure):
     _fields_ = [("la", ctypes.c_int), ("lb", ctypes.c_int), ("lc", ctypes.c_int), ("ld", ctypes.c_int),
                 ("launches", ctypes.c_int64), ("quartets", ctypes.c_int64), ("prim_quartets", ctypes.c_int64),
-                ("eris", ctypes.c_double), ("flops", ctypes.c_double), ("ms", ctypes.c_double)]
+                ("eris", ctypes.c_double), ("flops", ctypes.c_double), ("ms", ctypes.c_double),
+                ("fp32", ctypes.c_int)]
 
     def name(self):
         s = "spdfg"
-        return "(%s%s|%s%s)" % (s[self.la], s[self.lb], s[self.lc], s[self.ld])
+        return "(%s%s|%s%s)%s" % (s[self.la], s[self.lb], s[self.lc], s[self.ld], "f32" if self.fp32 else "")
 
 
 def library_path():
@@ -226,8 +228,8 @@ class Engine:
         return (r, D) if want_density else r
 
     def kernel_stats(self):
-        arr = (KernelStat * 64)()
-        k = lib().hf_kernel_stats(self.h, arr, 64)
+        arr = (KernelStat * 128)()
+        k = lib().hf_kernel_stats(self.h, arr, 128)
         return [arr[i] for i in range(k)]
 
     def reset_kernel_stats(self):

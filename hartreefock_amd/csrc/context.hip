@@ -1,6 +1,7 @@
 // libhfgpu context: molecule upload, shell-pair data, Schwarz screening, task lists,
 // and the ERI / J-K entry points of the C ABI (include/hfgpu.h).
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -72,6 +73,9 @@ PairClassDev dev_view(const PairClassHost& pc) {
     return d;
 }
 
+// Statistics slot of a class launch: FP64 launches use [0, kNumClasses), FP32 ones the next kNumClasses.
+int stat_slot(int mode, int cls) { return mode == MODE_JK32 ? kNumClasses + cls : cls; }
+
 void add_stats(hf_ctx* ctx, int mode, const QuartetClassTasks& qc, const SinkArgs& sink) {
     const ClassInfo& ci = kClasses[qc.cls];
     const int pf = ctx->opt.exact_transfer && ci.exact_differs ? ci.prim_flops_exact : ci.prim_flops_ref;
@@ -83,8 +87,9 @@ void add_stats(hf_ctx* ctx, int mode, const QuartetClassTasks& qc, const SinkArg
     const int nc = (ci.lc + 1) * (ci.lc + 2) / 2, nd = (ci.ld + 1) * (ci.ld + 2) / 2;
     const double nout = (double)na * nb * nc * nd;
     double sink_flops = 0;
-    if (mode == MODE_JK) sink_flops = nout * 2.0 * (2 + 4 * sink.ndens);   // 2 J + 4 K contractions per density
-    KernelStat& ks = ctx->stats[qc.cls];
+    if (mode == MODE_JK || mode == MODE_JK32)
+        sink_flops = nout * 2.0 * (2 + 4 * sink.ndens);   // 2 J + 4 K contractions per density
+    KernelStat& ks = ctx->stats[stat_slot(mode, qc.cls)];
     ks.launches += 1;
     ks.quartets += qc.ntasks;
     ks.prim_quartets += qc.prim_quartets;
@@ -98,6 +103,7 @@ TaskList task_list(hf_ctx* ctx, const TaskSet& ts, const QuartetClassTasks& qc) 
     tl.ket = dev_view(ctx->pcls[qc.ket]);
     tl.prefix = ts.d_prefix.p;
     tl.koff = ts.d_koff.p;
+    tl.kbeg = ts.d_kbeg.p;
     tl.kgroup = ctx->pcls[qc.ket].d_group.p;
     tl.seg = ts.d_seg.p + qc.seg_off;
     tl.nseg = qc.nseg;
@@ -107,17 +113,23 @@ TaskList task_list(hf_ctx* ctx, const TaskSet& ts, const QuartetClassTasks& qc) 
     return tl;
 }
 
-// Launch every quartet class of a task set in `mode`; with record, add to the per-class statistics.
-int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, hipStream_t st, bool record) {
+// Launch every quartet class of the given (mode, task set) jobs on the context's side streams
+// between one fork from and one join into st; with record, add to the per-class statistics.
+int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& jobs, const SinkArgs& sink,
+                hipStream_t st, bool record) {
     ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
-    if (ctx->stats.size() != (size_t)kNumClasses) {
-        ctx->stats.assign(kNumClasses, KernelStat{});
-        for (int c = 0; c < kNumClasses; ++c) ctx->stats[c].cls = c;
+    const int nslot = 2 * kNumClasses;
+    if (ctx->stats.size() != (size_t)nslot) {
+        ctx->stats.assign(nslot, KernelStat{});
+        for (int c = 0; c < nslot; ++c) {
+            ctx->stats[c].cls = c % kNumClasses;
+            ctx->stats[c].fp32 = c >= kNumClasses;
+        }
     }
     const bool prof = ctx->opt.profile != 0 && record;
-    if (prof && ctx->ev.size() < 2 * (size_t)kNumClasses) {
+    if (prof && ctx->ev.size() < 2 * (size_t)nslot) {
         for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
-        ctx->ev.assign(2 * kNumClasses, nullptr);
+        ctx->ev.assign(2 * nslot, nullptr);
         for (auto& e : ctx->ev) HF_HIP_CHECK(ctx, hipEventCreate(&e));
     }
     // fork: every side stream waits for the work already queued on st
@@ -130,49 +142,45 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
     }
     HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[0], st));
     for (hipStream_t s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamWaitEvent(s2, ctx->side_ev[0], 0));
-    // largest estimated work first, each to the least-loaded side stream
-    // Every class runs concurrently: the high-L classes use the one-wavefront-per-quartet LDS
-    // kernel and no class needs more than a few hundred bytes of scratch per lane.
-    std::vector<const QuartetClassTasks*> order, serial;
-    for (const QuartetClassTasks& qc : ts.classes)
-        if (qc.ntasks > 0) order.push_back(&qc);
-    auto cost = [](const QuartetClassTasks* q) {
-        const ClassInfo& ci = kClasses[q->cls];
-        return (double)q->prim_quartets * (ci.prim_flops_ref + 60) + (double)q->ntasks * ci.hrr_flops;
+    // Every class launch runs concurrently with the others (the high-L classes use the
+    // one-wavefront-per-quartet LDS kernel, so no launch needs much scratch): largest estimated
+    // work first, each to the least-loaded side stream.
+    struct Launch {
+        int mode;
+        const TaskSet* ts;
+        const QuartetClassTasks* qc;
+        double cost;
     };
-    std::stable_sort(order.begin(), order.end(),
-                     [&](const QuartetClassTasks* a, const QuartetClassTasks* b) { return cost(a) > cost(b); });
+    std::vector<Launch> order;
+    for (const auto& job : jobs)
+        for (const QuartetClassTasks& qc : job.second->classes) {
+            if (qc.ntasks <= 0) continue;
+            const ClassInfo& ci = kClasses[qc.cls];
+            const double cost = (double)qc.prim_quartets * (ci.prim_flops_ref + 60) + (double)qc.ntasks * ci.hrr_flops;
+            order.push_back({job.first, job.second, &qc, cost});
+        }
+    std::stable_sort(order.begin(), order.end(), [](const Launch& a, const Launch& b) { return a.cost > b.cost; });
     std::vector<double> load(kSide, 0.0);
     std::vector<int> launched;
-    for (const QuartetClassTasks* qcp : order) {
-        const QuartetClassTasks& qc = *qcp;
+    for (const Launch& L : order) {
+        const QuartetClassTasks& qc = *L.qc;
         const ClassInfo& ci = kClasses[qc.cls];
         const int si = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[si] += cost(qcp);
+        load[si] += L.cost;
         hipStream_t ks = ctx->side[si];
-        const TaskList tl = task_list(ctx, ts, qc);
-        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls], ks));
-        hipError_t e = ci.fn(mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ks);
+        const TaskList tl = task_list(ctx, *L.ts, qc);
+        const int slot = stat_slot(L.mode, qc.cls);
+        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * slot], ks));
+        hipError_t e = ci.fn(L.mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ks);
         if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));
-        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls + 1], ks));
-        launched.push_back(qc.cls);
-        if (record) add_stats(ctx, mode, qc, sink);
+        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * slot + 1], ks));
+        launched.push_back(slot);
+        if (record) add_stats(ctx, L.mode, qc, sink);
     }
     // join: st continues after every side stream's work
     for (int k = 0; k < kSide; ++k) {
         HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[k + 1], ctx->side[k]));
         HF_HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->side_ev[k + 1], 0));
-    }
-    for (const QuartetClassTasks* qcp : serial) {
-        const QuartetClassTasks& qc = *qcp;
-        const ClassInfo& ci = kClasses[qc.cls];
-        const TaskList tl = task_list(ctx, ts, qc);
-        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls], st));
-        hipError_t e = ci.fn(mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, st);
-        if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));
-        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * qc.cls + 1], st));
-        launched.push_back(qc.cls);
-        if (record) add_stats(ctx, mode, qc, sink);
     }
     if (prof) {
         HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
@@ -183,6 +191,10 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
         }
     }
     return HF_OK;
+}
+
+int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, hipStream_t st, bool record) {
+    return run_classes(ctx, {{mode, &ts}}, sink, st, record);
 }
 
 // Shell pairs, primitive pairs, Schwarz bounds, sorted pair lists.
@@ -205,6 +217,9 @@ int setup_molecule(hf_ctx* ctx) {
         hfhost::boys_table(tab);
         HF_HIP_CHECK(ctx, ctx->d_boys.alloc(tab.size()));
         HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_boys.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<float> tab32(tab.begin(), tab.end());
+        HF_HIP_CHECK(ctx, ctx->d_boys32.alloc(tab32.size()));
+        HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_boys32.p, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     // pairs (La >= Lb; for equal L, first shell index >= second)
     std::vector<PrimPair> pp;
@@ -308,33 +323,64 @@ int setup_molecule(hf_ctx* ctx) {
     }
     ctx->tasks_all.threshold = -1;
     ctx->tasks_by_level.clear();
+    ctx->tasks_mixed_hi.clear();
+    ctx->tasks_mixed_lo.clear();
     ctx->d_oe_L.release();
     ctx->d_oe_prim.release();
     ctx->have_mol = true;
     return HF_OK;
 }
 
-int ensure_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
-    if (ts.threshold == thr && !ts.classes.empty()) return HF_OK;
-    return hf_build_tasks(ctx, thr, ts);
+int ensure_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper = 0.0) {
+    if (ts.threshold == thr && ts.upper == upper && !ts.classes.empty()) return HF_OK;
+    return hf_build_tasks(ctx, thr, ts, upper);
 }
 
-// Screened task set for threshold thr, rounded down to the grid 10^(k/4) (never screens more than
-// asked); built once per level and molecule.
+// Threshold grid 10^(k/4): level of thr rounded down (never screens more than asked); thresholds
+// <= 0 (no screening) map to kNoScreen.
+constexpr int kNoScreen = -100000;
+int threshold_level(double thr) {
+    if (!(thr > 0)) return kNoScreen;
+    return std::max(kNoScreen + 1, (int)std::floor(4.0 * std::log10(thr) + 1e-9));
+}
+double level_threshold(int level) { return level == kNoScreen ? 0.0 : std::pow(10.0, level / 4.0); }
+
+// Screened task set for threshold thr, built once per level and molecule.
 int screened_tasks(hf_ctx* ctx, double thr, TaskSet*& out) {
-    const int level = (int)std::floor(4.0 * std::log10(thr) + 1e-9);
+    const int level = threshold_level(thr);
     TaskSet& ts = ctx->tasks_by_level[level];
     out = &ts;
-    return ensure_tasks(ctx, std::pow(10.0, level / 4.0), ts);
+    return ensure_tasks(ctx, level_threshold(level), ts);
+}
+
+// Mixed precision: the tasks at threshold thr split at the FP32 bound t32 into the FP64 part
+// (Q_p Q_q >= t32) and the FP32 band (thr <= Q_p Q_q < t32).
+int mixed_tasks(hf_ctx* ctx, double thr, double t32, TaskSet*& hi, TaskSet*& lo) {
+    const int level = threshold_level(thr);
+    const double t = level_threshold(level);
+    const int64_t key = (int64_t)level * 1000000 + threshold_level(t32);
+    hi = &ctx->tasks_mixed_hi[key];
+    lo = &ctx->tasks_mixed_lo[key];
+    if (!(t32 > t)) {                     // nothing below the FP32 bound: all FP64
+        if (int rc = ensure_tasks(ctx, t, *hi)) return rc;
+        return ensure_tasks(ctx, std::numeric_limits<double>::max(), *lo);     // empty
+    }
+    if (int rc = ensure_tasks(ctx, t32, *hi)) return rc;
+    return ensure_tasks(ctx, t, *lo, t32);
 }
 
 }  // namespace
 
-int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
+int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper) {
     ts.classes.clear();
     std::vector<int64_t> prefix_all;
-    std::vector<int> koff_all;
+    std::vector<int> koff_all, kbeg_all;
     std::vector<TaskSeg> seg_all;
+    // count of q in [k0, k1) (Q descending) with Q_q >= x
+    auto head = [](const PairClassHost& K, int k0, int k1, double x) {
+        return (int)(std::upper_bound(K.Q.begin() + k0, K.Q.begin() + k1, x, [](double v, double y) { return v > y; }) -
+                     (K.Q.begin() + k0));
+    };
     for (int c = 0; c < kNumClasses; ++c) {
         const ClassInfo& ci = kClasses[c];
         QuartetClassTasks qc;
@@ -355,6 +401,7 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
             sg.nkg = nkg;
             sg.prefix_off = (int64_t)prefix_all.size();
             sg.koff_off = (int64_t)koff_all.size();
+            sg.kbeg_off = (int64_t)kbeg_all.size();
             int64_t acc = 0;
             prefix_all.push_back(0);
             for (int p = b0; p < b1; ++p) {
@@ -362,23 +409,21 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
                 koff_all.push_back(0);
                 for (int kg = 0; kg < nkg; ++kg) {
                     const int k0 = K.group[kg], k1 = K.group[kg + 1];
-                    int cnt = 0;
+                    int beg = 0, end = 0;
                     if (!(same && kg > bg)) {       // one class: each unordered pair of pairs once (q <= p)
-                        cnt = k1 - k0;
-                        if (thr > 0) {
-                            if (B.Q[p] <= 0) cnt = 0;
-                            else {
-                                const double need = thr / B.Q[p];
-                                // Q descending inside the group: count of q with Q_q >= need
-                                cnt = (int)(std::upper_bound(K.Q.begin() + k0, K.Q.begin() + k1, need,
-                                                             [](double v, double x) { return v > x; }) -
-                                            (K.Q.begin() + k0));
-                            }
+                        end = k1 - k0;
+                        if (thr > 0) end = B.Q[p] > 0 ? head(K, k0, k1, thr / B.Q[p]) : 0;
+                        if (upper > 0) beg = B.Q[p] > 0 ? head(K, k0, k1, upper / B.Q[p]) : 0;
+                        if (same && kg == bg) {
+                            end = std::min(end, p - b0 + 1);
+                            beg = std::min(beg, end);
                         }
-                        if (same && kg == bg) cnt = std::min(cnt, p - b0 + 1);
+                        if (beg > end) beg = end;
                     }
+                    const int cnt = end - beg;
                     own += cnt;
                     koff_all.push_back(own);
+                    kbeg_all.push_back(beg);
                     qc.prim_quartets += (int64_t)cnt * B.nprim[p] * K.nprim[k0];
                 }
                 acc += own;
@@ -387,6 +432,7 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
             if (acc == 0) {
                 prefix_all.resize(sg.prefix_off);
                 koff_all.resize(sg.koff_off);
+                kbeg_all.resize(sg.kbeg_off);
                 continue;
             }
             sg.ntasks = acc;
@@ -401,28 +447,41 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts) {
     }
     HF_HIP_CHECK(ctx, ts.d_prefix.alloc(std::max<size_t>(prefix_all.size(), 1)));
     HF_HIP_CHECK(ctx, ts.d_koff.alloc(std::max<size_t>(koff_all.size(), 1)));
+    HF_HIP_CHECK(ctx, ts.d_kbeg.alloc(std::max<size_t>(kbeg_all.size(), 1)));
     HF_HIP_CHECK(ctx, ts.d_seg.alloc(std::max<size_t>(seg_all.size(), 1)));
     if (!prefix_all.empty())
         HF_HIP_CHECK(ctx, hipMemcpy(ts.d_prefix.p, prefix_all.data(), prefix_all.size() * sizeof(int64_t),
                                     hipMemcpyHostToDevice));
     if (!koff_all.empty())
         HF_HIP_CHECK(ctx, hipMemcpy(ts.d_koff.p, koff_all.data(), koff_all.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!kbeg_all.empty())
+        HF_HIP_CHECK(ctx, hipMemcpy(ts.d_kbeg.p, kbeg_all.data(), kbeg_all.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!seg_all.empty())
         HF_HIP_CHECK(ctx, hipMemcpy(ts.d_seg.p, seg_all.data(), seg_all.size() * sizeof(TaskSeg),
                                     hipMemcpyHostToDevice));
     ts.threshold = thr;
+    ts.upper = upper;
     return HF_OK;
 }
 
 int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st, double thr,
-                      bool accumulate) {
+                      bool accumulate, bool fp32) {
     if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
     if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
     const int n = ctx->mol.nbf;
     const size_t nn = (size_t)n * n;
-    TaskSet* ts = nullptr;
-    int rc = screened_tasks(ctx, thr > 0 ? thr : ctx->opt.schwarz_threshold, ts);
-    if (rc) return rc;
+    if (thr <= 0) thr = ctx->opt.schwarz_threshold;
+    std::vector<std::pair<int, const TaskSet*>> jobs;
+    if (fp32) {
+        TaskSet *hi = nullptr, *lo = nullptr;
+        if (int rc = mixed_tasks(ctx, thr, ctx->opt.fp32_threshold, hi, lo)) return rc;
+        jobs.push_back({MODE_JK, hi});
+        jobs.push_back({MODE_JK32, lo});
+    } else {
+        TaskSet* ts = nullptr;
+        if (int rc = screened_tasks(ctx, thr, ts)) return rc;
+        jobs.push_back({MODE_JK, ts});
+    }
     HF_HIP_CHECK(ctx, ctx->d_Jp.alloc(nn));
     HF_HIP_CHECK(ctx, ctx->d_Kp.alloc(2 * nn));
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(nn)), dim3(256), 0, st, ctx->d_Jp.p, nn, 0.0);
@@ -433,8 +492,8 @@ int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, do
     sink.D = d_D;
     sink.J = ctx->d_Jp.p;
     sink.K = ctx->d_Kp.p;
-    rc = run_classes(ctx, MODE_JK, *ts, sink, st, true);
-    if (rc) return rc;
+    sink.boys32 = ctx->d_boys32.p;
+    if (int rc = run_classes(ctx, jobs, sink, st, true)) return rc;
     hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(nn)), dim3(256), 0, st, ctx->d_Jp.p, d_J, n, 1,
                        accumulate ? 1 : 0);
     hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(ndens * nn)), dim3(256), 0, st, ctx->d_Kp.p, d_K, n, ndens,
@@ -567,7 +626,8 @@ int hf_eri_packed(hf_ctx* ctx, double* out, uint64_t len) {
 int hf_build_jk_device(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, void* stream) {
     if (!ctx) return HF_E_ARG;
     if (!d_D || !d_J || !d_K) return ctx->fail(HF_E_ARG, "null device buffer");
-    return hf_jk_device_impl(ctx, ndens, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream, 0.0, false);
+    return hf_jk_device_impl(ctx, ndens, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream, 0.0, false,
+                             ctx->opt.precision == HF_PREC_MIXED);
 }
 
 int hf_build_jk(hf_ctx* ctx, int ndens, const double* D, double* J, double* K) {
@@ -579,7 +639,8 @@ int hf_build_jk(hf_ctx* ctx, int ndens, const double* D, double* J, double* K) {
     HF_HIP_CHECK(ctx, ctx->d_J.alloc(nn));
     HF_HIP_CHECK(ctx, ctx->d_K.alloc(2 * nn));
     HF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_D.p, D, ndens * nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    int rc = hf_jk_device_impl(ctx, ndens, ctx->d_D.p, ctx->d_J.p, ctx->d_K.p, ctx->stream, 0.0, false);
+    int rc = hf_jk_device_impl(ctx, ndens, ctx->d_D.p, ctx->d_J.p, ctx->d_K.p, ctx->stream, 0.0, false,
+                               ctx->opt.precision == HF_PREC_MIXED);
     if (rc) return rc;
     HF_HIP_CHECK(ctx, hipMemcpyAsync(J, ctx->d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HF_HIP_CHECK(ctx, hipMemcpyAsync(K, ctx->d_K.p, ndens * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -619,6 +680,7 @@ int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max) {
         out[k].eris = s.eris;
         out[k].flops = s.flops;
         out[k].ms = s.ms;
+        out[k].fp32 = s.fp32;
         ++k;
     }
     return k;
@@ -627,9 +689,10 @@ int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max) {
 int hf_kernel_stats_reset(hf_ctx* ctx) {
     if (!ctx) return HF_E_ARG;
     for (KernelStat& s : ctx->stats) {
-        const int c = s.cls;
+        const int c = s.cls, f = s.fp32;
         s = KernelStat{};
         s.cls = c;
+        s.fp32 = f;
     }
     return HF_OK;
 }

@@ -114,8 +114,7 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
         const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
         const int pl = bra_of_task(pre, t - sg.task0, 0, sg.nbra);
         p = sg.bra0 + pl;
-        q = ket_of_rank(tl.koff + sg.koff_off + (int64_t)pl * (sg.nkg + 1), tl.kgroup, sg.nkg,
-                        (int)(t - sg.task0 - pre[pl]));
+        q = ket_of_task(tl, sg, pre, pl, t - sg.task0);
     }
     const int A = tl.bra.sa[p], B = tl.bra.sb[p];
     const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];

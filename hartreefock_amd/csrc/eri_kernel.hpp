@@ -53,12 +53,72 @@ __host__ __device__ __forceinline__ void boys_scaled(double T, double pref, cons
     for (int m = 0; m <= L; ++m) F[m] *= pref;
 }
 
+// 1/sqrt(x) for normal positive x: the hardware estimate refined by two Newton steps
+// r <- r (3 - x r^2) / 2 (each step squares the relative error), on the device; exact on the host.
+__host__ __device__ __forceinline__ double rsqrt_f64(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * (1.5 - 0.5 * x * r * r);
+    r = r * (1.5 - 0.5 * x * r * r);
+    return r;
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
+// FP32 evaluation of F_m(T) * pref (quartets of the mixed-precision tier): a 6-term Taylor
+// expansion of F_L from an FP32 copy of the table (truncation < 3e-9 relative for |x| <= 0.05),
+// FP32 exp and downward recursion below BOYS_TMAX; above it the asymptotic form
+// F_0 = sqrt(pi/T)/2 and the upward recursion F_{m+1} = (2m+1) F_m / (2T) in FP64, rounded.  The
+// e^{-T} term of that recursion is below 1.6e-15 of (2m+1) F_m for T >= 36 and is left out.
+template <int L>
+__host__ __device__ __forceinline__ void boys_scaled_f32(float T, float pref, const float* __restrict__ tab, float* F) {
+    if (T < (float)BOYS_TMAX) {
+        const int k = (int)rintf(T * (float)BOYS_INV_DT);
+        const float x = (float)k * (float)BOYS_DT - T;
+        const float* row = tab + k * BOYS_NCOL + L;
+        float f = row[5];
+        f = row[4] + x * f * (1.0f / 5.0f);
+        f = row[3] + x * f * (1.0f / 4.0f);
+        f = row[2] + x * f * (1.0f / 3.0f);
+        f = row[1] + x * f * (1.0f / 2.0f);
+        f = row[0] + x * f;
+        F[L] = f;
+        if (L > 0) {
+            const float e = expf(-T);
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = (2.0f * T * F[m + 1] + e) * (1.0f / (2 * m + 1));
+        }
+#pragma unroll
+        for (int m = 0; m <= L; ++m) F[m] *= pref;
+    } else {
+        const double Td = T, r = rsqrt_f64(Td), inv2T = 0.5 * r * r;
+        double Fd = kSqrtPiHalf * r;
+        F[0] = (float)(Fd * pref);
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            Fd = (2 * m + 1) * Fd * inv2T;
+            F[m + 1] = (float)(Fd * pref);
+        }
+    }
+}
+
+// Reciprocal: exact division in FP64; the hardware reciprocal (1 ulp) in FP32 on the device.
+__host__ __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__host__ __device__ __forceinline__ float recip(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 __device__ __forceinline__ uint64_t two_index(uint64_t i, uint64_t j) {
     return i < j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j;
 }
 
-template <class C>
-__device__ __forceinline__ void sink_jk(const double* __restrict__ v, int a0, int b0, int c0, int d0, double w,
+template <class C, typename V>
+__device__ __forceinline__ void sink_jk(const V* __restrict__ v, int a0, int b0, int c0, int d0, double w,
                                         const SinkArgs& s) {
     constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
     const int n = s.n;
@@ -177,8 +237,8 @@ __device__ __forceinline__ void sink_jk(const double* __restrict__ v, int a0, in
 // exchange blocks K'_ac, K'_ad, K'_bc, K'_bd all land in those rows.  The ket-side Coulomb block
 // J'_cd goes to global memory; the bra-side Coulomb block J'_ab is returned in jab for a
 // reduction across the wavefront.
-template <class C>
-__device__ __forceinline__ void sink_jk_rows(const double* __restrict__ v, double* __restrict__ rows, int a0, int b0,
+template <class C, typename V>
+__device__ __forceinline__ void sink_jk_rows(const V* __restrict__ v, double* __restrict__ rows, int a0, int b0,
                                              int c0, int d0, double w, const SinkArgs& s, double* __restrict__ jab) {
     constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
     const int n = s.n;
@@ -317,38 +377,43 @@ __device__ __forceinline__ void sink_packed(const double* __restrict__ v, int A,
 // One contracted shell quartet: primitive loops (Boys, VRR, transfer, contraction) then HRR and the
 // component normalisation.  A..D are the four centres; out receives NA*NB*NC*ND values, index
 // ((ia*NB + ib)*NC + ic)*ND + id.
-template <class C>
+template <class C, typename R = double>
 __host__ __device__ __forceinline__ void eval_quartet(const double* __restrict__ A, const double* __restrict__ B,
                                                      const double* __restrict__ Cc, const double* __restrict__ Dd,
                                                      const PrimPair* __restrict__ bra, int npb,
                                                      const PrimPair* __restrict__ ket, int npk,
-                                                     const double* __restrict__ boys, double* __restrict__ out) {
-    double acc[C::NACC];
+                                                     const R* __restrict__ boys, R* __restrict__ out) {
+    R acc[C::NACC];
 #pragma unroll
-    for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
+    for (int i = 0; i < C::NACC; ++i) acc[i] = R(0);
     for (int i = 0; i < npb; ++i) {
         const PrimPair P = bra[i];
-        const double PAx = P.Px - A[0], PAy = P.Py - A[1], PAz = P.Pz - A[2];
-        const double inv2z = 0.5 * P.inv_zeta;
+        const R PAx = R(P.Px - A[0]), PAy = R(P.Py - A[1]), PAz = R(P.Pz - A[2]);
+        const R zeta = R(P.zeta), inv2z = R(0.5 * P.inv_zeta), KP = R(P.Kz);
         for (int j = 0; j < npk; ++j) {
             const PrimPair Q = ket[j];
-            const double inv_s = 1.0 / (P.zeta + Q.zeta);
-            const double rho = P.zeta * Q.zeta * inv_s;
-            const double PQx = P.Px - Q.Px, PQy = P.Py - Q.Py, PQz = P.Pz - Q.Pz;
-            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = kTwoPi25 * P.Kz * Q.Kz * sqrt(inv_s);
-            double F[C::L + 1];
-            boys_scaled<C::L>(T, pref, boys, F);
-            const double rz = Q.zeta * inv_s;                       // rho / zeta
-            const double zoe = P.zeta * Q.inv_zeta;                 // zeta / eta
-            const double inv2e = 0.5 * Q.inv_zeta;
-            const double dlx = zoe * PAx + (Q.Px - Cc[0]);          // delta = (zeta/eta) PA + QC
-            const double dly = zoe * PAy + (Q.Py - Cc[1]);
-            const double dlz = zoe * PAz + (Q.Pz - Cc[2]);
+            const R eta = R(Q.zeta), inv_eta = R(Q.inv_zeta);
+            const R inv_s = recip(zeta + eta);
+            const R rho = zeta * eta * inv_s;
+            const R PQx = R(P.Px - Q.Px), PQy = R(P.Py - Q.Py), PQz = R(P.Pz - Q.Pz);
+            const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            const R pref = R(kTwoPi25) * KP * R(Q.Kz) * sqrt(inv_s);
+            R F[C::L + 1];
+            if constexpr (sizeof(R) == sizeof(double))
+                boys_scaled<C::L>(T, pref, boys, F);
+            else
+                boys_scaled_f32<C::L>(T, pref, boys, F);
+            const R rz = eta * inv_s;                       // rho / zeta
+            const R zoe = zeta * inv_eta;                   // zeta / eta
+            const R inv2e = R(0.5) * inv_eta;
+            const R dlx = zoe * PAx + R(Q.Px - Cc[0]);      // delta = (zeta/eta) PA + QC
+            const R dly = zoe * PAy + R(Q.Py - Cc[1]);
+            const R dlz = zoe * PAz + R(Q.Pz - Cc[2]);
             C::prim(F, PAx, PAy, PAz, -rz * PQx, -rz * PQy, -rz * PQz, inv2z, rz, dlx, dly, dlz, zoe, inv2e, acc);
         }
     }
-    C::hrr(acc, A[0] - B[0], A[1] - B[1], A[2] - B[2], Cc[0] - Dd[0], Cc[1] - Dd[1], Cc[2] - Dd[2], out);
+    C::hrr(acc, R(A[0] - B[0]), R(A[1] - B[1]), R(A[2] - B[2]), R(Cc[0] - Dd[0]), R(Cc[1] - Dd[1]), R(Cc[2] - Dd[2]),
+           out);
 }
 
 // A class whose reference convention truncates the transfer step (C::TRUNCATED) has values that
@@ -381,17 +446,18 @@ __host__ __device__ __forceinline__ void eval_quartet_reference_order(
                 }
 }
 
-template <class C>
+template <class C, typename R = double>
 __device__ __forceinline__ void quartet_values(const TaskList& tl, const ShellsDev& sh, const PrimPair* __restrict__ pp,
-                                               const double* __restrict__ boys, int p, int q, int A, int B, int Cs,
-                                               int Ds, int a0, int b0, int c0, int d0, double* out) {
+                                               const R* __restrict__ boys, int p, int q, int A, int B, int Cs,
+                                               int Ds, int a0, int b0, int c0, int d0, R* out) {
     if constexpr (C::TRUNCATED) {
+        static_assert(sizeof(R) == sizeof(double), "the reference-convention classes are FP64 only");
         eval_quartet_reference_order<C>(A, Cs, sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds,
                                         pp + tl.bra.prim0[p], tl.bra.nprim[p], pp + tl.ket.prim0[q],
                                         tl.ket.nprim[q], a0, b0, c0, d0, boys, out);
     } else {
-        eval_quartet<C>(sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds, pp + tl.bra.prim0[p],
-                        tl.bra.nprim[p], pp + tl.ket.prim0[q], tl.ket.nprim[q], boys, out);
+        eval_quartet<C, R>(sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds, pp + tl.bra.prim0[p],
+                           tl.bra.nprim[p], pp + tl.ket.prim0[q], tl.ket.nprim[q], boys, out);
     }
 }
 
@@ -401,9 +467,9 @@ __device__ __forceinline__ void quartet_values(const TaskList& tl, const ShellsD
 // the block adds the slots to K' with column-coalesced global atomics (one per touched element
 // instead of one per quartet and component).  J'_ab is summed across the wavefront when the
 // wavefront shares one slot.  Quartets of further bra pairs use the plain global-atomic sink.
-template <class C>
+template <class C, typename R>
 __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev& sh, const PrimPair* __restrict__ pp,
-                                             const double* __restrict__ boys, const SinkArgs& sink) {
+                                             const R* __restrict__ boys, const SinkArgs& sink) {
     extern __shared__ double rows[];
     constexpr int NA = C::NA, NB = C::NB, NAB = C::NA * C::NB;
     const int n = sink.n;
@@ -429,24 +495,23 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
     if (active) {
         const int pl = bra_of_task(pre, t, pb - sg.bra0, pe - sg.bra0 + 1);
         const int p = sg.bra0 + pl;
-        const int q = ket_of_rank(tl.koff + sg.koff_off + (int64_t)pl * (sg.nkg + 1), tl.kgroup, sg.nkg,
-                                  (int)(t - pre[pl]));
+        const int q = ket_of_task(tl, sg, pre, pl, t);
         const int A = tl.bra.sa[p], B = tl.bra.sb[p];
         const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
         a0 = sh.bf[A];
         b0 = sh.bf[B];
         const int c0 = sh.bf[Cs], d0 = sh.bf[Ds];
-        double out[C::NOUT];
-        quartet_values<C>(tl, sh, pp, boys, p, q, A, B, Cs, Ds, a0, b0, c0, d0, out);
+        R out[C::NOUT];
+        quartet_values<C, R>(tl, sh, pp, boys, p, q, A, B, Cs, Ds, a0, b0, c0, d0, out);
         double w = 1.0;
         if (A == B) w *= 0.5;
         if (Cs == Ds) w *= 0.5;
         if (tl.same_class && p == q) w *= 0.5;
         if (p - pb < nslot) {
             key = p - pb;
-            sink_jk_rows<C>(out, rows + (size_t)key * slot_len, a0, b0, c0, d0, w, sink, jab);
+            sink_jk_rows<C, R>(out, rows + (size_t)key * slot_len, a0, b0, c0, d0, w, sink, jab);
         } else {
-            sink_jk<C>(out, a0, b0, c0, d0, w, sink);
+            sink_jk<C, R>(out, a0, b0, c0, d0, w, sink);
         }
     }
     // J'_ab: lanes without a slot hold zeros; lane 0 is active whenever any lane is
@@ -493,7 +558,9 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
                                                         const double* __restrict__ boys, SinkArgs sink) {
     using C = EriClass<LA, LB, LC, LD, EXACT>;
     if constexpr (MODE == MODE_JK) {
-        eri_jk_block<C>(tl, sh, pp, boys, sink);
+        eri_jk_block<C, double>(tl, sh, pp, boys, sink);
+    } else if constexpr (MODE == MODE_JK32) {
+        eri_jk_block<C, float>(tl, sh, pp, sink.boys32, sink);
     } else {
         int p, q;
         if (MODE == MODE_DIAG) {
@@ -507,7 +574,7 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
             const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
             const int pl = bra_of_task(pre, t, 0, sg.nbra);
             p = sg.bra0 + pl;
-            q = ket_of_rank(tl.koff + sg.koff_off + (int64_t)pl * (sg.nkg + 1), tl.kgroup, sg.nkg, (int)(t - pre[pl]));
+            q = ket_of_task(tl, sg, pre, pl, t);
         }
         const int A = tl.bra.sa[p], B = tl.bra.sb[p];
         const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
@@ -537,7 +604,7 @@ inline hipError_t launch_eri(const TaskList& tl, const ShellsDev& sh, const Prim
     const int64_t blocks = MODE == MODE_DIAG ? (tl.ntasks + 255) / 256 : tl.nblocks;
     SinkArgs s = sink;
     size_t lds = 0;
-    if (MODE == MODE_JK) {
+    if (MODE == MODE_JK || MODE == MODE_JK32) {
         using C = EriClass<LA, LB, LC, LD, 0>;
         // up to 4 bra pairs per block, within 40 KB of LDS (a block of 256 tasks rarely spans more)
         const size_t slot_bytes = (size_t)sink.ndens * (C::NA + C::NB) * sink.n * sizeof(double);

@@ -141,7 +141,7 @@ class ClassGen:
                 e2 = add(e1, i, -1)
                 c = vrr(e2, m)
                 d = vrr(e2, m + 1)
-                expr += " + %d.0 * inv2z * (%s - rz * %s)" % (e1[i], c, d)
+                expr += " + %s(%s - rz * %s)" % (ifac(e1[i], "inv2z"), c, d)
                 fl[0] += 5
             name = "v%s_%d" % (cname(e), m)
             lines.append("        const double %s = %s;" % (name, expr))
@@ -170,11 +170,11 @@ class ClassGen:
             if e[i] > 0:
                 t = etr(add(e, i, -1), f1)
                 if t is not None:
-                    terms.append("+ %d.0 * inv2e * %s" % (e[i], t))
+                    terms.append("+ %s%s" % (ifac(e[i], "inv2e"), t))
             if f1[i] > 0:
                 t = etr(e, add(f1, i, -1))
                 if t is not None:
-                    terms.append("+ %d.0 * inv2e * %s" % (f1[i], t))
+                    terms.append("+ %s%s" % (ifac(f1[i], "inv2e"), t))
             if not terms:
                 etr_done[key] = None
                 return None
@@ -256,7 +256,7 @@ class ClassGen:
                         if s == 1.0:
                             lines.append("        out[%d] = %s;" % (idx, v))
                         else:
-                            lines.append("        out[%d] = %.17g * %s;" % (idx, s, v))
+                            lines.append("        out[%d] = R(%.17g) * %s;" % (idx, s, v))
                             fl += 1
         self.flops_hrr = fl
         return lines
@@ -342,7 +342,8 @@ def main():
                 cg.truncated = differs and not ex
                 write_if_changed(os.path.join(outdir, "coop_%s_%d.inc" % (tag, ex)), cg.emit("%s_%d" % (tag, ex)))
                 coop_flops[(la, lb, lc, ld, ex)] = (cg.flops_prim, cg.flops_hrr)
-        for mode in ("MODE_JK", "MODE_PACKED") + (("MODE_DIAG",) if diag else ()):
+        modes = ("MODE_JK", "MODE_PACKED") + (("MODE_DIAG",) if diag else ()) + (() if coop else ("MODE_JK32",))
+        for mode in modes:
             for ex in ((0, 1) if differs else (0,)):
                 fn = "run_%s_%s_%d" % (tag, mode.split("_")[1].lower(), ex)
                 variants.append((mode, ex, fn))
@@ -363,6 +364,8 @@ def main():
             body.append("hipError_t %s%s;" % (fn, sig))
         body.append("hipError_t launch_class_%s(int mode, int exact, const TaskList& tl, const ShellsDev& sh, "
                     "const PrimPair* pp, const double* boys, const SinkArgs& s, hipStream_t st) {" % tag)
+        if coop:
+            body.append("    if (mode == MODE_JK32) mode = MODE_JK;   // no FP32 variant of the LDS kernels")
         for (mode, ex, fn) in variants:
             cond = "mode == %s" % mode
             if differs:
@@ -413,8 +416,24 @@ def main():
                           % (cond, cls))
     hv.append("    return -1;")
     hv.append("}")
+    hv.append("inline int eval_quartet_host_f32(int la, int lb, int lc, int ld, const double* A, const double* B, "
+              "const double* C, const double* D, const PrimPair* bra, int npb, const PrimPair* ket, int npk, "
+              "const float* boys32, float* out) {")
+    for (la, lb, lc, ld, differs, fr, fe, fh) in info:
+        if la + lb + lc + ld >= COOP_MIN_L:
+            continue
+        cond = "la == %d && lb == %d && lc == %d && ld == %d" % (la, lb, lc, ld)
+        hv.append("    if (%s) { eval_quartet<EriClass<%d, %d, %d, %d, 0>, float>(A, B, C, D, bra, npb, ket, npk, "
+                  "boys32, out); return 0; }" % (cond, la, lb, lc, ld))
+    hv.append("    return -1;")
+    hv.append("}")
     hv.append("}  // namespace hfgpu")
     write_if_changed(os.path.join(outdir, "host_eval.inc"), "\n".join(hv) + "\n")
+
+
+def ifac(k, name):
+    """`k * name * ` with an integer factor, so the product keeps the arithmetic type R."""
+    return "%s * " % name if k == 1 else "%d * %s * " % (k, name)
 
 
 def write_if_changed(path, text):

@@ -40,14 +40,16 @@ struct ShellsDev {
     const int* bf;        // first basis function
 };
 
-enum EriMode { MODE_JK = 0, MODE_PACKED = 1, MODE_DIAG = 2 };
+// MODE_JK32: J/K digestion (in FP64) of quartets evaluated in FP32 (mixed-precision tier).
+enum EriMode { MODE_JK = 0, MODE_PACKED = 1, MODE_DIAG = 2, MODE_JK32 = 3 };
 
 // Within a pair class, pairs are grouped by contraction length (number of primitive pairs,
 // descending) and sorted by Schwarz bound (descending) inside each group.  A segment is one bra
 // group of a quartet class, so every bra in it has the same primitive-pair count.  Its tasks are
 // enumerated by a prefix array over the bra group: bra p = bra0 + i owns tasks
 // [prefix[i], prefix[i+1]), and those run over the ket groups in order, taking in each ket
-// group g the screened head (Q descending) of length koff[i][g+1] - koff[i][g].  Consecutive
+// group g the koff[i][g+1] - koff[i][g] kets that start kbeg[i][g] places into the group (Q
+// descending: the screened head, or for a band of bounds a range further in).  Consecutive
 // tasks of one bra therefore share the bra and, except at a group change, the ket contraction
 // length: wavefronts see little divergence in the primitive loops and the bra-side J/K
 // contributions of a block stay on one or two bra pairs.
@@ -55,6 +57,7 @@ struct TaskSeg {
     int bra0, nbra, nkg, pad;
     int64_t prefix_off;      // first of the segment's nbra + 1 prefix entries
     int64_t koff_off;        // first of the segment's nbra * (nkg + 1) ket-group offsets
+    int64_t kbeg_off;        // first of the segment's nbra * nkg ket-range starts
     int64_t ntasks;
     int64_t task0;           // first task of the segment within the launch
     int64_t block0;          // first 256-task block (segments padded to whole blocks)
@@ -66,6 +69,7 @@ struct TaskList {
     PairClassDev bra, ket;
     const int64_t* prefix;   // prefix arrays of every segment
     const int* koff;         // per-bra ket-group offsets of every segment
+    const int* kbeg;         // per-bra ket-range starts within each ket group
     const int* kgroup;       // first pair of each ket group (ket class), nkg + 1 entries
     const TaskSeg* seg;
     int nseg;
@@ -84,11 +88,20 @@ __device__ __forceinline__ int segment_of(const TaskSeg* __restrict__ seg, int n
     return lo;
 }
 
-// Ket pair for rank r (0 <= r < koff[nkg]) in the ket list of one bra, whose group offsets are ko.
-__device__ __forceinline__ int ket_of_rank(const int* __restrict__ ko, const int* __restrict__ kgroup, int nkg, int r) {
+// Ket pair for rank r (0 <= r < ko[nkg]) in the ket list of one bra, whose group offsets are ko
+// and range starts kb.
+__device__ __forceinline__ int ket_of_rank(const int* __restrict__ ko, const int* __restrict__ kb,
+                                           const int* __restrict__ kgroup, int nkg, int r) {
     int g = 0;
     while (g + 1 < nkg && ko[g + 1] <= r) ++g;
-    return kgroup[g] + (r - ko[g]);
+    return kgroup[g] + kb[g] + (r - ko[g]);
+}
+
+// Ket pair of segment-local task t owned by bra index pl of segment sg.
+__device__ __forceinline__ int ket_of_task(const TaskList& tl, const TaskSeg& sg, const int64_t* __restrict__ pre, int pl,
+                                           int64_t t) {
+    return ket_of_rank(tl.koff + sg.koff_off + (int64_t)pl * (sg.nkg + 1), tl.kbeg + sg.kbeg_off + (int64_t)pl * sg.nkg,
+                       tl.kgroup, sg.nkg, (int)(t - pre[pl]));
 }
 
 // Bra pair (index within the segment's bra group) owning task t: the largest i in [lo, hi) with
@@ -111,6 +124,7 @@ struct SinkArgs {
     uint64_t packed_len;
     double* Qout;            // MODE_DIAG output, one per bra pair
     int lds_slots;           // MODE_JK: bra pairs per block whose exchange rows are staged in LDS
+    const float* boys32;     // MODE_JK32: FP32 copy of the Boys table
 };
 
 }  // namespace hfgpu

@@ -90,10 +90,11 @@ struct QuartetClassTasks {
 };
 
 struct TaskSet {
-    double threshold = -1;
+    double threshold = -1;           // tasks with threshold <= Q_p Q_q < upper
+    double upper = 0;
     std::vector<QuartetClassTasks> classes;
     DevBuf<int64_t> d_prefix;
-    DevBuf<int> d_koff;
+    DevBuf<int> d_koff, d_kbeg;
     DevBuf<hfgpu::TaskSeg> d_seg;
 };
 
@@ -108,6 +109,7 @@ struct PairClassHost {
 
 struct KernelStat {
     int cls = 0;
+    int fp32 = 0;             // quartets evaluated in FP32 (mixed-precision tier)
     int64_t launches = 0, quartets = 0, prim_quartets = 0;
     double eris = 0, flops = 0, ms = 0;
 };
@@ -124,9 +126,11 @@ struct hf_ctx {
     DevBuf<int> d_shell_bf;
     DevBuf<hfgpu::PrimPair> d_pp;
     DevBuf<double> d_boys;
+    DevBuf<float> d_boys32;
     std::vector<PairClassHost> pcls;
     TaskSet tasks_all;                     // unscreened (packed ERIs)
     std::map<int, TaskSet> tasks_by_level; // screened, by threshold level k (10^(k/4))
+    std::map<int64_t, TaskSet> tasks_mixed_hi, tasks_mixed_lo;   // mixed precision: FP64 part, FP32 band
     // J/K scratch
     DevBuf<double> d_Jp, d_Kp, d_D, d_J, d_K;
     // one-electron kernel inputs: per-shell L/bf/prim0/nprim, and alpha|coef|Z|atom xyz
@@ -149,10 +153,12 @@ struct hf_ctx {
 
 // Implemented in context.hip
 // J/K of the densities d_D (ndens x n x n) on stream st, screened at Schwarz product thr (0: the
-// context's threshold); accumulate adds to d_J/d_K instead of overwriting them.
+// context's threshold); accumulate adds to d_J/d_K instead of overwriting them; fp32 evaluates
+// the quartets whose bound is below the context's fp32_threshold in FP32.
 int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st, double thr,
-                      bool accumulate);
-int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts);
+                      bool accumulate, bool fp32);
+// Tasks with threshold <= Q_p Q_q (< upper when upper > 0).
+int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts, double upper = 0.0);
 // Implemented in one_electron.hip: S, T, V (n x n, device) on stream st.
 int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st);
 

@@ -470,6 +470,7 @@ void hf_scf_defaults(hf_scf_params* p) {
     p->density_method = 0;
     p->incremental_fock = 0;
     p->full_rebuild_every = 8;
+    p->final_fp64_iterations = 3;
 }
 
 int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out) {
@@ -558,6 +559,11 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     double quartets = 0, last_dmax = 0, last_quartets = 0;
     const bool incremental = p->incremental_fock != 0;
     const int rebuild_every = p->full_rebuild_every > 0 ? p->full_rebuild_every : 8;
+    // mixed-precision tier: quartets below the context's fp32_threshold in FP32 until fp64_phase
+    const bool mixed = ctx->opt.precision == HF_PREC_MIXED;
+    const int final_fp64 = p->final_fp64_iterations > 0 ? p->final_fp64_iterations : 3;
+    bool fp64_phase = !mixed, last_fp32 = false;
+    int fp64_steps = 0, fp32_builds = 0;
     auto quartet_count = [&]() {
         double q = 0;
         for (const auto& ks : ctx->stats) q += (double)ks.quartets;
@@ -577,7 +583,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             const double q0 = quartet_count();
             const bool full = !incremental || force_full || full_builds == 0 || since_full >= rebuild_every;
             if (full) {
-                if (int rc = hf_jk_device_impl(ctx, nsp, st.D.p, st.J.p, st.K.p, s, 0.0, false)) return rc;
+                if (int rc = hf_jk_device_impl(ctx, nsp, st.D.p, st.J.p, st.K.p, s, 0.0, false, !fp64_phase)) return rc;
                 ++full_builds;
                 since_full = 0;
                 force_full = false;
@@ -591,12 +597,20 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 last_dmax = dmax;
                 if (dmax > 0) {
                     const double thr = ctx->opt.schwarz_threshold / std::min(1.0, dmax);
-                    if (int rc = hf_jk_device_impl(ctx, nsp, st.dD.p, st.J.p, st.K.p, s, thr, true)) return rc;
+                    if (int rc = hf_jk_device_impl(ctx, nsp, st.dD.p, st.J.p, st.K.p, s, thr, true, !fp64_phase))
+                        return rc;
                 }
                 ++incr_builds;
                 ++since_full;
             }
             last_incremental = !full;
+            last_fp32 = !fp64_phase;
+            if (last_fp32) {
+                ++fp32_builds;
+                fp64_steps = 0;
+            } else {
+                ++fp64_steps;
+            }
             if (incremental)
                 HF_HIP_CHECK(ctx, hipMemcpyAsync(st.Dprev.p, st.D.p, nsp * nn * sizeof(double), hipMemcpyDeviceToDevice, s));
             HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -695,16 +709,30 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     int iter = 0;
     bool converged = false, cancelled = false;
     auto stop = [&]() { return p->terminate && *p->terminate; };
+    // Convergence is accepted only on a step whose J/K build was full (not incremental) and, in
+    // the mixed-precision tier, all-FP64: the tier switches to FP64 once the criteria are met
+    // within a factor 100 (or met outright) and then needs final_fp64_iterations FP64 steps.
+    auto accept = [&](bool met, bool nearly) {
+        if (mixed && !fp64_phase && (nearly || met)) {
+            fp64_phase = true;
+            return false;
+        }
+        if (!met) return false;
+        if (last_incremental) {
+            force_full = true;
+            return false;
+        }
+        return !(mixed && (last_fp32 || fp64_steps < final_fp64));
+    };
     for (; iter < p->max_iterations; ++iter) {
         if (int rc = step(iter, rms)) return rc;
         cur = energy;
         trace_line(iter, rms);
-        if (std::fabs(prev - cur) <= (useDIIS ? etolD : etolN) && rms < dtol && lastErr < itol) {
-            if (!last_incremental) {
-                converged = true;
-                break;
-            }
-            force_full = true;     // confirm on a step with a full J/K build
+        const double de = std::fabs(prev - cur), etol = useDIIS ? etolD : etolN;
+        if (accept(de <= etol && rms < dtol && lastErr < itol,
+                   de <= 100 * etol && rms < 100 * dtol && lastErr < 100 * itol)) {
+            converged = true;
+            break;
         }
         if (stop()) { cancelled = true; break; }
         prev = cur;
@@ -725,9 +753,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             if (int rc = step(iter, rms)) return rc;
             cur = energy;
             trace_line(iter, rms);
-            if (std::fabs(prev - cur) <= etolN && rms < dtol) {
-                if (!last_incremental) { converged = true; break; }
-                force_full = true;
+            const double de = std::fabs(prev - cur);
+            if (accept(de <= etolN && rms < dtol, de <= 100 * etolN && rms < 100 * dtol)) {
+                converged = true;
+                break;
             }
             if (stop()) { cancelled = true; break; }
             prev = cur;
@@ -756,6 +785,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     r->full_builds = full_builds;
     r->incremental_builds = incr_builds;
     r->quartets = quartets;
+    r->fp32_builds = fp32_builds;
     if (uhf) {
         // <S^2> = ((Na-Nb)/2)^2 + (Na+Nb)/2 - sum_ij ((C_a,occ)^T S C_b,occ)_ij^2
         if (occ[0] > 0 && occ[1] > 0) {

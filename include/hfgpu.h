@@ -138,6 +138,8 @@ typedef struct hf_scf_params {
                                       at threshold / max|dD|; convergence is only accepted on a step
                                       with a full build.  0 (default): full J/K every step. */
     int full_rebuild_every;        /* incremental steps between full builds (default 8) */
+    int final_fp64_iterations;     /* mixed precision (context option): all-FP64 steps required at the
+                                      end before convergence is accepted (default 3) */
 } hf_scf_params;
 
 typedef struct hf_scf_result {
@@ -170,6 +172,7 @@ typedef struct hf_kernel_stat {
     double eris;                   /* Cartesian ERIs evaluated (quartets x components) */
     double flops;                  /* algorithmic flops (generated-code operation count) */
     double ms;                     /* kernel time from HIP events on the launch stream (profile = 1), else 0 */
+    int fp32;                      /* 1: quartets evaluated in FP32 (mixed precision) */
 } hf_kernel_stat;
 int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max);   /* returns count */
 int hf_kernel_stats_reset(hf_ctx* ctx);

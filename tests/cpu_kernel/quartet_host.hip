@@ -15,6 +15,13 @@ extern "C" int hfq_eval(int la, int lb, int lc, int ld, int exact, const double*
                                     c0, d0);
 }
 
+// The FP32 evaluation of the mixed-precision tier (classes with L < 6); boys32 is the FP32 table.
+extern "C" int hfq_eval_f32(int la, int lb, int lc, int ld, const double* A, const double* B, const double* C,
+                            const double* D, const hfgpu::PrimPair* bra, int npb, const hfgpu::PrimPair* ket, int npk,
+                            const float* boys32, float* out) {
+    return hfgpu::eval_quartet_host_f32(la, lb, lc, ld, A, B, C, D, bra, npb, ket, npk, boys32, out);
+}
+
 extern "C" void hfq_boys_table(double* out) {
     std::vector<double> t;
     hfhost::boys_table(t);

@@ -116,3 +116,49 @@ def test_uhf_energy_matches_oracle(gpu_engine_factory, name, bfile, atoms):
     assert abs(r.energy - ref.energy) <= E_TOL, (r.energy, ref.energy)
     assert abs(r.s2 - ref.s2) <= 1e-6
     assert 0.75 <= r.s2 < 0.80          # doublet with small spin contamination
+
+
+MIXED_CASES = [
+    ("ethane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(2, seed=5)),
+    ("water4_631gss", "6-31g_st__st_.nw", g.water_cluster(4, seed=2024)),
+]
+MIXED_KW = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+
+
+@pytest.mark.parametrize("name,bfile,atoms", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
+def test_mixed_jk_close_to_fp64(gpu_engine_factory, name, bfile, atoms):
+    """Mixed-precision J/K (quartets with Schwarz bound below 1e-6 in FP32) against the FP64
+    build on the converged density: the FP32 band's integrals are below 1e-6 with errors below
+    1e-3 of that, so J and K differ by far less than 1e-7."""
+    _print_case(name)
+    e64 = gpu_engine_factory().set_molecule(basis(bfile), atoms)
+    e32 = gpu_engine_factory(precision=1, fp32_threshold=1e-6).set_molecule(basis(bfile), atoms)
+    _, D = e64.scf(want_density=True, **MIXED_KW)
+    D = D[0]
+    J64, K64 = e64.build_jk(D)
+    e32.reset_kernel_stats()
+    J32, K32 = e32.build_jk(D)
+    dj, dk = np.abs(J32 - J64).max(), np.abs(K32 - K64).max()
+    print("max|dJ| %.3e max|dK| %.3e" % (dj, dk), flush=True)
+    fp32_quartets = sum(s.quartets for s in e32.kernel_stats() if s.fp32)
+    fp64_quartets = sum(s.quartets for s in e32.kernel_stats() if not s.fp32)
+    print("fp32 quartets %d fp64 quartets %d" % (fp32_quartets, fp64_quartets), flush=True)
+    assert fp32_quartets > 0
+    assert dj <= 1e-7 and dk <= 1e-7, (dj, dk)
+
+
+@pytest.mark.parametrize("name,bfile,atoms", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
+def test_mixed_scf_energy_matches_fp64(gpu_engine_factory, name, bfile, atoms):
+    """Mixed-precision SCF: FP32 band until the criteria are nearly met, then all-FP64 steps (at
+    least 3) before convergence is accepted.  The energy must agree with the all-FP64 run to
+    1e-6 Eh (the tier's target); in practice the final FP64 steps bring it to the FP64 result."""
+    _print_case(name)
+    e64 = gpu_engine_factory().set_molecule(basis(bfile), atoms)
+    e32 = gpu_engine_factory(precision=1, fp32_threshold=1e-6).set_molecule(basis(bfile), atoms)
+    r64 = e64.scf(**MIXED_KW)
+    r32 = e32.scf(**MIXED_KW)
+    print("E64 %.12f E32 %.12f dE %.3e iters %d/%d fp32 builds %d" % (
+        r64.energy, r32.energy, r32.energy - r64.energy, r64.iterations, r32.iterations, r32.fp32_builds), flush=True)
+    assert r64.converged and r32.converged
+    assert r32.fp32_builds > 0
+    assert abs(r32.energy - r64.energy) <= 1e-6

@@ -28,6 +28,8 @@ def harness():
         L.hfq_eval.argtypes = [ctypes.c_int] * 5 + [dp] * 4 + [dp, ctypes.c_int, dp, ctypes.c_int, dp, dp] + \
             [ctypes.c_int] * 6
         L.hfq_boys_table.argtypes = [dp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.hfq_eval_f32.argtypes = [ctypes.c_int] * 4 + [dp] * 4 + [dp, ctypes.c_int, dp, ctypes.c_int, fp, fp]
         _H = L
     return _H
 
@@ -257,3 +259,70 @@ def test_cooperative_truncated_dddd_matches_oracle_reference_mode():
                 ref = mol.eri(A["bf"] + ia, B["bf"] + ib, C["bf"] + ic, D["bf"] + id_)
                 worst = max(worst, abs(res[idx] - ref))
     assert worst < 1e-12, worst
+
+
+def block_f32(A, B, C, D, boys32):
+    nA, nB, nC, nD = (((s["L"] + 1) * (s["L"] + 2)) // 2 for s in (A, B, C, D))
+    out = np.zeros(nA * nB * nC * nD, dtype=np.float32)
+    bp, kp = prim_pairs(A, B), prim_pairs(C, D)
+    fp = ctypes.POINTER(ctypes.c_float)
+    rc = harness().hfq_eval_f32(A["L"], B["L"], C["L"], D["L"], _dp(A["xyz"]), _dp(B["xyz"]), _dp(C["xyz"]),
+                                _dp(D["xyz"]), _dp(bp), len(bp), _dp(kp), len(kp), boys32.ctypes.data_as(fp),
+                                out.ctypes.data_as(fp))
+    assert rc == 0
+    return out.astype(np.float64).reshape(nA, nB, nC, nD)
+
+
+def schwarz(A, B, boys):
+    """Schwarz factor sqrt(max |(ab|ab)|) of a shell pair (FP64 harness)."""
+    d = block(A, B, A, B, 1, boys)
+    n1, n2 = d.shape[0], d.shape[1]
+    return math.sqrt(max(abs(d[i, j, i, j]) for i in range(n1) for j in range(n2)))
+
+
+@pytest.mark.parametrize("name,bfile,atoms", CASES, ids=[c[0] for c in CASES])
+def test_fp32_classes_relative_accuracy(name, bfile, atoms):
+    """The FP32 evaluation used for small quartets in the mixed-precision tier (every class the
+    thread-per-quartet kernels serve, L < 6), against the oracle's FP64 integrals.  FP32 rounding
+    (6e-8) is amplified by the electron-transfer step, whose (zeta/eta) factor reaches 1e3-1e4 for
+    a tight bra primitive against a diffuse ket one (the same cancellation costs FP64 about four
+    digits); the error of a shell quartet stays below 1e-3 of its Schwarz bound Q_ab Q_cd, which
+    for the quartets under the 1e-6 bound that select FP32 is below 1e-9 absolute.  Classes with
+    an s-type ket (no transfer step) stay near FP32 rounding (< 1e-5 of the bound)."""
+    shells = shells_of(basis(bfile), atoms)
+    mol = refcpu.Molecule(basis(bfile), atoms)
+    boys = boys_table()
+    boys32 = boys.astype(np.float32)
+    rng = np.random.default_rng(11)
+    by_l = {}
+    for i, s in enumerate(shells):
+        by_l.setdefault(s["L"], []).append(i)
+    pairs = [(a, b) for a in range(3) for b in range(a + 1)]
+    worst = 0.0
+    worst_low = 0.0
+    checked = 0
+    refcpu.set_exact_transfer(True)
+    try:
+        for (la, lb) in pairs:
+            for (lc, ld) in pairs:
+                if (lc + ld, lc) > (la + lb, la) or la + lb + lc + ld >= 6:
+                    continue
+                if not (by_l.get(la) and by_l.get(lb) and by_l.get(lc) and by_l.get(ld)):
+                    continue
+                for _ in range(3):
+                    A, B, C, D = (shells[int(rng.choice(by_l[x]))] for x in (la, lb, lc, ld))
+                    blk = block_f32(A, B, C, D, boys32)
+                    ref = np.zeros_like(blk)
+                    for idx in np.ndindex(*blk.shape):
+                        ref[idx] = mol.eri(A["bf"] + idx[0], B["bf"] + idx[1], C["bf"] + idx[2], D["bf"] + idx[3])
+                    scale = schwarz(A, B, boys) * schwarz(C, D, boys)
+                    err = np.abs(blk - ref).max() / scale
+                    worst = max(worst, err)
+                    if lc + ld == 0:
+                        worst_low = max(worst_low, err)
+                    checked += 1
+    finally:
+        refcpu.set_exact_transfer(False)
+    assert checked >= 10
+    assert worst_low < 1e-5, worst_low
+    assert worst < 1e-3, worst
