@@ -42,10 +42,12 @@ void boys_host(int mmax, double T, double* F) {
 void boys_table(std::vector<double>& tab) {
     using hfgpu::BOYS_NCOL;
     using hfgpu::BOYS_NROW;
-    tab.assign((size_t)BOYS_NROW * BOYS_NCOL, 0.0);
+    using hfgpu::BOYS_STRIDE;
+    tab.assign((size_t)BOYS_NROW * BOYS_STRIDE, 0.0);
     for (int k = 0; k < BOYS_NROW; ++k) {
         const long double T = (long double)k * 0.1L;
-        for (int m = 0; m < BOYS_NCOL; ++m) tab[(size_t)k * BOYS_NCOL + m] = (double)series(m, T);
+        for (int m = 0; m < BOYS_NCOL; ++m) tab[(size_t)k * BOYS_STRIDE + m] = (double)series(m, T);
+        tab[(size_t)k * BOYS_STRIDE + BOYS_NCOL] = (double)std::exp(-T);
     }
 }
 

@@ -23,36 +23,6 @@ namespace hfgpu {
 constexpr double kTwoPi25 = 34.98683665524972497;   // 2 pi^{5/2}
 constexpr double kSqrtPiHalf = 0.88622692545275801365;  // sqrt(pi)/2
 
-// F_m(T) * pref for m = 0..L.
-template <int L>
-__host__ __device__ __forceinline__ void boys_scaled(double T, double pref, const double* __restrict__ tab, double* F) {
-    if (T < BOYS_TMAX) {
-        const int k = (int)rint(T * BOYS_INV_DT);
-        const double x = k * BOYS_DT - T;          // F_m(T) = sum_j F_{m+j}(T_k) x^j / j!
-        const double* row = tab + k * BOYS_NCOL + L;
-        double f = row[7];
-        f = row[6] + x * f * (1.0 / 7.0);
-        f = row[5] + x * f * (1.0 / 6.0);
-        f = row[4] + x * f * (1.0 / 5.0);
-        f = row[3] + x * f * (1.0 / 4.0);
-        f = row[2] + x * f * (1.0 / 3.0);
-        f = row[1] + x * f * (1.0 / 2.0);
-        f = row[0] + x * f;
-        const double e = exp(-T);
-        F[L] = f;
-#pragma unroll
-        for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) * (1.0 / (2 * m + 1));
-    } else {
-        const double e = exp(-T);
-        const double inv2T = 0.5 / T;
-        F[0] = kSqrtPiHalf / sqrt(T);
-#pragma unroll
-        for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * inv2T;
-    }
-#pragma unroll
-    for (int m = 0; m <= L; ++m) F[m] *= pref;
-}
-
 // 1/sqrt(x) for normal positive x: the hardware estimate refined by two Newton steps
 // r <- r (3 - x r^2) / 2 (each step squares the relative error), on the device; exact on the host.
 __host__ __device__ __forceinline__ double rsqrt_f64(double x) {
@@ -66,6 +36,59 @@ __host__ __device__ __forceinline__ double rsqrt_f64(double x) {
 #endif
 }
 
+__host__ __device__ __forceinline__ float rsqrt_f32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+
+// F_m(T) * pref for m = 0..L.  Below BOYS_TMAX: F_L by an 8-term Taylor expansion about the nearest
+// grid point T_k (|x| = |T_k - T| <= 0.05, truncation below 1e-17 relative) and downward
+// recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1), with e^{-T} = e^{-T_k} e^x from the table and
+// a 9-term series for e^x.  Beyond: F_0 = sqrt(pi/T)/2 and upward recursion.
+template <int L>
+__host__ __device__ __forceinline__ void boys_scaled(double T, double pref, const double* __restrict__ tab, double* F) {
+    if (T < BOYS_TMAX) {
+        const int k = (int)rint(T * BOYS_INV_DT);
+        const double x = k * BOYS_DT - T;          // F_m(T) = sum_j F_{m+j}(T_k) x^j / j!
+        const double* row = tab + k * BOYS_STRIDE;
+        double f = row[L + 7];
+        f = row[L + 6] + x * f * (1.0 / 7.0);
+        f = row[L + 5] + x * f * (1.0 / 6.0);
+        f = row[L + 4] + x * f * (1.0 / 5.0);
+        f = row[L + 3] + x * f * (1.0 / 4.0);
+        f = row[L + 2] + x * f * (1.0 / 3.0);
+        f = row[L + 1] + x * f * (1.0 / 2.0);
+        f = row[L] + x * f;
+        F[L] = f;
+        if (L > 0) {
+            double ex = 1.0 + x * (1.0 / 8.0);
+            ex = 1.0 + x * (1.0 / 7.0) * ex;
+            ex = 1.0 + x * (1.0 / 6.0) * ex;
+            ex = 1.0 + x * (1.0 / 5.0) * ex;
+            ex = 1.0 + x * (1.0 / 4.0) * ex;
+            ex = 1.0 + x * (1.0 / 3.0) * ex;
+            ex = 1.0 + x * (1.0 / 2.0) * ex;
+            ex = 1.0 + x * ex;
+            const double e = row[BOYS_NCOL] * ex;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double r = rsqrt_f64(T), inv2T = 0.5 * r * r;
+        F[0] = kSqrtPiHalf * r;
+        if (L > 0) {
+            const double e = exp(-T);
+#pragma unroll
+            for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * inv2T;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) F[m] *= pref;
+}
+
 // FP32 evaluation of F_m(T) * pref (quartets of the mixed-precision tier): a 6-term Taylor
 // expansion of F_L from an FP32 copy of the table (truncation < 3e-9 relative for |x| <= 0.05),
 // FP32 exp and downward recursion below BOYS_TMAX; above it the asymptotic form
@@ -76,7 +99,7 @@ __host__ __device__ __forceinline__ void boys_scaled_f32(float T, float pref, co
     if (T < (float)BOYS_TMAX) {
         const int k = (int)rintf(T * (float)BOYS_INV_DT);
         const float x = (float)k * (float)BOYS_DT - T;
-        const float* row = tab + k * BOYS_NCOL + L;
+        const float* row = tab + k * BOYS_STRIDE + L;
         float f = row[5];
         f = row[4] + x * f * (1.0f / 5.0f);
         f = row[3] + x * f * (1.0f / 4.0f);
@@ -103,15 +126,8 @@ __host__ __device__ __forceinline__ void boys_scaled_f32(float T, float pref, co
     }
 }
 
-// Reciprocal: exact division in FP64; the hardware reciprocal (1 ulp) in FP32 on the device.
-__host__ __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
-__host__ __device__ __forceinline__ float recip(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_rcpf(x);
-#else
-    return 1.0f / x;
-#endif
-}
+__host__ __device__ __forceinline__ double rsqrt_r(double x) { return rsqrt_f64(x); }
+__host__ __device__ __forceinline__ float rsqrt_r(float x) { return rsqrt_f32(x); }
 
 __device__ __forceinline__ uint64_t two_index(uint64_t i, uint64_t j) {
     return i < j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j;
@@ -393,11 +409,12 @@ __host__ __device__ __forceinline__ void eval_quartet(const double* __restrict__
         for (int j = 0; j < npk; ++j) {
             const PrimPair Q = ket[j];
             const R eta = R(Q.zeta), inv_eta = R(Q.inv_zeta);
-            const R inv_s = recip(zeta + eta);
+            const R rs = rsqrt_r(zeta + eta);               // (zeta + eta)^{-1/2}
+            const R inv_s = rs * rs;
             const R rho = zeta * eta * inv_s;
             const R PQx = R(P.Px - Q.Px), PQy = R(P.Py - Q.Py), PQz = R(P.Pz - Q.Pz);
             const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const R pref = R(kTwoPi25) * KP * R(Q.Kz) * sqrt(inv_s);
+            const R pref = R(kTwoPi25) * KP * R(Q.Kz) * rs;
             R F[C::L + 1];
             if constexpr (sizeof(R) == sizeof(double))
                 boys_scaled<C::L>(T, pref, boys, F);

@@ -6,11 +6,14 @@
 
 namespace hfgpu {
 
-// Boys table: F_m(T_k), T_k = k * BOYS_DT, k = 0..BOYS_NROW-1, m = 0..BOYS_NCOL-1.
+// Boys table: row k (T_k = k * BOYS_DT, k = 0..BOYS_NROW-1) holds F_m(T_k) for m = 0..BOYS_NCOL-1
+// and e^{-T_k} in column BOYS_NCOL; rows are BOYS_STRIDE doubles apart.
 // The kernels use an 8-term Taylor expansion about the nearest grid point for
-// F_L(T) and downward recursion for lower m while T < BOYS_TMAX, and the
-// asymptotic F_0 with upward recursion beyond.
+// F_L(T), e^{-T} = e^{-T_k} e^{T_k - T} (Taylor for the small factor) and downward
+// recursion for lower m while T < BOYS_TMAX, and the asymptotic F_0 with upward
+// recursion beyond.
 constexpr int BOYS_NCOL = 16;
+constexpr int BOYS_STRIDE = BOYS_NCOL + 1;
 constexpr int BOYS_TAYLOR = 8;
 constexpr double BOYS_DT = 0.1;
 constexpr double BOYS_INV_DT = 10.0;

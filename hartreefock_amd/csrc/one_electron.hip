@@ -22,7 +22,7 @@ __device__ void boys_rt(double T, int mmax, const double* __restrict__ tab, doub
     if (T < BOYS_TMAX) {
         const int k = (int)rint(T * BOYS_INV_DT);
         const double x = k * BOYS_DT - T;
-        const double* row = tab + k * BOYS_NCOL + mmax;
+        const double* row = tab + k * BOYS_STRIDE + mmax;
         double f = row[7];
         for (int j = 6; j >= 0; --j) f = row[j] + x * f / (j + 1);
         const double e = exp(-T);

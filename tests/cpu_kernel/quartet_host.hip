@@ -28,4 +28,4 @@ extern "C" void hfq_boys_table(double* out) {
     for (size_t i = 0; i < t.size(); ++i) out[i] = t[i];
 }
 
-extern "C" int hfq_boys_table_size() { return hfgpu::BOYS_NROW * hfgpu::BOYS_NCOL; }
+extern "C" int hfq_boys_table_size() { return hfgpu::BOYS_NROW * hfgpu::BOYS_STRIDE; }

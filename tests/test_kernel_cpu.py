@@ -79,7 +79,7 @@ def block(A, B, C, D, exact, boys):
 
 
 def boys_table():
-    t = np.zeros(361 * 16)
+    t = np.zeros(harness().hfq_boys_table_size())
     harness().hfq_boys_table(_dp(t))
     return t
 
