@@ -79,6 +79,35 @@ def cpu_baseline(bfile, atoms, iterations, budget_s):
                        % (cnt, unique, stride, t_eri, eri_phase, rows, n, t_g, fock, iterations, total))}
 
 
+def copy_calibration():
+    """A plain device copy of known size: 1 GiB read + 1 GiB written per launch, three launches.
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this gives the counters' scale for HBM bytes."""
+    import torch
+    n = (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}), flush=True)
+
+
+def timed_scfs(eng, steps, warmup):
+    for _ in range(warmup):
+        eng.scf(**SCF_KW)
+    eng.reset_kernel_stats()
+    lat, res = [], []
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        r = eng.scf(**SCF_KW)
+        lat.append(time.perf_counter() - s0)
+        res.append(r)
+    hip_sync()
+    return time.perf_counter() - t0, lat, res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,7 +116,13 @@ def main():
     ap.add_argument("--workload", default="decane")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (one molecule's SCF per GPU; "
               "multi-GPU J/K sharding is a design note, not implemented). Use --gpus 1.", file=sys.stderr)
@@ -96,22 +131,9 @@ def main():
     label, bfile, atoms = workload(a.workload)
     # exact integrals: the reference's (dd|dd) transfer convention makes this SCF diverge
     eng = hf.Engine(schwarz_threshold=1e-12, profile=1, exact_transfer=1).set_molecule(bfile, atoms)
-    for _ in range(a.warmup):
-        eng.scf(**SCF_KW)
-    eng.reset_kernel_stats()
-    lat = []
-    energies = []
-    iters = []
-    hip_sync()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        s0 = time.perf_counter()
-        r = eng.scf(**SCF_KW)
-        lat.append(time.perf_counter() - s0)
-        energies.append(r.energy)
-        iters.append(r.iterations)
-    hip_sync()
-    elapsed = time.perf_counter() - t0
+    elapsed, lat, res = timed_scfs(eng, a.steps, a.warmup)
+    energies = [r.energy for r in res]
+    iters = [r.iterations for r in res]
     stats = eng.kernel_stats()
     eris = sum(s.eris for s in stats)
     dom = max(stats, key=lambda s: s.ms)
@@ -120,6 +142,14 @@ def main():
     achieved = per_launch_flops / (per_launch_ms * 1e-3) / 1e12
     kernel_ms = sum(s.ms for s in stats)
     lat_sorted = sorted(lat)
+    # HBM bytes per launch of the dominant kernel: FETCH_SIZE + WRITE_SIZE from the committed
+    # rocprofv3 --pmc runs on this workload (calibrated against a 1 GiB copy), when available
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_decane_631gs.json")
+    if a.workload == "decane" and os.path.exists(tfile):
+        with open(tfile) as fh:
+            rec = json.load(fh)["kernels"].get(dom.name())
+        traffic = rec["total"] if rec else None
 
     def pct(p):
         k = min(len(lat_sorted) - 1, max(0, int(round(p / 100.0 * (len(lat_sorted) - 1)))))
@@ -148,10 +178,18 @@ def main():
         "eri_kernel_share": kernel_ms * 1e-3 / elapsed,
         "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "eri_class_kernel" + dom.name() + " (J/K)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "flops_per_launch": per_launch_flops, "ms_per_launch": per_launch_ms,
                      "launches": dom.launches},
     }
+    if not a.no_mixed:
+        # mixed-precision tier: quartets with Schwarz bound < 1e-6 in FP32, final iterations FP64
+        em = hf.Engine(schwarz_threshold=1e-12, profile=1, exact_transfer=1, precision=1,
+                       fp32_threshold=1e-6).set_molecule(bfile, atoms)
+        el_m, lat_m, res_m = timed_scfs(em, a.steps, a.warmup)
+        out["mixed"] = {"value": a.steps / el_m, "unit": "molecules/s", "vs_fp64": (a.steps / el_m) / out["value"],
+                        "abs_dE_vs_fp64_Eh": abs(res_m[-1].energy - energies[-1]), "iterations": res_m[-1].iterations,
+                        "fp32_builds": res_m[-1].fp32_builds, "fp32_threshold": 1e-6, "final_fp64_iterations": 3}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(bfile, atoms, iters[-1], a.cpu_budget)
     print(json.dumps(out), flush=True)
