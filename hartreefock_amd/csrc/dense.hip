@@ -1,0 +1,246 @@
+// Dense FP64 kernels for the SCF step on small matrices (n up to a few hundred), where a
+// library GEMM launches too few workgroups to use the chip:
+//   * dgemm_small: C = alpha op(A) op(B) + beta C (column-major, as BLAS), one wavefront per
+//     16 x 16 tile of C on the FP64 matrix cores (v_mfma_f64_16x16x4_f64);
+//   * the SP2 purification step fused into one such product (see sp2_step_kernel).
+//
+// Matrix-core lane layout of __builtin_amdgcn_mfma_f64_16x16x4f64 (A 16x4, B 4x16, D 16x16):
+//   lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; element r of its 4-vector result is
+//   D[l / 16 + 4 r][l % 16].
+#include "hf_internal.h"
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(64) dgemm16_kernel(int m, int n, int k, double alpha, const double* __restrict__ A,
+                                                     int lda, const double* __restrict__ B, int ldb, double beta,
+                                                     double* __restrict__ C, int ldc) {
+    const int l = threadIdx.x, lr = l & 15, lk = l >> 4;
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+    const int ia = i0 + lr, jb = j0 + lr;      // this lane's row of A and column of B
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < k; k0 += 16) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kk = k0 + 4 * u + lk;
+            const bool ok = kk < k;
+            a[u] = ok && ia < m ? (TA ? A[kk + (size_t)ia * lda] : A[ia + (size_t)kk * lda]) : 0.0;
+            b[u] = ok && jb < n ? (TB ? B[jb + (size_t)kk * ldb] : B[kk + (size_t)jb * ldb]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = i0 + lk + 4 * r, j = j0 + lr;
+        if (i < m && j < n) {
+            double* c = C + i + (size_t)j * ldc;
+            *c = beta == 0.0 ? alpha * acc[r] : alpha * acc[r] + beta * *c;
+        }
+    }
+}
+
+// ---- SP2 purification, one kernel per step ----
+// Step s forms X_s from the previous step's X_{s-1} and X_{s-1}^2 (or, for s = 0, from the
+// orthogonalised Fock matrix and its spectral bounds) on the fly, writes X_s, and multiplies
+// X_s X_s on the matrix cores; the wavefronts on diagonal tiles write partial traces of X_s and
+// X_s^2 for the next step's decision.  Everything a step needs from the previous one is read
+// after a kernel boundary, so one launch per step needs no host round trip.
+//
+// Step s (s >= 1) first decides, identically in every wavefront, for the previous matrix:
+//   e_{s-1} = tr X_{s-1} - tr X_{s-1}^2,   square = |tr X^2 - nocc| <= |2 tr X - tr X^2 - nocc|,
+//   stop_{s-1} = stop_{s-2} or (e_{s-1} < 1e-4 and e_{s-1} >= e_{s-3})
+// (the stopping rule of Kruchinina, Rudberg and Rubensson, JCTC 12, 5788 (2016)).  If stop_{s-2}
+// held, X_{s-1} was already final and the step only carries the stop flag forward.  Otherwise
+// X_s = X_{s-1}^2 or
+// 2 X_{s-1} - X_{s-1}^2 is written, and if stop_{s-1} holds it is the result (final = s).
+//
+// hist (device, doubles): [0] lo, [1] hi, [2] final step (-1 while running), then per step s at
+// 8 + 2 s: e_s, stop_s.  part: per step s at 2 * nt * s: nt partial traces of X_s, then nt of X_s^2.
+constexpr int kSp2Hist = 8;
+
+__global__ void __launch_bounds__(256) sp2_bounds_kernel(const double* __restrict__ F, int n, double* __restrict__ hist) {
+    __shared__ double slo[256], shi[256];
+    double lo = 1e300, hi = -1e300;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {     // column j (F symmetric), coalesced over j
+        double r = 0;
+        for (int i = 0; i < n; ++i)
+            if (i != j) r += fabs(F[(size_t)i * n + j]);
+        const double c = F[(size_t)j * n + j];
+        lo = fmin(lo, c - r);
+        hi = fmax(hi, c + r);
+    }
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + s]);
+            shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        lo = slo[0];
+        hi = shi[0];
+        if (hi - lo < 1e-12) hi = lo + 1.0;
+        hist[0] = lo;
+        hist[1] = hi;
+        hist[2] = -1.0;
+    }
+}
+
+struct Sp2Args {
+    int n, nt, s;
+    double nocc;
+    const double* F;         // orthogonalised Fock matrix (step 0)
+    const double* Xp;        // X_{s-1}
+    const double* X2p;       // X_{s-1}^2
+    double* X;               // X_s (out)
+    double* X2;              // X_s^2 (out)
+    double* hist;
+    double* part;
+};
+
+__global__ void __launch_bounds__(64) sp2_step_kernel(Sp2Args g) {
+    const int n = g.n, nt = g.nt, s = g.s;
+    bool square = false;
+    if (s > 0) {
+        const double* pp = g.part + (size_t)2 * nt * (s - 1);
+        double t1 = 0, t2 = 0;
+        for (int i = 0; i < nt; ++i) {
+            t1 += pp[i];
+            t2 += pp[nt + i];
+        }
+        const double e = t1 - t2;
+        square = fabs(t2 - g.nocc) <= fabs(2.0 * t1 - t2 - g.nocc);
+        const bool stop_pp = s >= 2 && g.hist[kSp2Hist + 2 * (s - 2) + 1] != 0;
+        if (stop_pp) {                                         // X_{s-1} already final
+            if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) g.hist[kSp2Hist + 2 * (s - 1) + 1] = 1.0;
+            return;
+        }
+        const double e3 = s >= 3 ? g.hist[kSp2Hist + 2 * (s - 3)] : 1e300;
+        const bool stop_p = e < 1e-4 && e >= e3;
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+            g.hist[kSp2Hist + 2 * (s - 1)] = e;
+            g.hist[kSp2Hist + 2 * (s - 1) + 1] = stop_p ? 1.0 : 0.0;
+            if (stop_p) g.hist[2] = s;
+        }
+        if (stop_p) {
+            // X_s is the result: write this tile of it and stop
+            const int l = threadIdx.x, lr = l & 15, lk = l >> 4;
+            const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + lk + 4 * r, j = j0 + lr;
+                if (i < n && j < n) {
+                    const size_t o = i + (size_t)j * n;
+                    g.X[o] = square ? g.X2p[o] : 2.0 * g.Xp[o] - g.X2p[o];
+                }
+            }
+            return;
+        }
+    }
+    const double lo = g.hist[0], hi = g.hist[1], inv = 1.0 / (hi - lo);
+    auto xval = [&](int i, int j) -> double {                   // element (i, j) of X_s
+        const size_t o = i + (size_t)j * n;
+        if (s == 0) return ((i == j ? hi : 0.0) - g.F[o]) * inv;
+        return square ? g.X2p[o] : 2.0 * g.Xp[o] - g.X2p[o];
+    };
+    const int l = threadIdx.x, lr = l & 15, lk = l >> 4;
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+    const int ia = i0 + lr, jb = j0 + lr;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kk = k0 + 4 * u + lk;
+            a[u] = kk < n && ia < n ? xval(ia, kk) : 0.0;
+            b[u] = kk < n && jb < n ? xval(kk, jb) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+    }
+    double d1 = 0, d2 = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = i0 + lk + 4 * r, j = j0 + lr;
+        if (i < n && j < n) {
+            const size_t o = i + (size_t)j * n;
+            const double x = xval(i, j);
+            g.X[o] = x;
+            g.X2[o] = acc[r];
+            if (i == j) {
+                d1 += x;
+                d2 += acc[r];
+            }
+        }
+    }
+    if (i0 == j0) {   // diagonal tile: partial traces, reduced over the wavefront in a fixed order
+#pragma unroll
+        for (int msk = 32; msk >= 1; msk >>= 1) {
+            d1 += __shfl_xor(d1, msk);
+            d2 += __shfl_xor(d2, msk);
+        }
+        if (l == 0) {
+            double* pp = g.part + (size_t)2 * nt * s;
+            pp[blockIdx.x] = d1;
+            pp[nt + blockIdx.x] = d2;
+        }
+    }
+}
+
+}  // namespace
+
+int hf_dgemm_small(hf_ctx* ctx, hipStream_t st, bool ta, bool tb, int m, int n, int k, double alpha, const double* A,
+                   int lda, const double* B, int ldb, double beta, double* C, int ldc) {
+    const dim3 grid((m + 15) / 16, (n + 15) / 16);
+    if (ta && tb) hipLaunchKernelGGL((dgemm16_kernel<true, true>), grid, dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (ta) hipLaunchKernelGGL((dgemm16_kernel<true, false>), grid, dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (tb) hipLaunchKernelGGL((dgemm16_kernel<false, true>), grid, dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else hipLaunchKernelGGL((dgemm16_kernel<false, false>), grid, dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    HF_HIP_CHECK(ctx, hipGetLastError());
+    return HF_OK;
+}
+
+int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp, double* X0, double* X1, double* W0,
+                 double* W1, double* hist, double* part, int max_steps, double*& result, int& steps) {
+    const int nt = (n + 15) / 16;
+    constexpr int kFirstCheck = 16, kEvery = 4;
+    hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(256), 0, st, Fp, n, hist);
+    Sp2Args g{};
+    g.n = n;
+    g.nt = nt;
+    g.nocc = nocc;
+    g.F = Fp;
+    g.hist = hist;
+    g.part = part;
+    double* Xb[2] = {X0, X1};
+    double* Wb[2] = {W0, W1};
+    result = nullptr;
+    steps = 0;
+    for (int s = 0; s <= max_steps; ++s) {
+        g.s = s;
+        g.Xp = Xb[(s + 1) & 1];
+        g.X2p = Wb[(s + 1) & 1];
+        g.X = Xb[s & 1];
+        g.X2 = Wb[s & 1];
+        hipLaunchKernelGGL(sp2_step_kernel, dim3(nt, nt), dim3(64), 0, st, g);
+        if ((s >= kFirstCheck && (s - kFirstCheck) % kEvery == 0) || s == max_steps) {
+            double fin = -1;
+            HF_HIP_CHECK(ctx, hipMemcpyAsync(&fin, hist + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+            HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
+            if (fin >= 0) {
+                steps = (int)fin;
+                result = Xb[steps & 1];
+                break;
+            }
+        }
+    }
+    HF_HIP_CHECK(ctx, hipGetLastError());
+    return HF_OK;
+}

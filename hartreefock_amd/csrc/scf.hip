@@ -172,6 +172,7 @@ int grid_for(size_t n) {
 // e_k = trace(X - X^2) is small, stop as soon as it no longer decreases (e_k >= e_{k-2}).
 // State (device, 8 doubles): lo, hi, done, e_{k-1}, e_{k-2}, steps, use_square, apply.
 enum { SP2_LO, SP2_HI, SP2_DONE, SP2_E1, SP2_E2, SP2_STEPS, SP2_SQUARE, SP2_APPLY };
+constexpr int kSp2MaxSteps = 100;
 
 __device__ double block_sum_256(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -335,7 +336,7 @@ struct Diis {
 struct ScfState {
     int n = 0;
     size_t nn = 0;
-    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, Dprev, dD, C, eps, T1, T2, T3, err, work, red, sp2;
+    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, Dprev, dD, C, eps, T1, T2, T3, err, work, red, sp2, sp2W, sp2part;
     DevBuf<rocblas_int> info;
 };
 
@@ -348,6 +349,8 @@ struct ScfState {
 
 int gemm(hf_ctx* ctx, bool ta, bool tb, int m, int n, int k, double alpha, const double* A, const double* B,
          double beta, double* C) {
+    if (std::max(m, std::max(n, k)) <= kSmallDense)
+        return hf_dgemm_small(ctx, ctx->stream, ta, tb, m, n, k, alpha, A, ta ? k : m, B, tb ? n : k, beta, C, m);
     BL(rocblas_dgemm(ctx->blas, ta ? rocblas_operation_transpose : rocblas_operation_none,
                      tb ? rocblas_operation_transpose : rocblas_operation_none, m, n, k, &alpha, A, ta ? k : m, B,
                      tb ? n : k, &beta, C, m));
@@ -505,7 +508,11 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     HF_HIP_CHECK(ctx, st.T1.alloc(nn));
     HF_HIP_CHECK(ctx, st.T2.alloc(nn));
     HF_HIP_CHECK(ctx, st.T3.alloc(nn));
-    HF_HIP_CHECK(ctx, st.sp2.alloc(8));
+    HF_HIP_CHECK(ctx, st.sp2.alloc(8 + 2 * (kSp2MaxSteps + 1)));
+    if (n <= kSmallDense) {
+        HF_HIP_CHECK(ctx, st.sp2W.alloc(2 * nn));
+        HF_HIP_CHECK(ctx, st.sp2part.alloc(2 * (size_t)((n + 15) / 16) * (kSp2MaxSteps + 1)));
+    }
     HF_HIP_CHECK(ctx, st.err.alloc(nn));
     HF_HIP_CHECK(ctx, st.work.alloc(n));
     HF_HIP_CHECK(ctx, st.red.alloc(16));
@@ -653,10 +660,23 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 HF_HIP_CHECK(ctx, hipMemsetAsync(Dk, 0, nn * sizeof(double), s));
                 ok = true;
             } else if (p->density_method == 0) {
-                if (int rc = sp2_projector(ctx, st, st.T2.p, occ[k], st.T1.p, st.T3.p, ok)) return rc;
+                const double* P = nullptr;
+                if (n <= kSmallDense) {
+                    double* res = nullptr;
+                    int steps = 0;
+                    if (int rc = hf_sp2_fused(ctx, s, n, occ[k], st.T2.p, st.T1.p, st.T3.p, st.sp2W.p, st.sp2W.p + nn,
+                                              st.sp2.p, st.sp2part.p, kSp2MaxSteps, res, steps))
+                        return rc;
+                    ok = res != nullptr;
+                    P = res;
+                } else {
+                    if (int rc = sp2_projector(ctx, st, st.T2.p, occ[k], st.T1.p, st.T3.p, ok)) return rc;
+                    P = st.T1.p;
+                }
                 if (ok) {
-                    if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.T1.p, st.X.p, 0.0, st.T3.p)) return rc;
-                    if (int rc = gemm(ctx, false, false, n, n, n, scale, st.X.p, st.T3.p, 0.0, Dk)) return rc;
+                    double* W = P == st.T1.p ? st.T3.p : st.T1.p;
+                    if (int rc = gemm(ctx, false, false, n, n, n, 1.0, P, st.X.p, 0.0, W)) return rc;
+                    if (int rc = gemm(ctx, false, false, n, n, n, scale, st.X.p, W, 0.0, Dk)) return rc;
                 }
             }
             last_diag[k] = !ok;
