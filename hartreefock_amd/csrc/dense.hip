@@ -1,7 +1,7 @@
 // Dense FP64 kernels for the SCF step on small matrices (n up to a few hundred), where a
 // library GEMM launches too few workgroups to use the chip:
-//   * dgemm_small: C = alpha op(A) op(B) + beta C (column-major, as BLAS), one wavefront per
-//     16 x 16 tile of C on the FP64 matrix cores (v_mfma_f64_16x16x4_f64);
+//   * dgemm_small: C = alpha op(A) op(B) + beta C (column-major, as BLAS), one block of four
+//     wavefronts per 16 x 16 tile of C, splitting k, on the FP64 matrix cores (v_mfma_f64_16x16x4_f64);
 //   * the SP2 purification step fused into one such product (see sp2_step_kernel).
 //
 // Matrix-core lane layout of __builtin_amdgcn_mfma_f64_16x16x4f64 (A 16x4, B 4x16, D 16x16):
@@ -13,26 +13,63 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <bool TA, bool TB>
-__global__ void __launch_bounds__(64) dgemm16_kernel(int m, int n, int k, double alpha, const double* __restrict__ A,
-                                                     int lda, const double* __restrict__ B, int ldb, double beta,
-                                                     double* __restrict__ C, int ldc) {
-    const int l = threadIdx.x, lr = l & 15, lk = l >> 4;
-    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
-    const int ia = i0 + lr, jb = j0 + lr;      // this lane's row of A and column of B
+// Partial product of a 16 x 16 tile over k in [kb, ke) for the calling wavefront: fa(i, k) and
+// fb(k, j) return elements of op(A) and op(B) (zero outside).  8 products per loop trip, with their
+// 16 loads issued first.
+template <class FA, class FB>
+__device__ __forceinline__ v4d tile_product(int kb, int ke, int i0, int j0, FA fa, FB fb) {
+    const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < k; k0 += 16) {
-        double a[4], b[4];
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        double a[8], b[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const int kk = k0 + 4 * u + lk;
-            const bool ok = kk < k;
-            a[u] = ok && ia < m ? (TA ? A[kk + (size_t)ia * lda] : A[ia + (size_t)kk * lda]) : 0.0;
-            b[u] = ok && jb < n ? (TB ? B[jb + (size_t)kk * ldb] : B[kk + (size_t)jb * ldb]) : 0.0;
+            a[u] = kk < ke ? fa(i0 + lr, kk) : 0.0;
+            b[u] = kk < ke ? fb(kk, j0 + lr) : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+        for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
     }
+    return acc;
+}
+
+// Sum of the four wavefronts' partial tiles, returned to wavefront 0 (others get garbage).
+__device__ __forceinline__ v4d reduce_waves(v4d acc, v4d (*part)[64]) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (w > 0) part[w - 1][l] = acc;
+    __syncthreads();
+    if (w == 0)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc += part[q][l];
+    return acc;
+}
+
+// k range of wavefront w of 4: contiguous strips of a multiple of 4.
+__device__ __forceinline__ void k_strip(int k, int w, int& kb, int& ke) {
+    const int q = ((k + 15) / 16) * 4;
+    kb = min(k, w * q);
+    ke = min(k, kb + q);
+}
+
+// One 16 x 16 tile of C per block of 4 wavefronts, which split the k range.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) dgemm16_kernel(int m, int n, int k, double alpha, const double* __restrict__ A,
+                                                      int lda, const double* __restrict__ B, int ldb, double beta,
+                                                      double* __restrict__ C, int ldc) {
+    __shared__ v4d part[3][64];
+    const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4, w = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+    int kb, ke;
+    k_strip(k, w, kb, ke);
+    auto fa = [&](int i, int kk) -> double {
+        return i < m ? (TA ? A[kk + (size_t)i * lda] : A[i + (size_t)kk * lda]) : 0.0;
+    };
+    auto fb = [&](int kk, int j) -> double {
+        return j < n ? (TB ? B[j + (size_t)kk * ldb] : B[kk + (size_t)j * ldb]) : 0.0;
+    };
+    v4d acc = reduce_waves(tile_product(kb, ke, i0, j0, fa, fb), part);
+    if (w != 0) return;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = i0 + lk + 4 * r, j = j0 + lr;
@@ -105,8 +142,12 @@ struct Sp2Args {
     double* part;
 };
 
-__global__ void __launch_bounds__(64) sp2_step_kernel(Sp2Args g) {
+__global__ void __launch_bounds__(256) sp2_step_kernel(Sp2Args g) {
+    __shared__ v4d part4[3][64];
     const int n = g.n, nt = g.nt, s = g.s;
+    const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4, w = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
     bool square = false;
     if (s > 0) {
         const double* pp = g.part + (size_t)2 * nt * (s - 1);
@@ -119,52 +160,43 @@ __global__ void __launch_bounds__(64) sp2_step_kernel(Sp2Args g) {
         square = fabs(t2 - g.nocc) <= fabs(2.0 * t1 - t2 - g.nocc);
         const bool stop_pp = s >= 2 && g.hist[kSp2Hist + 2 * (s - 2) + 1] != 0;
         if (stop_pp) {                                         // X_{s-1} already final
-            if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) g.hist[kSp2Hist + 2 * (s - 1) + 1] = 1.0;
+            if (lead) g.hist[kSp2Hist + 2 * (s - 1) + 1] = 1.0;
             return;
         }
         const double e3 = s >= 3 ? g.hist[kSp2Hist + 2 * (s - 3)] : 1e300;
         const bool stop_p = e < 1e-4 && e >= e3;
-        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (lead) {
             g.hist[kSp2Hist + 2 * (s - 1)] = e;
             g.hist[kSp2Hist + 2 * (s - 1) + 1] = stop_p ? 1.0 : 0.0;
             if (stop_p) g.hist[2] = s;
         }
         if (stop_p) {
             // X_s is the result: write this tile of it and stop
-            const int l = threadIdx.x, lr = l & 15, lk = l >> 4;
-            const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+            if (w == 0)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = i0 + lk + 4 * r, j = j0 + lr;
-                if (i < n && j < n) {
-                    const size_t o = i + (size_t)j * n;
-                    g.X[o] = square ? g.X2p[o] : 2.0 * g.Xp[o] - g.X2p[o];
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + lk + 4 * r, j = j0 + lr;
+                    if (i < n && j < n) {
+                        const size_t o = i + (size_t)j * n;
+                        g.X[o] = square ? g.X2p[o] : 2.0 * g.Xp[o] - g.X2p[o];
+                    }
                 }
-            }
             return;
         }
     }
     const double lo = g.hist[0], hi = g.hist[1], inv = 1.0 / (hi - lo);
-    auto xval = [&](int i, int j) -> double {                   // element (i, j) of X_s
+    auto xval = [&](int i, int j) -> double {                   // element (i, j) of X_s, zero outside
+        if (i >= n || j >= n) return 0.0;
         const size_t o = i + (size_t)j * n;
         if (s == 0) return ((i == j ? hi : 0.0) - g.F[o]) * inv;
         return square ? g.X2p[o] : 2.0 * g.Xp[o] - g.X2p[o];
     };
-    const int l = threadIdx.x, lr = l & 15, lk = l >> 4;
-    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
-    const int ia = i0 + lr, jb = j0 + lr;
-    v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < n; k0 += 16) {
-        double a[4], b[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int kk = k0 + 4 * u + lk;
-            a[u] = kk < n && ia < n ? xval(ia, kk) : 0.0;
-            b[u] = kk < n && jb < n ? xval(kk, jb) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
-    }
+    // X_s is symmetric: the B operand is read along a column, like A
+    auto xcol = [&](int kk, int j) -> double { return xval(j, kk); };
+    int kb, ke;
+    k_strip(n, w, kb, ke);
+    v4d acc = reduce_waves(tile_product(kb, ke, i0, j0, xval, xcol), part4);
+    if (w != 0) return;
     double d1 = 0, d2 = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -199,18 +231,20 @@ __global__ void __launch_bounds__(64) sp2_step_kernel(Sp2Args g) {
 int hf_dgemm_small(hf_ctx* ctx, hipStream_t st, bool ta, bool tb, int m, int n, int k, double alpha, const double* A,
                    int lda, const double* B, int ldb, double beta, double* C, int ldc) {
     const dim3 grid((m + 15) / 16, (n + 15) / 16);
-    if (ta && tb) hipLaunchKernelGGL((dgemm16_kernel<true, true>), grid, dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (ta) hipLaunchKernelGGL((dgemm16_kernel<true, false>), grid, dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (tb) hipLaunchKernelGGL((dgemm16_kernel<false, true>), grid, dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else hipLaunchKernelGGL((dgemm16_kernel<false, false>), grid, dim3(64), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (ta && tb) hipLaunchKernelGGL((dgemm16_kernel<true, true>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (ta) hipLaunchKernelGGL((dgemm16_kernel<true, false>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (tb) hipLaunchKernelGGL((dgemm16_kernel<false, true>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else hipLaunchKernelGGL((dgemm16_kernel<false, false>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     HF_HIP_CHECK(ctx, hipGetLastError());
     return HF_OK;
 }
 
 int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp, double* X0, double* X1, double* W0,
-                 double* W1, double* hist, double* part, int max_steps, double*& result, int& steps) {
+                 double* W1, double* hist, double* part, int max_steps, int hint, double*& result, int& steps) {
     const int nt = (n + 15) / 16;
-    constexpr int kFirstCheck = 16, kEvery = 4;
+    // the stop flag is read back from step kFirstCheck on (or from the step count of the previous
+    // call, hint, when given) and then every kEvery steps
+    const int kFirstCheck = hint > 0 ? std::max(8, hint) : 16, kEvery = hint > 0 ? 2 : 4;
     hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(256), 0, st, Fp, n, hist);
     Sp2Args g{};
     g.n = n;
@@ -229,7 +263,7 @@ int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp,
         g.X2p = Wb[(s + 1) & 1];
         g.X = Xb[s & 1];
         g.X2 = Wb[s & 1];
-        hipLaunchKernelGGL(sp2_step_kernel, dim3(nt, nt), dim3(64), 0, st, g);
+        hipLaunchKernelGGL(sp2_step_kernel, dim3(nt, nt), dim3(256), 0, st, g);
         if ((s >= kFirstCheck && (s - kFirstCheck) % kEvery == 0) || s == max_steps) {
             double fin = -1;
             HF_HIP_CHECK(ctx, hipMemcpyAsync(&fin, hist + 2, sizeof(double), hipMemcpyDeviceToHost, st));

@@ -159,9 +159,10 @@ int hf_dgemm_small(hf_ctx* ctx, hipStream_t st, bool ta, bool tb, int m, int n, 
 // SP2 purification of the n x n orthogonalised Fock matrix Fp to the projector onto its nocc lowest
 // eigenvectors, one fused kernel per step; X0, X1, W0, W1 are n x n work matrices, hist holds
 // 8 + 2 (max_steps + 1) doubles, part 2 ceil(n/16) (max_steps + 1).  result points at the projector
-// (one of X0, X1), or is null when the stopping rule was not met within max_steps.
+// (one of X0, X1), or is null when the stopping rule was not met within max_steps.  hint: the step
+// count of a previous, similar call (0: none), used only to decide when to poll the stop flag.
 int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp, double* X0, double* X1, double* W0,
-                 double* W1, double* hist, double* part, int max_steps, double*& result, int& steps);
+                 double* W1, double* hist, double* part, int max_steps, int hint, double*& result, int& steps);
 constexpr int kSmallDense = 384;   // the matrix-core kernels above serve n <= this
 
 // J/K of the densities d_D (ndens x n x n) on stream st, screened at Schwarz product thr (0: the

@@ -77,6 +77,23 @@ __global__ void sym_kernel(double* __restrict__ A, int n) {
     }
 }
 
+// A = c B, and I (identity) into E
+__global__ void scale_ident_kernel(const double* __restrict__ B, double c, double* __restrict__ A,
+                                   double* __restrict__ E, int n) {
+    const size_t nn = (size_t)n * n;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
+        A[idx] = c * B[idx];
+        E[idx] = idx / n == idx % n ? 1.0 : 0.0;
+    }
+}
+
+// T <- (3 I - T) / 2
+__global__ void ns_step_kernel(double* __restrict__ T, int n) {
+    const size_t nn = (size_t)n * n;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x)
+        T[idx] = 0.5 * ((idx / n == idx % n ? 3.0 : 0.0) - T[idx]);
+}
+
 // W[:, j] = U[:, j] / sqrt(s_j)   (column-major)
 __global__ void scale_cols_kernel(const double* __restrict__ U, const double* __restrict__ s, double* __restrict__ W,
                                   int n) {
@@ -379,6 +396,66 @@ int syevd(hf_ctx* ctx, ScfState& st, double* A, double* w) {
     return HF_OK;
 }
 
+// S^{-1/2} into st.X.  The reference diagonalises S (HF/HartreeFockAlgorithm.cpp:48-59); the
+// symmetric inverse square root is unique, and here it comes from the coupled Newton-Schulz
+// iteration (Higham, Numer. Algorithms 15, 227 (1997)) on Y0 = c S, c = 1/||S||_F (spectrum in
+// (0, 1]):  T = (3I - Z Y)/2,  Y <- Y T,  Z <- T Z;  Z -> (cS)^{-1/2}, so S^{-1/2} = sqrt(c) Z.
+// Three products per step (small matrices: matrix-core kernel); converged when ||I - Z Y||_F is
+// below 1e-14 sqrt(n) or, once below 1e-8, stops halving (the rounding floor of an ill-conditioned
+// S); if neither happens within 80 steps (S numerically singular), dsyevd.
+int inverse_sqrt(hf_ctx* ctx, ScfState& st) {
+    hipStream_t s = ctx->stream;
+    const int n = st.n;
+    const size_t nn = st.nn;
+    DotJobs jobs{};
+    jobs.m = 1;
+    jobs.x[0] = st.S.p;
+    jobs.y[0] = st.S.p;
+    hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, nn, st.red.p);
+    double fro2 = 0;
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(&fro2, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    const double c = 1.0 / std::sqrt(fro2);
+    double *Y = st.T1.p, *Z = st.X.p, *T = st.T2.p, *W = st.T3.p, *E = st.err.p;
+    hipLaunchKernelGGL(scale_ident_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.S.p, c, Y, Z, n);
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(E, Z, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+    bool ok = false;
+    double prev = 1e300;
+    for (int it = 1; it <= 80 && !ok; ++it) {
+        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, Z, Y, 0.0, T)) return rc;
+        if (it % 4 == 0 || it > 24) {
+            jobs.x[0] = T;
+            jobs.y[0] = E;
+            jobs.diff[0] = 1;
+            hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, nn, st.red.p);
+            double r2 = 0;
+            HF_HIP_CHECK(ctx, hipMemcpyAsync(&r2, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
+            HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+            const double r = std::sqrt(r2);
+            if (r < 1e-14 * std::sqrt((double)n) || (r < 1e-8 && r >= 0.5 * prev)) ok = true;
+            prev = r;
+        }
+        hipLaunchKernelGGL(ns_step_kernel, dim3(grid_for(nn)), dim3(256), 0, s, T, n);
+        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, Y, T, 0.0, W)) return rc;
+        std::swap(Y, W);
+        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, T, Z, 0.0, W)) return rc;
+        std::swap(Z, W);
+    }
+    if (ok) {
+        hipLaunchKernelGGL(scale_ident_kernel, dim3(grid_for(nn)), dim3(256), 0, s, Z, std::sqrt(c), st.C.p, T, n);
+        HF_HIP_CHECK(ctx, hipMemcpyAsync(st.X.p, st.C.p, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        // S^{-1/2} = U s^{-1/2} U^T
+        HF_HIP_CHECK(ctx, hipMemcpyAsync(st.T1.p, st.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (int rc = syevd(ctx, st, st.T1.p, st.eps.p)) return rc;
+        hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.eps.p, st.T2.p, n);
+        if (int rc = gemm(ctx, false, true, n, n, n, 1.0, st.T2.p, st.T1.p, 0.0, st.X.p)) return rc;
+    }
+    hipLaunchKernelGGL(sym_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.X.p, n);
+    HF_HIP_CHECK(ctx, hipGetLastError());
+    return HF_OK;
+}
+
 // Aufbau projector of the orthogonalised Fock matrix Fp (not modified) onto its nocc lowest
 // eigenvectors, by SP2 steps, into P; W is workspace.  ok = false when the stopping rule was not
 // met within kMaxSteps (a vanishing HOMO-LUMO gap leaves fractional eigenvalues).
@@ -520,12 +597,8 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     // S, T, V on the device; h = T + V
     if (int rc = hf_one_electron_device(ctx, st.S.p, st.T1.p, st.T2.p, s)) return rc;
     hipLaunchKernelGGL(add_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.T2.p, st.h.p, nn);
-    // S^{-1/2} = U s^{-1/2} U^T, symmetrised (:48-59)
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(st.T1.p, st.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-    if (int rc = syevd(ctx, st, st.T1.p, st.eps.p)) return rc;
-    hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.eps.p, st.T2.p, n);
-    if (int rc = gemm(ctx, false, true, n, n, n, 1.0, st.T2.p, st.T1.p, 0.0, st.X.p)) return rc;
-    hipLaunchKernelGGL(sym_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.X.p, n);
+    // S^{-1/2}, symmetrised (:48-59)
+    if (int rc = inverse_sqrt(ctx, st)) return rc;
     // occupations (HartreeFock/RestrictedHartreeFock.cpp:18-32, UnrestrictedHartreeFock.cpp:20-60)
     const int ne = ctx->mol.electrons();
     int occ[2];
@@ -558,7 +631,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     const double dtol = p->density_tolerance > 0 ? p->density_tolerance : 1e-10;
     const double itol = p->diis_tolerance > 0 ? p->diis_tolerance : 1e-10;
     double energy = 0, homo = 0, lastErr = 0, jk_ms = 0;
-    int diag_steps = 0;
+    int diag_steps = 0, sp2_steps[2] = {0, 0};
     bool last_diag[2] = {false, false};
     // incremental Fock build state: J, K hold the J/K of Dprev
     int full_builds = 0, incr_builds = 0, since_full = 0;
@@ -665,9 +738,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                     double* res = nullptr;
                     int steps = 0;
                     if (int rc = hf_sp2_fused(ctx, s, n, occ[k], st.T2.p, st.T1.p, st.T3.p, st.sp2W.p, st.sp2W.p + nn,
-                                              st.sp2.p, st.sp2part.p, kSp2MaxSteps, res, steps))
+                                              st.sp2.p, st.sp2part.p, kSp2MaxSteps, sp2_steps[k], res, steps))
                         return rc;
                     ok = res != nullptr;
+                    sp2_steps[k] = ok ? steps : 0;
                     P = res;
                 } else {
                     if (int rc = sp2_projector(ctx, st, st.T2.p, occ[k], st.T1.p, st.T3.p, ok)) return rc;
@@ -782,13 +856,13 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             prev = cur;
         }
     }
-    // orbital energies (HOMO) and coefficients of the last Fock matrices
+    // orbital energies (HOMO) of the last Fock matrices: eigenvalues only
     for (int k = 0; k < nsp; ++k) {
         if (last_diag[k]) continue;
         if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.Fa.p + k * nn, st.X.p, 0.0, st.T1.p)) return rc;
         if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
-        if (int rc = syevd(ctx, st, st.T2.p, st.eps.p + k * n)) return rc;
-        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T2.p, 0.0, st.C.p + k * nn)) return rc;
+        BL(rocsolver_dsyevd(ctx->blas, rocblas_evect_none, rocblas_fill_upper, n, st.T2.p, n, st.eps.p + k * n,
+                            st.work.p, st.info.p));
     }
     HF_HIP_CHECK(ctx, hipMemcpyAsync(eps_h.data(), st.eps.p, nsp * n * sizeof(double), hipMemcpyDeviceToHost, s));
     HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -808,17 +882,19 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     r->fp32_builds = fp32_builds;
     if (uhf) {
         // <S^2> = ((Na-Nb)/2)^2 + (Na+Nb)/2 - sum_ij ((C_a,occ)^T S C_b,occ)_ij^2
+        //       = ((Na-Nb)/2)^2 + (Na+Nb)/2 - Tr(P_a S P_b S), P = C_occ C_occ^T the last new densities
+        double ov = 0;
         if (occ[0] > 0 && occ[1] > 0) {
-            if (int rc = gemm(ctx, false, false, n, occ[1], n, 1.0, st.S.p, st.C.p + nn, 0.0, st.T1.p)) return rc;
-            if (int rc = gemm(ctx, true, false, occ[0], occ[1], n, 1.0, st.C.p, st.T1.p, 0.0, st.T2.p)) return rc;
-        }
-        std::vector<double> M((size_t)occ[0] * occ[1]);
-        if (!M.empty()) {
-            HF_HIP_CHECK(ctx, hipMemcpyAsync(M.data(), st.T2.p, M.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.Dn.p, st.S.p, 0.0, st.T1.p)) return rc;
+            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.S.p, st.Dn.p + nn, 0.0, st.T2.p)) return rc;
+            DotJobs jobs{};
+            jobs.m = 1;
+            jobs.x[0] = st.T1.p;
+            jobs.y[0] = st.T2.p;
+            hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, nn, st.red.p);
+            HF_HIP_CHECK(ctx, hipMemcpyAsync(&ov, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
             HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
         }
-        double ov = 0;
-        for (double x : M) ov += x * x;
         const double sz = 0.5 * (occ[0] - occ[1]);
         r->s2 = sz * sz + 0.5 * (occ[0] + occ[1]) - ov;
     }
