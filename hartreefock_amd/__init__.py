@@ -4,11 +4,11 @@ Python binding of the C ABI in include/hfgpu.h (libhfgpu.so, built in-tree by
 `python -m hartreefock_amd.build`).  There is no CPU fallback: every call runs the
 HIP library, and importing the binding raises if libhfgpu.so is missing.
 """
-from ._lib import (Engine, HFError, KernelStat, Options, ScfParams, ScfResult, lib, library_path,
-                   packed_index, packed_len, EXPORTED_SYMBOLS)
+from ._lib import (Engine, HFError, KernelStat, Options, ScfParams, ScfResult, Server, ServeResult, lib,
+                   library_path, packed_index, packed_len, serve_footprint, EXPORTED_SYMBOLS)
 
-__all__ = ["Engine", "HFError", "KernelStat", "Options", "ScfParams", "ScfResult", "lib", "library_path",
-           "packed_index", "packed_len", "EXPORTED_SYMBOLS", "basis_path"]
+__all__ = ["Engine", "HFError", "KernelStat", "Options", "ScfParams", "ScfResult", "Server", "ServeResult", "lib",
+           "library_path", "packed_index", "packed_len", "serve_footprint", "EXPORTED_SYMBOLS", "basis_path"]
 
 
 def basis_path(name):

@@ -13,6 +13,8 @@ EXPORTED_SYMBOLS = [
     "hf_nbf", "hf_nshell", "hf_electrons", "hf_nuclear_repulsion", "hf_packed_index", "hf_packed_len",
     "hf_eri_packed", "hf_eri_packed_device", "hf_build_jk", "hf_build_jk_device", "hf_one_electron",
     "hf_scf_defaults", "hf_scf", "hf_kernel_stats", "hf_kernel_stats_reset",
+    "hf_serve_default_options", "hf_serve_create", "hf_serve_submit", "hf_serve_poll", "hf_serve_pending",
+    "hf_serve_footprint", "hf_serve_destroy",
 ]
 
 
@@ -54,6 +56,17 @@ class ScfResult(ctypes.Structure):
                 ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("jk_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("fp32_builds", ctypes.c_int), ("diag_steps", ctypes.c_int),
                 ("full_builds", ctypes.c_int), ("incremental_builds", ctypes.c_int), ("quartets", ctypes.c_double)]
+
+
+class ServeOptions(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_double), ("ctx", Options), ("scf", ScfParams)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("status", ctypes.c_int), ("nbf", ctypes.c_int),
+                ("energy", ctypes.c_double), ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("arrival_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("footprint", ctypes.c_double)]
 
 
 class KernelStat(ctypes.Structure):
@@ -108,6 +121,16 @@ def lib():
     L.hf_scf.argtypes = [vp, ctypes.POINTER(ScfParams), ctypes.POINTER(ScfResult), dp]
     L.hf_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int]
     L.hf_kernel_stats_reset.argtypes = [vp]
+    L.hf_serve_default_options.argtypes = [ctypes.POINTER(ServeOptions)]
+    L.hf_serve_default_options.restype = None
+    L.hf_serve_create.argtypes = [ctypes.POINTER(ServeOptions), ctypes.POINTER(vp)]
+    L.hf_serve_submit.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ip, dp, ctypes.c_int, ctypes.c_uint64,
+                                  ctypes.POINTER(ctypes.c_uint64)]
+    L.hf_serve_poll.argtypes = [vp, ctypes.POINTER(ServeResult), ctypes.c_int, ctypes.c_int]
+    L.hf_serve_pending.argtypes = [vp]
+    L.hf_serve_footprint.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.hf_serve_footprint.restype = ctypes.c_double
+    L.hf_serve_destroy.argtypes = [vp]
     _LIB = L
     return L
 
@@ -234,3 +257,65 @@ class Engine:
 
     def reset_kernel_stats(self):
         self._check(lib().hf_kernel_stats_reset(self.h))
+
+
+class Server:
+    """Continuous-batching SCF server on one GPU (hf_serve_* in include/hfgpu.h).
+
+    Server(slots=4, hbm_budget=0, options={...context options...}, scf={...SCF parameters...})
+    submit(basis_file, atoms, unrestricted=False, tag=0) -> id;  poll(max, wait_ms) -> [ServeResult]
+    """
+
+    def __init__(self, slots=4, hbm_budget=0.0, options=None, scf=None):
+        L = lib()
+        o = ServeOptions()
+        L.hf_serve_default_options(ctypes.byref(o))
+        o.slots = slots
+        o.hbm_budget = hbm_budget
+        for k, v in (options or {}).items():
+            if not hasattr(o.ctx, k):
+                raise TypeError("unknown option %s" % k)
+            setattr(o.ctx, k, v)
+        for k, v in (scf or {}).items():
+            if not hasattr(o.scf, k):
+                raise TypeError("unknown SCF parameter %s" % k)
+            setattr(o.scf, k, v)
+        h = ctypes.c_void_p()
+        rc = L.hf_serve_create(ctypes.byref(o), ctypes.byref(h))
+        if rc != 0:
+            raise HFError(rc, "hf_serve_create failed (is a GPU visible?)")
+        self.h = h
+
+    def submit(self, basis_file, atoms, unrestricted=False, tag=0):
+        z = np.ascontiguousarray([a[0] for a in atoms], dtype=np.int32)
+        xyz = np.ascontiguousarray([c for a in atoms for c in a[1]], dtype=np.float64)
+        rid = ctypes.c_uint64()
+        rc = lib().hf_serve_submit(self.h, basis_file.encode(), len(atoms), _ip(z), _dp(xyz), 1 if unrestricted else 0,
+                                   tag, ctypes.byref(rid))
+        if rc != 0:
+            raise HFError(rc, "hf_serve_submit failed")
+        return rid.value
+
+    def poll(self, max_results=64, wait_ms=100):
+        arr = (ServeResult * max_results)()
+        k = lib().hf_serve_poll(self.h, arr, max_results, wait_ms)
+        return [arr[i] for i in range(k)]
+
+    def pending(self):
+        return lib().hf_serve_pending(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hf_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_footprint(nbf, unrestricted=False):
+    """Device bytes the server's admission rule assumes for one SCF on nbf basis functions."""
+    return lib().hf_serve_footprint(nbf, 1 if unrestricted else 0)

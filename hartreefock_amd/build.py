@@ -66,7 +66,7 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
         units.append((src, True, [os.path.join(CSRC, "hf_device.h")]))
     for name in ("context.hip", "scf.hip", "one_electron.hip", "dense.hip"):
         units.append((os.path.join(CSRC, name), True, hdeps))
-    for name in ("basis.cpp", "boys.cpp"):
+    for name in ("basis.cpp", "boys.cpp", "serve.cpp"):
         units.append((os.path.join(CSRC, name), False, hdeps))
     objs = []
     # biggest kernels first so the parallel build ends sooner
@@ -83,7 +83,7 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
                 print("  %-40s %6.1fs" % (os.path.basename(src), dt), flush=True)
     if force or _newer(objs, LIB):
         cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + \
-              ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-Wl,-rpath,/opt/rocm/lib"]
+              ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-pthread", "-Wl,-rpath,/opt/rocm/lib"]
         p = subprocess.run(cmd, capture_output=True, text=True)
         if p.returncode != 0:
             raise RuntimeError("link failed: %s\n%s" % (p.stdout, p.stderr))

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 
 #include "hf_internal.h"
@@ -120,10 +121,24 @@ double radial_norm(double alpha, int L) {
     return std::pow(2. * alpha / M_PI, 3. / 4.) * std::pow(4. * alpha, L / 2.);
 }
 
+// Parsed basis files, by path: parsed once per process (the serving loop loads the same few files
+// for every request).  Guarded for concurrent contexts on different host threads.
+static const std::map<int, std::vector<FileShell>>* cached_basis(const char* path, std::string& err) {
+    static std::mutex mu;
+    static std::map<std::string, std::map<int, std::vector<FileShell>>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(path);
+    if (it != cache.end()) return &it->second;
+    std::map<int, std::vector<FileShell>> atoms;
+    if (!parse_file(path, atoms, err)) return nullptr;
+    return &cache.emplace(path, std::move(atoms)).first->second;
+}
+
 bool load_molecule(const char* basis_file, int natoms, const int* Z, const double* xyz, Molecule& mol,
                    std::string& err) {
-    std::map<int, std::vector<FileShell>> atoms;
-    if (!parse_file(basis_file, atoms, err)) return false;
+    const std::map<int, std::vector<FileShell>>* cached = cached_basis(basis_file, err);
+    if (!cached) return false;
+    const std::map<int, std::vector<FileShell>>& atoms = *cached;
     mol = Molecule();
     int bf = 0;
     for (int a = 0; a < natoms; ++a) {

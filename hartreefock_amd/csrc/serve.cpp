@@ -1,0 +1,206 @@
+// Serving loop: continuous batching of SCF requests on one GPU (declared in include/hfgpu.h).
+//
+// `slots` worker threads each own a context (stream, side streams, device memory).  Requests wait
+// in a FIFO queue; the head request is admitted by the first free worker once its estimated HBM
+// footprint fits in the budget together with the running requests (no overtaking, so a large
+// request is not starved).  Admission happens whenever a worker or memory frees up, so requests
+// that arrive while others run join them, and the class kernels of all running requests overlap
+// on the device through their contexts' streams.  Results go to a completion queue read by
+// hf_serve_poll.  All times are steady-clock milliseconds since the server was created.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hf_internal.h"
+
+namespace {
+
+struct Request {
+    uint64_t id = 0, tag = 0;
+    std::string basis;
+    std::vector<int> Z;
+    std::vector<double> xyz;
+    int unrestricted = 0;
+    int nbf = 0;
+    double footprint = 0;
+    double arrival_ms = 0;
+};
+
+}  // namespace
+
+struct hf_server {
+    hf_serve_options opt;
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request> queue;
+    std::deque<hf_serve_result> done;
+    std::vector<std::thread> workers;
+    std::vector<hf_ctx*> ctxs;
+    double in_use = 0, budget = 0;
+    uint64_t next_id = 1;
+    int pending = 0;
+    bool stop = false;
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void run(int w) {
+        hf_ctx* ctx = ctxs[w];
+        for (;;) {
+            Request rq;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                cv_work.wait(lock, [&] {
+                    return stop || (!queue.empty() && (in_use == 0 || in_use + queue.front().footprint <= budget));
+                });
+                if (stop) return;
+                rq = std::move(queue.front());
+                queue.pop_front();
+                in_use += rq.footprint;
+            }
+            hf_serve_result r{};
+            r.id = rq.id;
+            r.tag = rq.tag;
+            r.nbf = rq.nbf;
+            r.footprint = rq.footprint;
+            r.arrival_ms = rq.arrival_ms;
+            r.admit_ms = now_ms();
+            int rc = hf_set_molecule(ctx, rq.basis.c_str(), (int)rq.Z.size(), rq.Z.data(), rq.xyz.data());
+            if (rc == HF_OK) {
+                hf_scf_params p = opt.scf;
+                p.unrestricted = rq.unrestricted;
+                p.terminate = nullptr;
+                hf_scf_result sr{};
+                rc = hf_scf(ctx, &p, &sr, nullptr);
+                r.energy = sr.energy;
+                r.iterations = sr.iterations;
+                r.converged = sr.converged;
+            }
+            r.status = rc;
+            r.done_ms = now_ms();
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                in_use -= rq.footprint;
+                done.push_back(r);
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+void hf_serve_default_options(hf_serve_options* o) {
+    o->slots = 4;
+    o->hbm_budget = 0;
+    hf_default_options(&o->ctx);
+    hf_scf_defaults(&o->scf);
+}
+
+double hf_serve_footprint(int nbf, int unrestricted) {
+    // SCF state on the device (S, h, S^-1/2, Fock, densities, DIIS ring of 6 per spin, SP2 and J/K
+    // work matrices: about 56 n x n FP64 matrices for RHF, 64 for UHF) plus pair and task data.
+    const double n2 = (double)nbf * nbf;
+    return 8.0 * n2 * (unrestricted ? 64 : 56) + 32.0 * (1 << 20);
+}
+
+int hf_serve_create(const hf_serve_options* o, hf_server** out) {
+    if (!o || !out || o->slots < 1) return HF_E_ARG;
+    *out = nullptr;
+    hf_server* s = new hf_server();
+    s->opt = *o;
+    s->t0 = std::chrono::steady_clock::now();
+    s->budget = o->hbm_budget;
+    if (s->budget <= 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipSetDevice(o->ctx.device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            delete s;
+            return HF_E_HIP;
+        }
+        s->budget = 0.8 * (double)free_b;
+    }
+    for (int w = 0; w < o->slots; ++w) {
+        hf_ctx* c = nullptr;
+        const int rc = hf_create(&o->ctx, &c);
+        if (rc != HF_OK) {
+            for (hf_ctx* x : s->ctxs) hf_destroy(x);
+            delete s;
+            return rc;
+        }
+        s->ctxs.push_back(c);
+    }
+    for (int w = 0; w < o->slots; ++w) s->workers.emplace_back([s, w] { s->run(w); });
+    *out = s;
+    return HF_OK;
+}
+
+int hf_serve_submit(hf_server* s, const char* basis_file, int natoms, const int* Z, const double* xyz,
+                    int unrestricted, uint64_t tag, uint64_t* id) {
+    if (!s || !basis_file || natoms <= 0 || !Z || !xyz) return HF_E_ARG;
+    Request rq;
+    rq.basis = basis_file;
+    rq.Z.assign(Z, Z + natoms);
+    rq.xyz.assign(xyz, xyz + 3 * (size_t)natoms);
+    rq.unrestricted = unrestricted ? 1 : 0;
+    rq.tag = tag;
+    hfhost::Molecule m;
+    std::string err;
+    if (!hfhost::load_molecule(basis_file, natoms, Z, xyz, m, err)) return HF_E_IO;
+    rq.nbf = m.nbf;
+    rq.footprint = hf_serve_footprint(m.nbf, rq.unrestricted);
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        rq.id = s->next_id++;
+        rq.arrival_ms = s->now_ms();
+        if (id) *id = rq.id;
+        s->queue.push_back(std::move(rq));
+        ++s->pending;
+    }
+    s->cv_work.notify_one();
+    return HF_OK;
+}
+
+int hf_serve_poll(hf_server* s, hf_serve_result* out, int max, int wait_ms) {
+    if (!s || !out || max <= 0) return 0;
+    std::unique_lock<std::mutex> lock(s->mu);
+    if (s->done.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lock, std::chrono::milliseconds(wait_ms), [&] { return !s->done.empty(); });
+    int k = 0;
+    while (k < max && !s->done.empty()) {
+        out[k++] = s->done.front();
+        s->done.pop_front();
+        --s->pending;
+    }
+    return k;
+}
+
+int hf_serve_pending(hf_server* s) {
+    if (!s) return 0;
+    std::lock_guard<std::mutex> lock(s->mu);
+    return s->pending;
+}
+
+int hf_serve_destroy(hf_server* s) {
+    if (!s) return HF_E_ARG;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+        s->queue.clear();
+    }
+    s->cv_work.notify_all();
+    for (std::thread& t : s->workers) t.join();
+    for (hf_ctx* c : s->ctxs) hf_destroy(c);
+    delete s;
+    return HF_OK;
+}
+
+}  // extern "C"

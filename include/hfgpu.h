@@ -177,6 +177,51 @@ typedef struct hf_kernel_stat {
 int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max);   /* returns count */
 int hf_kernel_stats_reset(hf_ctx* ctx);
 
+/* ---- Serving: continuous batching of SCF requests on one GPU ----
+ * No counterpart in the reference's library (its GUI runs one SCF per std::thread,
+ * HartreeFock/HartreeFockDoc.cpp:379-401).  A server owns `slots` worker threads, each with its own
+ * context (stream, device memory, SCF state kept on the device between requests).  A request is
+ * admitted to a free slot, first come first served, when its estimated HBM footprint fits in
+ * hbm_budget together with the requests already running; requests that arrive while others run
+ * are admitted as slots and memory free up, and the kernels of concurrent requests overlap on the
+ * device.  Each result is the same as the molecule's SCF run alone (to the atomics' rounding). */
+typedef struct hf_server hf_server;
+
+typedef struct hf_serve_options {
+    int slots;                     /* concurrent requests (worker threads / contexts), default 4 */
+    double hbm_budget;             /* bytes for admitted requests; 0 = 80% of the device's free memory */
+    hf_options ctx;                /* context options of every slot (precision, thresholds, ...) */
+    hf_scf_params scf;             /* SCF parameters of every request (unrestricted is per request) */
+} hf_serve_options;
+
+typedef struct hf_serve_result {
+    uint64_t id;                   /* from hf_serve_submit */
+    uint64_t tag;                  /* caller's tag */
+    int status;                    /* HF_OK, HF_E_NOT_CONVERGED or another error */
+    int nbf;
+    double energy;
+    int iterations;
+    int converged;
+    double arrival_ms;             /* submit time, ms since the server was created */
+    double admit_ms;               /* start of its work on a slot */
+    double done_ms;                /* completion; latency = done_ms - arrival_ms */
+    double footprint;              /* estimated device bytes while it ran */
+} hf_serve_result;
+
+void hf_serve_default_options(hf_serve_options* o);
+int hf_serve_create(const hf_serve_options* o, hf_server** out);
+/* Queue one molecule (atoms in bohr); never blocks on the GPU. */
+int hf_serve_submit(hf_server* s, const char* basis_file, int natoms, const int* Z, const double* xyz,
+                    int unrestricted, uint64_t tag, uint64_t* id);
+/* Up to max finished results, waiting up to wait_ms for the first one; returns the count. */
+int hf_serve_poll(hf_server* s, hf_serve_result* out, int max, int wait_ms);
+/* Requests submitted and not yet returned by hf_serve_poll. */
+int hf_serve_pending(hf_server* s);
+/* Estimated device bytes of an SCF on n basis functions (the admission rule). */
+double hf_serve_footprint(int nbf, int unrestricted);
+/* Waits for the running requests, drops the queued ones, frees everything. */
+int hf_serve_destroy(hf_server* s);
+
 #ifdef __cplusplus
 }
 #endif

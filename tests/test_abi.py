@@ -31,6 +31,10 @@ def test_pure_helpers_without_gpu():
     assert hf.packed_index(3, 2, 1, 0) == hf.packed_index(0, 1, 2, 3)
     n = 7
     assert hf.packed_len(n) == hf.packed_index(n, n, n, n) + 1
+    # serving admission estimate: 56 (RHF) / 64 (UHF) n x n FP64 matrices + 32 MiB of pair and task data
+    assert hf.serve_footprint(200) == 8.0 * 200 * 200 * 56 + 32 * 2 ** 20
+    assert hf.serve_footprint(200, unrestricted=True) == 8.0 * 200 * 200 * 64 + 32 * 2 ** 20
+    assert hf.serve_footprint(800) > hf.serve_footprint(200)
 
 
 def test_options_defaults():

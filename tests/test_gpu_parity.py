@@ -162,3 +162,67 @@ def test_mixed_scf_energy_matches_fp64(gpu_engine_factory, name, bfile, atoms):
     assert r64.converged and r32.converged
     assert r32.fp32_builds > 0
     assert abs(r32.energy - r64.energy) <= 1e-6
+
+
+SERVE_MOLS = [("water1", g.water_cluster(1, seed=11)), ("ethane", g.alkane_zigzag(2, seed=12)),
+              ("water3", g.water_cluster(3, seed=13)), ("butane", g.alkane_zigzag(4, seed=14)),
+              ("water2", g.water_cluster(2, seed=15)), ("methane", g.alkane_zigzag(1, seed=16))]
+SERVE_SCF = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0, energy_tolerance=1e-8,
+                 density_tolerance=1e-6, diis_tolerance=1e-6)
+
+
+def _serve_all(server, bfile, mols):
+    ids = {}
+    for i, (name, atoms) in enumerate(mols):
+        ids[server.submit(bfile, atoms, tag=i)] = name
+    out = {}
+    while len(out) < len(mols):
+        for r in server.poll(16, 2000):
+            out[ids[r.id]] = r
+    return out
+
+
+def test_serving_results_match_alone(gpu_engine_factory):
+    """Continuous batching: six molecules submitted at once to three slots run concurrently; each
+    energy equals the molecule's SCF run alone (1e-9 Eh: only the order of FP64 atomics differs)."""
+    import hartreefock_amd as hf
+    bfile = basis("6-31g.1.nw")
+    alone = {}
+    for name, atoms in SERVE_MOLS:
+        _print_case(name)
+        alone[name] = gpu_engine_factory().set_molecule(bfile, atoms).scf(**SERVE_SCF).energy
+    srv = hf.Server(slots=3, scf=SERVE_SCF)
+    try:
+        got = _serve_all(srv, bfile, SERVE_MOLS)
+    finally:
+        srv.close()
+    overlap = 0
+    for name, _ in SERVE_MOLS:
+        r = got[name]
+        print("%-8s alone %.10f served %.10f  arrive %.1f admit %.1f done %.1f" % (
+            name, alone[name], r.energy, r.arrival_ms, r.admit_ms, r.done_ms), flush=True)
+        assert r.status == 0 and r.converged
+        assert abs(r.energy - alone[name]) <= 1e-9
+        overlap += sum(1 for o in got.values() if o is not r and o.admit_ms < r.done_ms and r.admit_ms < o.done_ms)
+    assert overlap > 0            # requests did run concurrently
+
+
+def test_serving_admission_by_footprint(gpu_engine_factory):
+    """With an HBM budget that holds one request at a time, requests run one after another (no two
+    admitted intervals overlap), first come first served, and the results are unchanged."""
+    import hartreefock_amd as hf
+    bfile = basis("6-31g.1.nw")
+    mols = SERVE_MOLS[:4]
+    budget = max(hf.serve_footprint(gpu_engine_factory().set_molecule(bfile, a).nbf) for _, a in mols) * 1.5
+    srv = hf.Server(slots=3, hbm_budget=budget, scf=SERVE_SCF)
+    try:
+        got = _serve_all(srv, bfile, mols)
+    finally:
+        srv.close()
+    rs = sorted(got.values(), key=lambda r: r.admit_ms)
+    for a, b in zip(rs, rs[1:]):
+        assert b.admit_ms >= a.done_ms - 1e-6
+        assert b.id > a.id
+    for name, atoms in mols:
+        e = gpu_engine_factory().set_molecule(bfile, atoms).scf(**SERVE_SCF).energy
+        assert abs(got[name].energy - e) <= 1e-9
