@@ -132,16 +132,25 @@ int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& 
         ctx->ev.assign(2 * nslot, nullptr);
         for (auto& e : ctx->ev) HF_HIP_CHECK(ctx, hipEventCreate(&e));
     }
+    // Small builds (few quartets: small molecules, served several at a time) go in order on st: the
+    // fork/join across side streams would cost more than the overlap gains.
+    int64_t total_tasks = 0;
+    for (const auto& job : jobs)
+        for (const QuartetClassTasks& qc : job.second->classes) total_tasks += qc.ntasks;
+    constexpr int64_t kConcurrentMinTasks = 1 << 18;
+    const bool fork = total_tasks >= kConcurrentMinTasks;
     // fork: every side stream waits for the work already queued on st
     constexpr int kSide = 4;
-    if (ctx->side.empty()) {
+    if (fork && ctx->side.empty()) {
         ctx->side.assign(kSide, nullptr);
         for (auto& s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         ctx->side_ev.assign(kSide + 1, nullptr);
         for (auto& e : ctx->side_ev) HF_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[0], st));
-    for (hipStream_t s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamWaitEvent(s2, ctx->side_ev[0], 0));
+    if (fork) {
+        HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[0], st));
+        for (hipStream_t s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamWaitEvent(s2, ctx->side_ev[0], 0));
+    }
     // Every class launch runs concurrently with the others (the high-L classes use the
     // one-wavefront-per-quartet LDS kernel, so no launch needs much scratch): largest estimated
     // work first, each to the least-loaded side stream.
@@ -167,7 +176,7 @@ int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& 
         const ClassInfo& ci = kClasses[qc.cls];
         const int si = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         load[si] += L.cost;
-        hipStream_t ks = ctx->side[si];
+        hipStream_t ks = fork ? ctx->side[si] : st;
         const TaskList tl = task_list(ctx, *L.ts, qc);
         const int slot = stat_slot(L.mode, qc.cls);
         if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * slot], ks));
@@ -178,7 +187,7 @@ int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& 
         if (record) add_stats(ctx, L.mode, qc, sink);
     }
     // join: st continues after every side stream's work
-    for (int k = 0; k < kSide; ++k) {
+    for (int k = 0; fork && k < kSide; ++k) {
         HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[k + 1], ctx->side[k]));
         HF_HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->side_ev[k + 1], 0));
     }

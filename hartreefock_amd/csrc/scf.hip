@@ -638,6 +638,18 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     bool force_full = false, last_incremental = false;
     double quartets = 0, last_dmax = 0, last_quartets = 0;
     const bool incremental = p->incremental_fock != 0;
+    // J/K build time from events on the SCF stream, read at the step's own synchronisation
+    struct EventPair {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~EventPair() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } jk_events;
+    HF_HIP_CHECK(ctx, hipEventCreate(&jk_events.e[0]));
+    HF_HIP_CHECK(ctx, hipEventCreate(&jk_events.e[1]));
+    hipEvent_t* jk_ev = jk_events.e;
+    bool jk_pending = false;
     const int rebuild_every = p->full_rebuild_every > 0 ? p->full_rebuild_every : 8;
     // mixed-precision tier: quartets below the context's fp32_threshold in FP32 until fp64_phase
     const bool mixed = ctx->opt.precision == HF_PREC_MIXED;
@@ -659,7 +671,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             for (int k = 0; k < nsp; ++k)
                 HF_HIP_CHECK(ctx, hipMemcpyAsync(F[k], st.F0.p + k * nn, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
         } else {
-            const auto t0 = std::chrono::steady_clock::now();
+            HF_HIP_CHECK(ctx, hipEventRecord(jk_ev[0], s));
             const double q0 = quartet_count();
             const bool full = !incremental || force_full || full_builds == 0 || since_full >= rebuild_every;
             if (full) {
@@ -693,10 +705,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             }
             if (incremental)
                 HF_HIP_CHECK(ctx, hipMemcpyAsync(st.Dprev.p, st.D.p, nsp * nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+            HF_HIP_CHECK(ctx, hipEventRecord(jk_ev[1], s));
+            jk_pending = true;
             last_quartets = quartet_count() - q0;
             quartets += last_quartets;
-            jk_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             for (int k = 0; k < nsp; ++k)
                 hipLaunchKernelGGL(fock_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.h.p, st.J.p, st.K.p + k * nn,
                                    uhf ? 1.0 : 0.5, F[k], n, uhf ? 0 : 1);
@@ -775,6 +787,12 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         double red[8];
         HF_HIP_CHECK(ctx, hipMemcpyAsync(red, st.red.p, jobs.m * sizeof(double), hipMemcpyDeviceToHost, s));
         HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        if (jk_pending) {
+            float ms = 0;
+            HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, jk_ev[0], jk_ev[1]));
+            jk_ms += ms;
+            jk_pending = false;
+        }
         double e = 0, rms2 = 0;
         for (int k = 0; k < nsp; ++k) {
             e += 0.5 * (red[3 * k] + red[3 * k + 1]);
