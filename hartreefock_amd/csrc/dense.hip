@@ -226,6 +226,112 @@ __global__ void __launch_bounds__(256) sp2_step_kernel(Sp2Args g) {
     }
 }
 
+// SP2 for n <= kSp2LocalMax in one workgroup: X and X^2 stay in LDS for every step, so the whole
+// purification is one launch with no host round trip (small molecules, served many at a time).
+// Same steps and stopping rule as sp2_step_kernel; 16 wavefronts share the 16 x 16 tiles of each
+// product on the matrix cores.  out[0] = final step (or -1 if the rule was not met), then X.
+constexpr int kSp2LocalMax = 96;
+
+__global__ void __launch_bounds__(1024) sp2_local_kernel(const double* __restrict__ Fp, int n, double nocc,
+                                                          int max_steps, double* __restrict__ out,
+                                                          double* __restrict__ Xout) {
+    extern __shared__ double lds[];
+    double* X = lds;              // n x n, column-major (symmetric)
+    double* X2 = lds + n * n;
+    __shared__ double red[2][32];
+    __shared__ double e_hist[3];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int l = tid & 63, lr = l & 15, lk = l >> 4, w = tid >> 6, nw = nthr >> 6;
+    // Gershgorin bounds
+    double lo = 1e300, hi = -1e300;
+    for (int j = tid; j < n; j += nthr) {
+        double r = 0;
+        for (int i = 0; i < n; ++i)
+            if (i != j) r += fabs(Fp[(size_t)i * n + j]);
+        const double c = Fp[(size_t)j * n + j];
+        lo = fmin(lo, c - r);
+        hi = fmax(hi, c + r);
+    }
+    for (int msk = 32; msk >= 1; msk >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, msk));
+        hi = fmax(hi, __shfl_xor(hi, msk));
+    }
+    if (l == 0) {
+        red[0][w] = lo;
+        red[1][w] = hi;
+    }
+    __syncthreads();
+    lo = red[0][0];
+    hi = red[1][0];
+    for (int q = 1; q < nw; ++q) {
+        lo = fmin(lo, red[0][q]);
+        hi = fmax(hi, red[1][q]);
+    }
+    if (hi - lo < 1e-12) hi = lo + 1.0;
+    const double inv = 1.0 / (hi - lo);
+    for (int o = tid; o < n * n; o += nthr) X[o] = ((o / n == o % n ? hi : 0.0) - Fp[o]) * inv;
+    if (tid == 0) e_hist[0] = e_hist[1] = e_hist[2] = 1e300;   // e_{s-1}, e_{s-2}
+    __syncthreads();
+    const int nt = (n + 15) / 16;
+    int final_step = -1;
+    for (int s = 0; s <= max_steps; ++s) {
+        // X2 = X X, one 16 x 16 tile per wavefront in turn
+        for (int tile = w; tile < nt * nt; tile += nw) {
+            const int i0 = (tile % nt) * 16, j0 = (tile / nt) * 16;
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < n; k0 += 4) {
+                const int kk = k0 + lk, ia = i0 + lr, jb = j0 + lr;
+                const double a = kk < n && ia < n ? X[ia + kk * n] : 0.0;
+                const double b = kk < n && jb < n ? X[jb + kk * n] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + lk + 4 * r, j = j0 + lr;
+                if (i < n && j < n) X2[i + j * n] = acc[r];
+            }
+        }
+        __syncthreads();
+        // traces, in a fixed order
+        double t1 = 0, t2 = 0;
+        for (int i = tid; i < n; i += nthr) {
+            t1 += X[i + i * n];
+            t2 += X2[i + i * n];
+        }
+        for (int msk = 32; msk >= 1; msk >>= 1) {
+            t1 += __shfl_xor(t1, msk);
+            t2 += __shfl_xor(t2, msk);
+        }
+        if (l == 0) {
+            red[0][w] = t1;
+            red[1][w] = t2;
+        }
+        __syncthreads();
+        t1 = 0;
+        t2 = 0;
+        for (int q = 0; q < nw; ++q) {
+            t1 += red[0][q];
+            t2 += red[1][q];
+        }
+        const double e = t1 - t2;
+        const bool square = fabs(t2 - nocc) <= fabs(2.0 * t1 - t2 - nocc);
+        const bool stop = e < 1e-4 && e >= e_hist[1];
+        for (int o = tid; o < n * n; o += nthr) X[o] = square ? X2[o] : 2.0 * X[o] - X2[o];
+        __syncthreads();
+        if (tid == 0) {
+            e_hist[1] = e_hist[0];
+            e_hist[0] = e;
+        }
+        __syncthreads();
+        if (stop) {
+            final_step = s + 1;
+            break;
+        }
+    }
+    for (int o = tid; o < n * n; o += nthr) Xout[o] = X[o];
+    if (tid == 0) out[0] = final_step;
+}
+
 }  // namespace
 
 int hf_dgemm_small(hf_ctx* ctx, hipStream_t st, bool ta, bool tb, int m, int n, int k, double alpha, const double* A,
@@ -242,6 +348,18 @@ int hf_dgemm_small(hf_ctx* ctx, hipStream_t st, bool ta, bool tb, int m, int n, 
 int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp, double* X0, double* X1, double* W0,
                  double* W1, double* hist, double* part, int max_steps, int hint, double*& result, int& steps) {
     const int nt = (n + 15) / 16;
+    const size_t lds = 2 * (size_t)n * n * sizeof(double);
+    if (n <= kSp2LocalMax &&
+        hipFuncSetAttribute((const void*)sp2_local_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+            hipSuccess) {
+        hipLaunchKernelGGL(sp2_local_kernel, dim3(1), dim3(1024), lds, st, Fp, n, (double)nocc, max_steps, hist + 2, X0);
+        double fin = -1;
+        HF_HIP_CHECK(ctx, hipMemcpyAsync(&fin, hist + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+        HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        result = fin >= 0 ? X0 : nullptr;
+        steps = (int)fin;
+        return HF_OK;
+    }
     // the stop flag is read back from step kFirstCheck on (or from the step count of the previous
     // call, hint, when given) and then every kEvery steps
     const int kFirstCheck = hint > 0 ? std::max(8, hint) : 16, kEvery = hint > 0 ? 2 : 4;
