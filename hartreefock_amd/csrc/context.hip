@@ -55,6 +55,18 @@ int upload_pairs(hf_ctx* ctx, PairClassHost& pc) {
     HF_HIP_CHECK(ctx, hipMemcpy(pc.d_prim0.p, pc.prim0.data(), np * sizeof(int), hipMemcpyHostToDevice));
     HF_HIP_CHECK(ctx, hipMemcpy(pc.d_nprim.p, pc.nprim.data(), np * sizeof(int), hipMemcpyHostToDevice));
     HF_HIP_CHECK(ctx, hipMemcpy(pc.d_Q.p, pc.Q.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<PairRec> rec(np);
+    for (size_t i = 0; i < np; ++i) {
+        rec[i].sa = pc.sa[i];
+        rec[i].sb = pc.sb[i];
+        rec[i].prim0 = pc.prim0[i];
+        rec[i].nprim = pc.nprim[i];
+        rec[i].a0 = ctx->mol.shells[pc.sa[i]].bf;
+        rec[i].b0 = ctx->mol.shells[pc.sb[i]].bf;
+        rec[i].pad0 = rec[i].pad1 = 0;
+    }
+    HF_HIP_CHECK(ctx, pc.d_rec.alloc(np));
+    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_rec.p, rec.data(), np * sizeof(PairRec), hipMemcpyHostToDevice));
     if (!pc.group.empty()) {
         HF_HIP_CHECK(ctx, pc.d_group.alloc(pc.group.size()));
         HF_HIP_CHECK(ctx, hipMemcpy(pc.d_group.p, pc.group.data(), pc.group.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -64,6 +76,7 @@ int upload_pairs(hf_ctx* ctx, PairClassHost& pc) {
 
 PairClassDev dev_view(const PairClassHost& pc) {
     PairClassDev d;
+    d.rec = pc.d_rec.p;
     d.sa = pc.d_sa.p;
     d.sb = pc.d_sb.p;
     d.prim0 = pc.d_prim0.p;
@@ -101,6 +114,7 @@ TaskList task_list(hf_ctx* ctx, const TaskSet& ts, const QuartetClassTasks& qc) 
     TaskList tl{};
     tl.bra = dev_view(ctx->pcls[qc.bra]);
     tl.ket = dev_view(ctx->pcls[qc.ket]);
+    tl.tasks = ts.d_tasks.p + qc.tasks_off;
     tl.prefix = ts.d_prefix.p;
     tl.koff = ts.d_koff.p;
     tl.kbeg = ts.d_kbeg.p;
@@ -380,6 +394,21 @@ int mixed_tasks(hf_ctx* ctx, double thr, double t32, TaskSet*& hi, TaskSet*& lo)
 
 }  // namespace
 
+// Expanded task list of one class: entry b * 256 + i is task i of launch block b, (-1, -1) in a
+// segment's padding.
+__global__ void __launch_bounds__(256) expand_tasks_kernel(TaskList tl, int2* __restrict__ out) {
+    const TaskSeg sg = tl.seg[segment_of(tl.seg, tl.nseg, blockIdx.x, true)];
+    const int64_t t = ((int64_t)blockIdx.x - sg.block0) * blockDim.x + threadIdx.x;
+    int2 r = make_int2(-1, -1);
+    if (t < sg.ntasks) {
+        const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
+        const int pl = bra_of_task(pre, t, 0, sg.nbra);
+        r.x = sg.bra0 + pl;
+        r.y = ket_of_task(tl, sg, pre, pl, t);
+    }
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = r;
+}
+
 int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper) {
     ts.classes.clear();
     std::vector<int64_t> prefix_all;
@@ -468,6 +497,21 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper) {
     if (!seg_all.empty())
         HF_HIP_CHECK(ctx, hipMemcpy(ts.d_seg.p, seg_all.data(), seg_all.size() * sizeof(TaskSeg),
                                     hipMemcpyHostToDevice));
+    // expand every class's tasks once (launch-order (bra, ket) pairs)
+    size_t total = 0;
+    for (QuartetClassTasks& qc : ts.classes) {
+        qc.tasks_off = total;
+        total += (size_t)qc.nblocks * 256;
+    }
+    HF_HIP_CHECK(ctx, ts.d_tasks.alloc(std::max<size_t>(total, 1)));
+    for (const QuartetClassTasks& qc : ts.classes) {
+        if (qc.nblocks == 0) continue;
+        TaskList tl = task_list(ctx, ts, qc);
+        hipLaunchKernelGGL(expand_tasks_kernel, dim3((unsigned)qc.nblocks), dim3(256), 0, ctx->stream, tl,
+                           ts.d_tasks.p + qc.tasks_off);
+    }
+    HF_HIP_CHECK(ctx, hipGetLastError());
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     ts.threshold = thr;
     ts.upper = upper;
     return HF_OK;

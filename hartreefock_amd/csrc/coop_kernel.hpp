@@ -104,28 +104,26 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
     extern __shared__ double val[];
     __shared__ double cf[kCoopCoef];
     const int lane = threadIdx.x;
-    const int64_t t = blockIdx.x;
-    if (t >= tl.ntasks) return;
     int p, q;
     if (MODE == MODE_DIAG) {
-        p = q = (int)t;
+        if ((int64_t)blockIdx.x >= tl.ntasks) return;
+        p = q = (int)blockIdx.x;
     } else {
-        const TaskSeg sg = tl.seg[segment_of(tl.seg, tl.nseg, t, false)];
-        const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
-        const int pl = bra_of_task(pre, t - sg.task0, 0, sg.nbra);
-        p = sg.bra0 + pl;
-        q = ket_of_task(tl, sg, pre, pl, t - sg.task0);
+        const int2 me = tl.tasks[blockIdx.x];
+        if (me.x < 0) return;                      // segment padding
+        p = me.x;
+        q = me.y;
     }
-    const int A = tl.bra.sa[p], B = tl.bra.sb[p];
-    const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
+    const PairRec bp = tl.bra.rec[p], kp = tl.ket.rec[q];
+    const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;
     const double* xa = sh.xyz + 3 * A;
     const double* xb = sh.xyz + 3 * B;
     const double* xc = sh.xyz + 3 * Cs;
     const double* xd = sh.xyz + 3 * Ds;
-    const PrimPair* bra = pp + tl.bra.prim0[p];
-    const PrimPair* ket = pp + tl.ket.prim0[q];
-    const int npb = tl.bra.nprim[p], npk = tl.ket.nprim[q];
-    const int a0 = sh.bf[A], b0 = sh.bf[B], c0 = sh.bf[Cs], d0 = sh.bf[Ds];
+    const PrimPair* bra = pp + bp.prim0;
+    const PrimPair* ket = pp + kp.prim0;
+    const int npb = bp.nprim, npk = kp.nprim;
+    const int a0 = bp.a0, b0 = bp.b0, c0 = kp.a0, d0 = kp.b0;
     constexpr int NA = K::NA, NB = K::NB, NC = K::NC, ND = K::ND;
     const unsigned short* os = K::out_slot();
     const double* on = K::out_norm();
@@ -252,7 +250,9 @@ inline hipError_t launch_coop(const TaskList& tl, const ShellsDev& sh, const Pri
                               const SinkArgs& sink, hipStream_t st) {
     if (tl.ntasks <= 0) return hipSuccess;
     const size_t lds = (K::NSLOT + (K::TRUNCATED ? K::NOUT : 0)) * sizeof(double);
-    hipLaunchKernelGGL((eri_coop_kernel<K, MODE>), dim3((unsigned)tl.ntasks), dim3(64), lds, st, tl, sh, pp, boys,
+    // one block per entry of the expanded task list (padding entries return at once)
+    const int64_t blocks = MODE == MODE_DIAG ? tl.ntasks : tl.nblocks * 256;
+    hipLaunchKernelGGL((eri_coop_kernel<K, MODE>), dim3((unsigned)blocks), dim3(64), lds, st, tl, sh, pp, boys,
                        sink);
     return hipGetLastError();
 }

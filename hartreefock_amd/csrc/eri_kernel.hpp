@@ -464,17 +464,16 @@ __host__ __device__ __forceinline__ void eval_quartet_reference_order(
 }
 
 template <class C, typename R = double>
-__device__ __forceinline__ void quartet_values(const TaskList& tl, const ShellsDev& sh, const PrimPair* __restrict__ pp,
-                                               const R* __restrict__ boys, int p, int q, int A, int B, int Cs,
-                                               int Ds, int a0, int b0, int c0, int d0, R* out) {
+__device__ __forceinline__ void quartet_values(const ShellsDev& sh, const PrimPair* __restrict__ pp,
+                                               const R* __restrict__ boys, const PairRec& bp, const PairRec& kp,
+                                               R* out) {
+    const double *xa = sh.xyz + 3 * bp.sa, *xb = sh.xyz + 3 * bp.sb, *xc = sh.xyz + 3 * kp.sa, *xd = sh.xyz + 3 * kp.sb;
     if constexpr (C::TRUNCATED) {
         static_assert(sizeof(R) == sizeof(double), "the reference-convention classes are FP64 only");
-        eval_quartet_reference_order<C>(A, Cs, sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds,
-                                        pp + tl.bra.prim0[p], tl.bra.nprim[p], pp + tl.ket.prim0[q],
-                                        tl.ket.nprim[q], a0, b0, c0, d0, boys, out);
+        eval_quartet_reference_order<C>(bp.sa, kp.sa, xa, xb, xc, xd, pp + bp.prim0, bp.nprim, pp + kp.prim0, kp.nprim,
+                                        bp.a0, bp.b0, kp.a0, kp.b0, boys, out);
     } else {
-        eval_quartet<C, R>(sh.xyz + 3 * A, sh.xyz + 3 * B, sh.xyz + 3 * Cs, sh.xyz + 3 * Ds, pp + tl.bra.prim0[p],
-                           tl.bra.nprim[p], pp + tl.ket.prim0[q], tl.ket.nprim[q], boys, out);
+        eval_quartet<C, R>(xa, xb, xc, xd, pp + bp.prim0, bp.nprim, pp + kp.prim0, kp.nprim, boys, out);
     }
 }
 
@@ -493,33 +492,32 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
     const size_t nn = (size_t)n * n;
     const int rows_per_slot = sink.ndens * (NA + NB);
     const int slot_len = rows_per_slot * n;
-    const TaskSeg sg = tl.seg[segment_of(tl.seg, tl.nseg, blockIdx.x, true)];
-    const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
-    const int64_t tb = ((int64_t)blockIdx.x - sg.block0) * blockDim.x;     // segment-local tasks
-    const int64_t tlast = (tb + blockDim.x < sg.ntasks ? tb + blockDim.x : sg.ntasks) - 1;
-    const int pb = sg.bra0 + bra_of_task(pre, tb, 0, sg.nbra);
-    const int pe = sg.bra0 + bra_of_task(pre, tlast, pb - sg.bra0, sg.nbra);
+    __shared__ int s_pe;
+    const int2 me = tl.tasks[(size_t)blockIdx.x * blockDim.x + threadIdx.x];
+    const bool active = me.x >= 0;
+    const int pb = tl.tasks[(size_t)blockIdx.x * blockDim.x].x;     // a block's first task is never padding
+    if (threadIdx.x == 0) s_pe = pb;
+    __syncthreads();
+    if (active && me.x > pb) atomicMax(&s_pe, me.x);
+    __syncthreads();
+    const int pe = s_pe;
     const int nslot = pe - pb + 1 < sink.lds_slots ? pe - pb + 1 : sink.lds_slots;
     for (int i = threadIdx.x; i < nslot * slot_len; i += blockDim.x) rows[i] = 0.0;
     __syncthreads();
 
-    const int64_t t = tb + threadIdx.x;
-    const bool active = t <= tlast;
     int key = -1, a0 = 0, b0 = 0;
     double jab[NAB];
 #pragma unroll
     for (int i = 0; i < NAB; ++i) jab[i] = 0.0;
     if (active) {
-        const int pl = bra_of_task(pre, t, pb - sg.bra0, pe - sg.bra0 + 1);
-        const int p = sg.bra0 + pl;
-        const int q = ket_of_task(tl, sg, pre, pl, t);
-        const int A = tl.bra.sa[p], B = tl.bra.sb[p];
-        const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
-        a0 = sh.bf[A];
-        b0 = sh.bf[B];
-        const int c0 = sh.bf[Cs], d0 = sh.bf[Ds];
+        const int p = me.x, q = me.y;
+        const PairRec bp = tl.bra.rec[p], kp = tl.ket.rec[q];
+        const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;
+        a0 = bp.a0;
+        b0 = bp.b0;
+        const int c0 = kp.a0, d0 = kp.b0;
         R out[C::NOUT];
-        quartet_values<C, R>(tl, sh, pp, boys, p, q, A, B, Cs, Ds, a0, b0, c0, d0, out);
+        quartet_values<C, R>(sh, pp, boys, bp, kp, out);
         double w = 1.0;
         if (A == B) w *= 0.5;
         if (Cs == Ds) w *= 0.5;
@@ -560,7 +558,7 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
         const int s = sr / rows_per_slot, kr = sr - s * rows_per_slot;
         const int k = kr / (NA + NB), r = kr - k * (NA + NB);
         const int p = pb + s;
-        const int grow = r < NA ? sh.bf[tl.bra.sa[p]] + r : sh.bf[tl.bra.sb[p]] + (r - NA);
+        const int grow = r < NA ? tl.bra.rec[p].a0 + r : tl.bra.rec[p].b0 + (r - NA);
         const double* src = rows + (size_t)sr * n;
         double* dst = sink.K + k * nn + (size_t)grow * n;
         for (int col = threadIdx.x; col < n; col += blockDim.x) {
@@ -585,19 +583,16 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
             if (t >= tl.ntasks) return;
             p = q = (int)t;
         } else {
-            const TaskSeg sg = tl.seg[segment_of(tl.seg, tl.nseg, blockIdx.x, true)];
-            const int64_t t = ((int64_t)blockIdx.x - sg.block0) * blockDim.x + threadIdx.x;
-            if (t >= sg.ntasks) return;
-            const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
-            const int pl = bra_of_task(pre, t, 0, sg.nbra);
-            p = sg.bra0 + pl;
-            q = ket_of_task(tl, sg, pre, pl, t);
+            const int2 me = tl.tasks[(size_t)blockIdx.x * blockDim.x + threadIdx.x];
+            if (me.x < 0) return;
+            p = me.x;
+            q = me.y;
         }
-        const int A = tl.bra.sa[p], B = tl.bra.sb[p];
-        const int Cs = tl.ket.sa[q], Ds = tl.ket.sb[q];
-        const int a0 = sh.bf[A], b0 = sh.bf[B], c0 = sh.bf[Cs], d0 = sh.bf[Ds];
+        const PairRec bp = tl.bra.rec[p], kp = tl.ket.rec[q];
+        const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;
+        const int a0 = bp.a0, b0 = bp.b0, c0 = kp.a0, d0 = kp.b0;
         double out[C::NOUT];
-        quartet_values<C>(tl, sh, pp, boys, p, q, A, B, Cs, Ds, a0, b0, c0, d0, out);
+        quartet_values<C>(sh, pp, boys, bp, kp, out);
         if constexpr (MODE == MODE_PACKED) {
             sink_packed<C>(out, A, B, Cs, Ds, a0, b0, c0, d0, tl.same_class && p == q, sink);
         } else {

@@ -28,8 +28,17 @@ struct alignas(64) PrimPair {
     double Kz, inv_zeta, pad0, pad1;
 };
 
+// Everything a quartet kernel needs about one shell pair, in one 32-byte record.
+struct alignas(32) PairRec {
+    int sa, sb;           // shells (first has L = La)
+    int prim0, nprim;     // primitive pairs
+    int a0, b0;           // first basis function of each shell
+    int pad0, pad1;
+};
+
 // One class of shell pairs (La, Lb), La >= Lb, sorted by Schwarz bound, descending.
 struct PairClassDev {
+    const PairRec* rec;
     const int* sa;        // first shell (L = La)
     const int* sb;        // second shell (L = Lb)
     const int* prim0;     // offset of the pair's primitive pairs
@@ -67,9 +76,12 @@ struct TaskSeg {
 };
 
 // The quartet tasks of one launch: the segments of one quartet class.  In MODE_DIAG task t is
-// the quartet (t|t) of the bra class and there are no segments.
+// the quartet (t|t) of the bra class and there are no segments.  `tasks` is the same list
+// expanded once per task set to (bra, ket) pairs in launch order, nblocks * 256 entries (segment
+// tails padded with (-1, -1)): a kernel reads its task with one load.
 struct TaskList {
     PairClassDev bra, ket;
+    const int2* tasks;
     const int64_t* prefix;   // prefix arrays of every segment
     const int* koff;         // per-bra ket-group offsets of every segment
     const int* kbeg;         // per-bra ket-range starts within each ket group

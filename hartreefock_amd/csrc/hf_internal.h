@@ -87,6 +87,7 @@ struct QuartetClassTasks {
     int64_t prim_quartets = 0;
     size_t seg_off = 0;       // first segment in the shared device segment buffer
     int nseg = 0;
+    size_t tasks_off = 0;     // first entry in the expanded task list (nblocks * 256 entries)
 };
 
 struct TaskSet {
@@ -96,6 +97,7 @@ struct TaskSet {
     DevBuf<int64_t> d_prefix;
     DevBuf<int> d_koff, d_kbeg;
     DevBuf<hfgpu::TaskSeg> d_seg;
+    DevBuf<int2> d_tasks;
 };
 
 struct PairClassHost {
@@ -103,6 +105,7 @@ struct PairClassHost {
     std::vector<double> Q;
     std::vector<int> group;   // contraction groups: pairs [group[g], group[g+1]) share nprim
     DevBuf<int> d_sa, d_sb, d_prim0, d_nprim, d_group;
+    DevBuf<hfgpu::PairRec> d_rec;
     DevBuf<double> d_Q;
     int npairs() const { return (int)sa.size(); }
 };
