@@ -179,8 +179,12 @@ int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& 
         for (const QuartetClassTasks& qc : job.second->classes) {
             if (qc.ntasks <= 0) continue;
             const ClassInfo& ci = kClasses[qc.cls];
-            const double cost = (double)qc.prim_quartets * (ci.prim_flops_ref + 60) + (double)qc.ntasks * ci.hrr_flops;
-            order.push_back({job.first, job.second, &qc, cost});
+            // estimated duration x aggregate rate: the launch's flops plus the serial work of one quartet
+            // times the ratio of the chip's aggregate to a single lane's dependent-FP64 rate (~2e4), so
+            // that the latency-bound classes (few, long quartets) start first and overlap the others
+            const double work = (double)qc.prim_quartets * (ci.prim_flops_ref + 60) + (double)qc.ntasks * ci.hrr_flops;
+            const double chain = work / (double)qc.ntasks;
+            order.push_back({job.first, job.second, &qc, work + 2e4 * chain});
         }
     std::stable_sort(order.begin(), order.end(), [](const Launch& a, const Launch& b) { return a.cost > b.cost; });
     std::vector<double> load(kSide, 0.0);
