@@ -80,7 +80,14 @@ __host__ __device__ __forceinline__ void boys_scaled(double T, double pref, cons
         const double r = rsqrt_f64(T), inv2T = 0.5 * r * r;
         F[0] = kSqrtPiHalf * r;
         if (L > 0) {
+            // e^{-T} < 2.4e-16 here and enters F_{m+1} = ((2m+1) F_m - e^{-T}) / 2T only as a correction
+            // below 1e-11 of (2m+1) F_m for m <= 8, so FP32 exp (relative error ~1e-7) is exact to
+            // FP64 rounding and avoids the FP64 exp polynomial
+#if defined(__HIP_DEVICE_COMPILE__)
+            const double e = (double)__expf(-(float)T);
+#else
             const double e = exp(-T);
+#endif
 #pragma unroll
             for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * inv2T;
         }
