@@ -106,7 +106,8 @@ def serving(a):
     mols = g.serving_mix(a.requests, seed=2024)
     rng = np.random.default_rng(77)
 
-    def run(prec, rate):
+    def run(prec, rate, tag):
+        shown = 0
         srv = hf.Server(slots=a.slots, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1),
                         scf=scf)
         try:
@@ -122,8 +123,12 @@ def serving(a):
                 srv.submit(bfile, atoms)
             while len(got) < len(mols):
                 got += srv.poll(64, 1000)
+                if len(got) // 250 != shown:
+                    shown = len(got) // 250
+                    print("serving %s: %d of %d done" % (tag, len(got), len(mols)), file=sys.stderr, flush=True)
         finally:
             srv.close()
+        print("serving %s: finished in %.1f s" % (tag, time.perf_counter() - t0), file=sys.stderr, flush=True)
         bad = [{"id": r.id, "nbf": r.nbf, "status": r.status, "iterations": r.iterations, "energy": r.energy}
                for r in got if r.status != 0 or not r.converged]
         first = min(r.arrival_ms for r in got)
@@ -134,10 +139,10 @@ def serving(a):
                 "p99_ms": float(np.percentile(lat, 99)), "not_converged": bad, "nbf_min": min(nbf), "nbf_max": max(nbf),
                 "energies": {r.id: r.energy for r in got if r.status == 0 and r.converged}}
 
-    cap = run(0, None)["molecules_per_s"]
+    cap = run(0, None, "capacity")["molecules_per_s"]
     rate = 0.8 * cap
-    r64 = run(0, rate)
-    r32 = run(1, rate)
+    r64 = run(0, rate, "fp64")
+    r32 = run(1, rate, "mixed")
     common = set(r64["energies"]) & set(r32["energies"])
     dmax = max(abs(r64["energies"][k] - r32["energies"][k]) for k in common)
     for r in (r64, r32):
@@ -242,6 +247,7 @@ def main():
         "latency_ms": {"p50": pct(50), "p99": pct(99)},
         "eri_per_s": eris / elapsed,
         "eri_kernel_share": kernel_ms * 1e-3 / elapsed,
+        "jk_ms_per_build": sum(r.jk_ms for r in res) / max(1, sum(r.full_builds + r.incremental_builds for r in res)),
         "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "eri_class_kernel" + dom.name() + " (J/K)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,

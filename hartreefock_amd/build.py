@@ -55,7 +55,7 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
     kdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "eri_kernel.hpp"),
              os.path.join(CSRC, "coop_kernel.hpp"), os.path.join(GEN, "eri_classes.inc")]
     kdeps += sorted(glob.glob(os.path.join(GEN, "coop_*.inc")))
-    hdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "hf_internal.h"),
+    hdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "hf_internal.h"), os.path.join(CSRC, "gershgorin.h"),
              os.path.join(GEN, "dispatch.inc"), os.path.join(ROOT, "include", "hfgpu.h")]
     for stale in glob.glob(os.path.join(OBJ, "k_????.hip.o")):
         os.remove(stale)                 # objects of the former one-file-per-class layout

@@ -7,6 +7,7 @@
 // Matrix-core lane layout of __builtin_amdgcn_mfma_f64_16x16x4f64 (A 16x4, B 4x16, D 16x16):
 //   lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; element r of its 4-vector result is
 //   D[l / 16 + 4 r][l % 16].
+#include "gershgorin.h"
 #include "hf_internal.h"
 
 namespace {
@@ -99,30 +100,10 @@ __global__ void __launch_bounds__(256) dgemm16_kernel(int m, int n, int k, doubl
 // 8 + 2 s: e_s, stop_s.  part: per step s at 2 * nt * s: nt partial traces of X_s, then nt of X_s^2.
 constexpr int kSp2Hist = 8;
 
-__global__ void __launch_bounds__(256) sp2_bounds_kernel(const double* __restrict__ F, int n, double* __restrict__ hist) {
-    __shared__ double slo[256], shi[256];
-    double lo = 1e300, hi = -1e300;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {     // column j (F symmetric), coalesced over j
-        double r = 0;
-        for (int i = 0; i < n; ++i)
-            if (i != j) r += fabs(F[(size_t)i * n + j]);
-        const double c = F[(size_t)j * n + j];
-        lo = fmin(lo, c - r);
-        hi = fmax(hi, c + r);
-    }
-    slo[threadIdx.x] = lo;
-    shi[threadIdx.x] = hi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + s]);
-            shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + s]);
-        }
-        __syncthreads();
-    }
+__global__ void __launch_bounds__(1024) sp2_bounds_kernel(const double* __restrict__ F, int n, double* __restrict__ hist) {
+    double lo, hi;
+    hfgpu::gershgorin_block(F, n, lo, hi);
     if (threadIdx.x == 0) {
-        lo = slo[0];
-        hi = shi[0];
         if (hi - lo < 1e-12) hi = lo + 1.0;
         hist[0] = lo;
         hist[1] = hi;
@@ -363,7 +344,7 @@ int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp,
     // the stop flag is read back from step kFirstCheck on (or from the step count of the previous
     // call, hint, when given) and then every kEvery steps
     const int kFirstCheck = hint > 0 ? std::max(8, hint) : 16, kEvery = hint > 0 ? 2 : 4;
-    hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(256), 0, st, Fp, n, hist);
+    hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(1024), 0, st, Fp, n, hist);
     Sp2Args g{};
     g.n = n;
     g.nt = nt;

@@ -25,6 +25,7 @@
 #include <deque>
 #include <limits>
 
+#include "gershgorin.h"
 #include "hf_internal.h"
 
 namespace {
@@ -123,36 +124,57 @@ __global__ void mix_kernel(const double* __restrict__ Dn, double* __restrict__ D
         D[idx] = replace ? Dn[idx] : a * Dn[idx] + (1.0 - a) * D[idx];
 }
 
-// Deterministic single-block reductions: out[r] = sum_i x_r[i] * y_r[i]  or  sum_i (x_r[i]-y_r[i])^2
+// Deterministic reductions: out[r] = sum_i x_r[i] * y_r[i]  or  sum_i (x_r[i]-y_r[i])^2, r < m <= 8.
+// kDotBlocks blocks each reduce one contiguous chunk of every job into part[r * kDotBlocks + b]; one
+// wavefront then sums the partials of each job in a fixed order.
 struct DotJobs {
     const double* x[8];
     const double* y[8];
     int diff[8];
     int m;
 };
-__global__ void __launch_bounds__(1024) dots_kernel(DotJobs jobs, size_t nn, double* __restrict__ out) {
-    __shared__ double part[1024];
+constexpr int kDotBlocks = 64;
+
+__global__ void __launch_bounds__(256) dots_partial_kernel(DotJobs jobs, size_t nn, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const size_t chunk = (nn + gridDim.x - 1) / gridDim.x;
+    const size_t lo = (size_t)blockIdx.x * chunk, hi = lo + chunk < nn ? lo + chunk : nn;
     for (int r = 0; r < jobs.m; ++r) {
         double acc = 0;
         const double* __restrict__ x = jobs.x[r];
         const double* __restrict__ y = jobs.y[r];
         if (jobs.diff[r]) {
-            for (size_t i = threadIdx.x; i < nn; i += blockDim.x) {
+            for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
                 const double d = x[i] - y[i];
                 acc += d * d;
             }
         } else {
-            for (size_t i = threadIdx.x; i < nn; i += blockDim.x) acc += x[i] * y[i];
+            for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) acc += x[i] * y[i];
         }
-        part[threadIdx.x] = acc;
+        sh[threadIdx.x] = acc;
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
             __syncthreads();
         }
-        if (threadIdx.x == 0) out[r] = part[0];
+        if (threadIdx.x == 0) part[r * gridDim.x + blockIdx.x] = sh[0];
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(64) dots_final_kernel(const double* __restrict__ part, int m, double* __restrict__ out) {
+    for (int r = 0; r < m; ++r) {
+        double v = part[r * kDotBlocks + threadIdx.x];
+#pragma unroll
+        for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
+        if (threadIdx.x == 0) out[r] = v;
+    }
+}
+
+// part: kDotBlocks * 8 doubles of workspace
+void launch_dots(hipStream_t s, const DotJobs& jobs, size_t nn, double* part, double* out) {
+    hipLaunchKernelGGL(dots_partial_kernel, dim3(kDotBlocks), dim3(256), 0, s, jobs, nn, part);
+    hipLaunchKernelGGL(dots_final_kernel, dim3(1), dim3(64), 0, s, part, jobs.m, out);
 }
 
 // out = D - Dp; *mx = max |D - Dp|
@@ -203,30 +225,10 @@ __device__ double block_sum_256(double v, double* sh) {
     return r;
 }
 
-__global__ void __launch_bounds__(256) sp2_bounds_kernel(const double* __restrict__ F, int n, double* __restrict__ st) {
-    __shared__ double slo[256], shi[256];
-    double lo = 1e300, hi = -1e300;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {     // column j (F symmetric), coalesced over j
-        double r = 0;
-        for (int i = 0; i < n; ++i)
-            if (i != j) r += fabs(F[(size_t)i * n + j]);
-        const double c = F[(size_t)j * n + j];
-        lo = fmin(lo, c - r);
-        hi = fmax(hi, c + r);
-    }
-    slo[threadIdx.x] = lo;
-    shi[threadIdx.x] = hi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + s]);
-            shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + s]);
-        }
-        __syncthreads();
-    }
+__global__ void __launch_bounds__(1024) sp2_bounds_kernel(const double* __restrict__ F, int n, double* __restrict__ st) {
+    double lo, hi;
+    hfgpu::gershgorin_block(F, n, lo, hi);
     if (threadIdx.x == 0) {
-        lo = slo[0];
-        hi = shi[0];
         if (hi - lo < 1e-12) hi = lo + 1.0;
         st[SP2_LO] = lo;
         st[SP2_HI] = hi;
@@ -353,7 +355,8 @@ struct Diis {
 struct ScfState {
     int n = 0;
     size_t nn = 0;
-    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, Dprev, dD, C, eps, T1, T2, T3, err, work, red, sp2, sp2W, sp2part;
+    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, Dprev, dD, C, eps, T1, T2, T3, err, work, red, redpart, sp2, sp2W,
+        sp2part;
     DevBuf<rocblas_int> info;
 };
 
@@ -411,7 +414,7 @@ int inverse_sqrt(hf_ctx* ctx, ScfState& st) {
     jobs.m = 1;
     jobs.x[0] = st.S.p;
     jobs.y[0] = st.S.p;
-    hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, nn, st.red.p);
+    launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
     double fro2 = 0;
     HF_HIP_CHECK(ctx, hipMemcpyAsync(&fro2, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
     HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -427,7 +430,7 @@ int inverse_sqrt(hf_ctx* ctx, ScfState& st) {
             jobs.x[0] = T;
             jobs.y[0] = E;
             jobs.diff[0] = 1;
-            hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, nn, st.red.p);
+            launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
             double r2 = 0;
             HF_HIP_CHECK(ctx, hipMemcpyAsync(&r2, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
             HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -463,7 +466,7 @@ int sp2_projector(hf_ctx* ctx, ScfState& st, const double* Fp, int nocc, double*
     hipStream_t s = ctx->stream;
     const int n = st.n;
     constexpr int kMaxSteps = 100, kFirstCheck = 16, kEvery = 4;
-    hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(256), 0, s, Fp, n, st.sp2.p);
+    hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(1024), 0, s, Fp, n, st.sp2.p);
     hipLaunchKernelGGL(sp2_init_kernel, dim3(grid_for(st.nn)), dim3(256), 0, s, Fp, P, n, st.sp2.p);
     ok = false;
     for (int k = 1; k <= kMaxSteps; ++k) {
@@ -499,7 +502,7 @@ int diis_add(hf_ctx* ctx, ScfState& st, Diis& d, const double* Fsrc, const doubl
         jobs.y[r] = d.E.p + d.order[r] * st.nn;
         jobs.diff[r] = 0;
     }
-    hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, st.nn, st.red.p);
+    launch_dots(s, jobs, st.nn, st.redpart.p, st.red.p);
     double h[8];
     HF_HIP_CHECK(ctx, hipMemcpyAsync(h, st.red.p, jobs.m * sizeof(double), hipMemcpyDeviceToHost, s));
     HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -593,6 +596,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     HF_HIP_CHECK(ctx, st.err.alloc(nn));
     HF_HIP_CHECK(ctx, st.work.alloc(n));
     HF_HIP_CHECK(ctx, st.red.alloc(16));
+    HF_HIP_CHECK(ctx, st.redpart.alloc(8 * kDotBlocks));
     HF_HIP_CHECK(ctx, st.info.alloc(2));
     // S, T, V on the device; h = T + V
     if (int rc = hf_one_electron_device(ctx, st.S.p, st.T1.p, st.T2.p, s)) return rc;
@@ -783,7 +787,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             jobs.x[jobs.m] = st.Dn.p + k * nn; jobs.y[jobs.m] = F[k]; jobs.diff[jobs.m++] = 0;
             jobs.x[jobs.m] = st.Dn.p + k * nn; jobs.y[jobs.m] = st.D.p + k * nn; jobs.diff[jobs.m++] = 1;
         }
-        hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, nn, st.red.p);
+        launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
         double red[8];
         HF_HIP_CHECK(ctx, hipMemcpyAsync(red, st.red.p, jobs.m * sizeof(double), hipMemcpyDeviceToHost, s));
         HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -909,7 +913,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             jobs.m = 1;
             jobs.x[0] = st.T1.p;
             jobs.y[0] = st.T2.p;
-            hipLaunchKernelGGL(dots_kernel, dim3(1), dim3(1024), 0, s, jobs, nn, st.red.p);
+            launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
             HF_HIP_CHECK(ctx, hipMemcpyAsync(&ov, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
             HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
         }
