@@ -354,6 +354,7 @@ int setup_molecule(hf_ctx* ctx) {
     ctx->tasks_mixed_lo.clear();
     ctx->d_oe_L.release();
     ctx->d_oe_prim.release();
+    ctx->d_oe_pairs.release();
     ctx->have_mol = true;
     return HF_OK;
 }

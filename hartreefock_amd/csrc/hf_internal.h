@@ -140,6 +140,8 @@ struct hf_ctx {
     DevBuf<int> d_oe_L;
     DevBuf<double> d_oe_prim;
     size_t oe_nprim = 0;
+    DevBuf<int2> d_oe_pairs;               // shell pairs by one-electron class (La, Lb), La >= Lb
+    std::vector<int> oe_class_off;
     std::vector<KernelStat> stats;
     std::vector<hipEvent_t> ev;
     // side streams: the class kernels of one pass run concurrently, forked from and joined
