@@ -22,7 +22,7 @@ namespace hfgpu {
 
 constexpr double kTwoPi25 = 34.98683665524972497;   // 2 pi^{5/2}
 constexpr double kSqrtPiHalf = 0.88622692545275801365;  // sqrt(pi)/2
-// Classes with at most this many VRR accumulators evaluate ket primitives two at a time.
+// Classes with at most this many VRR accumulators evaluate bra primitives two at a time.
 constexpr int kKetPairMaxAcc = 27;
 
 // 1/sqrt(x) for normal positive x: the hardware estimate refined by two Newton steps
@@ -411,44 +411,57 @@ __host__ __device__ __forceinline__ void eval_quartet(const double* __restrict__
     R acc[C::NACC];
 #pragma unroll
     for (int i = 0; i < C::NACC; ++i) acc[i] = R(0);
-    for (int i = 0; i < npb; ++i) {
-        const PrimPair P = bra[i];
+    auto prim = [&](const PrimPair& P, const PrimPair& Q) __attribute__((always_inline)) {
         const R PAx = R(P.Px - A[0]), PAy = R(P.Py - A[1]), PAz = R(P.Pz - A[2]);
-        const R zeta = R(P.zeta), inv2z = R(0.5 * P.inv_zeta), KP = R(P.Kz);
-        auto prim = [&](const PrimPair& Q) __attribute__((always_inline)) {
-            const R eta = R(Q.zeta), inv_eta = R(Q.inv_zeta);
-            const R rs = rsqrt_r(zeta + eta);               // (zeta + eta)^{-1/2}
-            const R inv_s = rs * rs;
-            const R rho = zeta * eta * inv_s;
-            const R PQx = R(P.Px - Q.Px), PQy = R(P.Py - Q.Py), PQz = R(P.Pz - Q.Pz);
-            const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const R pref = R(kTwoPi25) * KP * R(Q.Kz) * rs;
-            R F[C::L + 1];
-            if constexpr (sizeof(R) == sizeof(double))
-                boys_scaled<C::L>(T, pref, boys, F);
-            else
-                boys_scaled_f32<C::L>(T, pref, boys, F);
-            const R rz = eta * inv_s;                       // rho / zeta
-            const R zoe = zeta * inv_eta;                   // zeta / eta
-            const R inv2e = R(0.5) * inv_eta;
-            const R dlx = zoe * PAx + R(Q.Px - Cc[0]);      // delta = (zeta/eta) PA + QC
-            const R dly = zoe * PAy + R(Q.Py - Cc[1]);
-            const R dlz = zoe * PAz + R(Q.Pz - Cc[2]);
-            C::prim(F, PAx, PAy, PAz, -rz * PQx, -rz * PQy, -rz * PQz, inv2z, rz, dlx, dly, dlz, zoe, inv2e, acc);
-        };
-        // The primitive loop is bound by load latency (ket primitive pair, then the Boys row it
-        // selects): small classes take ket primitives two at a time so both chains are in flight
-        // together.  (Measured on C10H22/6-31G*: (ss|ss) -11%, (ps|ps) -20%; flattening the bra x
-        // ket loop or four at a time raised register use and ran slower.)
+        const R zeta = R(P.zeta), inv2z = R(0.5 * P.inv_zeta);
+        const R eta = R(Q.zeta), inv_eta = R(Q.inv_zeta), inv2e = R(0.5) * inv_eta;
+        const R rs = rsqrt_r(zeta + eta);               // (zeta + eta)^{-1/2}
+        const R inv_s = rs * rs;
+        const R rho = zeta * eta * inv_s;
+        const R PQx = R(P.Px - Q.Px), PQy = R(P.Py - Q.Py), PQz = R(P.Pz - Q.Pz);
+        const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+        const R pref = R(kTwoPi25) * R(P.Kz) * R(Q.Kz) * rs;
+        R F[C::L + 1];
+        if constexpr (sizeof(R) == sizeof(double))
+            boys_scaled<C::L>(T, pref, boys, F);
+        else
+            boys_scaled_f32<C::L>(T, pref, boys, F);
+        const R rz = eta * inv_s;                       // rho / zeta
+        const R zoe = zeta * inv_eta;                   // zeta / eta
+        const R dlx = zoe * PAx + R(Q.Px - Cc[0]);      // delta = (zeta/eta) PA + QC
+        const R dly = zoe * PAy + R(Q.Py - Cc[1]);
+        const R dlz = zoe * PAz + R(Q.Pz - Cc[2]);
+        C::prim(F, PAx, PAy, PAz, -rz * PQx, -rz * PQy, -rz * PQz, inv2z, rz, dlx, dly, dlz, zoe, inv2e, acc);
+    };
+    // The loop is bound by load latency (primitive pair, then the Boys row it selects).  The lanes of
+    // a wavefront hold different kets but (tasks being ordered by bra pair) mostly the same bra, so
+    // ket primitives go in the outer loop: each lane loads each of its ket primitive pairs once, and
+    // the inner loop's bra loads are the same address across the wavefront.  Small classes take the
+    // inner primitives two at a time so both chains are in flight together; a one-primitive bra
+    // (d shells, outer valence shells) pairs the ket primitives instead.
+    constexpr bool kPairs = C::NACC <= kKetPairMaxAcc;
+    if (kPairs && npb == 1) {
+        const PrimPair P = bra[0];
         int j = 0;
-        if constexpr (C::NACC <= kKetPairMaxAcc) {
-            for (; j + 1 < npk; j += 2) {
-                const PrimPair Q0 = ket[j], Q1 = ket[j + 1];
-                prim(Q0);
-                prim(Q1);
-            }
+        for (; j + 1 < npk; j += 2) {
+            const PrimPair Q0 = ket[j], Q1 = ket[j + 1];
+            prim(P, Q0);
+            prim(P, Q1);
         }
-        for (; j < npk; ++j) prim(ket[j]);
+        if (j < npk) prim(P, ket[j]);
+    } else {
+        for (int j = 0; j < npk; ++j) {
+            const PrimPair Q = ket[j];
+            int i = 0;
+            if constexpr (kPairs) {
+                for (; i + 1 < npb; i += 2) {
+                    const PrimPair P0 = bra[i], P1 = bra[i + 1];
+                    prim(P0, Q);
+                    prim(P1, Q);
+                }
+            }
+            for (; i < npb; ++i) prim(bra[i], Q);
+        }
     }
     C::hrr(acc, R(A[0] - B[0]), R(A[1] - B[1]), R(A[2] - B[2]), R(Cc[0] - Dd[0]), R(Cc[1] - Dd[1]), R(Cc[2] - Dd[2]),
            out);
