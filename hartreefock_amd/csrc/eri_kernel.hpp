@@ -24,6 +24,16 @@ constexpr double kTwoPi25 = 34.98683665524972497;   // 2 pi^{5/2}
 constexpr double kSqrtPiHalf = 0.88622692545275801365;  // sqrt(pi)/2
 // Classes with at most this many VRR accumulators evaluate bra primitives two at a time.
 constexpr int kKetPairMaxAcc = 27;
+// J/K kernels of classes with L up to kBoysLdsMaxL and a bra pair of at most kBoysLdsMaxBra basis
+// functions stage their slice of the Boys table (F_L..F_{L+7} and e^{-T_k}, kBoysLdsStride doubles
+// per row, 26 KB) in LDS; larger bras need the LDS for their exchange-row slots.
+constexpr int kBoysLdsMaxL = 3;
+constexpr int kBoysLdsMaxBra = 6;
+template <class C>
+constexpr bool boys_in_lds() { return C::L <= kBoysLdsMaxL && C::NA + C::NB <= kBoysLdsMaxBra && !C::TRUNCATED; }
+constexpr int kBoysLdsStride = 9;
+constexpr size_t kBoysLdsBytes = (size_t)BOYS_NROW * kBoysLdsStride * sizeof(double);
+constexpr int kBoysLdsStride32 = 7;    // FP32 tier: F_L..F_{L+5}
 
 // 1/sqrt(x) for normal positive x: the hardware estimate refined by two Newton steps
 // r <- r (3 - x r^2) / 2 (each step squares the relative error), on the device; exact on the host.
@@ -50,20 +60,22 @@ __host__ __device__ __forceinline__ float rsqrt_f32(float x) {
 // grid point T_k (|x| = |T_k - T| <= 0.05, truncation below 1e-17 relative) and downward
 // recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1), with e^{-T} = e^{-T_k} e^x from the table and
 // a 9-term series for e^x.  Beyond: F_0 = sqrt(pi/T)/2 and upward recursion.
-template <int L>
+// STRIDE, C0, ECOL: row stride of the table, column of F_L, column of e^{-T_k} (the device table, or
+// the class's slice of it staged in LDS).
+template <int L, int STRIDE = BOYS_STRIDE, int C0 = L, int ECOL = BOYS_NCOL>
 __host__ __device__ __forceinline__ void boys_scaled(double T, double pref, const double* __restrict__ tab, double* F) {
     if (T < BOYS_TMAX) {
         const int k = (int)rint(T * BOYS_INV_DT);
         const double x = k * BOYS_DT - T;          // F_m(T) = sum_j F_{m+j}(T_k) x^j / j!
-        const double* row = tab + k * BOYS_STRIDE;
-        double f = row[L + 7];
-        f = row[L + 6] + x * f * (1.0 / 7.0);
-        f = row[L + 5] + x * f * (1.0 / 6.0);
-        f = row[L + 4] + x * f * (1.0 / 5.0);
-        f = row[L + 3] + x * f * (1.0 / 4.0);
-        f = row[L + 2] + x * f * (1.0 / 3.0);
-        f = row[L + 1] + x * f * (1.0 / 2.0);
-        f = row[L] + x * f;
+        const double* row = tab + k * STRIDE;
+        double f = row[C0 + 7];
+        f = row[C0 + 6] + x * f * (1.0 / 7.0);
+        f = row[C0 + 5] + x * f * (1.0 / 6.0);
+        f = row[C0 + 4] + x * f * (1.0 / 5.0);
+        f = row[C0 + 3] + x * f * (1.0 / 4.0);
+        f = row[C0 + 2] + x * f * (1.0 / 3.0);
+        f = row[C0 + 1] + x * f * (1.0 / 2.0);
+        f = row[C0] + x * f;
         F[L] = f;
         if (L > 0) {
             double ex = 1.0 + x * (1.0 / 8.0);
@@ -74,7 +86,7 @@ __host__ __device__ __forceinline__ void boys_scaled(double T, double pref, cons
             ex = 1.0 + x * (1.0 / 3.0) * ex;
             ex = 1.0 + x * (1.0 / 2.0) * ex;
             ex = 1.0 + x * ex;
-            const double e = row[BOYS_NCOL] * ex;
+            const double e = row[ECOL] * ex;
 #pragma unroll
             for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) * (1.0 / (2 * m + 1));
         }
@@ -103,12 +115,12 @@ __host__ __device__ __forceinline__ void boys_scaled(double T, double pref, cons
 // FP32 exp and downward recursion below BOYS_TMAX; above it the asymptotic form
 // F_0 = sqrt(pi/T)/2 and the upward recursion F_{m+1} = (2m+1) F_m / (2T) in FP64, rounded.  The
 // e^{-T} term of that recursion is below 1.6e-15 of (2m+1) F_m for T >= 36 and is left out.
-template <int L>
+template <int L, int STRIDE = BOYS_STRIDE, int C0 = L>
 __host__ __device__ __forceinline__ void boys_scaled_f32(float T, float pref, const float* __restrict__ tab, float* F) {
     if (T < (float)BOYS_TMAX) {
         const int k = (int)rintf(T * (float)BOYS_INV_DT);
         const float x = (float)k * (float)BOYS_DT - T;
-        const float* row = tab + k * BOYS_STRIDE + L;
+        const float* row = tab + k * STRIDE + C0;
         float f = row[5];
         f = row[4] + x * f * (1.0f / 5.0f);
         f = row[3] + x * f * (1.0f / 4.0f);
@@ -402,7 +414,7 @@ __device__ __forceinline__ void sink_packed(const double* __restrict__ v, int A,
 // One contracted shell quartet: primitive loops (Boys, VRR, transfer, contraction) then HRR and the
 // component normalisation.  A..D are the four centres; out receives NA*NB*NC*ND values, index
 // ((ia*NB + ib)*NC + ic)*ND + id.
-template <class C, typename R = double>
+template <class C, typename R = double, bool BOYS_LDS = false>
 __host__ __device__ __forceinline__ void eval_quartet(const double* __restrict__ A, const double* __restrict__ B,
                                                      const double* __restrict__ Cc, const double* __restrict__ Dd,
                                                      const PrimPair* __restrict__ bra, int npb,
@@ -422,8 +434,12 @@ __host__ __device__ __forceinline__ void eval_quartet(const double* __restrict__
         const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
         const R pref = R(kTwoPi25) * R(P.Kz) * R(Q.Kz) * rs;
         R F[C::L + 1];
-        if constexpr (sizeof(R) == sizeof(double))
+        if constexpr (sizeof(R) == sizeof(double) && BOYS_LDS)
+            boys_scaled<C::L, kBoysLdsStride, 0, 8>(T, pref, boys, F);
+        else if constexpr (sizeof(R) == sizeof(double))
             boys_scaled<C::L>(T, pref, boys, F);
+        else if constexpr (BOYS_LDS)
+            boys_scaled_f32<C::L, kBoysLdsStride32, 0>(T, pref, boys, F);
         else
             boys_scaled_f32<C::L>(T, pref, boys, F);
         const R rz = eta * inv_s;                       // rho / zeta
@@ -497,7 +513,7 @@ __host__ __device__ __forceinline__ void eval_quartet_reference_order(
                 }
 }
 
-template <class C, typename R = double>
+template <class C, typename R = double, bool BOYS_LDS = false>
 __device__ __forceinline__ void quartet_values(const ShellsDev& sh, const PrimPair* __restrict__ pp,
                                                const R* __restrict__ boys, const PairRec& bp, const PairRec& kp,
                                                R* out) {
@@ -507,7 +523,7 @@ __device__ __forceinline__ void quartet_values(const ShellsDev& sh, const PrimPa
         eval_quartet_reference_order<C>(bp.sa, kp.sa, xa, xb, xc, xd, pp + bp.prim0, bp.nprim, pp + kp.prim0, kp.nprim,
                                         bp.a0, bp.b0, kp.a0, kp.b0, boys, out);
     } else {
-        eval_quartet<C, R>(xa, xb, xc, xd, pp + bp.prim0, bp.nprim, pp + kp.prim0, kp.nprim, boys, out);
+        eval_quartet<C, R, BOYS_LDS>(xa, xb, xc, xd, pp + bp.prim0, bp.nprim, pp + kp.prim0, kp.nprim, boys, out);
     }
 }
 
@@ -537,6 +553,19 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
     const int pe = s_pe;
     const int nslot = pe - pb + 1 < sink.lds_slots ? pe - pb + 1 : sink.lds_slots;
     for (int i = threadIdx.x; i < nslot * slot_len; i += blockDim.x) rows[i] = 0.0;
+    constexpr bool kLdsBoys = boys_in_lds<C>();
+    R* btab = (R*)(rows + (size_t)sink.lds_slots * slot_len);
+    if constexpr (kLdsBoys && sizeof(R) == sizeof(double)) {
+        for (int i = threadIdx.x; i < BOYS_NROW * kBoysLdsStride; i += blockDim.x) {
+            const int r = i / kBoysLdsStride, c = i - r * kBoysLdsStride;
+            btab[i] = boys[r * BOYS_STRIDE + (c < 8 ? C::L + c : BOYS_NCOL)];
+        }
+    } else if constexpr (kLdsBoys) {
+        for (int i = threadIdx.x; i < BOYS_NROW * kBoysLdsStride32; i += blockDim.x) {
+            const int r = i / kBoysLdsStride32, c = i - r * kBoysLdsStride32;
+            btab[i] = c < 6 ? boys[r * BOYS_STRIDE + C::L + c] : R(0);
+        }
+    }
     __syncthreads();
 
     int key = -1, a0 = 0, b0 = 0;
@@ -551,7 +580,10 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
         b0 = bp.b0;
         const int c0 = kp.a0, d0 = kp.b0;
         R out[C::NOUT];
-        quartet_values<C, R>(sh, pp, boys, bp, kp, out);
+        if constexpr (kLdsBoys)
+            quartet_values<C, R, true>(sh, pp, (const R*)btab, bp, kp, out);
+        else
+            quartet_values<C, R>(sh, pp, boys, bp, kp, out);
         double w = 1.0;
         if (A == B) w *= 0.5;
         if (Cs == Ds) w *= 0.5;
@@ -657,6 +689,7 @@ inline hipError_t launch_eri(const TaskList& tl, const ShellsDev& sh, const Prim
         const size_t cap = 40 * 1024;
         s.lds_slots = (int)std::min<size_t>(4, cap / slot_bytes);
         lds = s.lds_slots * slot_bytes;
+        if (boys_in_lds<C>()) lds += kBoysLdsBytes;    // (the FP32 slice needs less)
     }
     hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, EXACT, MODE>), dim3((unsigned)blocks), dim3(256), lds, st, tl,
                        sh, pp, boys, s);
