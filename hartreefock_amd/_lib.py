@@ -55,7 +55,8 @@ class ScfResult(ctypes.Structure):
     _fields_ = [("energy", ctypes.c_double), ("homo", ctypes.c_double), ("s2", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("jk_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("fp32_builds", ctypes.c_int), ("diag_steps", ctypes.c_int),
-                ("full_builds", ctypes.c_int), ("incremental_builds", ctypes.c_int), ("quartets", ctypes.c_double)]
+                ("full_builds", ctypes.c_int), ("incremental_builds", ctypes.c_int), ("quartets", ctypes.c_double),
+                ("sp2_steps", ctypes.c_int)]
 
 
 class ServeOptions(ctypes.Structure):

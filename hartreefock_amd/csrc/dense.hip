@@ -123,12 +123,35 @@ struct Sp2Args {
     double* part;
 };
 
+// Operand loads of one wavefront's k strip, issued before the step's decision is read (the decision
+// needs the previous step's partial traces, one more round trip): up to kSp2Trips x 8 k values.
+constexpr int kSp2Trips = ((kSmallDense + 15) / 16 * 4 + 31) / 32;
+
 __global__ void __launch_bounds__(256) sp2_step_kernel(Sp2Args g) {
     __shared__ v4d part4[3][64];
     const int n = g.n, nt = g.nt, s = g.s;
     const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4, w = threadIdx.x >> 6;
     const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
     const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
+    int kb, ke;
+    k_strip(n, w, kb, ke);
+    // raw operands: element (i, kk) of A and (j, kk) of B (X_s symmetric), from X_{s-1} and
+    // X_{s-1}^2, or from F for s = 0
+    double ra[kSp2Trips][8], ra2[kSp2Trips][8], rb[kSp2Trips][8], rb2[kSp2Trips][8];
+    const double* P1 = s == 0 ? g.F : g.Xp;
+    const double* P2 = s == 0 ? g.F : g.X2p;
+#pragma unroll
+    for (int q = 0; q < kSp2Trips; ++q)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = kb + 32 * q + 4 * u + lk, ia = i0 + lr, jb = j0 + lr;
+            const bool okk = kk < ke;
+            const size_t oa = ia + (size_t)kk * n, ob = jb + (size_t)kk * n;
+            ra[q][u] = okk && ia < n ? P1[oa] : 0.0;
+            ra2[q][u] = okk && ia < n && s > 0 ? P2[oa] : 0.0;
+            rb[q][u] = okk && jb < n ? P1[ob] : 0.0;
+            rb2[q][u] = okk && jb < n && s > 0 ? P2[ob] : 0.0;
+        }
     bool square = false;
     if (s > 0) {
         const double* pp = g.part + (size_t)2 * nt * (s - 1);
@@ -166,18 +189,29 @@ __global__ void __launch_bounds__(256) sp2_step_kernel(Sp2Args g) {
         }
     }
     const double lo = g.hist[0], hi = g.hist[1], inv = 1.0 / (hi - lo);
+    auto combine = [&](double x, double x2, int i, int kk) -> double {   // element of X_s from the raw values
+        if (s == 0) return ((i == kk ? hi : 0.0) - x) * inv;
+        return square ? x2 : 2.0 * x - x2;
+    };
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kSp2Trips; ++q)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = kb + 32 * q + 4 * u + lk, ia = i0 + lr, jb = j0 + lr;
+            const bool oka = kk < ke && ia < n, okb = kk < ke && jb < n;
+            const double a = oka ? combine(ra[q][u], ra2[q][u], ia, kk) : 0.0;
+            const double b = okb ? combine(rb[q][u], rb2[q][u], jb, kk) : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+    acc = reduce_waves(acc, part4);
+    if (w != 0) return;
     auto xval = [&](int i, int j) -> double {                   // element (i, j) of X_s, zero outside
         if (i >= n || j >= n) return 0.0;
         const size_t o = i + (size_t)j * n;
         if (s == 0) return ((i == j ? hi : 0.0) - g.F[o]) * inv;
         return square ? g.X2p[o] : 2.0 * g.Xp[o] - g.X2p[o];
     };
-    // X_s is symmetric: the B operand is read along a column, like A
-    auto xcol = [&](int kk, int j) -> double { return xval(j, kk); };
-    int kb, ke;
-    k_strip(n, w, kb, ke);
-    v4d acc = reduce_waves(tile_product(kb, ke, i0, j0, xval, xcol), part4);
-    if (w != 0) return;
     double d1 = 0, d2 = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {

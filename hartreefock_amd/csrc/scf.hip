@@ -635,7 +635,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     const double dtol = p->density_tolerance > 0 ? p->density_tolerance : 1e-10;
     const double itol = p->diis_tolerance > 0 ? p->diis_tolerance : 1e-10;
     double energy = 0, homo = 0, lastErr = 0, jk_ms = 0;
-    int diag_steps = 0, sp2_steps[2] = {0, 0};
+    int diag_steps = 0, sp2_steps[2] = {0, 0}, sp2_total = 0;
     bool last_diag[2] = {false, false};
     // incremental Fock build state: J, K hold the J/K of Dprev
     int full_builds = 0, incr_builds = 0, since_full = 0;
@@ -758,6 +758,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                         return rc;
                     ok = res != nullptr;
                     sp2_steps[k] = ok ? steps : 0;
+                    sp2_total += ok ? steps + 1 : 0;
                     P = res;
                 } else {
                     if (int rc = sp2_projector(ctx, st, st.T2.p, occ[k], st.T1.p, st.T3.p, ok)) return rc;
@@ -898,6 +899,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     r->converged = converged ? 1 : 0;
     r->jk_ms = jk_ms;
     r->diag_steps = diag_steps;
+    r->sp2_steps = sp2_total;
     r->full_builds = full_builds;
     r->incremental_builds = incr_builds;
     r->quartets = quartets;

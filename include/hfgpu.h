@@ -155,6 +155,7 @@ typedef struct hf_scf_result {
     int full_builds;               /* J/K builds of the full density */
     int incremental_builds;        /* J/K builds of a density change */
     double quartets;               /* shell quartets evaluated over all J/K builds */
+    int sp2_steps;                 /* purification steps over all density steps (their final steps) */
 } hf_scf_result;
 
 void hf_scf_defaults(hf_scf_params* p);
