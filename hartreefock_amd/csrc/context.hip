@@ -129,6 +129,20 @@ TaskList task_list(hf_ctx* ctx, const TaskSet& ts, const QuartetClassTasks& qc) 
 
 // Launch every quartet class of the given (mode, task set) jobs on the context's side streams
 // between one fork from and one join into st; with record, add to the per-class statistics.
+// Adds the HIP-event times of the last profiled build's launches to the statistics.  Called when
+// the next build starts (by then the SCF step has long waited for that build) and when the
+// statistics are read, so profiling adds no synchronisation to a J/K build.
+int flush_kernel_times(hf_ctx* ctx) {
+    for (int c : ctx->ev_pending) {
+        HF_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[2 * c + 1]));
+        float ms = 0;
+        HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * c], ctx->ev[2 * c + 1]));
+        ctx->stats[c].ms += ms;
+    }
+    ctx->ev_pending.clear();
+    return HF_OK;
+}
+
 int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& jobs, const SinkArgs& sink,
                 hipStream_t st, bool record) {
     ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
@@ -141,6 +155,7 @@ int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& 
         }
     }
     const bool prof = ctx->opt.profile != 0 && record;
+    if (int rc = flush_kernel_times(ctx)) return rc;
     if (prof && ctx->ev.size() < 2 * (size_t)nslot) {
         for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
         ctx->ev.assign(2 * nslot, nullptr);
@@ -197,11 +212,15 @@ int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& 
         hipStream_t ks = fork ? ctx->side[si] : st;
         const TaskList tl = task_list(ctx, *L.ts, qc);
         const int slot = stat_slot(L.mode, qc.cls);
-        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * slot], ks));
+        const int key = ci.la * 1000 + ci.lb * 100 + ci.lc * 10 + ci.ld;
+        const bool timed = prof && (ctx->opt.profile_class < 0 || ctx->opt.profile_class == key);
+        if (timed) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * slot], ks));
         hipError_t e = ci.fn(L.mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ks);
         if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));
-        if (prof) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * slot + 1], ks));
-        launched.push_back(slot);
+        if (timed) {
+            HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * slot + 1], ks));
+            launched.push_back(slot);
+        }
         if (record) add_stats(ctx, L.mode, qc, sink);
     }
     // join: st continues after every side stream's work
@@ -209,14 +228,7 @@ int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& 
         HF_HIP_CHECK(ctx, hipEventRecord(ctx->side_ev[k + 1], ctx->side[k]));
         HF_HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->side_ev[k + 1], 0));
     }
-    if (prof) {
-        HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
-        for (int c : launched) {
-            float ms = 0;
-            HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * c], ctx->ev[2 * c + 1]));
-            ctx->stats[c].ms += ms;
-        }
-    }
+    if (prof) ctx->ev_pending = launched;
     return HF_OK;
 }
 
@@ -570,6 +582,7 @@ void hf_default_options(hf_options* o) {
     o->precision = HF_PREC_FP64;
     o->fp32_threshold = 1e-6;
     o->profile = 0;
+    o->profile_class = -1;
 }
 
 int hf_create(const hf_options* opt, hf_ctx** out) {
@@ -723,6 +736,7 @@ int hf_one_electron(hf_ctx* ctx, double* S, double* T, double* V) {
 
 int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max) {
     if (!ctx || !out) return 0;
+    if (flush_kernel_times(const_cast<hf_ctx*>(ctx)) != HF_OK) return 0;
     int k = 0;
     for (const KernelStat& s : ctx->stats) {
         if (k >= max) break;
@@ -746,6 +760,7 @@ int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max) {
 
 int hf_kernel_stats_reset(hf_ctx* ctx) {
     if (!ctx) return HF_E_ARG;
+    if (int rc = flush_kernel_times(ctx)) return rc;
     for (KernelStat& s : ctx->stats) {
         const int c = s.cls, f = s.fp32;
         s = KernelStat{};

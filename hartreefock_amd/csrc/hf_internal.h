@@ -144,6 +144,7 @@ struct hf_ctx {
     std::vector<int> oe_class_off;
     std::vector<KernelStat> stats;
     std::vector<hipEvent_t> ev;
+    std::vector<int> ev_pending;         // stat slots whose last launch's events are not yet read
     // side streams: the class kernels of one pass run concurrently, forked from and joined
     // back to the caller's stream with events
     std::vector<hipStream_t> side;

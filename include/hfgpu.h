@@ -45,7 +45,10 @@ typedef struct hf_options {
     int exact_transfer;        /* 0: the reference's ERI convention (see hf_eri_packed); 1: exact ERIs */
     int precision;             /* HF_PREC_FP64 or HF_PREC_MIXED */
     double fp32_threshold;     /* mixed: quartets whose bound is below this run in FP32 */
-    int profile;               /* 1: time every class kernel with HIP events (hf_kernel_stats) */
+    int profile;               /* 1: time class kernels with HIP events (hf_kernel_stats) */
+    int profile_class;         /* with profile = 1: time only the class la*1000 + lb*100 + lc*10 + ld
+                                  (-1, the default: every class); each timed launch is bracketed by
+                                  two events, which cost the build a few per cent */
 } hf_options;
 
 void hf_default_options(hf_options* opt);
