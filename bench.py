@@ -184,7 +184,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving workload: number of requests")
-    ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent requests")
+    ap.add_argument("--slots", type=int, default=8, help="serving workload: concurrent requests")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()

@@ -264,11 +264,11 @@ class Engine:
 class Server:
     """Continuous-batching SCF server on one GPU (hf_serve_* in include/hfgpu.h).
 
-    Server(slots=4, hbm_budget=0, options={...context options...}, scf={...SCF parameters...})
+    Server(slots=8, hbm_budget=0, options={...context options...}, scf={...SCF parameters...})
     submit(basis_file, atoms, unrestricted=False, tag=0) -> id;  poll(max, wait_ms) -> [ServeResult]
     """
 
-    def __init__(self, slots=4, hbm_budget=0.0, options=None, scf=None):
+    def __init__(self, slots=8, hbm_budget=0.0, options=None, scf=None):
         L = lib()
         o = ServeOptions()
         L.hf_serve_default_options(ctypes.byref(o))

@@ -100,7 +100,7 @@ struct hf_server {
 extern "C" {
 
 void hf_serve_default_options(hf_serve_options* o) {
-    o->slots = 4;
+    o->slots = 8;
     o->hbm_budget = 0;
     hf_default_options(&o->ctx);
     hf_scf_defaults(&o->scf);

@@ -192,7 +192,7 @@ int hf_kernel_stats_reset(hf_ctx* ctx);
 typedef struct hf_server hf_server;
 
 typedef struct hf_serve_options {
-    int slots;                     /* concurrent requests (worker threads / contexts), default 4 */
+    int slots;                     /* concurrent requests (worker threads / contexts), default 8 */
     double hbm_budget;             /* bytes for admitted requests; 0 = 80% of the device's free memory */
     hf_options ctx;                /* context options of every slot (precision, thresholds, ...) */
     hf_scf_params scf;             /* SCF parameters of every request (unrestricted is per request) */
