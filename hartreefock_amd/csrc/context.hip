@@ -104,10 +104,10 @@ void add_stats(hf_ctx* ctx, int mode, const QuartetClassTasks& qc, const SinkArg
         sink_flops = nout * 2.0 * (2 + 4 * sink.ndens);   // 2 J + 4 K contractions per density
     KernelStat& ks = ctx->stats[stat_slot(mode, qc.cls)];
     ks.launches += 1;
-    ks.quartets += qc.ntasks;
+    ks.quartets += qc.quartets;
     ks.prim_quartets += qc.prim_quartets;
-    ks.eris += (double)qc.ntasks * nout;
-    ks.flops += (double)qc.prim_quartets * (pf + boys_flops) + (double)qc.ntasks * (ci.hrr_flops + sink_flops);
+    ks.eris += (double)qc.quartets * nout;
+    ks.flops += (double)qc.prim_quartets * (pf + boys_flops) + (double)qc.quartets * (ci.hrr_flops + sink_flops);
 }
 
 TaskList task_list(hf_ctx* ctx, const TaskSet& ts, const QuartetClassTasks& qc) {
@@ -124,11 +124,10 @@ TaskList task_list(hf_ctx* ctx, const TaskSet& ts, const QuartetClassTasks& qc) 
     tl.ntasks = qc.ntasks;
     tl.nblocks = qc.nblocks;
     tl.same_class = qc.bra == qc.ket;
+    tl.slice_prims = qc.slice_prims;
     return tl;
 }
 
-// Launch every quartet class of the given (mode, task set) jobs on the context's side streams
-// between one fork from and one join into st; with record, add to the per-class statistics.
 // Adds the HIP-event times of the last profiled build's launches to the statistics.  Called when
 // the next build starts (by then the SCF step has long waited for that build) and when the
 // statistics are read, so profiling adds no synchronisation to a J/K build.
@@ -143,6 +142,8 @@ int flush_kernel_times(hf_ctx* ctx) {
     return HF_OK;
 }
 
+// Launch every quartet class of the given (mode, task set) jobs on the context's side streams
+// between one fork from and one join into st; with record, add to the per-class statistics.
 int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& jobs, const SinkArgs& sink,
                 hipStream_t st, bool record) {
     ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
@@ -375,9 +376,9 @@ int setup_molecule(hf_ctx* ctx) {
     return HF_OK;
 }
 
-int ensure_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper = 0.0) {
-    if (ts.threshold == thr && ts.upper == upper && !ts.classes.empty()) return HF_OK;
-    return hf_build_tasks(ctx, thr, ts, upper);
+int ensure_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper = 0.0, bool sliced = true) {
+    if (ts.threshold == thr && ts.upper == upper && ts.sliced == sliced && !ts.classes.empty()) return HF_OK;
+    return hf_build_tasks(ctx, thr, ts, upper, sliced);
 }
 
 // Threshold grid 10^(k/4): level of thr rounded down (never screens more than asked); thresholds
@@ -417,6 +418,8 @@ int mixed_tasks(hf_ctx* ctx, double thr, double t32, TaskSet*& hi, TaskSet*& lo)
 
 // Expanded task list of one class: entry b * 256 + i is task i of launch block b, (-1, -1) in a
 // segment's padding.
+// With slices (tl.slice_prims > 0) a (bra, ket group) block of cnt kets with S slices holds cnt * S
+// tasks, slice-major (the lanes of a wavefront share a slice and hold consecutive kets).
 __global__ void __launch_bounds__(256) expand_tasks_kernel(TaskList tl, int2* __restrict__ out) {
     const TaskSeg sg = tl.seg[segment_of(tl.seg, tl.nseg, blockIdx.x, true)];
     const int64_t t = ((int64_t)blockIdx.x - sg.block0) * blockDim.x + threadIdx.x;
@@ -425,13 +428,33 @@ __global__ void __launch_bounds__(256) expand_tasks_kernel(TaskList tl, int2* __
         const int64_t* __restrict__ pre = tl.prefix + sg.prefix_off;
         const int pl = bra_of_task(pre, t, 0, sg.nbra);
         r.x = sg.bra0 + pl;
-        r.y = ket_of_task(tl, sg, pre, pl, t);
+        if (tl.slice_prims <= 0) {
+            r.y = ket_of_task(tl, sg, pre, pl, t);
+        } else {
+            const int* ko = tl.koff + sg.koff_off + (int64_t)pl * (sg.nkg + 1);
+            const int* kb = tl.kbeg + sg.kbeg_off + (int64_t)pl * sg.nkg;
+            const int rank = (int)(t - pre[pl]);
+            int g = 0;
+            while (g + 1 < sg.nkg && ko[g + 1] <= rank) ++g;
+            const int npb = tl.bra.rec[r.x].nprim, npk = tl.ket.rec[tl.kgroup[g]].nprim;
+            const int S = slice_count(npb, npk, tl.slice_prims);
+            const int cnt = (ko[g + 1] - ko[g]) / S, lr = rank - ko[g];
+            const int sl = lr / cnt, q = tl.kgroup[g] + kb[g] + (lr - sl * cnt);
+            r.y = S > 1 ? q | ((sl + 1) << kTaskKetBits) : q;
+        }
     }
     out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = r;
 }
 
-int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper) {
+// J/K task sets slice heavily contracted quartets (kSliceTargetPrims primitive quartets per task)
+// in classes with fewer than kSliceClassMaxQuartets quartets; larger classes fill the chip with
+// whole quartets, and slicing them would only multiply their digestion atomics.
+constexpr int64_t kSliceClassMaxQuartets = 1 << 18;
+constexpr int kSliceTargetPrims = 16;
+
+int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper, bool sliced) {
     ts.classes.clear();
+    ts.sliced = sliced;
     std::vector<int64_t> prefix_all;
     std::vector<int> koff_all, kbeg_all;
     std::vector<TaskSeg> seg_all;
@@ -452,6 +475,32 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper) {
         qc.seg_off = seg_all.size();
         const int nbg = B.group.empty() ? 0 : (int)B.group.size() - 1;
         const int nkg = K.group.empty() ? 0 : (int)K.group.size() - 1;
+        // kets [beg, end) of ket group kg (ranks within the group) for bra p of bra group bg
+        auto range = [&](int p, int bg, int kg, int& beg, int& end) {
+            const int b0 = B.group[bg], k0 = K.group[kg], k1 = K.group[kg + 1];
+            beg = end = 0;
+            if (same && kg > bg) return;        // one class: each unordered pair of pairs once (q <= p)
+            end = k1 - k0;
+            if (thr > 0) end = B.Q[p] > 0 ? head(K, k0, k1, thr / B.Q[p]) : 0;
+            if (upper > 0) beg = B.Q[p] > 0 ? head(K, k0, k1, upper / B.Q[p]) : 0;
+            if (same && kg == bg) {
+                end = std::min(end, p - b0 + 1);
+                beg = std::min(beg, end);
+            }
+            if (beg > end) beg = end;
+        };
+        // primitive slices only where the class has too few quartets to fill the chip by itself
+        if (sliced && ci.la + ci.lb + ci.lc + ci.ld < kCoopMinL) {
+            int64_t nq = 0;
+            for (int bg = 0; bg < nbg; ++bg)
+                for (int p = B.group[bg]; p < B.group[bg + 1]; ++p)
+                    for (int kg = 0; kg < nkg; ++kg) {
+                        int beg, end;
+                        range(p, bg, kg, beg, end);
+                        nq += end - beg;
+                    }
+            qc.slice_prims = nq < kSliceClassMaxQuartets ? kSliceTargetPrims : 0;
+        }
         for (int bg = 0; bg < nbg; ++bg) {
             const int b0 = B.group[bg], b1 = B.group[bg + 1];
             TaskSeg sg{};
@@ -467,20 +516,12 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper) {
                 int own = 0;
                 koff_all.push_back(0);
                 for (int kg = 0; kg < nkg; ++kg) {
-                    const int k0 = K.group[kg], k1 = K.group[kg + 1];
-                    int beg = 0, end = 0;
-                    if (!(same && kg > bg)) {       // one class: each unordered pair of pairs once (q <= p)
-                        end = k1 - k0;
-                        if (thr > 0) end = B.Q[p] > 0 ? head(K, k0, k1, thr / B.Q[p]) : 0;
-                        if (upper > 0) beg = B.Q[p] > 0 ? head(K, k0, k1, upper / B.Q[p]) : 0;
-                        if (same && kg == bg) {
-                            end = std::min(end, p - b0 + 1);
-                            beg = std::min(beg, end);
-                        }
-                        if (beg > end) beg = end;
-                    }
+                    const int k0 = K.group[kg];
+                    int beg, end;
+                    range(p, bg, kg, beg, end);
                     const int cnt = end - beg;
-                    own += cnt;
+                    own += cnt * slice_count(B.nprim[p], K.nprim[k0], qc.slice_prims);
+                    qc.quartets += cnt;
                     koff_all.push_back(own);
                     kbeg_all.push_back(beg);
                     qc.prim_quartets += (int64_t)cnt * B.nprim[p] * K.nprim[k0];
@@ -675,7 +716,7 @@ int hf_eri_packed_device(hf_ctx* ctx, double* d_out, uint64_t len, void* stream)
     const uint64_t need = hf_packed_index(n - 1, n - 1, n - 1, n - 1) + 1;
     if (!d_out || len < need) return ctx->fail(HF_E_ARG, "packed buffer too small");
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    int rc = ensure_tasks(ctx, 0.0, ctx->tasks_all);
+    int rc = ensure_tasks(ctx, 0.0, ctx->tasks_all, 0.0, false);   // packed store: whole quartets
     if (rc) return rc;
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(len)), dim3(256), 0, st, d_out, (size_t)len, 0.0);
     SinkArgs sink{};

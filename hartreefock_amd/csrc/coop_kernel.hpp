@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
         const int2 me = tl.tasks[blockIdx.x];
         if (me.x < 0) return;                      // segment padding
         p = me.x;
-        q = me.y;
+        q = me.y & kTaskKetMask;                   // (these classes are never sliced)
     }
     const PairRec bp = tl.bra.rec[p], kp = tl.ket.rec[q];
     const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;

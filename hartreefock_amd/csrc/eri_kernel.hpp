@@ -513,17 +513,24 @@ __host__ __device__ __forceinline__ void eval_quartet_reference_order(
                 }
 }
 
+// sl: primitive slice + 1 of a sliced J/K task (0: the whole quartet); target: the class's slice size.
 template <class C, typename R = double, bool BOYS_LDS = false>
 __device__ __forceinline__ void quartet_values(const ShellsDev& sh, const PrimPair* __restrict__ pp,
                                                const R* __restrict__ boys, const PairRec& bp, const PairRec& kp,
-                                               R* out) {
+                                               R* out, int sl = 0, int target = 0) {
     const double *xa = sh.xyz + 3 * bp.sa, *xb = sh.xyz + 3 * bp.sb, *xc = sh.xyz + 3 * kp.sa, *xd = sh.xyz + 3 * kp.sb;
     if constexpr (C::TRUNCATED) {
         static_assert(sizeof(R) == sizeof(double), "the reference-convention classes are FP64 only");
         eval_quartet_reference_order<C>(bp.sa, kp.sa, xa, xb, xc, xd, pp + bp.prim0, bp.nprim, pp + kp.prim0, kp.nprim,
                                         bp.a0, bp.b0, kp.a0, kp.b0, boys, out);
     } else {
-        eval_quartet<C, R, BOYS_LDS>(xa, xb, xc, xd, pp + bp.prim0, bp.nprim, pp + kp.prim0, kp.nprim, boys, out);
+        int b_off = 0, nb = bp.nprim;
+        if (sl > 0) {
+            const int chunk = slice_chunk(bp.nprim, kp.nprim, target);
+            b_off = (sl - 1) * chunk;
+            nb = min(chunk, bp.nprim - b_off);
+        }
+        eval_quartet<C, R, BOYS_LDS>(xa, xb, xc, xd, pp + bp.prim0 + b_off, nb, pp + kp.prim0, kp.nprim, boys, out);
     }
 }
 
@@ -573,7 +580,7 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
 #pragma unroll
     for (int i = 0; i < NAB; ++i) jab[i] = 0.0;
     if (active) {
-        const int p = me.x, q = me.y;
+        const int p = me.x, q = me.y & kTaskKetMask, sl = me.y >> kTaskKetBits;
         const PairRec bp = tl.bra.rec[p], kp = tl.ket.rec[q];
         const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;
         a0 = bp.a0;
@@ -581,9 +588,9 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
         const int c0 = kp.a0, d0 = kp.b0;
         R out[C::NOUT];
         if constexpr (kLdsBoys)
-            quartet_values<C, R, true>(sh, pp, (const R*)btab, bp, kp, out);
+            quartet_values<C, R, true>(sh, pp, (const R*)btab, bp, kp, out, sl, tl.slice_prims);
         else
-            quartet_values<C, R>(sh, pp, boys, bp, kp, out);
+            quartet_values<C, R>(sh, pp, boys, bp, kp, out, sl, tl.slice_prims);
         double w = 1.0;
         if (A == B) w *= 0.5;
         if (Cs == Ds) w *= 0.5;
@@ -652,7 +659,7 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
             const int2 me = tl.tasks[(size_t)blockIdx.x * blockDim.x + threadIdx.x];
             if (me.x < 0) return;
             p = me.x;
-            q = me.y;
+            q = me.y & kTaskKetMask;
         }
         const PairRec bp = tl.bra.rec[p], kp = tl.ket.rec[q];
         const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;

@@ -388,6 +388,7 @@ def main():
     disp.append("struct ClassInfo { int la, lb, lc, ld, exact_differs, prim_flops_ref, prim_flops_exact, hrr_flops; "
                 "LaunchFn fn; };")
     disp.append("constexpr int kLmax = %d;" % lmax)
+    disp.append("constexpr int kCoopMinL = %d;   // classes with LA+LB+LC+LD >= this use eri_coop_kernel" % COOP_MIN_L)
     disp.append("constexpr int kNumPairClasses = %d;" % len(pairs))
     disp.append("static const int kPairClassL[%d][2] = {%s};" % (len(pairs), ", ".join("{%d, %d}" % p for p in pairs)))
     disp.append("static const ClassInfo kClasses[] = {")

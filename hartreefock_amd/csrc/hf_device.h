@@ -91,6 +91,7 @@ struct TaskList {
     int64_t ntasks;          // sum over segments (MODE_DIAG: bra pairs)
     int64_t nblocks;         // sum over segments of whole 256-task blocks
     int same_class;          // bra and ket are one class: apply the p==q degeneracy
+    int slice_prims;         // primitive-slice target of this class's tasks (0: whole quartets)
 };
 
 // Segment containing launch unit x: the largest s with key(s) <= x, key = block0 (by_block) or task0.
@@ -127,6 +128,26 @@ __device__ __forceinline__ int bra_of_task(const int64_t* __restrict__ prefix, i
         if (prefix[mid] <= t) lo = mid; else hi = mid;
     }
     return lo;
+}
+
+// Primitive slices of a J/K task.  A quartet with many primitive quartets (npb x npk) is one long
+// dependent chain in one thread; in a class with few quartets (a small molecule) that chain is the
+// launch's duration.  J/K tasks of such a class split the bra primitives of a quartet with at
+// least 1.5 x target primitive quartets into slices of about target primitive quartets, one task
+// each: every slice evaluates its partial contracted block and digests it (J and K are linear in
+// the integrals).  A task's ket word carries the slice + 1 above kTaskKetBits (0: whole quartet).
+constexpr int kTaskKetBits = 24;
+constexpr int kTaskKetMask = (1 << kTaskKetBits) - 1;
+__host__ __device__ __forceinline__ int slice_chunk(int npb, int npk, int target) {   // bra primitives per slice
+    const int tot = npb * npk;
+    if (target <= 0 || npb < 2 || 2 * tot < 3 * target) return npb;
+    int s = (tot + target - 1) / target;
+    if (s > npb) s = npb;
+    return (npb + s - 1) / s;
+}
+__host__ __device__ __forceinline__ int slice_count(int npb, int npk, int target) {
+    const int c = slice_chunk(npb, npk, target);
+    return (npb + c - 1) / c;
 }
 
 struct SinkArgs {

@@ -82,7 +82,9 @@ struct DevBuf {
 struct QuartetClassTasks {
     int cls = -1;             // index into kClasses
     int bra = -1, ket = -1;   // pair-class indices
-    int64_t ntasks = 0;
+    int64_t ntasks = 0;       // tasks (a sliced quartet counts once per slice)
+    int64_t quartets = 0;     // shell quartets
+    int slice_prims = 0;      // primitive-slice target of the tasks (0: whole quartets)
     int64_t nblocks = 0;      // whole 256-task blocks over the segments
     int64_t prim_quartets = 0;
     size_t seg_off = 0;       // first segment in the shared device segment buffer
@@ -93,6 +95,7 @@ struct QuartetClassTasks {
 struct TaskSet {
     double threshold = -1;           // tasks with threshold <= Q_p Q_q < upper
     double upper = 0;
+    bool sliced = false;             // J/K tasks: heavily contracted quartets split into primitive slices
     std::vector<QuartetClassTasks> classes;
     DevBuf<int64_t> d_prefix;
     DevBuf<int> d_koff, d_kbeg;
@@ -177,7 +180,7 @@ constexpr int kSmallDense = 384;   // the matrix-core kernels above serve n <= t
 int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st, double thr,
                       bool accumulate, bool fp32);
 // Tasks with threshold <= Q_p Q_q (< upper when upper > 0).
-int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts, double upper = 0.0);
+int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts, double upper = 0.0, bool sliced = false);
 // Implemented in one_electron.hip: S, T, V (n x n, device) on stream st.
 int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st);
 
