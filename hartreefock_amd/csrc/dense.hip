@@ -101,8 +101,9 @@ __global__ void __launch_bounds__(256) dgemm16_kernel(int m, int n, int k, doubl
 constexpr int kSp2Hist = 8;
 
 __global__ void __launch_bounds__(1024) sp2_bounds_kernel(const double* __restrict__ F, int n, double* __restrict__ hist) {
+    __shared__ double part[1024];
     double lo, hi;
-    hfgpu::gershgorin_block(F, n, lo, hi);
+    hfgpu::gershgorin_block(F, n, part, lo, hi);
     if (threadIdx.x == 0) {
         if (hi - lo < 1e-12) hi = lo + 1.0;
         hist[0] = lo;
@@ -257,31 +258,9 @@ __global__ void __launch_bounds__(1024) sp2_local_kernel(const double* __restric
     __shared__ double e_hist[3];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int l = tid & 63, lr = l & 15, lk = l >> 4, w = tid >> 6, nw = nthr >> 6;
-    // Gershgorin bounds
-    double lo = 1e300, hi = -1e300;
-    for (int j = tid; j < n; j += nthr) {
-        double r = 0;
-        for (int i = 0; i < n; ++i)
-            if (i != j) r += fabs(Fp[(size_t)i * n + j]);
-        const double c = Fp[(size_t)j * n + j];
-        lo = fmin(lo, c - r);
-        hi = fmax(hi, c + r);
-    }
-    for (int msk = 32; msk >= 1; msk >>= 1) {
-        lo = fmin(lo, __shfl_xor(lo, msk));
-        hi = fmax(hi, __shfl_xor(hi, msk));
-    }
-    if (l == 0) {
-        red[0][w] = lo;
-        red[1][w] = hi;
-    }
-    __syncthreads();
-    lo = red[0][0];
-    hi = red[1][0];
-    for (int q = 1; q < nw; ++q) {
-        lo = fmin(lo, red[0][q]);
-        hi = fmax(hi, red[1][q]);
-    }
+    // Gershgorin bounds (16 row groups per column, so each thread reads ~n/16 elements)
+    double lo, hi;
+    hfgpu::gershgorin_block(Fp, n, lds, lo, hi);      // (X and X^2 are not set up yet)
     if (hi - lo < 1e-12) hi = lo + 1.0;
     const double inv = 1.0 / (hi - lo);
     for (int o = tid; o < n * n; o += nthr) X[o] = ((o / n == o % n ? hi : 0.0) - Fp[o]) * inv;
@@ -363,7 +342,7 @@ int hf_dgemm_small(hf_ctx* ctx, hipStream_t st, bool ta, bool tb, int m, int n, 
 int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp, double* X0, double* X1, double* W0,
                  double* W1, double* hist, double* part, int max_steps, int hint, double*& result, int& steps) {
     const int nt = (n + 15) / 16;
-    const size_t lds = 2 * (size_t)n * n * sizeof(double);
+    const size_t lds = std::max<size_t>(2 * (size_t)n * n, 1024) * sizeof(double);   // >= the bounds' workspace
     if (n <= kSp2LocalMax &&
         hipFuncSetAttribute((const void*)sp2_local_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
             hipSuccess) {

@@ -6,9 +6,10 @@ namespace hfgpu {
 
 // Gershgorin interval [lo, hi] of a symmetric n x n row-major matrix, by one block of 1024 threads:
 // 64 consecutive columns per pass (coalesced rows), 16 row groups per column, partial row sums
-// combined in a fixed order.  Every thread returns the block's lo and hi.
-__device__ __forceinline__ void gershgorin_block(const double* __restrict__ F, int n, double& lo_out, double& hi_out) {
-    __shared__ double part[1024], slo[1024], shi[1024];
+// combined in a fixed order.  part: at least 1024 doubles of LDS (free again on return).  Every
+// thread returns the block's lo and hi.
+__device__ __forceinline__ void gershgorin_block(const double* __restrict__ F, int n, double* part, double& lo_out,
+                                                 double& hi_out) {
     const int t = threadIdx.x, c = t & 63, g = t >> 6;
     double lo = 1e300, hi = -1e300;
     for (int j0 = 0; j0 < n; j0 += 64) {
@@ -28,18 +29,20 @@ __device__ __forceinline__ void gershgorin_block(const double* __restrict__ F, i
         }
         __syncthreads();
     }
-    slo[t] = lo;
-    shi[t] = hi;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s) {
-            slo[t] = fmin(slo[t], slo[t + s]);
-            shi[t] = fmax(shi[t], shi[t + s]);
+    if (g == 0) {                      // the first wavefront holds every column's bounds
+        for (int m = 32; m >= 1; m >>= 1) {
+            lo = fmin(lo, __shfl_xor(lo, m));
+            hi = fmax(hi, __shfl_xor(hi, m));
         }
-        __syncthreads();
+        if (c == 0) {
+            part[0] = lo;
+            part[1] = hi;
+        }
     }
-    lo_out = slo[0];
-    hi_out = shi[0];
+    __syncthreads();
+    lo_out = part[0];
+    hi_out = part[1];
+    __syncthreads();
 }
 
 }  // namespace hfgpu

@@ -226,8 +226,9 @@ __device__ double block_sum_256(double v, double* sh) {
 }
 
 __global__ void __launch_bounds__(1024) sp2_bounds_kernel(const double* __restrict__ F, int n, double* __restrict__ st) {
+    __shared__ double part[1024];
     double lo, hi;
-    hfgpu::gershgorin_block(F, n, lo, hi);
+    hfgpu::gershgorin_block(F, n, part, lo, hi);
     if (threadIdx.x == 0) {
         if (hi - lo < 1e-12) hi = lo + 1.0;
         st[SP2_LO] = lo;
