@@ -220,6 +220,7 @@ def main():
     elapsed, lat, res = timed_scfs(eng, a.steps, a.warmup)
     energies = [r.energy for r in res]
     iters = [r.iterations for r in res]
+    nbf = eng.nbf
     eng.close()
     # roofline pass (two more SCFs): only the dominant class carries HIP events, and it is launched
     # first in each build, so the events bracket its execution rather than a wait for CUs
@@ -259,7 +260,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded geometry)",
-        "config": {"workload": label, "nbf": eng.nbf, "schwarz_threshold": 1e-12, "energy_tolerance": 1e-8,
+        "config": {"workload": label, "nbf": nbf, "schwarz_threshold": 1e-12, "energy_tolerance": 1e-8,
                    "eri_convention": "exact",
                    "scf": "alpha=0.5, core guess, DIIS(6/5), normalIterAfterDIIS=0", "parallelism": "single GPU"},
         "energy_Eh": energies[-1],
