@@ -12,6 +12,7 @@ EXPORTED_SYMBOLS = [
     "hf_default_options", "hf_create", "hf_destroy", "hf_last_error", "hf_set_basis", "hf_set_molecule",
     "hf_nbf", "hf_nshell", "hf_electrons", "hf_nuclear_repulsion", "hf_packed_index", "hf_packed_len",
     "hf_eri_packed", "hf_eri_packed_device", "hf_build_jk", "hf_build_jk_device", "hf_one_electron",
+    "hf_pair_matrix_len", "hf_eri_pairs_device", "hf_build_jk_incore_device", "hf_build_jk_incore",
     "hf_scf_defaults", "hf_scf", "hf_kernel_stats", "hf_kernel_stats_reset",
     "hf_serve_default_options", "hf_serve_create", "hf_serve_submit", "hf_serve_poll", "hf_serve_pending",
     "hf_serve_footprint", "hf_serve_destroy",
@@ -49,7 +50,8 @@ class ScfParams(ctypes.Structure):
                 ("density_tolerance", ctypes.c_double), ("diis_tolerance", ctypes.c_double),
                 ("terminate", ctypes.POINTER(ctypes.c_int)), ("density_method", ctypes.c_int),
                 ("incremental_fock", ctypes.c_int), ("full_rebuild_every", ctypes.c_int),
-                ("final_fp64_iterations", ctypes.c_int)]
+                ("final_fp64_iterations", ctypes.c_int), ("incore", ctypes.c_int),
+                ("incore_max_bytes", ctypes.c_double)]
 
 
 class ScfResult(ctypes.Structure):
@@ -57,7 +59,7 @@ class ScfResult(ctypes.Structure):
                 ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("jk_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("fp32_builds", ctypes.c_int), ("diag_steps", ctypes.c_int),
                 ("full_builds", ctypes.c_int), ("incremental_builds", ctypes.c_int), ("quartets", ctypes.c_double),
-                ("sp2_steps", ctypes.c_int)]
+                ("sp2_steps", ctypes.c_int), ("incore", ctypes.c_int), ("eri_ms", ctypes.c_double)]
 
 
 class ServeOptions(ctypes.Structure):
@@ -117,6 +119,11 @@ def lib():
     L.hf_eri_packed_device.argtypes = [vp, vp, ctypes.c_uint64, vp]
     L.hf_build_jk.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.hf_build_jk_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+    L.hf_pair_matrix_len.argtypes = [ctypes.c_int]
+    L.hf_pair_matrix_len.restype = ctypes.c_uint64
+    L.hf_eri_pairs_device.argtypes = [vp, vp, ctypes.c_uint64, vp]
+    L.hf_build_jk_incore_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
+    L.hf_build_jk_incore.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.hf_one_electron.argtypes = [vp, dp, dp, dp]
     L.hf_scf_defaults.argtypes = [ctypes.POINTER(ScfParams)]
     L.hf_scf_defaults.restype = None
@@ -229,6 +236,21 @@ class Engine:
         """Device pointers (ints) in, device pointers out; stream is a hipStream_t as int (None: the engine's)."""
         self._check(lib().hf_build_jk_device(self.h, ndens, ctypes.c_void_p(d_D), ctypes.c_void_p(d_J),
                                              ctypes.c_void_p(d_K), ctypes.c_void_p(stream or 0)))
+
+    def build_jk_incore(self, D):
+        """J/K from the in-core ERI store (evaluated in the engine's device memory first)."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        n = self.nbf
+        nd = 1 if D.ndim == 2 else D.shape[0]
+        D3 = D.reshape(nd, n, n)
+        J = np.zeros((n, n))
+        K = np.zeros((nd, n, n))
+        self._check(lib().hf_build_jk_incore(self.h, nd, _dp(D3), _dp(J), _dp(K)))
+        return J, (K[0] if D.ndim == 2 else K)
+
+    def eri_pairs_device(self, d_G, length, stream=None):
+        """Fill the device pair matrix G (hf_pair_matrix_len(n) doubles) with the ERIs."""
+        self._check(lib().hf_eri_pairs_device(self.h, ctypes.c_void_p(d_G), length, ctypes.c_void_p(stream or 0)))
 
     def one_electron(self):
         n = self.nbf

@@ -369,6 +369,7 @@ int setup_molecule(hf_ctx* ctx) {
         if (rc) return rc;
     }
     ctx->tasks_all.threshold = -1;
+    ctx->tasks_incore.threshold = -1;
     ctx->tasks_by_level.clear();
     ctx->tasks_mixed_hi.clear();
     ctx->tasks_mixed_lo.clear();
@@ -620,6 +621,21 @@ int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, do
     return HF_OK;
 }
 
+int hf_eri_pairs_impl(hf_ctx* ctx, double* d_G, uint64_t len, hipStream_t st) {
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    const uint64_t n = (uint64_t)ctx->mol.nbf, np = n * (n + 1) / 2;
+    if (!d_G || len < np * np) return ctx->fail(HF_E_ARG, "pair-matrix buffer too small");
+    // whole quartets (the store needs complete values), screened at the context's threshold
+    if (int rc = ensure_tasks(ctx, ctx->opt.schwarz_threshold, ctx->tasks_incore, 0.0, false)) return rc;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(np * np)), dim3(256), 0, st, d_G, (size_t)(np * np), 0.0);
+    SinkArgs sink{};
+    sink.n = (int)n;
+    sink.packed = d_G;
+    sink.packed_len = np * np;
+    sink.pair_stride = np;
+    return run_classes(ctx, MODE_PACKED, ctx->tasks_incore, sink, st, true);
+}
+
 extern "C" {
 
 void hf_default_options(hf_options* o) {
@@ -738,6 +754,47 @@ int hf_eri_packed(hf_ctx* ctx, double* out, uint64_t len) {
     int rc = hf_eri_packed_device(ctx, d.p, len, nullptr);
     if (rc) return rc;
     HF_HIP_CHECK(ctx, hipMemcpyAsync(out, d.p, len * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return HF_OK;
+}
+
+uint64_t hf_pair_matrix_len(int nbf) {
+    const uint64_t np = (uint64_t)nbf * (nbf + 1) / 2;
+    return np * np;
+}
+
+int hf_eri_pairs_device(hf_ctx* ctx, double* d_G, uint64_t len, void* stream) {
+    if (!ctx) return HF_E_ARG;
+    return hf_eri_pairs_impl(ctx, d_G, len, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int hf_build_jk_incore_device(hf_ctx* ctx, int ndens, const double* d_G, const double* d_D, double* d_J, double* d_K,
+                              void* stream) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!d_G || !d_D || !d_J || !d_K) return ctx->fail(HF_E_ARG, "null device buffer");
+    return hf_incore_jk_impl(ctx, ndens, d_G, nullptr, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream,
+                             false);
+}
+
+int hf_build_jk_incore(hf_ctx* ctx, int ndens, const double* D, double* J, double* K) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!D || !J || !K || ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "bad arguments");
+    const int n = ctx->mol.nbf;
+    if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "in-core J/K takes at most 512 basis functions");
+    const size_t nn = (size_t)n * n;
+    const uint64_t len = hf_pair_matrix_len(n);
+    HF_HIP_CHECK(ctx, ctx->d_G.alloc(len));
+    if (int rc = hf_eri_pairs_impl(ctx, ctx->d_G.p, len, ctx->stream)) return rc;
+    HF_HIP_CHECK(ctx, ctx->d_D.alloc(2 * nn));
+    HF_HIP_CHECK(ctx, ctx->d_J.alloc(nn));
+    HF_HIP_CHECK(ctx, ctx->d_K.alloc(2 * nn));
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_D.p, D, ndens * nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = hf_incore_jk_impl(ctx, ndens, ctx->d_G.p, nullptr, ctx->d_D.p, ctx->d_J.p, ctx->d_K.p, ctx->stream, false);
+    if (rc) return rc;
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(J, ctx->d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(K, ctx->d_K.p, ndens * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return HF_OK;
 }

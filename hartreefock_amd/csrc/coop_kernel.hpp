@@ -174,8 +174,17 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
             if (Cs == Ds && c < d) continue;
             const uint64_t ab = two_index(a, b), cd = two_index(c, d);
             if (same_pair && ab < cd) continue;
-            const uint64_t pidx = two_index(ab, cd);
-            if (pidx < sink.packed_len) sink.packed[pidx] = v(idx);
+            if (sink.pair_stride) {
+                const uint64_t i1 = ab * sink.pair_stride + cd, i2 = cd * sink.pair_stride + ab;
+                if (i1 < sink.packed_len && i2 < sink.packed_len) {
+                    const double x = v(idx);
+                    sink.packed[i1] = x;
+                    sink.packed[i2] = x;
+                }
+            } else {
+                const uint64_t pidx = two_index(ab, cd);
+                if (pidx < sink.packed_len) sink.packed[pidx] = v(idx);
+            }
         }
     } else {
         double w = 1.0;

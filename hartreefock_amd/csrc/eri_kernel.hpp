@@ -405,8 +405,17 @@ __device__ __forceinline__ void sink_packed(const double* __restrict__ v, int A,
                     if (Cs == Ds && c < d) continue;
                     const uint64_t cd = two_index(c, d);
                     if (same_pair && ab < cd) continue;
-                    const uint64_t idx = two_index(ab, cd);
-                    if (idx < s.packed_len) s.packed[idx] = v[((ia * NB + ib) * NC + ic) * ND + id];
+                    const double x = v[((ia * NB + ib) * NC + ic) * ND + id];
+                    if (s.pair_stride) {
+                        const uint64_t i1 = ab * s.pair_stride + cd, i2 = cd * s.pair_stride + ab;
+                        if (i1 < s.packed_len && i2 < s.packed_len) {
+                            s.packed[i1] = x;
+                            s.packed[i2] = x;
+                        }
+                    } else {
+                        const uint64_t idx = two_index(ab, cd);
+                        if (idx < s.packed_len) s.packed[idx] = x;
+                    }
                 }
         }
 }

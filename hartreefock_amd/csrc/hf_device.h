@@ -158,6 +158,8 @@ struct SinkArgs {
     double* K;               // ndens x n x n accumulators (K' ; final K = K' + K'^T)
     double* packed;          // MODE_PACKED output (reference 8-fold packed order)
     uint64_t packed_len;
+    uint64_t pair_stride;    // MODE_PACKED: 0 = reference packed order; > 0 = symmetric pair matrix,
+                             // value at [ab * pair_stride + cd] and [cd * pair_stride + ab]
     double* Qout;            // MODE_DIAG output, one per bra pair
     int lds_slots;           // MODE_JK: bra pairs per block whose exchange rows are staged in LDS
     const float* boys32;     // MODE_JK32: FP32 copy of the Boys table

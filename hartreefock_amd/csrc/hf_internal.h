@@ -137,6 +137,11 @@ struct hf_ctx {
     TaskSet tasks_all;                     // unscreened (packed ERIs)
     std::map<int, TaskSet> tasks_by_level; // screened, by threshold level k (10^(k/4))
     std::map<int64_t, TaskSet> tasks_mixed_hi, tasks_mixed_lo;   // mixed precision: FP64 part, FP32 band
+    TaskSet tasks_incore;                  // screened, whole quartets (in-core ERI matrix)
+    // in-core ERIs: symmetric matrix over canonical function pairs, FP64 and its FP32 copy
+    DevBuf<double> d_G;
+    DevBuf<float> d_G32;
+    DevBuf<double> d_Dw;                   // packed weighted total density of an in-core J/K build
     // J/K scratch
     DevBuf<double> d_Jp, d_Kp, d_D, d_J, d_K;
     // one-electron kernel inputs: per-shell L/bf/prim0/nprim, and alpha|coef|Z|atom xyz
@@ -179,6 +184,16 @@ constexpr int kSmallDense = 384;   // the matrix-core kernels above serve n <= t
 // the quartets whose bound is below the context's fp32_threshold in FP32.
 int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, hipStream_t st, double thr,
                       bool accumulate, bool fp32);
+// In-core ERIs: G[P * np + Q] = (ij|kl) for canonical pairs P = i(i+1)/2 + j (i >= j) and
+// Q = k(k+1)/2 + l (k >= l), np = n(n+1)/2; quartets below the context's Schwarz threshold are 0.
+// len must be at least np * np.
+int hf_eri_pairs_impl(hf_ctx* ctx, double* d_G, uint64_t len, hipStream_t st);
+// J/K from in-core ERIs (G FP64, or FP32 when g32 is set) on stream st: J of the summed densities,
+// K per density, both complete (no symmetrisation needed); accumulate adds to d_J/d_K.
+int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
+                      double* d_K, hipStream_t st, bool accumulate);
+// Largest basis the in-core contraction kernel takes (its column registers cover 64 * 8 functions).
+constexpr int kIncoreMaxN = 512;
 // Tasks with threshold <= Q_p Q_q (< upper when upper > 0).
 int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts, double upper = 0.0, bool sliced = false);
 // Implemented in one_electron.hip: S, T, V (n x n, device) on stream st.

@@ -1,0 +1,241 @@
+// In-core J/K: the ERIs are evaluated once per SCF into HBM and every Fock build contracts them
+// with the density, which is the reference's own scheme (store every unique ERI,
+// HartreeFock/IntegralsRepository.cpp:566-584, then loop over the store each iteration,
+// RestrictedHartreeFock::InitFockMatrix, HartreeFock/RestrictedHartreeFock.cpp:189-202, and
+// UnrestrictedHartreeFock::InitFockMatrices, HartreeFock/UnrestrictedHartreeFock.cpp:228-238).
+//
+// Layout.  G is the symmetric matrix of ERIs over canonical function pairs,
+// G[P * np + Q] = (ij|kl), P = i(i+1)/2 + j (i >= j), Q = k(k+1)/2 + l (k >= l), np = n(n+1)/2:
+// twice the reference's 8-fold packed store, so that a Fock build reads every row P as one
+// contiguous stream and needs no scatter.  Row P (the bra pair (i,j)) is the lower triangle of the
+// symmetric n x n matrix B_P[k][l] = (ij|kl), stored row by row.
+//
+// Per row P, with D symmetric:
+//   J[i][j] = J[j][i] = sum_{k>=l} w_kl B_P[k][l] Dtot[k][l]      (w = 2 off the diagonal, 1 on it)
+//   K_d[i][:] += B_P D_d[j][:],   K_d[j][:] += B_P D_d[i][:]  (the second only when i != j)
+// B_P x splits into a row part (y[k] += B[k][l] x[l], l <= k: a sum across the lanes that hold one
+// row) and a column part (y[l] += B[k][l] x[k], l < k: a register of the lane that owns column l).
+// One workgroup per P: four wavefronts take rows k = w (mod 4), four rows at a time; lane L owns
+// columns L + 64 m.  The 8 (RHF) or 16 (UHF) row sums of four rows are reduced together by
+// recursive halving (each exchange step halves the values a lane carries), 10 or 17 shuffles
+// instead of 6 per value.  J_ij needs no atomics (one block per P); each block adds its two K rows
+// with coalesced FP64 atomics.
+//
+// The kernel streams np^2 elements (8 or 4 bytes) per build: HBM-bound, ~2 + 4 ndens flops per
+// element.  Each element is used once per density, so there is no operand reuse for the matrix cores.
+#include <hip/hip_runtime.h>
+
+#include "hf_internal.h"
+
+namespace {
+
+__global__ void incore_fill_kernel(double* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// Dw[k(k+1)/2 + l] = w_kl * sum_d D_d[k][l]
+__global__ void incore_dw_kernel(const double* __restrict__ D, int n, int ndens, double* __restrict__ Dw) {
+    const int k = blockIdx.x;
+    const size_t nn = (size_t)n * n;
+    for (int l = threadIdx.x; l <= k; l += blockDim.x) {
+        double s = 0.0;
+        for (int d = 0; d < ndens; ++d) s += D[d * nn + (size_t)k * n + l];
+        Dw[(size_t)k * (k + 1) / 2 + l] = (l == k ? 1.0 : 2.0) * s;
+    }
+}
+
+// Sum of V[0..CNT) across the wavefront by recursive halving: at exchange step S a lane keeps the
+// half of its values selected by bit S of its lane id and adds the partner's copy of that half.
+// Afterwards V[0] of lane L holds the total of value index L >> (6 - log2 CNT).
+template <int CNT, int S>
+struct Halving {
+    template <int NV>
+    __device__ static __forceinline__ void run(double (&V)[NV], int lane) {
+        if constexpr (CNT > 1) {
+            constexpr int H = CNT / 2;
+            const bool up = (lane & S) != 0;
+#pragma unroll
+            for (int t = 0; t < H; ++t) {
+                const double keep = up ? V[H + t] : V[t];
+                const double send = up ? V[t] : V[H + t];
+                V[t] = keep + __shfl_xor(send, S, 64);
+            }
+            if constexpr (S > 1) Halving<H, S / 2>::run(V, lane);
+        } else {
+            V[0] += __shfl_xor(V[0], S, 64);
+            if constexpr (S > 1) Halving<1, S / 2>::run(V, lane);
+        }
+    }
+};
+
+template <int NV>
+constexpr int log2c() {
+    return NV <= 1 ? 0 : 1 + log2c<NV / 2>();
+}
+
+template <int NDENS, int M, typename T>
+__global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t np,
+                                                        const double* __restrict__ D, const double* __restrict__ Dw,
+                                                        double* __restrict__ J, double* __restrict__ K) {
+    extern __shared__ double lds[];
+    const int64_t P = blockIdx.x;
+    int i = (int)((sqrt(8.0 * (double)P + 1.0) - 1.0) * 0.5);
+    if ((int64_t)i * (i + 1) / 2 > P) --i;
+    if ((int64_t)(i + 1) * (i + 2) / 2 <= P) ++i;
+    const int j = (int)(P - (int64_t)i * (i + 1) / 2);
+    double* xj = lds;               // D_d[j][:]
+    double* xi = xj + NDENS * n;    // D_d[i][:]
+    double* rI = xi + NDENS * n;    // additions to K_d[i][:]
+    double* rJ = rI + NDENS * n;    // additions to K_d[j][:]
+    double* red = rJ + NDENS * n;   // per-wavefront partials of J_ij
+    const size_t nn = (size_t)n * n;
+    for (int t = threadIdx.x; t < NDENS * n; t += blockDim.x) {
+        const int d = t / n, c = t - d * n;
+        xj[t] = D[d * nn + (size_t)j * n + c];
+        xi[t] = D[d * nn + (size_t)i * n + c];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double xjl[NDENS][M], xil[NDENS][M], cI[NDENS][M], cJ[NDENS][M];
+#pragma unroll
+    for (int d = 0; d < NDENS; ++d)
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int l = lane + 64 * m;
+            xjl[d][m] = l < n ? xj[d * n + l] : 0.0;
+            xil[d][m] = l < n ? xi[d * n + l] : 0.0;
+            cI[d][m] = 0.0;
+            cJ[d][m] = 0.0;
+        }
+    double jacc = 0.0;
+    const T* __restrict__ row0 = G + P * np;
+    constexpr int NV = 8 * NDENS;                  // 4 rows x (K_i, K_j) x densities
+    constexpr int SH = 6 - log2c<NV>();
+    for (int k0 = w; k0 < n; k0 += 16) {
+        double V[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) V[v] = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int k = k0 + 4 * r;
+            if (k >= n) break;                     // uniform across the wavefront
+            const int64_t base = (int64_t)k * (k + 1) / 2;
+            const T* __restrict__ rowp = row0 + base;
+            double xkj[NDENS], xki[NDENS];
+#pragma unroll
+            for (int d = 0; d < NDENS; ++d) {
+                xkj[d] = xj[d * n + k];
+                xki[d] = xi[d * n + k];
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (64 * m > k) break;             // uniform
+                const int l = lane + 64 * m;
+                if (l <= k) {
+                    const double g = (double)rowp[l];
+                    jacc = fma(g, Dw[base + l], jacc);
+#pragma unroll
+                    for (int d = 0; d < NDENS; ++d) {
+                        V[(2 * r) * NDENS + d] = fma(g, xjl[d][m], V[(2 * r) * NDENS + d]);
+                        V[(2 * r + 1) * NDENS + d] = fma(g, xil[d][m], V[(2 * r + 1) * NDENS + d]);
+                    }
+                    if (l < k) {
+#pragma unroll
+                        for (int d = 0; d < NDENS; ++d) {
+                            cI[d][m] = fma(g, xkj[d], cI[d][m]);
+                            cJ[d][m] = fma(g, xki[d], cJ[d][m]);
+                        }
+                    }
+                }
+            }
+        }
+        Halving<NV, 32>::run(V, lane);
+        if ((lane & ((1 << SH) - 1)) == 0) {
+            const int idx = lane >> SH;
+            const int r = idx / (2 * NDENS), a = (idx / NDENS) & 1, d = idx % NDENS;
+            const int k = k0 + 4 * r;
+            if (k < n) (a ? rJ : rI)[d * n + k] = V[0];   // row k belongs to this wavefront alone
+        }
+    }
+    // J_ij: wavefront sums, then the block's
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) jacc += __shfl_xor(jacc, s, 64);
+    if (lane == 0) red[w] = jacc;
+    __syncthreads();
+    // column parts of every wavefront
+#pragma unroll
+    for (int d = 0; d < NDENS; ++d)
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int l = lane + 64 * m;
+            if (l < n) {
+                atomicAdd(&rI[d * n + l], cI[d][m]);
+                atomicAdd(&rJ[d * n + l], cJ[d][m]);
+            }
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double v = (red[0] + red[1]) + (red[2] + red[3]);
+        J[(size_t)i * n + j] += v;
+        if (i != j) J[(size_t)j * n + i] += v;
+    }
+    for (int t = threadIdx.x; t < NDENS * n; t += blockDim.x) {
+        const int d = t / n, c = t - d * n;
+        atomicAdd(&K[d * nn + (size_t)i * n + c], rI[t]);
+        if (i != j) atomicAdd(&K[d * nn + (size_t)j * n + c], rJ[t]);
+    }
+}
+
+template <int NDENS, typename T>
+hipError_t launch_incore(const T* G, int n, int64_t np, const double* D, const double* Dw, double* J, double* K,
+                         hipStream_t st) {
+    const size_t lds = (4 * (size_t)NDENS * n + 4) * sizeof(double);
+    const dim3 grid((unsigned)np), block(256);
+    switch ((n + 63) / 64) {
+#define HF_INCORE_CASE(MM)                                                                                    \
+    case MM:                                                                                                  \
+        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MM, T>), grid, block, lds, st, G, n, np, D, Dw, J, K); \
+        break;
+        HF_INCORE_CASE(1)
+        HF_INCORE_CASE(2)
+        HF_INCORE_CASE(3)
+        HF_INCORE_CASE(4)
+        HF_INCORE_CASE(5)
+        HF_INCORE_CASE(6)
+        HF_INCORE_CASE(7)
+        HF_INCORE_CASE(8)
+#undef HF_INCORE_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
+                      double* d_K, hipStream_t st, bool accumulate) {
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
+    const int n = ctx->mol.nbf;
+    if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "in-core J/K takes at most 512 basis functions");
+    const int64_t np = (int64_t)n * (n + 1) / 2;
+    const size_t nn = (size_t)n * n;
+    HF_HIP_CHECK(ctx, ctx->d_Dw.alloc((size_t)np));
+    hipLaunchKernelGGL(incore_dw_kernel, dim3(n), dim3(256), 0, st, d_D, n, ndens, ctx->d_Dw.p);
+    if (!accumulate) {
+        hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, d_J, nn);
+        hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((ndens * nn + 255) / 256)), dim3(256), 0, st, d_K,
+                           ndens * nn);
+    }
+    hipError_t e;
+    if (g32)
+        e = ndens == 1 ? launch_incore<1>(g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st)
+                       : launch_incore<2>(g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
+    else
+        e = ndens == 1 ? launch_incore<1>(G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st)
+                       : launch_incore<2>(G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
+    if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("in-core J/K: ") + hipGetErrorString(e));
+    return HF_OK;
+}

@@ -46,6 +46,11 @@ __global__ void add_kernel(const double* __restrict__ a, const double* __restric
         c[idx] = a[idx] + b[idx];
 }
 
+__global__ void to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, size_t n) {
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x)
+        b[idx] = (float)a[idx];
+}
+
 // F = g S_ij (h_ii + h_jj)/2 (g > 0) or h; Fa = F plus `asym` on (0,1) and (1,0).
 __global__ void guess_kernel(const double* __restrict__ S, const double* __restrict__ h, double g, int n, double asym,
                              double* __restrict__ Fa, double* __restrict__ Fb) {
@@ -555,6 +560,8 @@ void hf_scf_defaults(hf_scf_params* p) {
     p->incremental_fock = 0;
     p->full_rebuild_every = 8;
     p->final_fp64_iterations = 3;
+    p->incore = 0;
+    p->incore_max_bytes = 0;
 }
 
 int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out) {
@@ -604,6 +611,41 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     hipLaunchKernelGGL(add_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.T2.p, st.h.p, nn);
     // S^{-1/2}, symmetrised (:48-59)
     if (int rc = inverse_sqrt(ctx, st)) return rc;
+    // in-core ERIs (the reference's store-once scheme) when the pair matrix fits the budget; the
+    // mixed-precision tier contracts an FP32 copy until its final FP64 phase
+    const bool mixed_tier = ctx->opt.precision == HF_PREC_MIXED;
+    bool incore = false;
+    double eri_ms = 0;
+    hipEvent_t eri_ev[2] = {nullptr, nullptr};
+    struct EventGuard {
+        hipEvent_t* e;
+        ~EventGuard() {
+            for (int k = 0; k < 2; ++k)
+                if (e[k]) (void)hipEventDestroy(e[k]);
+        }
+    } eri_guard{eri_ev};
+    if (p->incore && n <= kIncoreMaxN) {
+        const uint64_t len = hf_pair_matrix_len(n);
+        size_t free_b = 0, total_b = 0;
+        HF_HIP_CHECK(ctx, hipMemGetInfo(&free_b, &total_b));
+        const double want = (double)len * (mixed_tier ? 12.0 : 8.0);
+        const double held = (double)ctx->d_G.n * 8.0 + (double)ctx->d_G32.n * 4.0;   // stores of an earlier SCF
+        const double cap = p->incore_max_bytes > 0 ? p->incore_max_bytes : 0.5 * ((double)free_b + held);
+        if (want <= cap) {
+            incore = ctx->d_G.alloc(len) == hipSuccess && (!mixed_tier || ctx->d_G32.alloc(len) == hipSuccess);
+            (void)hipGetLastError();
+        }
+        if (incore) {
+            HF_HIP_CHECK(ctx, hipEventCreate(&eri_ev[0]));
+            HF_HIP_CHECK(ctx, hipEventCreate(&eri_ev[1]));
+            HF_HIP_CHECK(ctx, hipEventRecord(eri_ev[0], s));
+            if (int rc = hf_eri_pairs_impl(ctx, ctx->d_G.p, len, s)) return rc;
+            if (mixed_tier)
+                hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for(len)), dim3(256), 0, s, ctx->d_G.p, ctx->d_G32.p,
+                                   (size_t)len);
+            HF_HIP_CHECK(ctx, hipEventRecord(eri_ev[1], s));
+        }
+    }
     // occupations (HartreeFock/RestrictedHartreeFock.cpp:18-32, UnrestrictedHartreeFock.cpp:20-60)
     const int ne = ctx->mol.electrons();
     int occ[2];
@@ -679,7 +721,14 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             HF_HIP_CHECK(ctx, hipEventRecord(jk_ev[0], s));
             const double q0 = quartet_count();
             const bool full = !incremental || force_full || full_builds == 0 || since_full >= rebuild_every;
-            if (full) {
+            if (full && incore) {
+                if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, fp64_phase ? nullptr : ctx->d_G32.p, st.D.p, st.J.p,
+                                               st.K.p, s, false))
+                    return rc;
+                ++full_builds;
+                since_full = 0;
+                force_full = false;
+            } else if (full) {
                 if (int rc = hf_jk_device_impl(ctx, nsp, st.D.p, st.J.p, st.K.p, s, 0.0, false, !fp64_phase)) return rc;
                 ++full_builds;
                 since_full = 0;
@@ -692,7 +741,11 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 HF_HIP_CHECK(ctx, hipMemcpyAsync(&dmax, st.red.p + 15, sizeof(double), hipMemcpyDeviceToHost, s));
                 HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
                 last_dmax = dmax;
-                if (dmax > 0) {
+                if (dmax > 0 && incore) {
+                    if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, fp64_phase ? nullptr : ctx->d_G32.p, st.dD.p,
+                                                   st.J.p, st.K.p, s, true))
+                        return rc;
+                } else if (dmax > 0) {
                     const double thr = ctx->opt.schwarz_threshold / std::min(1.0, dmax);
                     if (int rc = hf_jk_device_impl(ctx, nsp, st.dD.p, st.J.p, st.K.p, s, thr, true, !fp64_phase))
                         return rc;
@@ -905,6 +958,13 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     r->incremental_builds = incr_builds;
     r->quartets = quartets;
     r->fp32_builds = fp32_builds;
+    r->incore = incore ? 1 : 0;
+    if (incore) {
+        float ms = 0;
+        HF_HIP_CHECK(ctx, hipEventSynchronize(eri_ev[1]));
+        HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, eri_ev[0], eri_ev[1]));
+        r->eri_ms = ms;
+    }
     if (uhf) {
         // <S^2> = ((Na-Nb)/2)^2 + (Na+Nb)/2 - sum_ij ((C_a,occ)^T S C_b,occ)_ij^2
         //       = ((Na-Nb)/2)^2 + (Na+Nb)/2 - Tr(P_a S P_b S), P = C_occ C_occ^T the last new densities

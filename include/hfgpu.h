@@ -114,6 +114,20 @@ int hf_build_jk(hf_ctx* ctx, int ndens, const double* D, double* J, double* K); 
 int hf_build_jk_device(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K,
                        void* stream);                                                       /* device */
 
+/* In-core J/K: the reference's scheme of storing every ERI once and contracting the store with the
+ * density in each Fock build (HartreeFock/IntegralsRepository.cpp:566-584, then
+ * HartreeFock/RestrictedHartreeFock.cpp:189-202 and HartreeFock/UnrestrictedHartreeFock.cpp:228-238),
+ * with the store in HBM.  The store is the symmetric matrix over canonical function pairs,
+ * G[P*np + Q] = (ij|kl), P = i(i+1)/2 + j (i >= j), Q = k(k+1)/2 + l (k >= l), np = n(n+1)/2
+ * (hf_pair_matrix_len(n) = np*np doubles: 2.9 GB for n = 194, 137 GB for n = 512), screened at the
+ * context's Schwarz threshold.  J and K as hf_build_jk; n <= 512. */
+uint64_t hf_pair_matrix_len(int nbf);
+int hf_eri_pairs_device(hf_ctx* ctx, double* d_G, uint64_t len, void* stream);
+int hf_build_jk_incore_device(hf_ctx* ctx, int ndens, const double* d_G, const double* d_D, double* d_J,
+                              double* d_K, void* stream);
+/* Host convenience: evaluates the store in the context's memory, then contracts. */
+int hf_build_jk_incore(hf_ctx* ctx, int ndens, const double* D, double* J, double* K);
+
 /* One-electron matrices S, T, V (HartreeFock/QuantumMatrix.cpp:34-155). Host buffers, n*n. */
 int hf_one_electron(hf_ctx* ctx, double* S, double* T, double* V);
 
@@ -143,6 +157,9 @@ typedef struct hf_scf_params {
     int full_rebuild_every;        /* incremental steps between full builds (default 8) */
     int final_fp64_iterations;     /* mixed precision (context option): all-FP64 steps required at the
                                       end before convergence is accepted (default 3) */
+    int incore;                    /* 1: evaluate the ERIs once per SCF into HBM (hf_eri_pairs_device)
+                                      and build J/K from the store when it fits; 0 (default): direct */
+    double incore_max_bytes;       /* largest store the SCF allocates (0: half the free device memory) */
 } hf_scf_params;
 
 typedef struct hf_scf_result {
@@ -159,6 +176,8 @@ typedef struct hf_scf_result {
     int incremental_builds;        /* J/K builds of a density change */
     double quartets;               /* shell quartets evaluated over all J/K builds */
     int sp2_steps;                 /* purification steps over all density steps (their final steps) */
+    int incore;                    /* 1: J/K came from the in-core ERI store */
+    double eri_ms;                 /* in-core: time to evaluate the store */
 } hf_scf_result;
 
 void hf_scf_defaults(hf_scf_params* p);

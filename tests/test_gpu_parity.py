@@ -279,3 +279,88 @@ def test_sp2_density_matches_diagonalisation(gpu_engine_factory, name, bfile, at
     assert r_sp2.converged and r_diag.converged
     assert abs(r_sp2.energy - r_diag.energy) <= 1e-8
     assert r_sp2.sp2_steps > 0
+
+
+# ---- in-core J/K: ERIs evaluated once into the pair matrix, contracted every build ----
+@pytest.mark.parametrize("name,bfile,atoms", SMALL[3:6], ids=[s[0] for s in SMALL[3:6]])
+@pytest.mark.parametrize("ndens", [1, 2])
+def test_incore_jk_matches_oracle(gpu_engine_factory, name, bfile, atoms, ndens):
+    """In-core J/K (pair-matrix store, one workgroup per bra pair) against the oracle's J/K loop over
+    its own packed ERIs (the reference's InitFockMatrix contraction), 1e-10 relative."""
+    _print_case(name)
+    eng = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis(bfile), atoms)
+    mol = refcpu.Molecule(basis(bfile), atoms)
+    n = eng.nbf
+    eri = mol.eri_packed()
+    rng = np.random.default_rng(17)
+    Ds = []
+    for _ in range(ndens):
+        X = rng.normal(size=(n, n))
+        Ds.append(0.5 * (X + X.T))
+    D = np.array(Ds) if ndens == 2 else Ds[0]
+    J, K = eng.build_jk_incore(D)
+    Jref = sum(refcpu.jk_from_packed(n, eri, d)[0] for d in Ds)
+    assert np.abs(J - Jref).max() <= 1e-10 * max(1.0, np.abs(Jref).max())
+    Kg = K if ndens == 2 else [K]
+    for d, k in zip(Ds, Kg):
+        Kref = refcpu.jk_from_packed(n, eri, d)[1]
+        assert np.abs(k - Kref).max() <= 1e-10 * max(1.0, np.abs(Kref).max())
+
+
+@pytest.mark.parametrize("name,bfile,atoms", SCALE_CASES, ids=[c[0] for c in SCALE_CASES])
+def test_incore_jk_at_scale_matches_packed_contraction(gpu_engine_factory, name, bfile, atoms):
+    """In-core J/K at 99 and 150 functions (two and three column registers per lane, rows longer than
+    a wavefront) against the oracle's J/K loop over the GPU's packed store, and against direct J/K."""
+    _print_case(name)
+    eng = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis(bfile), atoms)
+    n = eng.nbf
+    eri = eng.eri_packed()
+    rng = np.random.default_rng(19)
+    X = rng.normal(size=(2, n, n)) * 0.1
+    D = X + X.transpose(0, 2, 1)
+    J, K = eng.build_jk_incore(D)
+    Jref = sum(refcpu.jk_from_packed(n, eri, d)[0] for d in D)
+    assert np.abs(J - Jref).max() <= 1e-11 * max(1.0, np.abs(Jref).max())
+    for d, k in zip(D, K):
+        Kref = refcpu.jk_from_packed(n, eri, d)[1]
+        assert np.abs(k - Kref).max() <= 1e-11 * max(1.0, np.abs(Kref).max())
+    Jd, Kd = eng.build_jk(D)
+    assert np.abs(J - Jd).max() <= 1e-11 * max(1.0, np.abs(Jd).max())
+    assert np.abs(K - Kd).max() <= 1e-11 * max(1.0, np.abs(Kd).max())
+
+
+INCORE_SCF_CASES = [
+    ("h2o_crawford_sto3g", "sto3g.txt", g.water_crawford(), False, dict(alpha=0.5, init_guess=0.7)),
+    ("ethane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(2, seed=5), False, dict(alpha=0.5, init_guess=0.0)),
+    ("ch3_631gs_uhf", "6-31g_st_.1.nw", UHF_CASES[1][2], True, dict(alpha=0.5, init_guess=0.0)),
+]
+
+
+@pytest.mark.parametrize("name,bfile,atoms,uhf,kw", INCORE_SCF_CASES, ids=[c[0] for c in INCORE_SCF_CASES])
+def test_incore_scf_matches_oracle(gpu_engine_factory, name, bfile, atoms, uhf, kw):
+    """SCF with in-core J/K: the oracle's energy within 1e-8 Eh (RHF and UHF)."""
+    _print_case(name)
+    kw = dict(kw, normal_iter_after_diis=0)
+    eng = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis(bfile), atoms)
+    r = eng.scf(unrestricted=uhf, incore=1, **kw)
+    assert r.incore == 1
+    ref, _ = refcpu.Molecule(basis(bfile), atoms).scf(unrestricted=uhf, **kw)
+    assert r.converged and ref.converged
+    assert abs(r.energy - ref.energy) <= E_TOL, (r.energy, ref.energy)
+
+
+@pytest.mark.parametrize("name,bfile,atoms", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
+def test_incore_mixed_scf_matches_fp64(gpu_engine_factory, name, bfile, atoms):
+    """Mixed tier with in-core J/K (FP32 copy of the store until the final FP64 steps) against the
+    all-FP64 direct SCF: |dE| <= 1e-6 Eh; and the FP64 in-core SCF against the direct one, 1e-8 Eh."""
+    _print_case(name)
+    kw = dict(MIXED_KW, energy_tolerance=1e-8, density_tolerance=1e-6, diis_tolerance=1e-6)
+    e64 = gpu_engine_factory(schwarz_threshold=1e-12, exact_transfer=1).set_molecule(basis(bfile), atoms).scf(**kw)
+    inc = gpu_engine_factory(schwarz_threshold=1e-12, exact_transfer=1).set_molecule(basis(bfile), atoms)
+    r64 = inc.scf(incore=1, **kw)
+    mix = gpu_engine_factory(schwarz_threshold=1e-12, exact_transfer=1, precision=1).set_molecule(basis(bfile), atoms)
+    rm = mix.scf(incore=1, **kw)
+    assert e64.converged and r64.converged and rm.converged
+    assert r64.incore == 1 and rm.incore == 1 and rm.fp32_builds > 0
+    assert abs(r64.energy - e64.energy) <= 1e-8
+    assert abs(rm.energy - e64.energy) <= 1e-6
