@@ -158,16 +158,17 @@ def serving(a):
     print(json.dumps(out), flush=True)
 
 
-def timed_scfs(eng, steps, warmup):
+def timed_scfs(eng, steps, warmup, **extra):
+    kw = dict(SCF_KW, **extra)
     for _ in range(warmup):
-        eng.scf(**SCF_KW)
+        eng.scf(**kw)
     eng.reset_kernel_stats()
     lat, res = [], []
     hip_sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         s0 = time.perf_counter()
-        r = eng.scf(**SCF_KW)
+        r = eng.scf(**kw)
         lat.append(time.perf_counter() - s0)
         res.append(r)
     hip_sync()
@@ -275,6 +276,35 @@ def main():
                      "flops_per_launch": per_launch_flops, "ms_per_launch": per_launch_ms,
                      "launches": dom.launches},
     }
+    # in-core leg: the ERIs evaluated once per SCF into a pair-matrix store in HBM (inside the timed
+    # step), every J/K build contracts the store; roofline of the contraction kernel from its HIP
+    # events in a profiled SCF (one launch per build on the SCF stream)
+    ei = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1).set_molecule(bfile, atoms)
+    el_i, lat_i, res_i = timed_scfs(ei, a.steps, a.warmup, incore=1)
+    ei.close()
+    if res_i[-1].incore:
+        ep = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, profile=1).set_molecule(bfile, atoms)
+        ep.scf(incore=1, **SCF_KW)
+        ep.reset_kernel_stats()
+        ep.scf(incore=1, **SCF_KW)
+        ist = [s for s in ep.kernel_stats() if s.la < 0 and not s.fp32][0]
+        ep.close()
+        bytes_launch = ist.eris / ist.launches * 8.0
+        ms_launch = ist.ms / ist.launches
+        itraffic = None
+        if a.workload == "decane" and os.path.exists(tfile):
+            with open(tfile) as fh:
+                rec = json.load(fh)["kernels"].get("incore_jk")
+            itraffic = rec["total"] if rec else None
+        out["incore"] = {
+            "value": a.steps / el_i, "unit": "molecules/s", "vs_direct": (a.steps / el_i) / out["value"],
+            "abs_dE_vs_direct_Eh": abs(res_i[-1].energy - energies[-1]), "iterations": res_i[-1].iterations,
+            "eri_store_ms": res_i[-1].eri_ms, "store_bytes": bytes_launch,
+            "jk_ms_per_build": res_i[-1].jk_ms / max(1, res_i[-1].full_builds + res_i[-1].incremental_builds),
+            "roofline": {"bound": "hbm", "kernel": "incore_jk_kernel (J/K from the store)",
+                         "achieved": bytes_launch / (ms_launch * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": bytes_launch / (ms_launch * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": itraffic,
+                         "ms_per_launch": ms_launch, "launches": ist.launches}}
     if not a.no_mixed:
         # mixed-precision tier: quartets with Schwarz bound < 1e-6 in FP32, final iterations FP64
         em = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, precision=1,
@@ -283,6 +313,16 @@ def main():
         out["mixed"] = {"value": a.steps / el_m, "unit": "molecules/s", "vs_fp64": (a.steps / el_m) / out["value"],
                         "abs_dE_vs_fp64_Eh": abs(res_m[-1].energy - energies[-1]), "iterations": res_m[-1].iterations,
                         "fp32_builds": res_m[-1].fp32_builds, "fp32_threshold": 1e-6, "final_fp64_iterations": 3}
+        # mixed tier with the in-core store: an FP32 copy of the store until the final FP64 steps
+        emi = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, precision=1,
+                        fp32_threshold=1e-6).set_molecule(bfile, atoms)
+        el_mi, lat_mi, res_mi = timed_scfs(emi, a.steps, a.warmup, incore=1)
+        emi.close()
+        out["mixed_incore"] = {"value": a.steps / el_mi, "unit": "molecules/s",
+                               "vs_fp64_direct": (a.steps / el_mi) / out["value"],
+                               "abs_dE_vs_fp64_Eh": abs(res_mi[-1].energy - energies[-1]),
+                               "iterations": res_mi[-1].iterations, "fp32_builds": res_mi[-1].fp32_builds,
+                               "incore": res_mi[-1].incore}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(bfile, atoms, iters[-1], a.cpu_budget)
     print(json.dumps(out), flush=True)

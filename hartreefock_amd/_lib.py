@@ -81,6 +81,8 @@ class KernelStat(ctypes.Structure):
 
     def name(self):
         s = "spdfg"
+        if self.la < 0:
+            return "incore_jk" + ("_f32" if self.fp32 else "")
         return "(%s%s|%s%s)%s" % (s[self.la], s[self.lb], s[self.lc], s[self.ld], "f32" if self.fp32 else "")
 
 

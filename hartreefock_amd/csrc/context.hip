@@ -139,6 +139,13 @@ int flush_kernel_times(hf_ctx* ctx) {
         ctx->stats[c].ms += ms;
     }
     ctx->ev_pending.clear();
+    if (ctx->incore_pending >= 0) {
+        HF_HIP_CHECK(ctx, hipEventSynchronize(ctx->incore_ev[1]));
+        float ms = 0;
+        HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->incore_ev[0], ctx->incore_ev[1]));
+        ctx->incore_stat[ctx->incore_pending].ms += ms;
+        ctx->incore_pending = -1;
+    }
     return HF_OK;
 }
 
@@ -681,6 +688,8 @@ int hf_destroy(hf_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->side_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->incore_ev)
+        if (e) (void)hipEventDestroy(e);
     for (hipStream_t s2 : ctx->side) (void)hipStreamDestroy(s2);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -860,6 +869,20 @@ int hf_kernel_stats(const hf_ctx* ctx, hf_kernel_stat* out, int max) {
         out[k].fp32 = s.fp32;
         ++k;
     }
+    // the in-core contraction, as class (-1 -1 | -1 -1)
+    for (int f = 0; f < 2 && k < max; ++f) {
+        const KernelStat& s = ctx->incore_stat[f];
+        if (s.launches == 0) continue;
+        out[k].la = out[k].lb = out[k].lc = out[k].ld = -1;
+        out[k].launches = s.launches;
+        out[k].quartets = 0;
+        out[k].prim_quartets = 0;
+        out[k].eris = s.eris;
+        out[k].flops = s.flops;
+        out[k].ms = s.ms;
+        out[k].fp32 = f;
+        ++k;
+    }
     return k;
 }
 
@@ -872,6 +895,7 @@ int hf_kernel_stats_reset(hf_ctx* ctx) {
         s.cls = c;
         s.fp32 = f;
     }
+    ctx->incore_stat[0] = ctx->incore_stat[1] = KernelStat{};
     return HF_OK;
 }
 

@@ -142,6 +142,11 @@ struct hf_ctx {
     DevBuf<double> d_G;
     DevBuf<float> d_G32;
     DevBuf<double> d_Dw;                   // packed weighted total density of an in-core J/K build
+    // in-core contraction statistics ([0] FP64 store, [1] FP32 store): launches, elements read
+    // (eris), algorithmic flops, and with profile = 1 HIP-event time
+    KernelStat incore_stat[2];
+    hipEvent_t incore_ev[2] = {nullptr, nullptr};
+    int incore_pending = -1;               // stat index whose last launch's events are not yet read
     // J/K scratch
     DevBuf<double> d_Jp, d_Kp, d_D, d_J, d_K;
     // one-electron kernel inputs: per-shell L/bf/prim0/nprim, and alpha|coef|Z|atom xyz

@@ -116,37 +116,49 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
         double V[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) V[v] = 0.0;
+        // rows k0 + 4r (r < 4); the four rows' elements of one column chunk are loaded together
+        int base[4], kr[4];                        // row offsets fit 32 bits: np < 2^31 for n <= 512
+        double xkj[4][NDENS], xki[4][NDENS];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int k = k0 + 4 * r;
-            if (k >= n) break;                     // uniform across the wavefront
-            const int64_t base = (int64_t)k * (k + 1) / 2;
-            const T* __restrict__ rowp = row0 + base;
-            double xkj[NDENS], xki[NDENS];
+            kr[r] = k < n ? k : -1;                // -1: past the last row (no element)
+            const int kk = k < n ? k : n - 1;
+            base[r] = kk * (kk + 1) / 2;
 #pragma unroll
             for (int d = 0; d < NDENS; ++d) {
-                xkj[d] = xj[d * n + k];
-                xki[d] = xi[d * n + k];
+                xkj[r][d] = xj[d * n + kk];
+                xki[r][d] = xi[d * n + kk];
+            }
+        }
+        const int klast = k0 + 12 < n ? k0 + 12 : n - 1;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            if (64 * m > klast) break;             // uniform across the wavefront
+            const int l = lane + 64 * m;
+            double gv[4], dv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int kk = kr[r] < 0 ? n - 1 : kr[r];
+                const int at = base[r] + (l <= kk ? l : kk);     // clamped: always inside the row
+                const double gx = (double)row0[at];
+                dv[r] = Dw[at];
+                gv[r] = l <= kr[r] ? gx : 0.0;
             }
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                if (64 * m > k) break;             // uniform
-                const int l = lane + 64 * m;
-                if (l <= k) {
-                    const double g = (double)rowp[l];
-                    jacc = fma(g, Dw[base + l], jacc);
+            for (int r = 0; r < 4; ++r) {
+                const double g = gv[r];
+                jacc = fma(g, dv[r], jacc);
 #pragma unroll
-                    for (int d = 0; d < NDENS; ++d) {
-                        V[(2 * r) * NDENS + d] = fma(g, xjl[d][m], V[(2 * r) * NDENS + d]);
-                        V[(2 * r + 1) * NDENS + d] = fma(g, xil[d][m], V[(2 * r + 1) * NDENS + d]);
-                    }
-                    if (l < k) {
+                for (int d = 0; d < NDENS; ++d) {
+                    V[(2 * r) * NDENS + d] = fma(g, xjl[d][m], V[(2 * r) * NDENS + d]);
+                    V[(2 * r + 1) * NDENS + d] = fma(g, xil[d][m], V[(2 * r + 1) * NDENS + d]);
+                }
+                const double gc = l < kr[r] ? g : 0.0;            // the diagonal element has no column part
 #pragma unroll
-                        for (int d = 0; d < NDENS; ++d) {
-                            cI[d][m] = fma(g, xkj[d], cI[d][m]);
-                            cJ[d][m] = fma(g, xki[d], cJ[d][m]);
-                        }
-                    }
+                for (int d = 0; d < NDENS; ++d) {
+                    cI[d][m] = fma(gc, xkj[r][d], cI[d][m]);
+                    cJ[d][m] = fma(gc, xki[r][d], cJ[d][m]);
                 }
             }
         }
@@ -229,6 +241,25 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
         hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((ndens * nn + 255) / 256)), dim3(256), 0, st, d_K,
                            ndens * nn);
     }
+    // statistics, and with profile = 1 events around the contraction (read at the next flush)
+    const int f = g32 ? 1 : 0;
+    const bool prof = ctx->opt.profile != 0;
+    if (prof) {
+        if (ctx->incore_pending >= 0) {
+            HF_HIP_CHECK(ctx, hipEventSynchronize(ctx->incore_ev[1]));
+            float ms = 0;
+            HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->incore_ev[0], ctx->incore_ev[1]));
+            ctx->incore_stat[ctx->incore_pending].ms += ms;
+            ctx->incore_pending = -1;
+        }
+        for (hipEvent_t& ev : ctx->incore_ev)
+            if (!ev) HF_HIP_CHECK(ctx, hipEventCreate(&ev));
+        HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[0], st));
+    }
+    KernelStat& ks = ctx->incore_stat[f];
+    ks.launches += 1;
+    ks.eris += (double)np * (double)np;                                  // elements streamed
+    ks.flops += (double)np * (double)np * (2.0 + 8.0 * ndens);           // J, and row + column parts of K
     hipError_t e;
     if (g32)
         e = ndens == 1 ? launch_incore<1>(g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st)
@@ -237,5 +268,9 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
         e = ndens == 1 ? launch_incore<1>(G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st)
                        : launch_incore<2>(G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
     if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("in-core J/K: ") + hipGetErrorString(e));
+    if (prof) {
+        HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[1], st));
+        ctx->incore_pending = f;
+    }
     return HF_OK;
 }
