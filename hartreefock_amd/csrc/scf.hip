@@ -684,7 +684,12 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     int full_builds = 0, incr_builds = 0, since_full = 0;
     bool force_full = false, last_incremental = false;
     double quartets = 0, last_dmax = 0, last_quartets = 0;
-    const bool incremental = p->incremental_fock != 0;
+    // In-core mixed tier: the first two builds are full FP64 (the density changes most there), then
+    // the J/K of each density change comes from the FP32 store, whose error scales with |D - Dprev|;
+    // full rebuilds and the final phase are FP64.
+    const bool incore_mixed = incore && mixed_tier;
+    const bool incremental = p->incremental_fock != 0 || incore_mixed;
+    const int first_full = incore_mixed ? 2 : 1;
     // J/K build time from events on the SCF stream, read at the step's own synchronisation
     struct EventPair {
         hipEvent_t e[2] = {nullptr, nullptr};
@@ -720,9 +725,11 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         } else {
             HF_HIP_CHECK(ctx, hipEventRecord(jk_ev[0], s));
             const double q0 = quartet_count();
-            const bool full = !incremental || force_full || full_builds == 0 || since_full >= rebuild_every;
+            const bool full = !incremental || force_full || full_builds < first_full || since_full >= rebuild_every;
+            bool used_fp32 = !fp64_phase;
             if (full && incore) {
-                if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, fp64_phase ? nullptr : ctx->d_G32.p, st.D.p, st.J.p,
+                used_fp32 = !fp64_phase && !incore_mixed;
+                if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, used_fp32 ? ctx->d_G32.p : nullptr, st.D.p, st.J.p,
                                                st.K.p, s, false))
                     return rc;
                 ++full_builds;
@@ -738,11 +745,13 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 hipLaunchKernelGGL(delta_kernel, dim3(1), dim3(1024), 0, s, st.D.p, st.Dprev.p, st.dD.p, nsp * nn,
                                    st.red.p + 15);
                 double dmax = 0;
-                HF_HIP_CHECK(ctx, hipMemcpyAsync(&dmax, st.red.p + 15, sizeof(double), hipMemcpyDeviceToHost, s));
-                HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
-                last_dmax = dmax;
-                if (dmax > 0 && incore) {
-                    if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, fp64_phase ? nullptr : ctx->d_G32.p, st.dD.p,
+                if (!incore) {   // the in-core contraction reads the whole store whatever max|dD| is
+                    HF_HIP_CHECK(ctx, hipMemcpyAsync(&dmax, st.red.p + 15, sizeof(double), hipMemcpyDeviceToHost, s));
+                    HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+                    last_dmax = dmax;
+                }
+                if (incore) {
+                    if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, used_fp32 ? ctx->d_G32.p : nullptr, st.dD.p,
                                                    st.J.p, st.K.p, s, true))
                         return rc;
                 } else if (dmax > 0) {
@@ -754,7 +763,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 ++since_full;
             }
             last_incremental = !full;
-            last_fp32 = !fp64_phase;
+            last_fp32 = used_fp32;
             if (last_fp32) {
                 ++fp32_builds;
                 fp64_steps = 0;
