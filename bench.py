@@ -106,26 +106,27 @@ def serving(a):
     mols = g.serving_mix(a.requests, seed=2024)
     rng = np.random.default_rng(77)
 
-    def run(prec, rate, tag):
+    def run(prec, rate, tag, slots=None, incore=None, reqs=None):
         shown = 0
-        srv = hf.Server(slots=a.slots, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1),
-                        scf=scf)
+        reqs = mols if reqs is None else reqs
+        srv = hf.Server(slots=slots or a.slots, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1),
+                        scf=dict(scf, incore=a.serve_incore if incore is None else incore))
         try:
-            gaps = rng.exponential(1.0 / rate, len(mols)) if rate else np.zeros(len(mols))
+            gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
             got = []
             t0 = time.perf_counter()
             due = 0.0
-            for (name, atoms), gap in zip(mols, gaps):
+            for (name, atoms), gap in zip(reqs, gaps):
                 due += gap
                 while time.perf_counter() - t0 < due:
                     got += srv.poll(64, 0)
                     time.sleep(min(0.001, max(0.0, due - (time.perf_counter() - t0))))
                 srv.submit(bfile, atoms)
-            while len(got) < len(mols):
+            while len(got) < len(reqs):
                 got += srv.poll(64, 1000)
                 if len(got) // 250 != shown:
                     shown = len(got) // 250
-                    print("serving %s: %d of %d done" % (tag, len(got), len(mols)), file=sys.stderr, flush=True)
+                    print("serving %s: %d of %d done" % (tag, len(got), len(reqs)), file=sys.stderr, flush=True)
         finally:
             srv.close()
         print("serving %s: finished in %.1f s" % (tag, time.perf_counter() - t0), file=sys.stderr, flush=True)
@@ -139,7 +140,10 @@ def serving(a):
                 "p99_ms": float(np.percentile(lat, 99)), "not_converged": bad, "nbf_min": min(nbf), "nbf_max": max(nbf),
                 "energies": {r.id: r.energy for r in got if r.status == 0 and r.converged}}
 
+    # the plain FP64 path: one molecule at a time, direct J/K, on the first plain_requests requests
+    plain = run(0, None, "plain", slots=1, incore=0, reqs=mols[:a.plain_requests])["molecules_per_s"]
     cap = run(0, None, "capacity")["molecules_per_s"]
+    cap_m = run(1, None, "capacity mixed")["molecules_per_s"]
     rate = 0.8 * cap
     r64 = run(0, rate, "fp64")
     r32 = run(1, rate, "mixed")
@@ -152,7 +156,12 @@ def serving(a):
            "dtype": "f64", "data": "synthetic (seeded geometries)",
            "config": {"workload": "serving mix: %d RHF requests, water clusters 1-12 / alkanes C1-C16, 6-31G" % a.requests,
                       "slots": a.slots, "arrival": "Poisson, rate 0.8 x closed-loop capacity", "seed": 77,
+                      "jk": "in-core ERI store" if a.serve_incore else "direct",
                       "closed_loop_capacity": cap, "rate": rate},
+           "plain_fp64_one_at_a_time": {"molecules_per_s": plain, "requests": a.plain_requests,
+                                        "jk": "direct", "slots": 1},
+           "capacity": {"fp64": cap, "mixed": cap_m},
+           "mixed_batched_vs_plain_fp64": cap_m / plain, "fp64_batched_vs_plain_fp64": cap / plain,
            "fp64": r64, "mixed": r32, "mixed_vs_fp64": r32["molecules_per_s"] / r64["molecules_per_s"],
            "max_abs_dE_mixed_vs_fp64_Eh": dmax}
     print(json.dumps(out), flush=True)
@@ -186,6 +195,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving workload: number of requests")
     ap.add_argument("--slots", type=int, default=8, help="serving workload: concurrent requests")
+    ap.add_argument("--plain-requests", type=int, default=300,
+                    help="serving workload: requests of the one-at-a-time plain FP64 reference run")
+    ap.add_argument("--serve-incore", type=int, default=1,
+                    help="serving workload: 1 = each request's ERIs evaluated once into an in-core store (0: direct)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()

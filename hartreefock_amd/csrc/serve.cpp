@@ -157,6 +157,8 @@ int hf_serve_submit(hf_server* s, const char* basis_file, int natoms, const int*
     if (!hfhost::load_molecule(basis_file, natoms, Z, xyz, m, err)) return HF_E_IO;
     rq.nbf = m.nbf;
     rq.footprint = hf_serve_footprint(m.nbf, rq.unrestricted);
+    if (s->opt.scf.incore && m.nbf <= 512)   // the in-core ERI store (and its FP32 copy in the mixed tier)
+        rq.footprint += (double)hf_pair_matrix_len(m.nbf) * (s->opt.ctx.precision == HF_PREC_MIXED ? 12.0 : 8.0);
     {
         std::lock_guard<std::mutex> lock(s->mu);
         rq.id = s->next_id++;

@@ -240,7 +240,8 @@ int hf_serve_submit(hf_server* s, const char* basis_file, int natoms, const int*
 int hf_serve_poll(hf_server* s, hf_serve_result* out, int max, int wait_ms);
 /* Requests submitted and not yet returned by hf_serve_poll. */
 int hf_serve_pending(hf_server* s);
-/* Estimated device bytes of an SCF on n basis functions (the admission rule). */
+/* Estimated device bytes of an SCF on n basis functions (the admission rule); a server whose SCF
+ * parameters ask for the in-core store adds hf_pair_matrix_len(n) doubles (x 1.5 in the mixed tier). */
 double hf_serve_footprint(int nbf, int unrestricted);
 /* Waits for the running requests, drops the queued ones, frees everything. */
 int hf_serve_destroy(hf_server* s);
