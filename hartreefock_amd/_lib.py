@@ -13,6 +13,7 @@ EXPORTED_SYMBOLS = [
     "hf_nbf", "hf_nshell", "hf_electrons", "hf_nuclear_repulsion", "hf_packed_index", "hf_packed_len",
     "hf_eri_packed", "hf_eri_packed_device", "hf_build_jk", "hf_build_jk_device", "hf_one_electron",
     "hf_pair_matrix_len", "hf_eri_pairs_device", "hf_build_jk_incore_device", "hf_build_jk_incore",
+    "hf_properties",
     "hf_scf_defaults", "hf_scf", "hf_kernel_stats", "hf_kernel_stats_reset",
     "hf_serve_default_options", "hf_serve_create", "hf_serve_submit", "hf_serve_poll", "hf_serve_pending",
     "hf_serve_footprint", "hf_serve_destroy",
@@ -51,7 +52,7 @@ class ScfParams(ctypes.Structure):
                 ("terminate", ctypes.POINTER(ctypes.c_int)), ("density_method", ctypes.c_int),
                 ("incremental_fock", ctypes.c_int), ("full_rebuild_every", ctypes.c_int),
                 ("final_fp64_iterations", ctypes.c_int), ("incore", ctypes.c_int),
-                ("incore_max_bytes", ctypes.c_double)]
+                ("incore_max_bytes", ctypes.c_double), ("mp2", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -59,7 +60,8 @@ class ScfResult(ctypes.Structure):
                 ("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("jk_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("fp32_builds", ctypes.c_int), ("diag_steps", ctypes.c_int),
                 ("full_builds", ctypes.c_int), ("incremental_builds", ctypes.c_int), ("quartets", ctypes.c_double),
-                ("sp2_steps", ctypes.c_int), ("incore", ctypes.c_int), ("eri_ms", ctypes.c_double)]
+                ("sp2_steps", ctypes.c_int), ("incore", ctypes.c_int), ("eri_ms", ctypes.c_double),
+                ("mp2_energy", ctypes.c_double)]
 
 
 class ServeOptions(ctypes.Structure):
@@ -127,6 +129,7 @@ def lib():
     L.hf_build_jk_incore_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
     L.hf_build_jk_incore.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.hf_one_electron.argtypes = [vp, dp, dp, dp]
+    L.hf_properties.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.hf_scf_defaults.argtypes = [ctypes.POINTER(ScfParams)]
     L.hf_scf_defaults.restype = None
     L.hf_scf.argtypes = [vp, ctypes.POINTER(ScfParams), ctypes.POINTER(ScfResult), dp]
@@ -196,6 +199,7 @@ class Engine:
             raise HFError(rc, lib().hf_last_error(self.h).decode())
 
     def set_molecule(self, basis_file, atoms):
+        self._natoms = len(atoms)
         z = np.ascontiguousarray([a[0] for a in atoms], dtype=np.int32)
         xyz = np.ascontiguousarray([c for a in atoms for c in a[1]], dtype=np.float64)
         self._check(lib().hf_set_molecule(self.h, basis_file.encode(), len(atoms), _ip(z), _dp(xyz)))
@@ -253,6 +257,16 @@ class Engine:
     def eri_pairs_device(self, d_G, length, stream=None):
         """Fill the device pair matrix G (hf_pair_matrix_len(n) doubles) with the ERIs."""
         self._check(lib().hf_eri_pairs_device(self.h, ctypes.c_void_p(d_G), length, ctypes.c_void_p(stream or 0)))
+
+    def properties(self, D):
+        """Mulliken charges (per atom) and dipole moment (au) of D: the RHF total density (n x n) or the
+        UHF alpha and beta densities (2 x n x n)."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        nd = 1 if D.ndim == 2 else D.shape[0]
+        q = np.zeros(self._natoms)
+        mu = np.zeros(3)
+        self._check(lib().hf_properties(self.h, nd, _dp(D), _dp(q), _dp(mu)))
+        return q, mu
 
     def one_electron(self):
         n = self.nbf

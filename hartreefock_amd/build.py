@@ -64,7 +64,7 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
         units.append((src, True, kdeps))
     for src in sorted(glob.glob(os.path.join(GEN, "d_*.hip"))):
         units.append((src, True, [os.path.join(CSRC, "hf_device.h")]))
-    for name in ("context.hip", "scf.hip", "one_electron.hip", "dense.hip", "incore.hip"):
+    for name in ("context.hip", "scf.hip", "one_electron.hip", "dense.hip", "incore.hip", "post.hip"):
         units.append((os.path.join(CSRC, name), True, hdeps))
     for name in ("basis.cpp", "boys.cpp", "serve.cpp"):
         units.append((os.path.join(CSRC, name), False, hdeps))

@@ -201,8 +201,15 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
 constexpr int kIncoreMaxN = 512;
 // Tasks with threshold <= Q_p Q_q (< upper when upper > 0).
 int hf_build_tasks(hf_ctx* ctx, double threshold, TaskSet& ts, double upper = 0.0, bool sliced = false);
-// Implemented in one_electron.hip: S, T, V (n x n, device) on stream st.
-int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st);
+// Implemented in one_electron.hip: S, T, V (n x n, device) on stream st; with dM (3 n x n) also
+// the dipole moment integrals about the origin (x, y, z).
+int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st, double* dM = nullptr);
+// Implemented in post.hip: MP2 sum over the nocc occupied orbitals of C (column-major, orbital p at
+// C + p n) with energies eps, from the in-core store G; same_spin: the reference's UHF per-spin sum.
+int hf_mp2_impl(hf_ctx* ctx, hipStream_t st, int nocc, const double* C, const double* eps, const double* G,
+                bool same_spin, double& e2);
+// Mulliken charges (natoms) and dipole (3) from the device densities d_D (ndens x n x n).
+int hf_properties_impl(hf_ctx* ctx, int ndens, const double* d_D, double* charges, double* dipole, hipStream_t st);
 
 #define HF_HIP_CHECK(ctx, call)                                                                         \
     do {                                                                                                \

@@ -93,6 +93,7 @@ struct OneEArgs {
     const double* Z;
     const double* boys;
     double *S, *T, *V;
+    double* M;               // optional: dipole integrals about the origin, x, y, z (3 n x n)
 };
 
 __global__ void zero_kernel(double* __restrict__ x, size_t n) {
@@ -114,8 +115,14 @@ __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a, const int
     const double AB[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
     const double C[3] = {a.axyz[3 * c], a.axyz[3 * c + 1], a.axyz[3 * c + 2]};
     double s[NA * NB], k[NA * NB], v[NA * NB];
+    double mom[FIRST ? 3 : 1][NA * NB];
 #pragma unroll
-    for (int i = 0; i < NA * NB; ++i) s[i] = k[i] = v[i] = 0.0;
+    for (int i = 0; i < NA * NB; ++i) {
+        s[i] = k[i] = v[i] = 0.0;
+#pragma unroll
+        for (int d = 0; d < (FIRST ? 3 : 1); ++d) mom[d][i] = 0.0;
+    }
+    const bool want_m = FIRST && a.M != nullptr;
     const int pa0 = a.sprim0[sa], pb0 = a.sprim0[sb], npa = a.snprim[sa], npb = a.snprim[sb];
     for (int ia = 0; ia < npa; ++ia)
         for (int ib = 0; ib < npb; ++ib) {
@@ -142,6 +149,18 @@ __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a, const int
                         }
                         s[i * NB + j] += w * s1[0] * s1[1] * s1[2];
                         k[i * NB + j] += w * (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]);
+                        if (want_m) {
+                            // <a|x|b> about the origin: x = (x - P_x) + P_x, and the integral of
+                            // (x - P_x) against the Hermite expansion keeps only t = 1:
+                            // (E_1 + P_x E_0) sqrt(pi/p) in the moment direction.
+#pragma unroll
+                            for (int d = 0; d < (FIRST ? 3 : 1); ++d) {
+                                const int li = kComp[LA][i][d], lj = kComp[LB][j][d];
+                                const double Pd = (ea * A[d] + eb * B[d]) / p;
+                                const double m1 = (E[d][li][lj][1] + Pd * E[d][li][lj][0]) * sq;
+                                mom[d][i * NB + j] += w * m1 * (s1[(d + 1) % 3] * s1[(d + 2) % 3]);
+                            }
+                        }
                     }
             }
             const double PC[3] = {(ea * A[0] + eb * B[0]) / p - C[0], (ea * A[1] + eb * B[1]) / p - C[1],
@@ -212,6 +231,12 @@ __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a, const int
             if constexpr (first) {
                 a.S[r * n + cc] = a.S[cc * n + r] = f * s[i * NB + j];
                 a.T[r * n + cc] = a.T[cc * n + r] = f * k[i * NB + j];
+                if (want_m) {
+                    const size_t nn = (size_t)n * n;
+#pragma unroll
+                    for (int d = 0; d < (FIRST ? 3 : 1); ++d)
+                        a.M[d * nn + r * n + cc] = a.M[d * nn + cc * n + r] = f * mom[d][i * NB + j];
+                }
             }
             unsafeAtomicAdd(&a.V[r * n + cc], f * v[i * NB + j]);
             if (sa != sb) unsafeAtomicAdd(&a.V[cc * n + r], f * v[i * NB + j]);
@@ -221,7 +246,7 @@ __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a, const int
 }  // namespace
 
 // Device S, T, V for the context's molecule (row-major n x n device buffers), on stream st.
-int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st) {
+int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st, double* dM) {
     const hfhost::Molecule& mol = ctx->mol;
     const int ns = (int)mol.shells.size();
     if (!ctx->d_oe_L.p) {
@@ -268,6 +293,7 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
     a.S = dS;
     a.T = dT;
     a.V = dV;
+    a.M = dM;
     // shell pairs by class (La, Lb), La >= Lb, built once per molecule
     constexpr int kOeClasses = 6;
     static const int kOeL[kOeClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};

@@ -562,6 +562,7 @@ void hf_scf_defaults(hf_scf_params* p) {
     p->final_fp64_iterations = 3;
     p->incore = 0;
     p->incore_max_bytes = 0;
+    p->mp2 = 0;
 }
 
 int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out) {
@@ -956,6 +957,35 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         homo = eps_h[occ[0] - 1];
     else
         homo = std::max(occ[0] ? eps_h[occ[0] - 1] : 0.0, occ[1] ? eps_h[n + occ[1] - 1] : 0.0);
+    // MP2 from the orbitals of the last Fock matrices (the reference's C and eigenvalues after
+    // Calculate(), used by CalculateMp2Energy) and the in-core ERI store
+    if (p->mp2 && !cancelled) {
+        for (int k = 0; k < nsp; ++k) {
+            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.Fa.p + k * nn, st.X.p, 0.0, st.T1.p)) return rc;
+            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
+            if (int rc = syevd(ctx, st, st.T2.p, st.eps.p + k * n)) return rc;
+            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T2.p, 0.0, st.C.p + k * nn)) return rc;
+        }
+        if (!incore) {
+            if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "MP2 needs the in-core ERI store (n <= 512)");
+            const uint64_t len = hf_pair_matrix_len(n);
+            HF_HIP_CHECK(ctx, ctx->d_G.alloc(len));
+            if (int rc = hf_eri_pairs_impl(ctx, ctx->d_G.p, len, s)) return rc;
+        }
+        double e2 = 0;
+        if (!uhf) {
+            if (int rc = hf_mp2_impl(ctx, s, occ[0], st.C.p, st.eps.p, ctx->d_G.p, false, e2)) return rc;
+        } else {
+            for (int k = 0; k < 2; ++k) {
+                double ek = 0;
+                if (int rc = hf_mp2_impl(ctx, s, occ[k], st.C.p + k * nn, st.eps.p + k * n, ctx->d_G.p, true, ek))
+                    return rc;
+                e2 += ek;
+            }
+            e2 *= 0.25;   // UnrestrictedHartreeFock::CalculateMp2Energy's formula, as written
+        }
+        r->mp2_energy = e2;
+    }
     r->energy = cur;
     r->homo = homo;
     r->iterations = iter + (converged ? 1 : 0);

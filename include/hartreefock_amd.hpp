@@ -210,6 +210,7 @@ public:
         p.max_diis_iterations = maxDIISiterations;
         p.normal_iter_after_diis = normalIterAfterDIIS;
         p.terminate = &terminate;
+        p.mp2 = wantMp2 ? 1 : 0;
         hf_scf_result r;
         const int n = numberOfOrbitals;
         density_.assign((Unrestricted() ? 2 : 1) * (size_t)n * n, 0.0);
@@ -219,12 +220,42 @@ public:
         converged = r.converged != 0;
         iterations = r.iterations;
         s2 = r.s2;
+        if (wantMp2) mp2Energy = r.mp2_energy;
         Publish();
         return totalEnergy;
     }
     double GetTotalEnergy() const { return totalEnergy; }
     int lastStatus = HF_OK;
     double s2 = 0;
+    double mp2Energy = 0;
+
+    // HartreeFock/RestrictedHartreeFock.cpp:251-288, UnrestrictedHartreeFock.cpp:295-375.  The
+    // library computes MP2 inside the SCF call (from the orbitals of its last step and the in-core
+    // ERI store), so this re-runs Calculate()'s SCF with MP2 requested and returns the correlation
+    // energy; the SCF members are refreshed with the identical result.
+    double CalculateMp2Energy() {
+        if (!inited) return 0;
+        wantMp2 = true;
+        Calculate();
+        wantMp2 = false;
+        return mp2Energy;
+    }
+    // HartreeFock/RestrictedHartreeFock.cpp:291-317 (UHF: UnrestrictedHartreeFock.cpp:381-406)
+    double CalculateAtomicCharge(int atom) const {
+        std::vector<double> q(integralsRepository.getMolecule() ? integralsRepository.getMolecule()->atoms.size() : 0);
+        if (atom < 0 || (size_t)atom >= q.size() || density_.empty()) return 0;
+        double mu[3];
+        if (hf_properties(integralsRepository.context(), Unrestricted() ? 2 : 1, density_.data(), q.data(), mu) != HF_OK)
+            return 0;
+        return q[atom];
+    }
+    // HartreeFock/RestrictedHartreeFock.cpp:319-335 (UHF: UnrestrictedHartreeFock.cpp:408-420)
+    Vector3D<double> GetMoment() const {
+        double mu[3] = {0, 0, 0};
+        if (!density_.empty())
+            (void)hf_properties(integralsRepository.context(), Unrestricted() ? 2 : 1, density_.data(), nullptr, mu);
+        return Vector3D<double>(mu[0], mu[1], mu[2]);
+    }
 
 protected:
     virtual bool Unrestricted() const = 0;
@@ -232,6 +263,7 @@ protected:
     virtual void Publish() = 0;
     int maxIterations;
     bool inited = false;
+    bool wantMp2 = false;
     double totalEnergy = std::numeric_limits<double>::infinity();
     std::vector<double> density_;
 };

@@ -160,6 +160,10 @@ typedef struct hf_scf_params {
     int incore;                    /* 1: evaluate the ERIs once per SCF into HBM (hf_eri_pairs_device)
                                       and build J/K from the store when it fits; 0 (default): direct */
     double incore_max_bytes;       /* largest store the SCF allocates (0: half the free device memory) */
+    int mp2;                       /* 1: after the SCF, the MP2 correlation energy as the reference's
+                                      CalculateMp2Energy (HartreeFock/RestrictedHartreeFock.cpp:251-288,
+                                      HartreeFock/UnrestrictedHartreeFock.cpp:295-375) from the in-core
+                                      store (evaluated for it if the SCF ran direct; n <= 512) */
 } hf_scf_params;
 
 typedef struct hf_scf_result {
@@ -178,11 +182,19 @@ typedef struct hf_scf_result {
     int sp2_steps;                 /* purification steps over all density steps (their final steps) */
     int incore;                    /* 1: J/K came from the in-core ERI store */
     double eri_ms;                 /* in-core: time to evaluate the store */
+    double mp2_energy;             /* with mp2 = 1: MP2 correlation energy, Eh */
 } hf_scf_result;
 
 void hf_scf_defaults(hf_scf_params* p);
 /* D_out (optional): n*n (RHF total density) or 2*n*n (UHF alpha then beta). */
 int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out);
+
+/* Mulliken charges (natoms) and dipole moment (3, au, about the origin) of the densities D (host,
+ * ndens x n x n: the RHF total density, or UHF alpha then beta), replacing
+ * RestrictedHartreeFock::CalculateAtomicCharge / GetMoment (HartreeFock/RestrictedHartreeFock.cpp:291-335)
+ * and the UnrestrictedHartreeFock ones (HartreeFock/UnrestrictedHartreeFock.cpp:381-420).  Overlap and
+ * dipole integrals are computed on the device.  charges or dipole may be null. */
+int hf_properties(hf_ctx* ctx, int ndens, const double* D, double* charges, double* dipole);
 
 /* Per-class ERI kernel statistics, summed over every launch since the last reset
  * (J/K builds, packed-ERI passes).  No counterpart in the reference (it has no timers);

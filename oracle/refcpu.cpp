@@ -916,6 +916,40 @@ int ref_one_electron(ref_mol* m, double* S, double* T, double* V) {
     return 0;
 }
 
+/* Dipole moment integrals about the origin, <a|x|b>, <a|y|b>, <a|z|b> (MomentMatrix::Calculate,
+ * HartreeFock/QuantumMatrix.cpp:61-90, GaussianMoment.cpp:15-90 with centre3 = 0 from
+ * GaussianMoment.h:38-49): the factor of the moment direction is s(i+1, j) + A s(i, j) (x = x_A + A),
+ * the other two directions are the 1-D overlaps. */
+int ref_moment(ref_mol* m, double* Mx, double* My, double* Mz) {
+    const int n = (int)m->bfs.size();
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j) {
+            const CGO* a = m->bfs[i];
+            const CGO* b = m->bfs[j];
+            double mx = 0, my = 0, mz = 0;
+            for (const Prim& g1 : a->g)
+                for (const Prim& g2 : b->g) {
+                    Mat sx, sy, sz;
+                    Overlap1D(g1.alpha, g2.alpha, g1.c.x, g2.c.x, g1.qn.l, g2.qn.l, sx);
+                    Overlap1D(g1.alpha, g2.alpha, g1.c.y, g2.c.y, g1.qn.m, g2.qn.m, sy);
+                    Overlap1D(g1.alpha, g2.alpha, g1.c.z, g2.c.z, g1.qn.n, g2.qn.n, sz);
+                    const V3 d = g1.c - g2.c;
+                    const double o = 1. / (g1.alpha + g2.alpha);
+                    const double f = g1.norm * g2.norm * g1.coef * g2.coef *
+                                     std::exp(-g1.alpha * g2.alpha * o * dot(d, d)) * std::pow(kPi * o, 1.5);
+                    const unsigned l1 = g1.qn.l, l2 = g2.qn.l, m1 = g1.qn.m, m2 = g2.qn.m, n1 = g1.qn.n, n2 = g2.qn.n;
+                    const double X = sx(l1, l2), Y = sy(m1, m2), Z = sz(n1, n2);
+                    mx += f * (sx(l1 + 1, l2) + g1.c.x * X) * Y * Z;
+                    my += f * X * (sy(m1 + 1, m2) + g1.c.y * Y) * Z;
+                    mz += f * X * Y * (sz(n1 + 1, n2) + g1.c.z * Z);
+                }
+            Mx[(size_t)i * n + j] = Mx[(size_t)j * n + i] = mx;
+            My[(size_t)i * n + j] = My[(size_t)j * n + i] = my;
+            Mz[(size_t)i * n + j] = Mz[(size_t)j * n + i] = mz;
+        }
+    return 0;
+}
+
 int ref_jk_from_packed(int n, const double* eri, const double* D, double* J, double* K) {
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {

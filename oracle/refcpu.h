@@ -64,6 +64,8 @@ int ref_rhf_g(int n, const double* eri, const double* D, double* G, int i0, int 
 
 /* One-electron matrices S, T, V (HartreeFock/QuantumMatrix.cpp:34-155). */
 int ref_one_electron(ref_mol* m, double* S, double* T, double* V);
+/* Dipole moment integrals about the origin (HartreeFock/QuantumMatrix.cpp:61-90). */
+int ref_moment(ref_mol* m, double* Mx, double* My, double* Mz);
 
 /* J/K from a packed ERI store: J_ij = sum_kl (ij|kl) D_kl, K_ij = sum_kl (il|kj) D_kl,
  * the N^4 loop of HartreeFock/RestrictedHartreeFock.cpp:189-202. */

@@ -57,6 +57,7 @@ def lib():
                                             ctypes.POINTER(ctypes.c_uint64)]
         L.ref_rhf_g.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.c_int]
         L.ref_one_electron.argtypes = [ctypes.c_void_p, dp, dp, dp]
+        L.ref_moment.argtypes = [ctypes.c_void_p, dp, dp, dp]
         L.ref_jk_from_packed.argtypes = [ctypes.c_int, dp, dp, dp, dp]
         L.ref_scf_defaults.argtypes = [ctypes.POINTER(ScfParams)]
         L.ref_scf.argtypes = [ctypes.c_void_p, ctypes.POINTER(ScfParams), ctypes.POINTER(ScfResult), dp]
@@ -142,6 +143,13 @@ class Molecule:
         lib().ref_one_electron(self.h, _dp(S), _dp(T), _dp(V))
         return S, T, V
 
+    def moment(self):
+        """Dipole moment integrals about the origin, (Mx, My, Mz) (HartreeFock/QuantumMatrix.cpp:61-90)."""
+        n = self.nbf
+        M = [np.zeros((n, n)) for _ in range(3)]
+        lib().ref_moment(self.h, _dp(M[0]), _dp(M[1]), _dp(M[2]))
+        return M
+
     def scf(self, unrestricted=False, **kw):
         p = ScfParams()
         lib().ref_scf_defaults(ctypes.byref(p))
@@ -188,3 +196,70 @@ def jk_from_packed(n, eri, D):
     D = np.ascontiguousarray(D, dtype=np.float64)
     lib().ref_jk_from_packed(n, _dp(eri), _dp(D), _dp(J), _dp(K))
     return J, K
+
+
+# ---- properties and MP2 (the steps after the SCF) ----
+
+def unpack_eri(n, eri):
+    """Full n^4 tensor g[i,j,k,l] from the 8-fold packed store (small n only)."""
+    i = np.arange(n)
+
+    def two(a, b):
+        hi, lo = np.maximum(a, b), np.minimum(a, b)
+        return hi * (hi + 1) // 2 + lo
+
+    ij = two(i[:, None], i[None, :])
+    return eri[two(ij[:, :, None, None], ij[None, None, :, :])]
+
+
+def mulliken_charges(D, S, Z, atom_of_bf):
+    """Z_A - sum over the functions of atom A of (D S)_mm, D the total density
+    (RestrictedHartreeFock::CalculateAtomicCharge, HartreeFock/RestrictedHartreeFock.cpp:291-317;
+    UnrestrictedHartreeFock.cpp:381-406 with D = D_alpha + D_beta)."""
+    ds = np.einsum("mn,nm->m", D, S)
+    return np.array([z - ds[atom_of_bf == a].sum() for a, z in enumerate(Z)])
+
+
+def dipole(D, M, Z, R):
+    """sum_A Z_A R_A - sum_mn D_mn M_mn (GetMoment, HartreeFock/RestrictedHartreeFock.cpp:319-335,
+    GetNuclearMoment, HartreeFock/HartreeFockAlgorithm.cpp:234-242)."""
+    nuc = np.einsum("a,ax->x", np.asarray(Z, float), np.asarray(R, float))
+    return nuc - np.array([np.sum(D * m) for m in M])
+
+
+def _orbitals(F, S):
+    import scipy.linalg
+    return scipy.linalg.eigh(F, S)
+
+
+def mp2_rhf(n, eri, D, h, S, nocc):
+    """RHF MP2 correlation energy as RestrictedHartreeFock::CalculateMp2Energy
+    (HartreeFock/RestrictedHartreeFock.cpp:251-288): sum over occupied i, j and virtual a, b of
+    (ia|jb) (2 (ia|jb) - (ib|ja)) / (e_i + e_j - e_a - e_b), the orbitals and energies those of the
+    converged Fock matrix h + J[D] - K[D]/2."""
+    J, K = jk_from_packed(n, eri, D)
+    eps, C = _orbitals(h + J - 0.5 * K, S)
+    g = unpack_eri(n, eri)
+    Co, Cv = C[:, :nocc], C[:, nocc:]
+    ovov = np.einsum("pqrs,pi,qa,rj,sb->iajb", g, Co, Cv, Co, Cv, optimize=True)
+    eo, ev = eps[:nocc], eps[nocc:]
+    den = eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]
+    return float(np.sum(ovov * (2.0 * ovov - ovov.transpose(0, 3, 2, 1)) / den))
+
+
+def mp2_uhf_reference(n, eri, Da, Db, h, S, na, nb):
+    """UHF MP2 as UnrestrictedHartreeFock::CalculateMp2Energy (HartreeFock/UnrestrictedHartreeFock.cpp:
+    295-375): for each spin, sum over same-spin occupied i, j and virtual a, b of
+    (ia|jb)^2 / (e_i + e_j - e_a - e_b), the two sums added and multiplied by 1/4.  (This is the
+    reference's formula as written: no antisymmetrisation and no opposite-spin term.)"""
+    J = jk_from_packed(n, eri, Da)[0] + jk_from_packed(n, eri, Db)[0]
+    g = unpack_eri(n, eri)
+    tot = 0.0
+    for Dsp, no in ((Da, na), (Db, nb)):
+        eps, C = _orbitals(h + J - jk_from_packed(n, eri, Dsp)[1], S)
+        Co, Cv = C[:, :no], C[:, no:]
+        ovov = np.einsum("pqrs,pi,qa,rj,sb->iajb", g, Co, Cv, Co, Cv, optimize=True)
+        eo, ev = eps[:no], eps[no:]
+        den = eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]
+        tot += float(np.sum(ovov * ovov / den))
+    return 0.25 * tot

@@ -27,6 +27,9 @@ int main(int argc, char** argv) {
     const double E = hf.Calculate();
     std::printf("E = %.12f converged = %d iterations = %d status = %d\n", E, hf.converged ? 1 : 0, hf.iterations,
                 hf.lastStatus);
+    const Vector3D<double> mu = hf.GetMoment();
+    std::printf("dipole = %.9f %.9f %.9f  q(O) = %.9f\n", mu.X, mu.Y, mu.Z, hf.CalculateAtomicCharge(0));
+    std::printf("E(MP2) = %.12f\n", hf.CalculateMp2Energy());
     hf.integralsRepository.CalculateElectronElectronIntegrals();
     std::printf("(11|11) = %.12f\n", hf.integralsRepository.getElectronElectron(0, 0, 0, 0));
     return hf.converged ? 0 : 1;

@@ -182,16 +182,18 @@ def _serve_all(server, bfile, mols):
     return out
 
 
-def test_serving_results_match_alone(gpu_engine_factory):
+@pytest.mark.parametrize("incore", [0, 1])
+def test_serving_results_match_alone(gpu_engine_factory, incore):
     """Continuous batching: six molecules submitted at once to three slots run concurrently; each
-    energy equals the molecule's SCF run alone (1e-9 Eh: only the order of FP64 atomics differs)."""
+    energy equals the molecule's SCF run alone (1e-9 Eh: only the order of FP64 atomics differs),
+    with direct J/K and with each request's in-core ERI store."""
     import hartreefock_amd as hf
     bfile = basis("6-31g.1.nw")
     alone = {}
     for name, atoms in SERVE_MOLS:
         _print_case(name)
         alone[name] = gpu_engine_factory().set_molecule(bfile, atoms).scf(**SERVE_SCF).energy
-    srv = hf.Server(slots=3, scf=SERVE_SCF)
+    srv = hf.Server(slots=3, scf=dict(SERVE_SCF, incore=incore))
     try:
         got = _serve_all(srv, bfile, SERVE_MOLS)
     finally:
@@ -364,3 +366,87 @@ def test_incore_mixed_scf_matches_fp64(gpu_engine_factory, name, bfile, atoms):
     assert r64.incore == 1 and rm.incore == 1 and rm.fp32_builds > 0
     assert abs(r64.energy - e64.energy) <= 1e-8
     assert abs(rm.energy - e64.energy) <= 1e-6
+
+
+# ---- after the SCF: properties and MP2 ----
+TIGHT = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0, energy_tolerance=1e-11, density_tolerance=1e-9,
+             diis_tolerance=1e-9)
+POST_CASES = [
+    ("h2o_crawford_sto3g", "sto3g.txt", g.water_crawford(), False),
+    ("ethane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(2, seed=5), False),
+    ("ch3_631gs_uhf", "6-31g_st_.1.nw", UHF_CASES[1][2], True),
+]
+
+
+@pytest.mark.parametrize("name,bfile,atoms,uhf", POST_CASES, ids=[c[0] for c in POST_CASES])
+def test_properties_match_oracle(gpu_engine_factory, name, bfile, atoms, uhf):
+    """Mulliken charges and dipole (device overlap and dipole integrals) against the oracle's on the
+    same density, 1e-10; H2O/STO-3G against Crawford's known answers."""
+    _print_case(name)
+    eng = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis(bfile), atoms)
+    r, D = eng.scf(unrestricted=uhf, want_density=True, **TIGHT)
+    assert r.converged
+    q, mu = eng.properties(D if uhf else D[0])
+    mol = refcpu.Molecule(basis(bfile), atoms)
+    S = mol.one_electron()[0]
+    Dt = D.sum(0)
+    Z = [a[0] for a in atoms]
+    R = [a[1] for a in atoms]
+    q_ref = refcpu.mulliken_charges(Dt, S, Z, mol.bf_info()[2])
+    mu_ref = refcpu.dipole(Dt, mol.moment(), Z, R)
+    assert np.abs(q - q_ref).max() <= 1e-10 and np.abs(mu - mu_ref).max() <= 1e-10
+    assert abs(q.sum() - (sum(Z) - sum(eng.electrons()))) <= 1e-10
+    if name == "h2o_crawford_sto3g":
+        assert np.abs(q - np.array([-0.253146052405, 0.126573026202, 0.126573026202])).max() <= 1e-8
+        assert abs(mu[1] - 0.603521296525) <= 1e-8 and abs(mu[0]) <= 1e-10 and abs(mu[2]) <= 1e-10
+
+
+@pytest.mark.parametrize("name,bfile,atoms,uhf", POST_CASES, ids=[c[0] for c in POST_CASES])
+@pytest.mark.parametrize("incore", [0, 1])
+def test_mp2_matches_oracle(gpu_engine_factory, name, bfile, atoms, uhf, incore):
+    """MP2 correlation energy (two FP64 GEMMs over the in-core store) against the oracle's restatement
+    of the reference's CalculateMp2Energy on the oracle's own SCF, 1e-8 Eh; H2O/STO-3G against Crawford's
+    -0.049149636120 Eh.  UHF follows the reference's formula as written (same-spin sums times 1/4)."""
+    _print_case(name)
+    eng = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis(bfile), atoms)
+    r = eng.scf(unrestricted=uhf, mp2=1, incore=incore, **TIGHT)
+    assert r.converged and r.mp2_energy < 0
+    mol = refcpu.Molecule(basis(bfile), atoms)
+    ref, D = mol.scf(unrestricted=uhf, alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+    assert ref.converged
+    n = mol.nbf
+    S, T, V = mol.one_electron()
+    eri = mol.eri_packed()
+    na, nb = mol.electrons()
+    if uhf:
+        e2 = refcpu.mp2_uhf_reference(n, eri, D[0], D[1], T + V, S, na, nb)
+    else:
+        e2 = refcpu.mp2_rhf(n, eri, D[0], T + V, S, na)
+    print("mp2 gpu %.12f oracle %.12f" % (r.mp2_energy, e2), flush=True)
+    assert abs(r.mp2_energy - e2) <= 1e-8
+    if name == "h2o_crawford_sto3g":
+        assert abs(r.mp2_energy - (-0.049149636120)) <= 1e-8
+
+
+def test_cpp_dropin_runs(tmp_path):
+    """The reference-style C++ caller (drop-in header over the C ABI) runs an RHF, its dipole, a Mulliken
+    charge and MP2 on the GPU, and its energy matches the oracle's."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "dropin_example"
+    cmd = ["g++", "-std=c++17", "-O1", "-I" + os.path.join(root, "include"),
+           os.path.join(root, "tests", "cpp", "dropin_example.cpp"),
+           "-L" + os.path.join(root, "hartreefock_amd"), "-lhfgpu", "-o", str(exe)]
+    assert subprocess.run(cmd, capture_output=True, text=True).returncode == 0
+    env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(root, "hartreefock_amd") + ":" +
+               os.environ.get("LD_LIBRARY_PATH", ""))
+    p = subprocess.run([str(exe), basis("sto3g.txt")], capture_output=True, text=True, env=env, timeout=100)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0
+    vals = dict(line.split(" = ", 1) for line in p.stdout.splitlines() if " = " in line)
+    e = float(vals["E"].split()[0])
+    ref, _ = refcpu.Molecule(basis("sto3g.txt"), g.water_cfg1()).scf(alpha=0.5, init_guess=0.0,
+                                                                     normal_iter_after_diis=0)
+    assert abs(e - ref.energy) <= 1e-8
+    assert float(vals["E(MP2)"]) < 0

@@ -232,3 +232,72 @@ def test_geometry_generators():
     o = np.array([p for z, p in w if z == 8]) * g.BOHR_ANGSTROM
     dist = np.linalg.norm(o[:, None] - o[None], axis=-1) + np.eye(len(o)) * 99
     assert dist.min() >= 2.6 - 1e-9
+
+
+# ---------------------------------------------------------------- properties and MP2
+def _crawford_water_state():
+    m = refcpu.Molecule(basis("sto3g.txt"), g.water_crawford())
+    r, D = m.scf(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+    assert r.converged
+    return m, D[0]
+
+
+def test_crawford_water_properties_known_answers():
+    """Crawford Project #3, H2O/STO-3G: Mulliken charges O -0.253146052405, H +0.126573026202 and the
+    dipole moment 0.603521296525 au along y (the reference quotes 0.603521 au for the same molecule,
+    HartreeFock/Test.cpp:660-661)."""
+    m, D = _crawford_water_state()
+    S = m.one_electron()[0]
+    L, lmn, atom_of_bf, sh = m.bf_info()
+    atoms = g.water_crawford()
+    Z = [a[0] for a in atoms]
+    R = [a[1] for a in atoms]
+    q = refcpu.mulliken_charges(D, S, Z, atom_of_bf)
+    assert np.abs(q - np.array([-0.253146052405, 0.126573026202, 0.126573026202])).max() < 1e-8
+    mu = refcpu.dipole(D, m.moment(), Z, R)
+    assert abs(mu[0]) < 1e-10 and abs(mu[2]) < 1e-10
+    assert abs(mu[1] - 0.603521296525) < 1e-8
+
+
+def test_moment_integrals_translation():
+    """<a|r|b> about the origin shifts by t S_ab when the molecule moves by t; the dipole of a neutral
+    molecule is unchanged."""
+    atoms = g.water_cluster(1, seed=3)
+    t = np.array([0.7, -1.3, 2.1])
+    moved = [(z, tuple(np.add(xyz, t))) for z, xyz in atoms]
+    m0 = refcpu.Molecule(basis("6-31g_st_.1.nw"), atoms)
+    m1 = refcpu.Molecule(basis("6-31g_st_.1.nw"), moved)
+    S = m0.one_electron()[0]
+    for a, b, tk in zip(m0.moment(), m1.moment(), t):
+        assert np.abs(b - (a + tk * S)).max() < 1e-12
+
+
+def test_crawford_water_mp2_known_answer():
+    """Crawford Project #4, H2O/STO-3G: MP2 correlation energy -0.049149636120 Eh (all electrons
+    correlated, as RestrictedHartreeFock::CalculateMp2Energy, HartreeFock/RestrictedHartreeFock.cpp:251-288)."""
+    m, D = _crawford_water_state()
+    n = m.nbf
+    S, T, V = m.one_electron()
+    e2 = refcpu.mp2_rhf(n, m.eri_packed(), D, T + V, S, m.electrons()[0])
+    assert abs(e2 - (-0.049149636120)) < 1e-9
+
+
+def test_uhf_mp2_reference_formula_closed_shell_limit():
+    """The reference's UHF MP2 formula (same-spin (ia|jb)^2 sums, times 1/4) on a closed shell with
+    equal alpha and beta densities equals half the RHF sum of (ia|jb)^2 / denominators (its formula
+    has no exchange and no opposite-spin term).  Pins the spin bookkeeping of the oracle's UHF MP2
+    against the RHF orbitals; the UHF formula itself is the reference's as written."""
+    m, D = _crawford_water_state()
+    n = m.nbf
+    S, T, V = m.one_electron()
+    eri = m.eri_packed()
+    nocc = m.electrons()[0]
+    e_u = refcpu.mp2_uhf_reference(n, eri, 0.5 * D, 0.5 * D, T + V, S, nocc, nocc)
+    J, K = refcpu.jk_from_packed(n, eri, D)
+    import scipy.linalg
+    eps, C = scipy.linalg.eigh(T + V + J - 0.5 * K, S)
+    G = refcpu.unpack_eri(n, eri)
+    ov = np.einsum("pqrs,pi,qa,rj,sb->iajb", G, C[:, :nocc], C[:, nocc:], C[:, :nocc], C[:, nocc:])
+    eo, ev = eps[:nocc], eps[nocc:]
+    den = eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]
+    assert abs(e_u - 0.5 * np.sum(ov * ov / den)) < 1e-13
