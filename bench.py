@@ -6,7 +6,7 @@ angles, +-0.05 A jitter), Schwarz threshold 1e-12, converged to 1e-8 Eh.  A step
 complete SCF solve (integrals recomputed every iteration: direct J/K).  The molecule's
 shells, primitive pairs and Schwarz table are resident in HBM before timing starts.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload decane|water|serving]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload decane|water|serving|water32|eicosyl]
 
 Prints ONE JSON line (rank 0).  `roofline` is the dominant ERI class kernel: algorithmic
 flops (exact operation count of the generated code, per launch) over its average launch
@@ -44,6 +44,10 @@ def workload(name):
         return ("C10H22/6-31G* direct RHF FP64", hf.basis_path("6-31g_st_.1.nw"), g.decane_cfg2())
     if name == "water":
         return ("H2O/STO-3G RHF FP64", hf.basis_path("sto3g.txt"), g.water_cfg1())
+    if name == "water32":
+        return ("(H2O)32/6-31G** direct RHF FP64", hf.basis_path("6-31g_st__st_.nw"), g.water32_cfg3())
+    if name == "eicosyl":
+        return ("C20H41/6-31+G* UHF FP64", hf.basis_path("6-31+g_st_.1.nw"), g.eicosyl_cfg5())
     raise SystemExit("unknown workload " + name)
 
 
@@ -167,6 +171,47 @@ def serving(a):
     print(json.dumps(out), flush=True)
 
 
+def large(a):
+    """The large single-molecule configurations, one SCF per leg: (H2O)32/6-31G** RHF (800 functions) and
+    C20H41/6-31+G* UHF (462 functions, 200 iterations at most: with the reference's SCF settings it does
+    not meet 1e-8 Eh).  Legs: direct FP64, in-core FP64 (when the store fits: n <= 512), mixed direct and
+    mixed in-core; each reports wall time, iterations, energy and |dE| against direct FP64."""
+    import hartreefock_amd as hf
+    label, bfile, atoms = workload(a.workload)
+    uhf = a.workload == "eicosyl"
+    kw = dict(SCF_KW, max_iterations=200 if uhf else 300)
+    legs = [("fp64_direct", 0, 0), ("fp64_incore", 0, 1), ("mixed_direct", 1, 0), ("mixed_incore", 1, 1)]
+    out = {"metric": "SCF wall time per molecule", "unit": "s", "higher_is_better": False, "n_gpus": 1,
+           "dtype": "f64", "data": "synthetic (seeded geometry)",
+           "config": {"workload": label, "schwarz_threshold": 1e-12, "energy_tolerance": 1e-8,
+                      "eri_convention": "exact", "scf": "alpha=0.5, core guess, DIIS(6/5), normalIterAfterDIIS=0"}}
+    e0 = None
+    for name, prec, inc in legs:
+        eng = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, precision=prec,
+                        fp32_threshold=1e-6).set_molecule(bfile, atoms)
+        out["config"]["nbf"] = eng.nbf
+        if inc and eng.nbf > 512:
+            eng.close()
+            out[name] = "store does not fit (n > 512)"
+            continue
+        print("large %s: %s" % (a.workload, name), file=sys.stderr, flush=True)
+        hip_sync()
+        t0 = time.perf_counter()
+        r = eng.scf(unrestricted=uhf, check=False, incore=inc, **kw)
+        hip_sync()
+        el = time.perf_counter() - t0
+        eng.close()
+        if e0 is None:
+            e0 = r.energy
+        out[name] = {"s": el, "iterations": r.iterations, "converged": r.converged, "energy_Eh": r.energy,
+                     "abs_dE_vs_fp64_direct_Eh": abs(r.energy - e0), "incore": r.incore, "eri_store_ms": r.eri_ms,
+                     "jk_ms": r.jk_ms, "fp32_builds": r.fp32_builds}
+        if uhf:
+            out[name]["s2"] = r.s2
+    out["value"] = out["fp64_direct"]["s"]
+    print(json.dumps(out), flush=True)
+
+
 def timed_scfs(eng, steps, warmup, **extra):
     kw = dict(SCF_KW, **extra)
     for _ in range(warmup):
@@ -211,6 +256,9 @@ def main():
         sys.exit(2)
     if a.workload == "serving":
         serving(a)
+        return
+    if a.workload in ("water32", "eicosyl"):
+        large(a)
         return
     import hartreefock_amd as hf
     label, bfile, atoms = workload(a.workload)

@@ -25,6 +25,10 @@
 // element.  Each element is used once per density, so there is no operand reuse for the matrix cores.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "hf_internal.h"
 
 namespace {
@@ -74,7 +78,7 @@ constexpr int log2c() {
     return NV <= 1 ? 0 : 1 + log2c<NV / 2>();
 }
 
-template <int NDENS, int M, typename T>
+template <int NDENS, int M, int ROWS, typename T>
 __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t np,
                                                         const double* __restrict__ D, const double* __restrict__ Dw,
                                                         double* __restrict__ J, double* __restrict__ K) {
@@ -110,17 +114,17 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
         }
     double jacc = 0.0;
     const T* __restrict__ row0 = G + P * np;
-    constexpr int NV = 8 * NDENS;                  // 4 rows x (K_i, K_j) x densities
+    constexpr int NV = 2 * ROWS * NDENS;           // ROWS rows x (K_i, K_j) x densities
     constexpr int SH = 6 - log2c<NV>();
-    for (int k0 = w; k0 < n; k0 += 16) {
+    for (int k0 = w; k0 < n; k0 += 4 * ROWS) {
         double V[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) V[v] = 0.0;
-        // rows k0 + 4r (r < 4); the four rows' elements of one column chunk are loaded together
-        int base[4], kr[4];                        // row offsets fit 32 bits: np < 2^31 for n <= 512
-        double xkj[4][NDENS], xki[4][NDENS];
+        // rows k0 + 4r (r < ROWS); the rows' elements of one column chunk are loaded together
+        int base[ROWS], kr[ROWS];                        // row offsets fit 32 bits: np < 2^31 for n <= 512
+        double xkj[ROWS][NDENS], xki[ROWS][NDENS];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < ROWS; ++r) {
             const int k = k0 + 4 * r;
             kr[r] = k < n ? k : -1;                // -1: past the last row (no element)
             const int kk = k < n ? k : n - 1;
@@ -131,22 +135,24 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
                 xki[r][d] = xi[d * n + kk];
             }
         }
-        const int klast = k0 + 12 < n ? k0 + 12 : n - 1;
+        const int klast = k0 + 4 * (ROWS - 1) < n ? k0 + 4 * (ROWS - 1) : n - 1;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
             if (64 * m > klast) break;             // uniform across the wavefront
             const int l = lane + 64 * m;
-            double gv[4], dv[4];
+            double gv[ROWS], dv[ROWS];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            for (int r = 0; r < ROWS; ++r) {
                 const int kk = kr[r] < 0 ? n - 1 : kr[r];
                 const int at = base[r] + (l <= kk ? l : kk);     // clamped: always inside the row
-                const double gx = (double)row0[at];
-                dv[r] = Dw[at];
+                // 32-bit byte offsets from the block-uniform row and density bases (address arithmetic
+                // stays in one VALU op; a row is at most np * 8 < 2^20 bytes)
+                const double gx = (double)*(const T*)((const char*)row0 + (unsigned)at * (unsigned)sizeof(T));
+                dv[r] = *(const double*)((const char*)Dw + (unsigned)at * 8u);
                 gv[r] = l <= kr[r] ? gx : 0.0;
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            for (int r = 0; r < ROWS; ++r) {
                 const double g = gv[r];
                 jacc = fma(g, dv[r], jacc);
 #pragma unroll
@@ -199,15 +205,15 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
     }
 }
 
-template <int NDENS, typename T>
+template <int NDENS, int ROWS, typename T>
 hipError_t launch_incore(const T* G, int n, int64_t np, const double* D, const double* Dw, double* J, double* K,
                          hipStream_t st) {
     const size_t lds = (4 * (size_t)NDENS * n + 4) * sizeof(double);
     const dim3 grid((unsigned)np), block(256);
     switch ((n + 63) / 64) {
-#define HF_INCORE_CASE(MM)                                                                                    \
-    case MM:                                                                                                  \
-        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MM, T>), grid, block, lds, st, G, n, np, D, Dw, J, K); \
+#define HF_INCORE_CASE(MM)                                                                                          \
+    case MM:                                                                                                        \
+        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MM, ROWS, T>), grid, block, lds, st, G, n, np, D, Dw, J, K); \
         break;
         HF_INCORE_CASE(1)
         HF_INCORE_CASE(2)
@@ -222,6 +228,75 @@ hipError_t launch_incore(const T* G, int n, int64_t np, const double* D, const d
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Row-group height of the contraction (rows of one wavefront whose loads are issued together and
+// whose row sums are reduced together): 2, 4 or 8.
+template <typename T>
+hipError_t launch_incore_rows(int rows, int ndens, const T* G, int n, int64_t np, const double* D, const double* Dw,
+                              double* J, double* K, hipStream_t st) {
+    if (ndens == 1) {
+        if (rows == 2) return launch_incore<1, 2>(G, n, np, D, Dw, J, K, st);
+        if (rows == 8) return launch_incore<1, 8>(G, n, np, D, Dw, J, K, st);
+        return launch_incore<1, 4>(G, n, np, D, Dw, J, K, st);
+    }
+    if (rows == 2) return launch_incore<2, 2>(G, n, np, D, Dw, J, K, st);
+    if (rows == 8) return launch_incore<2, 8>(G, n, np, D, Dw, J, K, st);
+    return launch_incore<2, 4>(G, n, np, D, Dw, J, K, st);
+}
+
+// Autotuner: the row-group height is picked per (column chunks M = ceil(n/64), densities, store
+// precision) by timing every candidate on the caller's own store, density and stream, once per
+// process (the table is shared by all contexts; a server's slots tune each key once).  Candidates
+// write to scratch J/K, so the build's outputs are untouched.
+struct IncoreTuning {
+    std::mutex mu;
+    std::map<int, int> rows;                  // key -> chosen rows
+};
+IncoreTuning& tuning() {
+    static IncoreTuning t;
+    return t;
+}
+
+template <typename T>
+int tuned_rows(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double* D, const double* Dw,
+               hipStream_t st) {
+    const int key = ((n + 63) / 64) * 100 + ndens * 10 + (sizeof(T) == 4 ? 1 : 0);
+    {
+        std::lock_guard<std::mutex> lock(tuning().mu);
+        auto it = tuning().rows.find(key);
+        if (it != tuning().rows.end()) return it->second;
+    }
+    const size_t nn = (size_t)n * n;
+    DevBuf<double> sJ, sK;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int best = 4;
+    float best_ms = 0;
+    bool ok = sJ.alloc(nn) == hipSuccess && sK.alloc(2 * nn) == hipSuccess &&
+              hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess;
+    for (int rows : {2, 4, 8}) {
+        if (!ok) break;
+        float t = 0;
+        for (int rep = 0; rep < 3 && ok; ++rep) {          // first run warms the caches and the code
+            ok = hipEventRecord(ev[0], st) == hipSuccess &&
+                 launch_incore_rows(rows, ndens, G, n, np, D, Dw, sJ.p, sK.p, st) == hipSuccess &&
+                 hipEventRecord(ev[1], st) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess;
+            float ms = 0;
+            if (ok) ok = hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess;
+            if (rep > 0) t = rep == 1 ? ms : std::min(t, ms);
+        }
+        if (ok && (best_ms == 0 || t < best_ms)) {
+            best_ms = t;
+            best = rows;
+        }
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipGetLastError();
+    if (!ok) return 4;                                     // tuning failed: the default, not recorded
+    std::lock_guard<std::mutex> lock(tuning().mu);
+    tuning().rows[key] = best;
+    return best;
 }
 
 }  // namespace
@@ -241,6 +316,9 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
         hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((ndens * nn + 255) / 256)), dim3(256), 0, st, d_K,
                            ndens * nn);
     }
+    // row-group height from the autotuner (first build of each kind in the process times the candidates)
+    const int rows = g32 ? tuned_rows(ctx, ndens, g32, n, np, d_D, ctx->d_Dw.p, st)
+                         : tuned_rows(ctx, ndens, G, n, np, d_D, ctx->d_Dw.p, st);
     // statistics, and with profile = 1 events around the contraction (read at the next flush)
     const int f = g32 ? 1 : 0;
     const bool prof = ctx->opt.profile != 0;
@@ -262,11 +340,9 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
     ks.flops += (double)np * (double)np * (2.0 + 8.0 * ndens);           // J, and row + column parts of K
     hipError_t e;
     if (g32)
-        e = ndens == 1 ? launch_incore<1>(g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st)
-                       : launch_incore<2>(g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
+        e = launch_incore_rows(rows, ndens, g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
     else
-        e = ndens == 1 ? launch_incore<1>(G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st)
-                       : launch_incore<2>(G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
+        e = launch_incore_rows(rows, ndens, G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
     if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("in-core J/K: ") + hipGetErrorString(e));
     if (prof) {
         HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[1], st));

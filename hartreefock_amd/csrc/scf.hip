@@ -686,11 +686,12 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     bool force_full = false, last_incremental = false;
     double quartets = 0, last_dmax = 0, last_quartets = 0;
     // In-core mixed tier: the first two builds are full FP64 (the density changes most there), then
-    // the J/K of each density change comes from the FP32 store, whose error scales with |D - Dprev|;
-    // full rebuilds and the final phase are FP64.
+    // the J/K of each density change comes from the FP32 store while max|D - Dprev| <= 1e-3 (its
+    // error scales with |D - Dprev|), FP64 otherwise; full rebuilds and the final phase are FP64.
     const bool incore_mixed = incore && mixed_tier;
     const bool incremental = p->incremental_fock != 0 || incore_mixed;
     const int first_full = incore_mixed ? 2 : 1;
+    constexpr double kIncoreFp32MaxDelta = 1e-3;   // largest max|D - Dprev| contracted with the FP32 store
     // J/K build time from events on the SCF stream, read at the step's own synchronisation
     struct EventPair {
         hipEvent_t e[2] = {nullptr, nullptr};
@@ -746,11 +747,16 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 hipLaunchKernelGGL(delta_kernel, dim3(1), dim3(1024), 0, s, st.D.p, st.Dprev.p, st.dD.p, nsp * nn,
                                    st.red.p + 15);
                 double dmax = 0;
-                if (!incore) {   // the in-core contraction reads the whole store whatever max|dD| is
+                // the in-core FP64 contraction reads the whole store whatever max|dD| is; the mixed tier
+                // needs max|dD| to decide whether the FP32 copy is safe
+                if (!incore || (incore_mixed && used_fp32)) {
                     HF_HIP_CHECK(ctx, hipMemcpyAsync(&dmax, st.red.p + 15, sizeof(double), hipMemcpyDeviceToHost, s));
                     HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
                     last_dmax = dmax;
                 }
+                // FP32 store only for small density changes: its rounding enters J/K scaled by |dD|, and
+                // near-linearly-dependent (diffuse) bases keep |dD| large for many iterations
+                if (incore_mixed && used_fp32 && !(dmax <= kIncoreFp32MaxDelta)) used_fp32 = false;
                 if (incore) {
                     if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, used_fp32 ? ctx->d_G32.p : nullptr, st.dD.p,
                                                    st.J.p, st.K.p, s, true))

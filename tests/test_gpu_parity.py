@@ -450,3 +450,21 @@ def test_cpp_dropin_runs(tmp_path):
                                                                      normal_iter_after_diis=0)
     assert abs(e - ref.energy) <= 1e-8
     assert float(vals["E(MP2)"]) < 0
+
+
+@pytest.mark.parametrize("prec,incr", [(0, 1), (1, 0)], ids=["fp64_incremental", "mixed"])
+def test_incore_uhf_density_changes_match_direct(gpu_engine_factory, prec, incr):
+    """UHF with the in-core store and J/K of density changes (FP64 incremental builds, and the mixed
+    tier's FP32 store on density changes): the direct FP64 energy within 1e-6 Eh."""
+    atoms = UHF_CASES[1][2]
+    _print_case("ch3_631gs_uhf prec=%d incremental=%d" % (prec, incr))
+    kw = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0, energy_tolerance=1e-8, density_tolerance=1e-6,
+              diis_tolerance=1e-6)
+    ref = gpu_engine_factory(schwarz_threshold=1e-12, exact_transfer=1).set_molecule(
+        basis("6-31g_st_.1.nw"), atoms).scf(unrestricted=True, **kw)
+    eng = gpu_engine_factory(schwarz_threshold=1e-12, exact_transfer=1, precision=prec).set_molecule(
+        basis("6-31g_st_.1.nw"), atoms)
+    r = eng.scf(unrestricted=True, incore=1, incremental_fock=incr, **kw)
+    print("uhf direct %.10f in-core %.10f s2 %.4f %.4f" % (ref.energy, r.energy, ref.s2, r.s2), flush=True)
+    assert ref.converged and r.converged and r.incore == 1
+    assert abs(r.energy - ref.energy) <= 1e-6 and abs(r.s2 - ref.s2) <= 1e-4
