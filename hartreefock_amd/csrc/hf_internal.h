@@ -197,6 +197,9 @@ int hf_eri_pairs_impl(hf_ctx* ctx, double* d_G, uint64_t len, hipStream_t st);
 // K per density, both complete (no symmetrisation needed); accumulate adds to d_J/d_K.
 int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
                       double* d_K, hipStream_t st, bool accumulate);
+// Frees the context's in-core stores when together they hold more than keep_bytes (a server calls
+// this after each request, so idle slots do not hold large stores outside its admission budget).
+void hf_release_incore(hf_ctx* ctx, size_t keep_bytes);
 // Largest basis the in-core contraction kernel takes (its column registers cover 64 * 8 functions).
 constexpr int kIncoreMaxN = 512;
 // Tasks with threshold <= Q_p Q_q (< upper when upper > 0).

@@ -301,6 +301,13 @@ int tuned_rows(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double* 
 
 }  // namespace
 
+void hf_release_incore(hf_ctx* ctx, size_t keep_bytes) {
+    if (ctx->d_G.n * sizeof(double) + ctx->d_G32.n * sizeof(float) > keep_bytes) {
+        ctx->d_G.release();
+        ctx->d_G32.release();
+    }
+}
+
 int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
                       double* d_K, hipStream_t st, bool accumulate) {
     if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");

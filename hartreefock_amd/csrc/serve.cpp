@@ -80,6 +80,9 @@ struct hf_server {
                 p.terminate = nullptr;
                 hf_scf_result sr{};
                 rc = hf_scf(ctx, &p, &sr, nullptr);
+                // the in-core store is counted in the request's footprint only while it runs: free
+                // stores above 1 GiB (smaller ones stay cached, at most slots x 1 GiB in all)
+                hf_release_incore(ctx, (size_t)1 << 30);
                 r.energy = sr.energy;
                 r.iterations = sr.iterations;
                 r.converged = sr.converged;
