@@ -630,7 +630,7 @@ int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, do
 
 int hf_eri_pairs_impl(hf_ctx* ctx, double* d_G, uint64_t len, hipStream_t st) {
     if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
-    const uint64_t n = (uint64_t)ctx->mol.nbf, np = n * (n + 1) / 2;
+    const uint64_t n = (uint64_t)ctx->mol.nbf, np = (uint64_t)pair_pad_base((int64_t)n);
     if (!d_G || len < np * np) return ctx->fail(HF_E_ARG, "pair-matrix buffer too small");
     // whole quartets (the store needs complete values), screened at the context's threshold
     if (int rc = ensure_tasks(ctx, ctx->opt.schwarz_threshold, ctx->tasks_incore, 0.0, false)) return rc;
@@ -768,7 +768,7 @@ int hf_eri_packed(hf_ctx* ctx, double* out, uint64_t len) {
 }
 
 uint64_t hf_pair_matrix_len(int nbf) {
-    const uint64_t np = (uint64_t)nbf * (nbf + 1) / 2;
+    const uint64_t np = nbf > 0 ? (uint64_t)pair_pad_base(nbf) : 0;
     return np * np;
 }
 

@@ -175,7 +175,8 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
             const uint64_t ab = two_index(a, b), cd = two_index(c, d);
             if (same_pair && ab < cd) continue;
             if (sink.pair_stride) {
-                const uint64_t i1 = ab * sink.pair_stride + cd, i2 = cd * sink.pair_stride + ab;
+                const uint64_t pab = (uint64_t)pair_pad_index(a, b), pcd = (uint64_t)pair_pad_index(c, d);
+                const uint64_t i1 = pab * sink.pair_stride + pcd, i2 = pcd * sink.pair_stride + pab;
                 if (i1 < sink.packed_len && i2 < sink.packed_len) {
                     const double x = v(idx);
                     sink.packed[i1] = x;

@@ -407,7 +407,8 @@ __device__ __forceinline__ void sink_packed(const double* __restrict__ v, int A,
                     if (same_pair && ab < cd) continue;
                     const double x = v[((ia * NB + ib) * NC + ic) * ND + id];
                     if (s.pair_stride) {
-                        const uint64_t i1 = ab * s.pair_stride + cd, i2 = cd * s.pair_stride + ab;
+                        const uint64_t pab = (uint64_t)pair_pad_index(a, b), pcd = (uint64_t)pair_pad_index(c, d);
+                        const uint64_t i1 = pab * s.pair_stride + pcd, i2 = pcd * s.pair_stride + pab;
                         if (i1 < s.packed_len && i2 < s.packed_len) {
                             s.packed[i1] = x;
                             s.packed[i2] = x;

@@ -150,6 +150,16 @@ __host__ __device__ __forceinline__ int slice_count(int npb, int npk, int target
     return (npb + c - 1) / c;
 }
 
+// In-core store index of the canonical function pair (k, l), k >= l: rows of the pair triangle padded
+// to an even length (row k starts at pair_pad_base(k)), so every row and every even column offset is
+// 16-byte aligned for FP64; n functions span pair_pad_base(n) positions.
+__host__ __device__ __forceinline__ int64_t pair_pad_base(int64_t k) {
+    return (k & 1) ? (k + 1) * (k + 1) / 2 : k * (k + 2) / 2;
+}
+__host__ __device__ __forceinline__ int64_t pair_pad_index(int64_t a, int64_t b) {
+    return a >= b ? pair_pad_base(a) + b : pair_pad_base(b) + a;
+}
+
 struct SinkArgs {
     int n;                   // basis functions
     int ndens;               // densities contracted in one pass (1 = RHF, 2 = UHF)
@@ -158,8 +168,9 @@ struct SinkArgs {
     double* K;               // ndens x n x n accumulators (K' ; final K = K' + K'^T)
     double* packed;          // MODE_PACKED output (reference 8-fold packed order)
     uint64_t packed_len;
-    uint64_t pair_stride;    // MODE_PACKED: 0 = reference packed order; > 0 = symmetric pair matrix,
-                             // value at [ab * pair_stride + cd] and [cd * pair_stride + ab]
+    uint64_t pair_stride;    // MODE_PACKED: 0 = reference packed order; > 0 = in-core store, the symmetric
+                             // matrix over padded pair indices (pair_pad_index): value at
+                             // [pab * pair_stride + pcd] and [pcd * pair_stride + pab]
     double* Qout;            // MODE_DIAG output, one per bra pair
     int lds_slots;           // MODE_JK: bra pairs per block whose exchange rows are staged in LDS
     const float* boys32;     // MODE_JK32: FP32 copy of the Boys table

@@ -4,24 +4,25 @@
 // RestrictedHartreeFock::InitFockMatrix, HartreeFock/RestrictedHartreeFock.cpp:189-202, and
 // UnrestrictedHartreeFock::InitFockMatrices, HartreeFock/UnrestrictedHartreeFock.cpp:228-238).
 //
-// Layout.  G is the symmetric matrix of ERIs over canonical function pairs,
-// G[P * np + Q] = (ij|kl), P = i(i+1)/2 + j (i >= j), Q = k(k+1)/2 + l (k >= l), np = n(n+1)/2:
-// twice the reference's 8-fold packed store, so that a Fock build reads every row P as one
-// contiguous stream and needs no scatter.  Row P (the bra pair (i,j)) is the lower triangle of the
-// symmetric n x n matrix B_P[k][l] = (ij|kl), stored row by row.
+// Layout.  G is the symmetric matrix of ERIs over canonical function pairs: G[P * np + Q] = (ij|kl)
+// with P = pair_pad_index(i, j), Q = pair_pad_index(k, l) (i >= j, k >= l): the pair triangle row by
+// row, each row padded to an even length, np = pair_pad_base(n) ~ n(n+1)/2 + n/2.  Each unique ERI
+// is held twice (twice the reference's 8-fold packed store), so a Fock build reads every row P as one
+// contiguous stream and needs no scatter; the padding makes every even column offset 16-byte aligned.
+// Row P (the bra pair (i,j)) is the lower triangle of the symmetric n x n matrix B_P[k][l] = (ij|kl).
 //
 // Per row P, with D symmetric:
 //   J[i][j] = J[j][i] = sum_{k>=l} w_kl B_P[k][l] Dtot[k][l]      (w = 2 off the diagonal, 1 on it)
 //   K_d[i][:] += B_P D_d[j][:],   K_d[j][:] += B_P D_d[i][:]  (the second only when i != j)
 // B_P x splits into a row part (y[k] += B[k][l] x[l], l <= k: a sum across the lanes that hold one
 // row) and a column part (y[l] += B[k][l] x[k], l < k: a register of the lane that owns column l).
-// One workgroup per P: four wavefronts take rows k = w (mod 4), four rows at a time; lane L owns
-// columns L + 64 m.  The 8 (RHF) or 16 (UHF) row sums of four rows are reduced together by
-// recursive halving (each exchange step halves the values a lane carries), 10 or 17 shuffles
-// instead of 6 per value.  J_ij needs no atomics (one block per P); each block adds its two K rows
-// with coalesced FP64 atomics.
+// One workgroup per P: four wavefronts take rows k = w (mod 4), ROWS rows at a time; lane L owns the
+// column pairs (2L, 2L+1) + 128 m and reads each pair with one load.  The 2 ROWS ndens row sums of a
+// group are reduced together by recursive halving (each exchange step halves the values a lane
+// carries), 10 shuffles for 8 values instead of 6 per value.  J_ij needs no atomics (one block per P);
+// each block adds its two K rows with coalesced FP64 atomics.
 //
-// The kernel streams np^2 elements (8 or 4 bytes) per build: HBM-bound, ~2 + 4 ndens flops per
+// The kernel streams np^2 elements (8 or 4 bytes) per build: HBM-bound, ~2 + 8 ndens flops per
 // element.  Each element is used once per density, so there is no operand reuse for the matrix cores.
 #include <hip/hip_runtime.h>
 
@@ -38,14 +39,16 @@ __global__ void incore_fill_kernel(double* __restrict__ p, size_t n) {
     if (i < n) p[i] = 0.0;
 }
 
-// Dw[k(k+1)/2 + l] = w_kl * sum_d D_d[k][l]
+// Dw[pair_pad_index(k, l)] = w_kl * sum_d D_d[k][l] (the store's padded row order; padding is 0)
 __global__ void incore_dw_kernel(const double* __restrict__ D, int n, int ndens, double* __restrict__ Dw) {
     const int k = blockIdx.x;
     const size_t nn = (size_t)n * n;
-    for (int l = threadIdx.x; l <= k; l += blockDim.x) {
+    const int64_t b = hfgpu::pair_pad_base(k), len = hfgpu::pair_pad_base(k + 1) - b;
+    for (int l = threadIdx.x; l < len; l += blockDim.x) {
         double s = 0.0;
-        for (int d = 0; d < ndens; ++d) s += D[d * nn + (size_t)k * n + l];
-        Dw[(size_t)k * (k + 1) / 2 + l] = (l == k ? 1.0 : 2.0) * s;
+        if (l <= k)
+            for (int d = 0; d < ndens; ++d) s += D[d * nn + (size_t)k * n + l];
+        Dw[b + l] = (l == k ? 1.0 : 2.0) * s;
     }
 }
 
@@ -78,16 +81,32 @@ constexpr int log2c() {
     return NV <= 1 ? 0 : 1 + log2c<NV / 2>();
 }
 
-template <int NDENS, int M, int ROWS, typename T>
-__global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t np,
+template <typename T>
+struct PairOf;
+template <>
+struct PairOf<double> {
+    using V = double2;
+};
+template <>
+struct PairOf<float> {
+    using V = float2;
+};
+
+// One workgroup per padded row index R of the store (R = pair_pad_index(i, j); padding rows return at
+// once).  M2 = ceil(n / 128) column chunks: lane L owns columns 2L + 128 m and 2L + 1 + 128 m and reads
+// both with one aligned 16-byte (FP64) or 8-byte (FP32) load.
+template <int NDENS, int M2, int ROWS, typename T>
+__global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t npad,
                                                         const double* __restrict__ D, const double* __restrict__ Dw,
                                                         double* __restrict__ J, double* __restrict__ K) {
     extern __shared__ double lds[];
-    const int64_t P = blockIdx.x;
-    int i = (int)((sqrt(8.0 * (double)P + 1.0) - 1.0) * 0.5);
-    if ((int64_t)i * (i + 1) / 2 > P) --i;
-    if ((int64_t)(i + 1) * (i + 2) / 2 <= P) ++i;
-    const int j = (int)(P - (int64_t)i * (i + 1) / 2);
+    const int64_t R = blockIdx.x;
+    int i = (int)sqrt(2.0 * (double)R) - 1;
+    if (i < 0) i = 0;
+    while (hfgpu::pair_pad_base(i + 1) <= R) ++i;
+    while (i > 0 && hfgpu::pair_pad_base(i) > R) --i;
+    const int j = (int)(R - hfgpu::pair_pad_base(i));
+    if (j > i) return;                                      // padding row: all zero (whole block)
     double* xj = lds;               // D_d[j][:]
     double* xi = xj + NDENS * n;    // D_d[i][:]
     double* rI = xi + NDENS * n;    // additions to K_d[i][:]
@@ -101,19 +120,23 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double xjl[NDENS][M], xil[NDENS][M], cI[NDENS][M], cJ[NDENS][M];
+    double xjl[NDENS][M2][2], xil[NDENS][M2][2], cI[NDENS][M2][2], cJ[NDENS][M2][2];
 #pragma unroll
     for (int d = 0; d < NDENS; ++d)
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const int l = lane + 64 * m;
-            xjl[d][m] = l < n ? xj[d * n + l] : 0.0;
-            xil[d][m] = l < n ? xi[d * n + l] : 0.0;
-            cI[d][m] = 0.0;
-            cJ[d][m] = 0.0;
-        }
+        for (int m = 0; m < M2; ++m)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int l = 2 * lane + 128 * m + h;
+                xjl[d][m][h] = l < n ? xj[d * n + l] : 0.0;
+                xil[d][m][h] = l < n ? xi[d * n + l] : 0.0;
+                cI[d][m][h] = 0.0;
+                cJ[d][m][h] = 0.0;
+            }
     double jacc = 0.0;
-    const T* __restrict__ row0 = G + P * np;
+    // byte offsets from the block-uniform row base (a padded row is at most ~1 MB)
+    const char* __restrict__ row0 = (const char*)(G + R * npad);
+    const char* __restrict__ dwb = (const char*)Dw;
     constexpr int NV = 2 * ROWS * NDENS;           // ROWS rows x (K_i, K_j) x densities
     constexpr int SH = 6 - log2c<NV>();
     for (int k0 = w; k0 < n; k0 += 4 * ROWS) {
@@ -121,14 +144,14 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
 #pragma unroll
         for (int v = 0; v < NV; ++v) V[v] = 0.0;
         // rows k0 + 4r (r < ROWS); the rows' elements of one column chunk are loaded together
-        int base[ROWS], kr[ROWS];                        // row offsets fit 32 bits: np < 2^31 for n <= 512
+        int base[ROWS], kr[ROWS];
         double xkj[ROWS][NDENS], xki[ROWS][NDENS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
             const int k = k0 + 4 * r;
             kr[r] = k < n ? k : -1;                // -1: past the last row (no element)
             const int kk = k < n ? k : n - 1;
-            base[r] = kk * (kk + 1) / 2;
+            base[r] = (int)hfgpu::pair_pad_base(kk);
 #pragma unroll
             for (int d = 0; d < NDENS; ++d) {
                 xkj[r][d] = xj[d * n + kk];
@@ -137,36 +160,40 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
         }
         const int klast = k0 + 4 * (ROWS - 1) < n ? k0 + 4 * (ROWS - 1) : n - 1;
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            if (64 * m > klast) break;             // uniform across the wavefront
-            const int l = lane + 64 * m;
-            double gv[ROWS], dv[ROWS];
+        for (int m = 0; m < M2; ++m) {
+            if (128 * m > klast) break;            // uniform across the wavefront
+            const int l0 = 2 * lane + 128 * m;
+            double g0[ROWS], g1[ROWS], d0[ROWS], d1[ROWS];
 #pragma unroll
             for (int r = 0; r < ROWS; ++r) {
                 const int kk = kr[r] < 0 ? n - 1 : kr[r];
-                const int at = base[r] + (l <= kk ? l : kk);     // clamped: always inside the row
-                // 32-bit byte offsets from the block-uniform row and density bases (address arithmetic
-                // stays in one VALU op; a row is at most np * 8 < 2^20 bytes)
-                const double gx = (double)*(const T*)((const char*)row0 + (unsigned)at * (unsigned)sizeof(T));
-                dv[r] = *(const double*)((const char*)Dw + (unsigned)at * 8u);
-                gv[r] = l <= kr[r] ? gx : 0.0;
+                const int kev = kk & ~1;           // last even column of the row: (kev, kev + 1) is in the row
+                const unsigned at = (unsigned)(base[r] + (l0 <= kev ? l0 : kev));
+                const typename PairOf<T>::V gx = *(const typename PairOf<T>::V*)(row0 + at * (unsigned)sizeof(T));
+                const double2 dx = *(const double2*)(dwb + at * 8u);
+                g0[r] = l0 <= kr[r] ? (double)gx.x : 0.0;
+                g1[r] = l0 + 1 <= kr[r] ? (double)gx.y : 0.0;
+                d0[r] = dx.x;
+                d1[r] = dx.y;
             }
 #pragma unroll
-            for (int r = 0; r < ROWS; ++r) {
-                const double g = gv[r];
-                jacc = fma(g, dv[r], jacc);
+            for (int r = 0; r < ROWS; ++r)
 #pragma unroll
-                for (int d = 0; d < NDENS; ++d) {
-                    V[(2 * r) * NDENS + d] = fma(g, xjl[d][m], V[(2 * r) * NDENS + d]);
-                    V[(2 * r + 1) * NDENS + d] = fma(g, xil[d][m], V[(2 * r + 1) * NDENS + d]);
-                }
-                const double gc = l < kr[r] ? g : 0.0;            // the diagonal element has no column part
+                for (int h = 0; h < 2; ++h) {
+                    const double g = h ? g1[r] : g0[r];
+                    jacc = fma(g, h ? d1[r] : d0[r], jacc);
 #pragma unroll
-                for (int d = 0; d < NDENS; ++d) {
-                    cI[d][m] = fma(gc, xkj[r][d], cI[d][m]);
-                    cJ[d][m] = fma(gc, xki[r][d], cJ[d][m]);
+                    for (int d = 0; d < NDENS; ++d) {
+                        V[(2 * r) * NDENS + d] = fma(g, xjl[d][m][h], V[(2 * r) * NDENS + d]);
+                        V[(2 * r + 1) * NDENS + d] = fma(g, xil[d][m][h], V[(2 * r + 1) * NDENS + d]);
+                    }
+                    const double gc = l0 + h < kr[r] ? g : 0.0;    // the diagonal element has no column part
+#pragma unroll
+                    for (int d = 0; d < NDENS; ++d) {
+                        cI[d][m][h] = fma(gc, xkj[r][d], cI[d][m][h]);
+                        cJ[d][m][h] = fma(gc, xki[r][d], cJ[d][m][h]);
+                    }
                 }
-            }
         }
         Halving<NV, 32>::run(V, lane);
         if ((lane & ((1 << SH) - 1)) == 0) {
@@ -185,13 +212,15 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
 #pragma unroll
     for (int d = 0; d < NDENS; ++d)
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const int l = lane + 64 * m;
-            if (l < n) {
-                atomicAdd(&rI[d * n + l], cI[d][m]);
-                atomicAdd(&rJ[d * n + l], cJ[d][m]);
+        for (int m = 0; m < M2; ++m)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int l = 2 * lane + 128 * m + h;
+                if (l < n) {
+                    atomicAdd(&rI[d * n + l], cI[d][m][h]);
+                    atomicAdd(&rJ[d * n + l], cJ[d][m][h]);
+                }
             }
-        }
     __syncthreads();
     if (threadIdx.x == 0) {
         const double v = (red[0] + red[1]) + (red[2] + red[3]);
@@ -206,23 +235,19 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
 }
 
 template <int NDENS, int ROWS, typename T>
-hipError_t launch_incore(const T* G, int n, int64_t np, const double* D, const double* Dw, double* J, double* K,
+hipError_t launch_incore(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
                          hipStream_t st) {
     const size_t lds = (4 * (size_t)NDENS * n + 4) * sizeof(double);
-    const dim3 grid((unsigned)np), block(256);
-    switch ((n + 63) / 64) {
-#define HF_INCORE_CASE(MM)                                                                                          \
-    case MM:                                                                                                        \
-        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MM, ROWS, T>), grid, block, lds, st, G, n, np, D, Dw, J, K); \
+    const dim3 grid((unsigned)npad), block(256);
+    switch ((n + 127) / 128) {
+#define HF_INCORE_CASE(MM)                                                                                            \
+    case MM:                                                                                                          \
+        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MM, ROWS, T>), grid, block, lds, st, G, n, npad, D, Dw, J, K); \
         break;
         HF_INCORE_CASE(1)
         HF_INCORE_CASE(2)
         HF_INCORE_CASE(3)
         HF_INCORE_CASE(4)
-        HF_INCORE_CASE(5)
-        HF_INCORE_CASE(6)
-        HF_INCORE_CASE(7)
-        HF_INCORE_CASE(8)
 #undef HF_INCORE_CASE
         default:
             return hipErrorInvalidValue;
@@ -245,7 +270,7 @@ hipError_t launch_incore_rows(int rows, int ndens, const T* G, int n, int64_t np
     return launch_incore<2, 4>(G, n, np, D, Dw, J, K, st);
 }
 
-// Autotuner: the row-group height is picked per (column chunks M = ceil(n/64), densities, store
+// Autotuner: the row-group height is picked per (column chunks M2 = ceil(n/128), densities, store
 // precision) by timing every candidate on the caller's own store, density and stream, once per
 // process (the table is shared by all contexts; a server's slots tune each key once).  Candidates
 // write to scratch J/K, so the build's outputs are untouched.
@@ -261,7 +286,7 @@ IncoreTuning& tuning() {
 template <typename T>
 int tuned_rows(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double* D, const double* Dw,
                hipStream_t st) {
-    const int key = ((n + 63) / 64) * 100 + ndens * 10 + (sizeof(T) == 4 ? 1 : 0);
+    const int key = ((n + 127) / 128) * 100 + ndens * 10 + (sizeof(T) == 4 ? 1 : 0);
     {
         std::lock_guard<std::mutex> lock(tuning().mu);
         auto it = tuning().rows.find(key);
@@ -314,7 +339,7 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
     if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
     const int n = ctx->mol.nbf;
     if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "in-core J/K takes at most 512 basis functions");
-    const int64_t np = (int64_t)n * (n + 1) / 2;
+    const int64_t np = hfgpu::pair_pad_base(n);      // padded pair positions (rows and columns of the store)
     const size_t nn = (size_t)n * n;
     HF_HIP_CHECK(ctx, ctx->d_Dw.alloc((size_t)np));
     hipLaunchKernelGGL(incore_dw_kernel, dim3(n), dim3(256), 0, st, d_D, n, ndens, ctx->d_Dw.p);

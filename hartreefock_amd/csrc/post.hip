@@ -5,11 +5,11 @@
 // UnrestrictedHartreeFock::CalculateMp2Energy, HartreeFock/UnrestrictedHartreeFock.cpp:295-375).  The
 // reference transforms ERIs to the MO basis one quarter at a time with hash-map caches
 // (HartreeFock/IntegralsRepository.h:190-291).  Here the transform is two FP64 GEMMs on the matrix
-// cores over the pair-matrix store G (symmetric over canonical AO pairs P = (m >= n)):
-//   C2[P][(i,a)] = C_mi C_na + C_ni C_ma (m > n), C_mi C_ma (m = n)
+// cores over the in-core store G (symmetric over canonical AO pairs P = (m >= n), padded positions):
+//   C2[P][(i,a)] = C_mi C_na + C_ni C_ma (m > n), C_mi C_ma (m = n), 0 at padding positions
 //   (ia|jb) = sum_{P,Q} C2[P][(ia)] G[P][Q] C2[Q][(jb)]  =  (C2^T G C2)[(ia)][(jb)]
-// 2 ov np^2 + 2 (ov)^2 np flops (o occupied, v virtual orbitals, np = n(n+1)/2), then one pass over
-// the (ov)^2 MO integrals for the energy.
+// 2 ov np^2 + 2 (ov)^2 np flops (o occupied, v virtual orbitals, np ~ n(n+1)/2 + n/2), then one pass
+// over the (ov)^2 MO integrals for the energy.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -20,23 +20,28 @@
 
 namespace {
 
-// C2 column (ia) = i * nv + a, row P; column-major with leading dimension np.  C is column-major:
+// C2 column (ia) = i * nv + a, row R = pair_pad_index(m, q) of the in-core store (padding rows 0);
+// column-major with leading dimension np (the store's padded pair count).  C is column-major:
 // orbital p's coefficients at C[p * n + m].
 __global__ void mp2_c2_kernel(const double* __restrict__ C, int n, int nocc, int nv, int64_t np,
                               double* __restrict__ C2) {
-    const int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= np) return;
-    int m = (int)((sqrt(8.0 * (double)P + 1.0) - 1.0) * 0.5);
-    if ((int64_t)m * (m + 1) / 2 > P) --m;
-    if ((int64_t)(m + 1) * (m + 2) / 2 <= P) ++m;
-    const int q = (int)(P - (int64_t)m * (m + 1) / 2);
+    const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (R >= np) return;
+    int m = (int)sqrt(2.0 * (double)R) - 1;
+    if (m < 0) m = 0;
+    while (hfgpu::pair_pad_base(m + 1) <= R) ++m;
+    while (m > 0 && hfgpu::pair_pad_base(m) > R) --m;
+    const int q = (int)(R - hfgpu::pair_pad_base(m));
     for (int ia = blockIdx.y; ia < nocc * nv; ia += gridDim.y) {
-        const int i = ia / nv, a = nocc + ia % nv;
-        const double* ci = C + (size_t)i * n;
-        const double* ca = C + (size_t)a * n;
-        double v = ci[m] * ca[q];
-        if (m != q) v += ci[q] * ca[m];
-        C2[(size_t)ia * np + P] = v;
+        double v = 0.0;
+        if (q <= m) {
+            const int i = ia / nv, a = nocc + ia % nv;
+            const double* ci = C + (size_t)i * n;
+            const double* ca = C + (size_t)a * n;
+            v = ci[m] * ca[q];
+            if (m != q) v += ci[q] * ca[m];
+        }
+        C2[(size_t)ia * np + R] = v;
     }
 }
 
@@ -117,7 +122,7 @@ int hf_mp2_impl(hf_ctx* ctx, hipStream_t st, int nocc, const double* C, const do
     const int nv = n - nocc;
     e2 = 0.0;
     if (nocc <= 0 || nv <= 0) return HF_OK;
-    const int64_t np = (int64_t)n * (n + 1) / 2, ov = (int64_t)nocc * nv;
+    const int64_t np = hfgpu::pair_pad_base(n), ov = (int64_t)nocc * nv;     // the store is np x np
     DevBuf<double> C2, T1, MO, part;
     HF_HIP_CHECK(ctx, C2.alloc((size_t)(np * ov)));
     HF_HIP_CHECK(ctx, T1.alloc((size_t)(np * ov)));
