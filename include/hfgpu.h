@@ -118,8 +118,10 @@ int hf_build_jk_device(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, d
  * density in each Fock build (HartreeFock/IntegralsRepository.cpp:566-584, then
  * HartreeFock/RestrictedHartreeFock.cpp:189-202 and HartreeFock/UnrestrictedHartreeFock.cpp:228-238),
  * with the store in HBM.  The store is the symmetric matrix over canonical function pairs,
- * G[P*np + Q] = (ij|kl), P = i(i+1)/2 + j (i >= j), Q = k(k+1)/2 + l (k >= l), np = n(n+1)/2
- * (hf_pair_matrix_len(n) = np*np doubles: 2.9 GB for n = 194, 137 GB for n = 512), screened at the
+ * G[P*np + Q] = (ij|kl) with P = pos(i, j) (i >= j), Q = pos(k, l) (k >= l), where the pair triangle is
+ * laid out row by row with each row padded to an even length: pos(k, l) = base(k) + l, base(k) =
+ * k(k+2)/2 for even k and (k+1)^2/2 for odd k, np = base(n) (padding entries are 0).
+ * hf_pair_matrix_len(n) = np*np doubles: 2.9 GB for n = 194, 137 GB for n = 512.  Screened at the
  * context's Schwarz threshold.  J and K as hf_build_jk; n <= 512. */
 uint64_t hf_pair_matrix_len(int nbf);
 int hf_eri_pairs_device(hf_ctx* ctx, double* d_G, uint64_t len, void* stream);
