@@ -80,9 +80,10 @@ struct hf_server {
                 p.terminate = nullptr;
                 hf_scf_result sr{};
                 rc = hf_scf(ctx, &p, &sr, nullptr);
-                // the in-core store is counted in the request's footprint only while it runs: free
-                // stores above 1 GiB (smaller ones stay cached, at most slots x 1 GiB in all)
-                hf_release_incore(ctx, (size_t)1 << 30);
+                // the in-core store is counted in the request's footprint only while it runs; a slot
+                // keeps stores up to 16 GiB for its next request (freeing synchronises the whole device,
+                // which stalls every other slot), and frees larger ones
+                hf_release_incore(ctx, (size_t)16 << 30);
                 r.energy = sr.energy;
                 r.iterations = sr.iterations;
                 r.converged = sr.converged;
