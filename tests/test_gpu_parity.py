@@ -468,3 +468,19 @@ def test_incore_uhf_density_changes_match_direct(gpu_engine_factory, prec, incr)
     print("uhf direct %.10f in-core %.10f s2 %.4f %.4f" % (ref.energy, r.energy, ref.s2, r.s2), flush=True)
     assert ref.converged and r.converged and r.incore == 1
     assert abs(r.energy - ref.energy) <= 1e-6 and abs(r.s2 - ref.s2) <= 1e-4
+
+
+def test_mp2_at_scale_matches_oracle_restatement(gpu_engine_factory):
+    """MP2 at 99 functions (d shells, 21 occupied and 78 virtual orbitals): the GPU's GEMM transform over the
+    in-core store against the oracle's restatement of the reference's MP2 applied to the GPU's own packed ERIs
+    and converged density (the oracle's ERIs take minutes at this size), 1e-8 Eh."""
+    _print_case("pentane_631gs mp2")
+    atoms = g.alkane_zigzag(5, seed=31)
+    eng = gpu_engine_factory(schwarz_threshold=0.0, exact_transfer=1).set_molecule(basis("6-31g_st_.1.nw"), atoms)
+    r, D = eng.scf(mp2=1, incore=1, want_density=True, **TIGHT)
+    assert r.converged and r.incore == 1
+    n = eng.nbf
+    S, T, V = eng.one_electron()
+    e2 = refcpu.mp2_rhf(n, eng.eri_packed(), D[0], T + V, S, eng.electrons()[0])
+    print("mp2 gpu %.12f oracle %.12f" % (r.mp2_energy, e2), flush=True)
+    assert abs(r.mp2_energy - e2) <= 1e-8
