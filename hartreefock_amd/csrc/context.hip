@@ -691,6 +691,10 @@ int hf_destroy(hf_ctx* ctx) {
     for (hipEvent_t e : ctx->incore_ev)
         if (e) (void)hipEventDestroy(e);
     for (hipStream_t s2 : ctx->side) (void)hipStreamDestroy(s2);
+    if (ctx->eri_stream) {
+        (void)hipStreamSynchronize(ctx->eri_stream);
+        (void)hipStreamDestroy(ctx->eri_stream);
+    }
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

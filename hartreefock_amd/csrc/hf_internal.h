@@ -142,6 +142,7 @@ struct hf_ctx {
     DevBuf<double> d_G;
     DevBuf<float> d_G32;
     DevBuf<double> d_Dw;                   // packed weighted total density of an in-core J/K build
+    hipStream_t eri_stream = nullptr;      // the SCF evaluates the in-core store here, beside its own stream
     // in-core contraction statistics ([0] FP64 store, [1] FP32 store): launches, elements read
     // (eris), algorithmic flops, and with profile = 1 HIP-event time
     KernelStat incore_stat[2];

@@ -608,14 +608,12 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     HF_HIP_CHECK(ctx, st.redpart.alloc(8 * kDotBlocks));
     HF_HIP_CHECK(ctx, st.info.alloc(2));
     // S, T, V on the device; h = T + V
-    if (int rc = hf_one_electron_device(ctx, st.S.p, st.T1.p, st.T2.p, s)) return rc;
-    hipLaunchKernelGGL(add_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.T2.p, st.h.p, nn);
-    // S^{-1/2}, symmetrised (:48-59)
-    if (int rc = inverse_sqrt(ctx, st)) return rc;
     // in-core ERIs (the reference's store-once scheme) when the pair matrix fits the budget; the
-    // mixed-precision tier contracts an FP32 copy until its final FP64 phase
+    // mixed-precision tier contracts an FP32 copy until its final FP64 phase.  The store is evaluated
+    // on its own stream, overlapping the one-electron integrals, S^-1/2 and iteration 0 (which needs
+    // no J/K); the first J/K build waits for it.
     const bool mixed_tier = ctx->opt.precision == HF_PREC_MIXED;
-    bool incore = false;
+    bool incore = false, store_pending = false;
     double eri_ms = 0;
     hipEvent_t eri_ev[2] = {nullptr, nullptr};
     struct EventGuard {
@@ -637,16 +635,23 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             (void)hipGetLastError();
         }
         if (incore) {
+            if (!ctx->eri_stream) HF_HIP_CHECK(ctx, hipStreamCreateWithFlags(&ctx->eri_stream, hipStreamNonBlocking));
+            hipStream_t es = ctx->eri_stream;
             HF_HIP_CHECK(ctx, hipEventCreate(&eri_ev[0]));
             HF_HIP_CHECK(ctx, hipEventCreate(&eri_ev[1]));
-            HF_HIP_CHECK(ctx, hipEventRecord(eri_ev[0], s));
-            if (int rc = hf_eri_pairs_impl(ctx, ctx->d_G.p, len, s)) return rc;
+            HF_HIP_CHECK(ctx, hipEventRecord(eri_ev[0], es));
+            if (int rc = hf_eri_pairs_impl(ctx, ctx->d_G.p, len, es)) return rc;
             if (mixed_tier)
-                hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for(len)), dim3(256), 0, s, ctx->d_G.p, ctx->d_G32.p,
+                hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for(len)), dim3(256), 0, es, ctx->d_G.p, ctx->d_G32.p,
                                    (size_t)len);
-            HF_HIP_CHECK(ctx, hipEventRecord(eri_ev[1], s));
+            HF_HIP_CHECK(ctx, hipEventRecord(eri_ev[1], es));
+            store_pending = true;
         }
     }
+    if (int rc = hf_one_electron_device(ctx, st.S.p, st.T1.p, st.T2.p, s)) return rc;
+    hipLaunchKernelGGL(add_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.T2.p, st.h.p, nn);
+    // S^{-1/2}, symmetrised (:48-59)
+    if (int rc = inverse_sqrt(ctx, st)) return rc;
     // occupations (HartreeFock/RestrictedHartreeFock.cpp:18-32, UnrestrictedHartreeFock.cpp:20-60)
     const int ne = ctx->mol.electrons();
     int occ[2];
@@ -729,6 +734,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             const double q0 = quartet_count();
             const bool full = !incremental || force_full || full_builds < first_full || since_full >= rebuild_every;
             bool used_fp32 = !fp64_phase;
+            if (incore && store_pending) {   // the store (own stream) must be complete
+                HF_HIP_CHECK(ctx, hipStreamWaitEvent(s, eri_ev[1], 0));
+                store_pending = false;
+            }
             if (full && incore) {
                 used_fp32 = !fp64_phase && !incore_mixed;
                 if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, used_fp32 ? ctx->d_G32.p : nullptr, st.D.p, st.J.p,
@@ -971,6 +980,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
             if (int rc = syevd(ctx, st, st.T2.p, st.eps.p + k * n)) return rc;
             if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T2.p, 0.0, st.C.p + k * nn)) return rc;
+        }
+        if (incore && store_pending) {
+            HF_HIP_CHECK(ctx, hipStreamWaitEvent(s, eri_ev[1], 0));
+            store_pending = false;
         }
         if (!incore) {
             if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "MP2 needs the in-core ERI store (n <= 512)");
