@@ -678,6 +678,11 @@ int hf_create(const hf_options* opt, hf_ctx** out) {
         return HF_E_HIP;
     }
     rocblas_set_stream(ctx->blas, ctx->stream);
+    // pinned buffer for the SCF's small read-backs (without it they go through pageable memory)
+    if (hipHostMalloc((void**)&ctx->hpin, kPinnedDoubles * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        ctx->hpin = nullptr;
+        (void)hipGetLastError();
+    }
     *out = ctx;
     return HF_OK;
 }
@@ -695,6 +700,7 @@ int hf_destroy(hf_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->eri_stream);
         (void)hipStreamDestroy(ctx->eri_stream);
     }
+    if (ctx->hpin) (void)hipHostFree(ctx->hpin);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -348,8 +348,7 @@ int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp,
             hipSuccess) {
         hipLaunchKernelGGL(sp2_local_kernel, dim3(1), dim3(1024), lds, st, Fp, n, (double)nocc, max_steps, hist + 2, X0);
         double fin = -1;
-        HF_HIP_CHECK(ctx, hipMemcpyAsync(&fin, hist + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-        HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        if (int rc = hf_readback(ctx, st, &fin, hist + 2, 1)) return rc;
         result = fin >= 0 ? X0 : nullptr;
         steps = (int)fin;
         return HF_OK;
@@ -378,8 +377,7 @@ int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp,
         hipLaunchKernelGGL(sp2_step_kernel, dim3(nt, nt), dim3(256), 0, st, g);
         if ((s >= kFirstCheck && (s - kFirstCheck) % kEvery == 0) || s == max_steps) {
             double fin = -1;
-            HF_HIP_CHECK(ctx, hipMemcpyAsync(&fin, hist + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-            HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
+            if (int rc = hf_readback(ctx, st, &fin, hist + 2, 1)) return rc;
             if (fin >= 0) {
                 steps = (int)fin;
                 result = Xb[steps & 1];

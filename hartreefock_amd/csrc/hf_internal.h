@@ -143,6 +143,7 @@ struct hf_ctx {
     DevBuf<float> d_G32;
     DevBuf<double> d_Dw;                   // packed weighted total density of an in-core J/K build
     hipStream_t eri_stream = nullptr;      // the SCF evaluates the in-core store here, beside its own stream
+    double* hpin = nullptr;                // kPinnedDoubles of pinned host memory for small read-backs
     // in-core contraction statistics ([0] FP64 store, [1] FP32 store): launches, elements read
     // (eris), algorithmic flops, and with profile = 1 HIP-event time
     KernelStat incore_stat[2];
@@ -221,3 +222,19 @@ int hf_properties_impl(hf_ctx* ctx, int ndens, const double* d_D, double* charge
         if (e_ != hipSuccess)                                                                           \
             return (ctx)->fail(HF_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));            \
     } while (0)
+
+// Small device-to-host read-backs of the SCF loop (reduction results, stop flags): through the
+// context's pinned buffer, which the copy engine writes directly (pageable memory would take a
+// staging copy on every read-back).  count <= kPinnedDoubles.
+constexpr int kPinnedDoubles = 64;
+inline int hf_readback(hf_ctx* ctx, hipStream_t st, double* dst, const double* src, int count) {
+    if (!ctx->hpin || count > kPinnedDoubles) {
+        HF_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, st));
+        HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        return HF_OK;
+    }
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->hpin, src, count * sizeof(double), hipMemcpyDeviceToHost, st));
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    for (int i = 0; i < count; ++i) dst[i] = ctx->hpin[i];
+    return HF_OK;
+}

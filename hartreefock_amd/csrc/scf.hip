@@ -422,8 +422,7 @@ int inverse_sqrt(hf_ctx* ctx, ScfState& st) {
     jobs.y[0] = st.S.p;
     launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
     double fro2 = 0;
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(&fro2, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
-    HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (int rc = hf_readback(ctx, s, &fro2, st.red.p, 1)) return rc;
     const double c = 1.0 / std::sqrt(fro2);
     double *Y = st.T1.p, *Z = st.X.p, *T = st.T2.p, *W = st.T3.p, *E = st.err.p;
     hipLaunchKernelGGL(scale_ident_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.S.p, c, Y, Z, n);
@@ -438,8 +437,7 @@ int inverse_sqrt(hf_ctx* ctx, ScfState& st) {
             jobs.diff[0] = 1;
             launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
             double r2 = 0;
-            HF_HIP_CHECK(ctx, hipMemcpyAsync(&r2, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
-            HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+            if (int rc = hf_readback(ctx, s, &r2, st.red.p, 1)) return rc;
             const double r = std::sqrt(r2);
             if (r < 1e-14 * std::sqrt((double)n) || (r < 1e-8 && r >= 0.5 * prev)) ok = true;
             prev = r;
@@ -481,8 +479,7 @@ int sp2_projector(hf_ctx* ctx, ScfState& st, const double* Fp, int nocc, double*
         hipLaunchKernelGGL(sp2_update_kernel, dim3(grid_for(st.nn)), dim3(256), 0, s, P, W, st.nn, st.sp2.p);
         if ((k >= kFirstCheck && (k - kFirstCheck) % kEvery == 0) || k == kMaxSteps) {
             double done = 0;
-            HF_HIP_CHECK(ctx, hipMemcpyAsync(&done, st.sp2.p + SP2_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
-            HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+            if (int rc = hf_readback(ctx, s, &done, st.sp2.p + SP2_DONE, 1)) return rc;
             if (done != 0) {
                 ok = true;
                 break;
@@ -510,8 +507,7 @@ int diis_add(hf_ctx* ctx, ScfState& st, Diis& d, const double* Fsrc, const doubl
     }
     launch_dots(s, jobs, st.nn, st.redpart.p, st.red.p);
     double h[8];
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(h, st.red.p, jobs.m * sizeof(double), hipMemcpyDeviceToHost, s));
-    HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (int rc = hf_readback(ctx, s, h, st.red.p, jobs.m)) return rc;
     for (int r = 0; r < jobs.m; ++r) d.B[slot][d.order[r]] = d.B[d.order[r]][slot] = h[r];
     ready = (int)d.order.size() >= Diis::kFirst;
     return HF_OK;
@@ -759,8 +755,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 // the in-core FP64 contraction reads the whole store whatever max|dD| is; the mixed tier
                 // needs max|dD| to decide whether the FP32 copy is safe
                 if (!incore || (incore_mixed && used_fp32)) {
-                    HF_HIP_CHECK(ctx, hipMemcpyAsync(&dmax, st.red.p + 15, sizeof(double), hipMemcpyDeviceToHost, s));
-                    HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+                    if (int rc = hf_readback(ctx, s, &dmax, st.red.p + 15, 1)) return rc;
                     last_dmax = dmax;
                 }
                 // FP32 store only for small density changes: its rounding enters J/K scaled by |dD|, and
@@ -869,8 +864,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         }
         launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
         double red[8];
-        HF_HIP_CHECK(ctx, hipMemcpyAsync(red, st.red.p, jobs.m * sizeof(double), hipMemcpyDeviceToHost, s));
-        HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        if (int rc = hf_readback(ctx, s, red, st.red.p, jobs.m)) return rc;
         if (jk_pending) {
             float ms = 0;
             HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, jk_ev[0], jk_ev[1]));
@@ -1035,8 +1029,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             jobs.x[0] = st.T1.p;
             jobs.y[0] = st.T2.p;
             launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
-            HF_HIP_CHECK(ctx, hipMemcpyAsync(&ov, st.red.p, sizeof(double), hipMemcpyDeviceToHost, s));
-            HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+            if (int rc = hf_readback(ctx, s, &ov, st.red.p, 1)) return rc;
         }
         const double sz = 0.5 * (occ[0] - occ[1]);
         r->s2 = sz * sz + 0.5 * (occ[0] + occ[1]) - ov;
