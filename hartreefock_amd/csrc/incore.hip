@@ -17,10 +17,12 @@
 // B_P x splits into a row part (y[k] += B[k][l] x[l], l <= k: a sum across the lanes that hold one
 // row) and a column part (y[l] += B[k][l] x[k], l < k: a register of the lane that owns column l).
 // One workgroup per P: four wavefronts take rows k = w (mod 4), ROWS rows at a time; lane L owns the
-// column pairs (2L, 2L+1) + 128 m and reads each pair with one load.  The 2 ROWS ndens row sums of a
-// group are reduced together by recursive halving (each exchange step halves the values a lane
-// carries), 10 shuffles for 8 values instead of 6 per value.  J_ij needs no atomics (one block per P);
-// each block adds its two K rows with coalesced FP64 atomics.
+// column pairs (2L, 2L+1) + 128 m and reads each pair with one load; the loads of every column chunk
+// of a group go out together (optionally the next group's too, before this group's arithmetic).
+// The 2 ROWS ndens row sums of a group are reduced together by recursive halving (each exchange step
+// halves the values a lane carries), 10 exchanges for 8 values instead of 6 per value, and every
+// exchange stays in registers (gfx950 permlane swaps and DPP moves, no LDS round trip).  J_ij needs no
+// atomics (one block per P); each block adds its two K rows with coalesced FP64 atomics.
 //
 // The kernel streams np^2 elements (8 or 4 bytes) per build: HBM-bound, ~2 + 8 ndens flops per
 // element.  Each element is used once per density, so there is no operand reuse for the matrix cores.
@@ -52,25 +54,76 @@ __global__ void incore_dw_kernel(const double* __restrict__ D, int n, int ndens,
     }
 }
 
+// Lane exchanges of gfx950 that stay in registers (no LDS round trip).  swap32 / swap16 are
+// v_permlane32_swap / v_permlane16_swap on both halves of a double: lanes 32..63 of a trade places with
+// lanes 0..31 of b (swap32), or the odd 16-lane rows of a with the even rows of b (swap16).
+__device__ __forceinline__ double join_halves(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int S>
+__device__ __forceinline__ void lane_swap(double& a, double& b) {
+    const unsigned long long ia = (unsigned long long)__double_as_longlong(a);
+    const unsigned long long ib = (unsigned long long)__double_as_longlong(b);
+    if constexpr (S == 32) {
+        const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ia, (unsigned)ib, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ia >> 32), (unsigned)(ib >> 32), false, false);
+        a = join_halves(lo[0], hi[0]);
+        b = join_halves(lo[1], hi[1]);
+    } else {
+        const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ia, (unsigned)ib, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ia >> 32), (unsigned)(ib >> 32), false, false);
+        a = join_halves(lo[0], hi[0]);
+        b = join_halves(lo[1], hi[1]);
+    }
+}
+// Value of the partner lane for the exchange steps below 16, by DPP: row_ror:8 (lane ^ 8),
+// row_half_mirror (lane ^ 7 within 8 lanes), quad_perm [2,3,0,1] (lane ^ 2), quad_perm [1,0,3,2] (lane ^ 1).
+template <int S>
+__device__ __forceinline__ double lane_partner(double v) {
+    constexpr int ctrl = S == 8 ? 0x128 : S == 4 ? 0x141 : S == 2 ? 0x4E : 0xB1;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), ctrl, 0xF, 0xF, false);
+    return join_halves((unsigned)lo, (unsigned)hi);
+}
+
 // Sum of V[0..CNT) across the wavefront by recursive halving: at exchange step S a lane keeps the
-// half of its values selected by bit S of its lane id and adds the partner's copy of that half.
-// Afterwards V[0] of lane L holds the total of value index L >> (6 - log2 CNT).
+// half of its values selected by bit S of its lane id and adds the partner's copy of that half.  The
+// partners are lane ^ 32, ^ 16, ^ 8, ^ 7, ^ 2, ^ 1: the masks are independent, so the six steps cover
+// all 64 lanes, and each partner differs from the lane in bit S.  Afterwards V[0] of lane L holds the
+// total of value index L >> (6 - log2 CNT).  Steps 32 and 16 trade the two halves whole: after the
+// swap the lane's kept half and the partner's copy of it are both in its registers (a + b).
 template <int CNT, int S>
 struct Halving {
     template <int NV>
     __device__ static __forceinline__ void run(double (&V)[NV], int lane) {
         if constexpr (CNT > 1) {
             constexpr int H = CNT / 2;
-            const bool up = (lane & S) != 0;
+            if constexpr (S >= 16) {
 #pragma unroll
-            for (int t = 0; t < H; ++t) {
-                const double keep = up ? V[H + t] : V[t];
-                const double send = up ? V[t] : V[H + t];
-                V[t] = keep + __shfl_xor(send, S, 64);
+                for (int t = 0; t < H; ++t) {
+                    double a = V[t], b = V[H + t];
+                    lane_swap<S>(a, b);
+                    V[t] = a + b;
+                }
+            } else {
+                const bool up = (lane & S) != 0;
+#pragma unroll
+                for (int t = 0; t < H; ++t) {
+                    const double keep = up ? V[H + t] : V[t];
+                    const double send = up ? V[t] : V[H + t];
+                    V[t] = keep + lane_partner<S>(send);
+                }
             }
             if constexpr (S > 1) Halving<H, S / 2>::run(V, lane);
         } else {
-            V[0] += __shfl_xor(V[0], S, 64);
+            if constexpr (S >= 16) {
+                double a = V[0], b = V[0];
+                lane_swap<S>(a, b);
+                V[0] = a + b;
+            } else {
+                V[0] += lane_partner<S>(V[0]);
+            }
             if constexpr (S > 1) Halving<1, S / 2>::run(V, lane);
         }
     }
@@ -95,7 +148,7 @@ struct PairOf<float> {
 // One workgroup per padded row index R of the store (R = pair_pad_index(i, j); padding rows return at
 // once).  M2 = ceil(n / 128) column chunks: lane L owns columns 2L + 128 m and 2L + 1 + 128 m and reads
 // both with one aligned 16-byte (FP64) or 8-byte (FP32) load.
-template <int NDENS, int M2, int ROWS, typename T>
+template <int NDENS, int M2, int ROWS, bool PREFETCH, typename T>
 __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t npad,
                                                         const double* __restrict__ D, const double* __restrict__ Dw,
                                                         double* __restrict__ J, double* __restrict__ K) {
@@ -139,19 +192,44 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
     const char* __restrict__ dwb = (const char*)Dw;
     constexpr int NV = 2 * ROWS * NDENS;           // ROWS rows x (K_i, K_j) x densities
     constexpr int SH = 6 - log2c<NV>();
+    using PV = typename PairOf<T>::V;
+    // The loads of a row group: every column chunk of its ROWS rows k0 + 4r, issued together.  Addresses
+    // are clamped into the row (rows past the last one read row n - 1), so every load is in bounds;
+    // elements outside the row are masked where they are used.
+    auto load = [&](int k0, PV (&g)[ROWS][M2], double2 (&dd)[ROWS][M2]) {
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int k = k0 + 4 * r;
+            const int kk = k < n ? k : n - 1;
+            const int b = (int)hfgpu::pair_pad_base(kk);
+            const int kev = kk & ~1;               // last even column of the row: (kev, kev + 1) is in the row
+#pragma unroll
+            for (int m = 0; m < M2; ++m) {
+                const int l0 = 2 * lane + 128 * m;
+                const unsigned at = (unsigned)(b + (l0 <= kev ? l0 : kev));
+                g[r][m] = *(const PV*)(row0 + at * (unsigned)sizeof(T));
+                dd[r][m] = *(const double2*)(dwb + at * 8u);
+            }
+        }
+    };
+    PV ga[ROWS][M2];
+    double2 da[ROWS][M2];
+    load(w, ga, da);
     for (int k0 = w; k0 < n; k0 += 4 * ROWS) {
+        // PREFETCH: the next group's loads go out before this group's arithmetic and reduction
+        PV gb[ROWS][M2];
+        double2 db[ROWS][M2];
+        if constexpr (PREFETCH) load(k0 + 4 * ROWS, gb, db);
         double V[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) V[v] = 0.0;
-        // rows k0 + 4r (r < ROWS); the rows' elements of one column chunk are loaded together
-        int base[ROWS], kr[ROWS];
+        int kr[ROWS];
         double xkj[ROWS][NDENS], xki[ROWS][NDENS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
             const int k = k0 + 4 * r;
             kr[r] = k < n ? k : -1;                // -1: past the last row (no element)
             const int kk = k < n ? k : n - 1;
-            base[r] = (int)hfgpu::pair_pad_base(kk);
 #pragma unroll
             for (int d = 0; d < NDENS; ++d) {
                 xkj[r][d] = xj[d * n + kk];
@@ -163,25 +241,13 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
         for (int m = 0; m < M2; ++m) {
             if (128 * m > klast) break;            // uniform across the wavefront
             const int l0 = 2 * lane + 128 * m;
-            double g0[ROWS], g1[ROWS], d0[ROWS], d1[ROWS];
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r) {
-                const int kk = kr[r] < 0 ? n - 1 : kr[r];
-                const int kev = kk & ~1;           // last even column of the row: (kev, kev + 1) is in the row
-                const unsigned at = (unsigned)(base[r] + (l0 <= kev ? l0 : kev));
-                const typename PairOf<T>::V gx = *(const typename PairOf<T>::V*)(row0 + at * (unsigned)sizeof(T));
-                const double2 dx = *(const double2*)(dwb + at * 8u);
-                g0[r] = l0 <= kr[r] ? (double)gx.x : 0.0;
-                g1[r] = l0 + 1 <= kr[r] ? (double)gx.y : 0.0;
-                d0[r] = dx.x;
-                d1[r] = dx.y;
-            }
 #pragma unroll
             for (int r = 0; r < ROWS; ++r)
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    const double g = h ? g1[r] : g0[r];
-                    jacc = fma(g, h ? d1[r] : d0[r], jacc);
+                    const double graw = h ? (double)ga[r][m].y : (double)ga[r][m].x;
+                    const double g = l0 + h <= kr[r] ? graw : 0.0;
+                    jacc = fma(g, h ? da[r][m].y : da[r][m].x, jacc);
 #pragma unroll
                     for (int d = 0; d < NDENS; ++d) {
                         V[(2 * r) * NDENS + d] = fma(g, xjl[d][m][h], V[(2 * r) * NDENS + d]);
@@ -202,10 +268,24 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
             const int k = k0 + 4 * r;
             if (k < n) (a ? rJ : rI)[d * n + k] = V[0];   // row k belongs to this wavefront alone
         }
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+                for (int m = 0; m < M2; ++m) {
+                    ga[r][m] = gb[r][m];
+                    da[r][m] = db[r][m];
+                }
+        } else if (k0 + 4 * ROWS < n) {
+            load(k0 + 4 * ROWS, ga, da);
+        }
     }
     // J_ij: wavefront sums, then the block's
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) jacc += __shfl_xor(jacc, s, 64);
+    {
+        double jv[1] = {jacc};
+        Halving<1, 32>::run(jv, lane);
+        jacc = jv[0];
+    }
     if (lane == 0) red[w] = jacc;
     __syncthreads();
     // column parts of every wavefront
@@ -234,15 +314,16 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
     }
 }
 
-template <int NDENS, int ROWS, typename T>
+template <int NDENS, int ROWS, bool PREFETCH, typename T>
 hipError_t launch_incore(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
                          hipStream_t st) {
     const size_t lds = (4 * (size_t)NDENS * n + 4) * sizeof(double);
     const dim3 grid((unsigned)npad), block(256);
     switch ((n + 127) / 128) {
-#define HF_INCORE_CASE(MM)                                                                                            \
-    case MM:                                                                                                          \
-        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MM, ROWS, T>), grid, block, lds, st, G, n, npad, D, Dw, J, K); \
+#define HF_INCORE_CASE(MM)                                                                                        \
+    case MM:                                                                                                      \
+        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MM, ROWS, PREFETCH, T>), grid, block, lds, st, G, n, npad, D, \
+                           Dw, J, K);                                                                             \
         break;
         HF_INCORE_CASE(1)
         HF_INCORE_CASE(2)
@@ -255,28 +336,33 @@ hipError_t launch_incore(const T* G, int n, int64_t npad, const double* D, const
     return hipGetLastError();
 }
 
-// Row-group height of the contraction (rows of one wavefront whose loads are issued together and
-// whose row sums are reduced together): 2, 4 or 8.
-template <typename T>
-hipError_t launch_incore_rows(int rows, int ndens, const T* G, int n, int64_t np, const double* D, const double* Dw,
-                              double* J, double* K, hipStream_t st) {
-    if (ndens == 1) {
-        if (rows == 2) return launch_incore<1, 2>(G, n, np, D, Dw, J, K, st);
-        if (rows == 8) return launch_incore<1, 8>(G, n, np, D, Dw, J, K, st);
-        return launch_incore<1, 4>(G, n, np, D, Dw, J, K, st);
-    }
-    if (rows == 2) return launch_incore<2, 2>(G, n, np, D, Dw, J, K, st);
-    if (rows == 8) return launch_incore<2, 8>(G, n, np, D, Dw, J, K, st);
-    return launch_incore<2, 4>(G, n, np, D, Dw, J, K, st);
+// Tile shapes of the contraction: rows of one wavefront whose loads are issued together and whose
+// row sums are reduced together, and whether the next group's loads are issued before the current
+// group's arithmetic (more bytes in flight per wavefront for more registers, so fewer wavefronts).
+enum IncoreShape { kRows2 = 0, kRows4 = 1, kRows2Prefetch = 2, kIncoreShapes = 3 };
+
+template <int NDENS, typename T>
+hipError_t launch_incore_nd(int shape, const T* G, int n, int64_t np, const double* D, const double* Dw, double* J,
+                            double* K, hipStream_t st) {
+    if (shape == kRows4) return launch_incore<NDENS, 4, false>(G, n, np, D, Dw, J, K, st);
+    if (shape == kRows2Prefetch) return launch_incore<NDENS, 2, true>(G, n, np, D, Dw, J, K, st);
+    return launch_incore<NDENS, 2, false>(G, n, np, D, Dw, J, K, st);
 }
 
-// Autotuner: the row-group height is picked per (column chunks M2 = ceil(n/128), densities, store
+template <typename T>
+hipError_t launch_incore_shape(int shape, int ndens, const T* G, int n, int64_t np, const double* D,
+                               const double* Dw, double* J, double* K, hipStream_t st) {
+    return ndens == 1 ? launch_incore_nd<1>(shape, G, n, np, D, Dw, J, K, st)
+                      : launch_incore_nd<2>(shape, G, n, np, D, Dw, J, K, st);
+}
+
+// Autotuner: the tile shape is picked per (column chunks M2 = ceil(n/128), densities, store
 // precision) by timing every candidate on the caller's own store, density and stream, once per
 // process (the table is shared by all contexts; a server's slots tune each key once).  Candidates
 // write to scratch J/K, so the build's outputs are untouched.
 struct IncoreTuning {
     std::mutex mu;
-    std::map<int, int> rows;                  // key -> chosen rows
+    std::map<int, int> shape;                 // key -> chosen IncoreShape
 };
 IncoreTuning& tuning() {
     static IncoreTuning t;
@@ -284,27 +370,27 @@ IncoreTuning& tuning() {
 }
 
 template <typename T>
-int tuned_rows(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double* D, const double* Dw,
+int tuned_shape(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double* D, const double* Dw,
                hipStream_t st) {
     const int key = ((n + 127) / 128) * 100 + ndens * 10 + (sizeof(T) == 4 ? 1 : 0);
     {
         std::lock_guard<std::mutex> lock(tuning().mu);
-        auto it = tuning().rows.find(key);
-        if (it != tuning().rows.end()) return it->second;
+        auto it = tuning().shape.find(key);
+        if (it != tuning().shape.end()) return it->second;
     }
     const size_t nn = (size_t)n * n;
     DevBuf<double> sJ, sK;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    int best = 4;
+    int best = kRows2;
     float best_ms = 0;
     bool ok = sJ.alloc(nn) == hipSuccess && sK.alloc(2 * nn) == hipSuccess &&
               hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess;
-    for (int rows : {2, 4, 8}) {
+    for (int shape = 0; shape < kIncoreShapes; ++shape) {
         if (!ok) break;
         float t = 0;
         for (int rep = 0; rep < 3 && ok; ++rep) {          // first run warms the caches and the code
             ok = hipEventRecord(ev[0], st) == hipSuccess &&
-                 launch_incore_rows(rows, ndens, G, n, np, D, Dw, sJ.p, sK.p, st) == hipSuccess &&
+                 launch_incore_shape(shape, ndens, G, n, np, D, Dw, sJ.p, sK.p, st) == hipSuccess &&
                  hipEventRecord(ev[1], st) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess;
             float ms = 0;
             if (ok) ok = hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess;
@@ -312,15 +398,15 @@ int tuned_rows(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double* 
         }
         if (ok && (best_ms == 0 || t < best_ms)) {
             best_ms = t;
-            best = rows;
+            best = shape;
         }
     }
     for (hipEvent_t e : ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipGetLastError();
-    if (!ok) return 4;                                     // tuning failed: the default, not recorded
+    if (!ok) return kRows2;                                // tuning failed: the default, not recorded
     std::lock_guard<std::mutex> lock(tuning().mu);
-    tuning().rows[key] = best;
+    tuning().shape[key] = best;
     return best;
 }
 
@@ -348,9 +434,9 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
         hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((ndens * nn + 255) / 256)), dim3(256), 0, st, d_K,
                            ndens * nn);
     }
-    // row-group height from the autotuner (first build of each kind in the process times the candidates)
-    const int rows = g32 ? tuned_rows(ctx, ndens, g32, n, np, d_D, ctx->d_Dw.p, st)
-                         : tuned_rows(ctx, ndens, G, n, np, d_D, ctx->d_Dw.p, st);
+    // tile shape from the autotuner (first build of each kind in the process times the candidates)
+    const int shape = g32 ? tuned_shape(ctx, ndens, g32, n, np, d_D, ctx->d_Dw.p, st)
+                          : tuned_shape(ctx, ndens, G, n, np, d_D, ctx->d_Dw.p, st);
     // statistics, and with profile = 1 events around the contraction (read at the next flush)
     const int f = g32 ? 1 : 0;
     const bool prof = ctx->opt.profile != 0;
@@ -372,9 +458,9 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
     ks.flops += (double)np * (double)np * (2.0 + 8.0 * ndens);           // J, and row + column parts of K
     hipError_t e;
     if (g32)
-        e = launch_incore_rows(rows, ndens, g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
+        e = launch_incore_shape(shape, ndens, g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
     else
-        e = launch_incore_rows(rows, ndens, G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
+        e = launch_incore_shape(shape, ndens, G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
     if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("in-core J/K: ") + hipGetErrorString(e));
     if (prof) {
         HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[1], st));
