@@ -52,10 +52,11 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
     jobs = jobs or min(16, os.cpu_count() or 4)
     os.makedirs(OBJ, exist_ok=True)
     subprocess.run([sys.executable, os.path.join(CSRC, "gen_eri.py"), GEN, str(LMAX)], check=True)
-    kdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "eri_kernel.hpp"),
+    kdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "wave_reduce.h"), os.path.join(CSRC, "eri_kernel.hpp"),
              os.path.join(CSRC, "coop_kernel.hpp"), os.path.join(GEN, "eri_classes.inc")]
     kdeps += sorted(glob.glob(os.path.join(GEN, "coop_*.inc")))
-    hdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "hf_internal.h"), os.path.join(CSRC, "gershgorin.h"),
+    hdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "wave_reduce.h"), os.path.join(CSRC, "hf_internal.h"),
+             os.path.join(CSRC, "gershgorin.h"),
              os.path.join(GEN, "dispatch.inc"), os.path.join(ROOT, "include", "hfgpu.h")]
     for stale in glob.glob(os.path.join(OBJ, "k_????.hip.o")):
         os.remove(stale)                 # objects of the former one-file-per-class layout

@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "hf_device.h"
+#include "wave_reduce.h"
 #include "generated/eri_classes.inc"
 
 namespace hfgpu {
@@ -615,20 +616,20 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
     // J'_ab: lanes without a slot hold zeros; lane 0 is active whenever any lane is
     const int key0 = __shfl(key, 0);
     if (__all(key == key0 || key < 0)) {
+        // the NAB sums by recursive halving in registers (padded to a power of two); sum v ends in lane
+        // v << SH, and those lanes add their sums to J' in one atomic instruction
+        static_assert(NAB <= 64, "one value per lane at most");
+        constexpr int CNT = NAB <= 1 ? 1 : NAB <= 2 ? 2 : NAB <= 4 ? 4 : NAB <= 8 ? 8 : NAB <= 16 ? 16 : NAB <= 32 ? 32 : 64;
+        constexpr int SH = 6 - log2c<CNT>();
+        double V[CNT];
 #pragma unroll
-        for (int i = 0; i < NAB; ++i) {
-            double x = jab[i];
-#pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
-            jab[i] = x;
-        }
-        if (key0 >= 0 && (threadIdx.x & 63) == 0) {
-#pragma unroll
-            for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-                for (int ib = 0; ib < NB; ++ib)
-                    unsafeAtomicAdd(&sink.J[(size_t)(a0 + ia) * n + b0 + ib], jab[ia * NB + ib]);
-        }
+        for (int i = 0; i < CNT; ++i) V[i] = i < NAB ? jab[i] : 0.0;
+        const int lane = threadIdx.x & 63;
+        Halving<CNT, 32>::run(V, lane);
+        const int v = lane >> SH;
+        const int ra0 = __builtin_amdgcn_readlane(a0, 0), rb0 = __builtin_amdgcn_readlane(b0, 0);
+        if (key0 >= 0 && (lane & ((1 << SH) - 1)) == 0 && v < NAB)
+            unsafeAtomicAdd(&sink.J[(size_t)(ra0 + v / NB) * n + rb0 + v % NB], V[0]);
     } else if (key >= 0) {
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia)

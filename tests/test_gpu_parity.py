@@ -455,11 +455,14 @@ def test_cpp_dropin_runs(tmp_path):
 @pytest.mark.parametrize("prec,incr", [(0, 1), (1, 0)], ids=["fp64_incremental", "mixed"])
 def test_incore_uhf_density_changes_match_direct(gpu_engine_factory, prec, incr):
     """UHF with the in-core store and J/K of density changes (FP64 incremental builds, and the mixed
-    tier's FP32 store on density changes): the direct FP64 energy within 1e-6 Eh."""
+    tier's FP32 store on density changes): the direct FP64 energy within 1e-8 Eh.  This SCF converges
+    slowly (~190 iterations): with criteria of 1e-8 Eh / 1e-6 its stopping point scatters by ~1e-6 Eh from
+    run to run (a plain FP64 in-core SCF and the direct one stopped 1.2e-6 Eh apart), so both SCFs are
+    converged tightly and land on the same stationary point (measured agreement 2e-11 Eh)."""
     atoms = UHF_CASES[1][2]
     _print_case("ch3_631gs_uhf prec=%d incremental=%d" % (prec, incr))
-    kw = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0, energy_tolerance=1e-8, density_tolerance=1e-6,
-              diis_tolerance=1e-6)
+    kw = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0, energy_tolerance=1e-10, density_tolerance=1e-8,
+              diis_tolerance=1e-8, max_iterations=600)
     ref = gpu_engine_factory(schwarz_threshold=1e-12, exact_transfer=1).set_molecule(
         basis("6-31g_st_.1.nw"), atoms).scf(unrestricted=True, **kw)
     eng = gpu_engine_factory(schwarz_threshold=1e-12, exact_transfer=1, precision=prec).set_molecule(
@@ -467,7 +470,7 @@ def test_incore_uhf_density_changes_match_direct(gpu_engine_factory, prec, incr)
     r = eng.scf(unrestricted=True, incore=1, incremental_fock=incr, **kw)
     print("uhf direct %.10f in-core %.10f s2 %.4f %.4f" % (ref.energy, r.energy, ref.s2, r.s2), flush=True)
     assert ref.converged and r.converged and r.incore == 1
-    assert abs(r.energy - ref.energy) <= 1e-6 and abs(r.s2 - ref.s2) <= 1e-4
+    assert abs(r.energy - ref.energy) <= 1e-8 and abs(r.s2 - ref.s2) <= 1e-6
 
 
 def test_mp2_at_scale_matches_oracle_restatement(gpu_engine_factory):
