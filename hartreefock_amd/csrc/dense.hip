@@ -9,6 +9,7 @@
 //   D[l / 16 + 4 r][l % 16].
 #include "gershgorin.h"
 #include "hf_internal.h"
+#include "wave_reduce.h"
 
 namespace {
 
@@ -229,16 +230,11 @@ __global__ void __launch_bounds__(256) sp2_step_kernel(Sp2Args g) {
         }
     }
     if (i0 == j0) {   // diagonal tile: partial traces, reduced over the wavefront in a fixed order
-#pragma unroll
-        for (int msk = 32; msk >= 1; msk >>= 1) {
-            d1 += __shfl_xor(d1, msk);
-            d2 += __shfl_xor(d2, msk);
-        }
-        if (l == 0) {
-            double* pp = g.part + (size_t)2 * nt * s;
-            pp[blockIdx.x] = d1;
-            pp[nt + blockIdx.x] = d2;
-        }
+        double V[2] = {d1, d2};
+        hfgpu::Halving<2, 32>::run(V, l);     // lanes 0..31 now hold the d1 total, lanes 32..63 the d2 total
+        double* pp = g.part + (size_t)2 * nt * s;
+        if (l == 0) pp[blockIdx.x] = V[0];
+        if (l == 32) pp[nt + blockIdx.x] = V[0];
     }
 }
 
@@ -292,13 +288,11 @@ __global__ void __launch_bounds__(1024) sp2_local_kernel(const double* __restric
             t1 += X[i + i * n];
             t2 += X2[i + i * n];
         }
-        for (int msk = 32; msk >= 1; msk >>= 1) {
-            t1 += __shfl_xor(t1, msk);
-            t2 += __shfl_xor(t2, msk);
-        }
-        if (l == 0) {
-            red[0][w] = t1;
-            red[1][w] = t2;
+        {
+            double V[2] = {t1, t2};
+            hfgpu::Halving<2, 32>::run(V, l);  // lanes 0..31: the t1 total, lanes 32..63: the t2 total
+            if (l == 0) red[0][w] = V[0];
+            if (l == 32) red[1][w] = V[0];
         }
         __syncthreads();
         t1 = 0;

@@ -27,6 +27,7 @@
 
 #include "gershgorin.h"
 #include "hf_internal.h"
+#include "wave_reduce.h"
 
 namespace {
 
@@ -169,10 +170,9 @@ __global__ void __launch_bounds__(256) dots_partial_kernel(DotJobs jobs, size_t 
 
 __global__ void __launch_bounds__(64) dots_final_kernel(const double* __restrict__ part, int m, double* __restrict__ out) {
     for (int r = 0; r < m; ++r) {
-        double v = part[r * kDotBlocks + threadIdx.x];
-#pragma unroll
-        for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
-        if (threadIdx.x == 0) out[r] = v;
+        double v[1] = {part[r * kDotBlocks + threadIdx.x]};
+        hfgpu::Halving<1, 32>::run(v, (int)threadIdx.x);
+        if (threadIdx.x == 0) out[r] = v[0];
     }
 }
 
