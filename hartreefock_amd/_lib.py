@@ -10,7 +10,8 @@ _LIB = None
 # every function include/hfgpu.h declares
 EXPORTED_SYMBOLS = [
     "hf_default_options", "hf_create", "hf_destroy", "hf_last_error", "hf_set_basis", "hf_set_molecule",
-    "hf_nbf", "hf_nshell", "hf_electrons", "hf_nuclear_repulsion", "hf_packed_index", "hf_packed_len",
+    "hf_nbf", "hf_nshell", "hf_electrons", "hf_nuclear_repulsion", "hf_set_electrons", "hf_set_electric_field",
+    "hf_nuclear_electric_field_energy", "hf_packed_index", "hf_packed_len",
     "hf_eri_packed", "hf_eri_packed_device", "hf_build_jk", "hf_build_jk_device", "hf_one_electron",
     "hf_pair_matrix_len", "hf_eri_pairs_device", "hf_build_jk_incore_device", "hf_build_jk_incore",
     "hf_properties",
@@ -115,6 +116,10 @@ def lib():
     L.hf_electrons.argtypes = [vp, ip, ip]
     L.hf_nuclear_repulsion.argtypes = [vp]
     L.hf_nuclear_repulsion.restype = ctypes.c_double
+    L.hf_set_electrons.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.hf_set_electric_field.argtypes = [vp, dp]
+    L.hf_nuclear_electric_field_energy.argtypes = [vp]
+    L.hf_nuclear_electric_field_energy.restype = ctypes.c_double
     L.hf_packed_index.argtypes = [ctypes.c_uint32] * 4
     L.hf_packed_index.restype = ctypes.c_uint64
     L.hf_packed_len.argtypes = [ctypes.c_int]
@@ -204,6 +209,41 @@ class Engine:
         xyz = np.ascontiguousarray([c for a in atoms for c in a[1]], dtype=np.float64)
         self._check(lib().hf_set_molecule(self.h, basis_file.encode(), len(atoms), _ip(z), _dp(xyz)))
         return self
+
+    def set_basis(self, shells, atoms):
+        """Install a molecule from shell arrays (hf_set_basis): shells is a list of
+        (L, atom index, [exponents], [coefficient x radial normalisation]) with SP already split, in the
+        basis-function order; atoms is [(Z, (x, y, z)), ...] in bohr."""
+        self._natoms = len(atoms)
+        L = np.ascontiguousarray([s[0] for s in shells], dtype=np.int32)
+        at = np.ascontiguousarray([s[1] for s in shells], dtype=np.int32)
+        npr = np.ascontiguousarray([len(s[2]) for s in shells], dtype=np.int32)
+        poff = np.ascontiguousarray(np.concatenate([[0], np.cumsum(npr)[:-1]]), dtype=np.int32)
+        nc = (L + 1) * (L + 2) // 2
+        bfo = np.ascontiguousarray(np.concatenate([[0], np.cumsum(nc)[:-1]]), dtype=np.int32)
+        al = np.ascontiguousarray([x for s in shells for x in s[2]], dtype=np.float64)
+        co = np.ascontiguousarray([x for s in shells for x in s[3]], dtype=np.float64)
+        z = np.ascontiguousarray([a[0] for a in atoms], dtype=np.int32)
+        axyz = np.ascontiguousarray([c for a in atoms for c in a[1]], dtype=np.float64)
+        sxyz = np.ascontiguousarray([c for s in shells for c in atoms[s[1]][1]], dtype=np.float64)
+        b = BasisSoa(len(shells), int(nc.sum()), _ip(L), _ip(npr), _ip(poff), _ip(bfo), _ip(at), _dp(sxyz), _dp(al),
+                     _dp(co), len(atoms), _ip(z), _dp(axyz))
+        self._check(lib().hf_set_basis(self.h, ctypes.byref(b)))
+        return self
+
+    def set_electrons(self, alpha, beta):
+        """Molecule::alphaElectrons / betaElectrons of the installed molecule (0, 0: neutral)."""
+        self._check(lib().hf_set_electrons(self.h, int(alpha), int(beta)))
+        return self
+
+    def set_electric_field(self, field):
+        """Molecule::ElectricField (au) of the installed molecule."""
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self._check(lib().hf_set_electric_field(self.h, _dp(f)))
+        return self
+
+    def nuclear_electric_field_energy(self):
+        return lib().hf_nuclear_electric_field_energy(self.h)
 
     @property
     def nbf(self):
@@ -333,7 +373,8 @@ class Server:
         rc = lib().hf_serve_submit(self.h, basis_file.encode(), len(atoms), _ip(z), _dp(xyz), 1 if unrestricted else 0,
                                    tag, ctypes.byref(rid))
         if rc != 0:
-            raise HFError(rc, "hf_serve_submit failed")
+            raise HFError(rc, "hf_serve_submit failed" + (" (footprint above the HBM budget: rejected)"
+                                                          if rc == -2 else ""))
         return rid.value
 
     def poll(self, max_results=64, wait_ms=100):

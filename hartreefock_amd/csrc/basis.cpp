@@ -175,12 +175,15 @@ bool molecule_from_soa(const hf_basis_soa* b, Molecule& mol, std::string& err) {
         for (int k = 0; k < 3; ++k) mol.xyz.push_back(b->atom_xyz[3 * a + k]);
     }
     int bf = 0;
+    for (int k = 0; k < 3 * b->natoms; ++k)
+        if (!std::isfinite(b->atom_xyz[k])) { err = "atom coordinate is not finite"; return false; }
     for (int s = 0; s < b->nshell; ++s) {
         Shell sh;
         sh.L = b->L[s];
         sh.atom = b->atom[s];
         sh.bf = b->bf_offset[s];
         if (sh.L < 0 || sh.L > 2) { err = "shell L outside 0..2 is not supported"; return false; }
+        if (sh.atom < 0 || sh.atom >= b->natoms) { err = "shell atom index out of range"; return false; }
         for (int k = 0; k < 3; ++k) sh.xyz[k] = b->xyz[3 * s + k];
         for (int p = 0; p < b->nprim[s]; ++p) {
             sh.alpha.push_back(b->alpha[b->prim_offset[s] + p]);

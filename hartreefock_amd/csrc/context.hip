@@ -255,6 +255,10 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
 int setup_molecule(hf_ctx* ctx) {
     const hfhost::Molecule& mol = ctx->mol;
     const int ns = (int)mol.shells.size();
+    // the device buffers of the previous molecule are reused: its work must be finished
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->eri_stream) HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->eri_stream));
+    for (hipStream_t s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamSynchronize(s2));
     if (mol.maxL > kLmax) return ctx->fail(HF_E_UNSUPPORTED, "basis has L > 2 shells; this build supports s, p, d");
     std::vector<double> sxyz(3 * ns);
     std::vector<int> sbf(ns);
@@ -277,8 +281,16 @@ int setup_molecule(hf_ctx* ctx) {
     }
     // pairs (La >= Lb; for equal L, first shell index >= second)
     std::vector<PrimPair> pp;
-    ctx->pcls.clear();
+    // pair classes keep their device buffers from the previous molecule (they only grow)
     ctx->pcls.resize(kNumPairClasses);
+    for (PairClassHost& pc : ctx->pcls) {
+        pc.sa.clear();
+        pc.sb.clear();
+        pc.prim0.clear();
+        pc.nprim.clear();
+        pc.Q.clear();
+        pc.group.clear();
+    }
     for (int sa = 0; sa < ns; ++sa)
         for (int sb = 0; sb < ns; ++sb) {
             const hfhost::Shell& A = mol.shells[sa];
@@ -321,11 +333,10 @@ int setup_molecule(hf_ctx* ctx) {
     }
     // Schwarz bounds from the diagonal quartets (ab|ab), always with exact integrals
     {
-        DevBuf<double> dQ;
-        TaskSet diag;
         size_t maxp = 1;
         for (auto& pc : ctx->pcls) maxp = std::max<size_t>(maxp, pc.sa.size());
-        HF_HIP_CHECK(ctx, dQ.alloc(maxp));
+        HF_HIP_CHECK(ctx, ctx->d_Qdiag.alloc(maxp));
+        double* dQ = ctx->d_Qdiag.p;
         ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
         for (int c = 0; c < kNumClasses; ++c) {
             const ClassInfo& ci = kClasses[c];
@@ -338,10 +349,10 @@ int setup_molecule(hf_ctx* ctx) {
             tl.ntasks = pc.npairs();
             tl.same_class = 1;
             SinkArgs sink{};
-            sink.Qout = dQ.p;
+            sink.Qout = dQ;
             hipError_t e = ci.fn(MODE_DIAG, 1, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ctx->stream);
             if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("Schwarz kernel: ") + hipGetErrorString(e));
-            HF_HIP_CHECK(ctx, hipMemcpyAsync(pc.Q.data(), dQ.p, pc.Q.size() * sizeof(double), hipMemcpyDeviceToHost,
+            HF_HIP_CHECK(ctx, hipMemcpyAsync(pc.Q.data(), dQ, pc.Q.size() * sizeof(double), hipMemcpyDeviceToHost,
                                              ctx->stream));
             HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         }
@@ -375,14 +386,17 @@ int setup_molecule(hf_ctx* ctx) {
         int rc = upload_pairs(ctx, pc);
         if (rc) return rc;
     }
-    ctx->tasks_all.threshold = -1;
-    ctx->tasks_incore.threshold = -1;
-    ctx->tasks_by_level.clear();
-    ctx->tasks_mixed_hi.clear();
-    ctx->tasks_mixed_lo.clear();
-    ctx->d_oe_L.release();
-    ctx->d_oe_prim.release();
-    ctx->d_oe_pairs.release();
+    // task sets of the previous molecule: marked stale (rebuilt on first use), device buffers kept
+    auto stale = [](TaskSet& ts) {
+        ts.threshold = -1;
+        ts.classes.clear();
+    };
+    stale(ctx->tasks_all);
+    stale(ctx->tasks_incore);
+    for (auto& kv : ctx->tasks_by_level) stale(kv.second);
+    for (auto& kv : ctx->tasks_mixed_hi) stale(kv.second);
+    for (auto& kv : ctx->tasks_mixed_lo) stale(kv.second);
+    ctx->oe_valid = false;
     ctx->have_mol = true;
     return HF_OK;
 }
@@ -701,6 +715,7 @@ int hf_destroy(hf_ctx* ctx) {
         (void)hipStreamDestroy(ctx->eri_stream);
     }
     if (ctx->hpin) (void)hipHostFree(ctx->hpin);
+    hf_scf_workspace_free(ctx);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -712,6 +727,8 @@ const char* hf_last_error(const hf_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 int hf_set_molecule(hf_ctx* ctx, const char* basis_file, int natoms, const int* Z, const double* xyz) {
     if (!ctx || !basis_file || natoms <= 0 || !Z || !xyz) return ctx ? ctx->fail(HF_E_ARG, "bad arguments") : HF_E_ARG;
     ctx->have_mol = false;
+    for (int k = 0; k < 3 * natoms; ++k)
+        if (!std::isfinite(xyz[k])) return ctx->fail(HF_E_ARG, "atom coordinate is not finite");
     std::string err;
     if (!hfhost::load_molecule(basis_file, natoms, Z, xyz, ctx->mol, err)) return ctx->fail(HF_E_IO, err);
     return setup_molecule(ctx);
@@ -731,9 +748,38 @@ int hf_nshell(const hf_ctx* ctx) { return ctx && ctx->have_mol ? (int)ctx->mol.s
 int hf_electrons(const hf_ctx* ctx, int* alpha, int* beta) {
     if (!ctx || !ctx->have_mol) return HF_E_STATE;
     const int ne = ctx->mol.electrons();
-    if (alpha) *alpha = ne / 2;
-    if (beta) *beta = ne - ne / 2;
+    const bool set = ctx->mol.alpha_set > 0 || ctx->mol.beta_set > 0;
+    if (alpha) *alpha = set ? ctx->mol.alpha_set : ne / 2;
+    if (beta) *beta = set ? ctx->mol.beta_set : ne - ne / 2;
     return ne;
+}
+
+int hf_set_electrons(hf_ctx* ctx, int alpha, int beta) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (alpha < 0 || beta < 0) return ctx->fail(HF_E_ARG, "negative electron count");
+    ctx->mol.alpha_set = alpha;
+    ctx->mol.beta_set = beta;
+    return HF_OK;
+}
+
+int hf_set_electric_field(hf_ctx* ctx, const double* field) {
+    if (!ctx || !field) return ctx ? ctx->fail(HF_E_ARG, "null field") : HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    for (int k = 0; k < 3; ++k) {
+        if (!std::isfinite(field[k])) return ctx->fail(HF_E_ARG, "field component is not finite");
+        ctx->mol.field[k] = field[k];
+    }
+    return HF_OK;
+}
+
+double hf_nuclear_electric_field_energy(const hf_ctx* ctx) {
+    // Molecule::NuclearElectricFieldEnergy (HartreeFock/Molecule.cpp:86-94): -sum_A Z_A R_A . E
+    if (!ctx || !ctx->have_mol) return 0.0;
+    double e = 0;
+    for (size_t a = 0; a < ctx->mol.Z.size(); ++a)
+        for (int k = 0; k < 3; ++k) e -= ctx->mol.Z[a] * ctx->mol.xyz[3 * a + k] * ctx->mol.field[k];
+    return e;
 }
 
 double hf_nuclear_repulsion(const hf_ctx* ctx) {

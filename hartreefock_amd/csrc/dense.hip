@@ -7,6 +7,8 @@
 // Matrix-core lane layout of __builtin_amdgcn_mfma_f64_16x16x4f64 (A 16x4, B 4x16, D 16x16):
 //   lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; element r of its 4-vector result is
 //   D[l / 16 + 4 r][l % 16].
+#include <mutex>
+
 #include "gershgorin.h"
 #include "hf_internal.h"
 #include "wave_reduce.h"
@@ -333,14 +335,38 @@ int hf_dgemm_small(hf_ctx* ctx, hipStream_t st, bool ta, bool tb, int m, int n, 
     return HF_OK;
 }
 
+namespace {
+
+// The dynamic-LDS limit of sp2_local_kernel is an attribute of the kernel for the whole process, not of
+// one launch.  It is set once, to the size of the largest matrix the kernel takes (n = kSp2LocalMax), before
+// any launch: setting it on every call to that call's size, from the concurrent contexts of a server's
+// slots, changed it under the other threads' launches of the same kernel.
+bool sp2_local_ready() {
+    static std::once_flag once;
+    static bool ok = false;
+    std::call_once(once, [] {
+        const size_t lds = 2 * (size_t)kSp2LocalMax * kSp2LocalMax * sizeof(double);
+        ok = hipFuncSetAttribute((const void*)sp2_local_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds) == hipSuccess;
+        (void)hipGetLastError();
+    });
+    return ok;
+}
+
+__global__ void set_flag_kernel(double* __restrict__ p, double v) { *p = v; }
+
+}  // namespace
+
 int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp, double* X0, double* X1, double* W0,
                  double* W1, double* hist, double* part, int max_steps, int hint, double*& result, int& steps) {
     const int nt = (n + 15) / 16;
-    const size_t lds = std::max<size_t>(2 * (size_t)n * n, 1024) * sizeof(double);   // >= the bounds' workspace
-    if (n <= kSp2LocalMax &&
-        hipFuncSetAttribute((const void*)sp2_local_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
-            hipSuccess) {
+    if (n <= kSp2LocalMax && sp2_local_ready()) {
+        const size_t lds = std::max<size_t>(2 * (size_t)n * n, 1024) * sizeof(double);   // >= the bounds' workspace
+        // the stop flag starts at "not met", so nothing a launch failed to write is read as a result
+        hipLaunchKernelGGL(set_flag_kernel, dim3(1), dim3(1), 0, st, hist + 2, -1.0);
         hipLaunchKernelGGL(sp2_local_kernel, dim3(1), dim3(1024), lds, st, Fp, n, (double)nocc, max_steps, hist + 2, X0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("single-workgroup SP2 launch: ") + hipGetErrorString(e));
         double fin = -1;
         if (int rc = hf_readback(ctx, st, &fin, hist + 2, 1)) return rc;
         result = fin >= 0 ? X0 : nullptr;

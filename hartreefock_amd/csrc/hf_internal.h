@@ -27,11 +27,17 @@ struct Molecule {
     std::vector<Shell> shells;
     int nbf = 0;
     int maxL = 0;
+    // Molecule::alphaElectrons / betaElectrons (HartreeFock/Molecule.h:16-17): both 0 means the neutral
+    // count, split by Molecule::Init (HartreeFock/Molecule.cpp:127-140)
+    int alpha_set = 0, beta_set = 0;
+    double field[3] = {0, 0, 0};  // Molecule::ElectricField, atomic units
     int electrons() const {
+        if (alpha_set > 0 || beta_set > 0) return alpha_set + beta_set;
         int e = 0;
         for (int z : Z) e += z;
         return e;
     }
+    bool has_field() const { return field[0] != 0 || field[1] != 0 || field[2] != 0; }
 };
 
 // Parse an NWChem basis file and build the molecule's shell list (basis.cpp).
@@ -120,6 +126,8 @@ struct KernelStat {
     double eris = 0, flops = 0, ms = 0;
 };
 
+struct ScfWorkspace;   // the SCF's device matrices and events, kept between hf_scf calls (scf.hip)
+
 struct hf_ctx {
     hf_options opt;
     std::string err;
@@ -157,6 +165,11 @@ struct hf_ctx {
     size_t oe_nprim = 0;
     DevBuf<int2> d_oe_pairs;               // shell pairs by one-electron class (La, Lb), La >= Lb
     std::vector<int> oe_class_off;
+    bool oe_valid = false;                 // the one-electron inputs above describe the current molecule
+    DevBuf<double> d_Qdiag;                // Schwarz diagonals of one pair class (molecule set-up)
+    // Device memory is kept between calls and molecules (buffers only grow): hipFree synchronises the
+    // whole device, which would stall the other contexts of a server at every request.
+    ScfWorkspace* scf_ws = nullptr;
     std::vector<KernelStat> stats;
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_pending;         // stat slots whose last launch's events are not yet read
@@ -199,6 +212,10 @@ int hf_eri_pairs_impl(hf_ctx* ctx, double* d_G, uint64_t len, hipStream_t st);
 // K per density, both complete (no symmetrisation needed); accumulate adds to d_J/d_K.
 int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
                       double* d_K, hipStream_t st, bool accumulate);
+// Frees the SCF workspace (hf_destroy).
+void hf_scf_workspace_free(hf_ctx* ctx);
+// Device bytes the context holds between calls: SCF workspace, in-core stores, J/K scratch.
+double hf_ctx_held_bytes(const hf_ctx* ctx);
 // Frees the context's in-core stores when together they hold more than keep_bytes (a server calls
 // this after each request, so idle slots do not hold large stores outside its admission budget).
 void hf_release_incore(hf_ctx* ctx, size_t keep_bytes);

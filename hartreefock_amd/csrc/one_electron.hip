@@ -249,7 +249,8 @@ __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a, const int
 int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipStream_t st, double* dM) {
     const hfhost::Molecule& mol = ctx->mol;
     const int ns = (int)mol.shells.size();
-    if (!ctx->d_oe_L.p) {
+    const bool rebuild = !ctx->oe_valid;   // inputs of the current molecule not uploaded yet
+    if (rebuild) {
         std::vector<int> L(ns), bf(ns), p0(ns), np(ns);
         std::vector<double> al, co;
         for (int s = 0; s < ns; ++s) {
@@ -297,7 +298,7 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
     // shell pairs by class (La, Lb), La >= Lb, built once per molecule
     constexpr int kOeClasses = 6;
     static const int kOeL[kOeClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
-    if (!ctx->d_oe_pairs.p) {
+    if (rebuild) {
         std::vector<std::vector<int2>> byc(kOeClasses);
         for (int x = 0; x < ns; ++x)
             for (int y = 0; y <= x; ++y) {
@@ -317,6 +318,7 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
         HF_HIP_CHECK(ctx, ctx->d_oe_pairs.alloc(std::max<size_t>(all.size(), 1)));
         if (!all.empty())
             HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_oe_pairs.p, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+        ctx->oe_valid = true;
     }
     const size_t nn = (size_t)mol.nbf * mol.nbf;
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, st, dV, nn);

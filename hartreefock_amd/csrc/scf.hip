@@ -348,7 +348,8 @@ std::vector<double> qr_solve(int m, std::vector<double> A, std::vector<double> b
 // DIIS history of one spin: device slots for F and e, host inner products.
 struct Diis {
     static constexpr int kKeep = 6, kFirst = 5;
-    DevBuf<double> F, E;          // kKeep * nn each
+    double* F = nullptr;          // kKeep * nn each (SCF workspace)
+    double* E = nullptr;
     std::deque<int> order;        // slot indices, oldest first
     double B[kKeep][kKeep] = {};  // B[slot][slot]
     size_t nn = 0;
@@ -495,14 +496,14 @@ int diis_add(hf_ctx* ctx, ScfState& st, Diis& d, const double* Fsrc, const doubl
     hipStream_t s = ctx->stream;
     const int slot = d.next_slot();
     if ((int)d.order.size() == Diis::kKeep) d.order.pop_front();
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(d.F.p + slot * st.nn, Fsrc, st.nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(d.E.p + slot * st.nn, esrc, st.nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(d.F + slot * st.nn, Fsrc, st.nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(d.E + slot * st.nn, esrc, st.nn * sizeof(double), hipMemcpyDeviceToDevice, s));
     d.order.push_back(slot);
     DotJobs jobs{};
     jobs.m = (int)d.order.size();
     for (int r = 0; r < jobs.m; ++r) {
-        jobs.x[r] = d.E.p + slot * st.nn;
-        jobs.y[r] = d.E.p + d.order[r] * st.nn;
+        jobs.x[r] = d.E + slot * st.nn;
+        jobs.y[r] = d.E + d.order[r] * st.nn;
         jobs.diff[r] = 0;
     }
     launch_dots(s, jobs, st.nn, st.redpart.p, st.red.p);
@@ -529,7 +530,7 @@ int diis_estimate(hf_ctx* ctx, ScfState& st, Diis& d, double* Fout, double& est)
     LinComb lc{};
     lc.m = m;
     for (int i = 0; i < m; ++i) {
-        lc.src[i] = d.F.p + d.order[i] * st.nn;
+        lc.src[i] = d.F + d.order[i] * st.nn;
         lc.c[i] = c[i];
     }
     hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(st.nn)), dim3(256), 0, ctx->stream, lc, Fout, st.nn);
@@ -538,6 +539,40 @@ int diis_estimate(hf_ctx* ctx, ScfState& st, Diis& d, double* Fout, double& est)
 }
 
 }  // namespace
+
+// Device matrices and events of hf_scf, kept in the context between calls (buffers only grow), so a
+// context that solves one molecule after another (a serving slot) allocates and frees nothing per SCF.
+struct ScfWorkspace {
+    ScfState st;
+    DevBuf<double> diisF[2], diisE[2];
+    hipEvent_t jk_ev[2] = {nullptr, nullptr};
+    hipEvent_t eri_ev[2] = {nullptr, nullptr};
+    ~ScfWorkspace() {
+        for (hipEvent_t e : jk_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : eri_ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+void hf_scf_workspace_free(hf_ctx* ctx) {
+    delete ctx->scf_ws;
+    ctx->scf_ws = nullptr;
+}
+
+double hf_ctx_held_bytes(const hf_ctx* ctx) {
+    double b = (double)ctx->d_G.n * 8.0 + (double)ctx->d_G32.n * 4.0 + (double)ctx->d_Dw.n * 8.0 +
+               8.0 * (double)(ctx->d_Jp.n + ctx->d_Kp.n + ctx->d_D.n + ctx->d_J.n + ctx->d_K.n);
+    if (const ScfWorkspace* w = ctx->scf_ws) {
+        const ScfState& st = w->st;
+        for (const DevBuf<double>* m : {&st.S, &st.h, &st.X, &st.F0, &st.Fa, &st.Fb, &st.J, &st.K, &st.D, &st.Dn,
+                                        &st.Dprev, &st.dD, &st.C, &st.eps, &st.T1, &st.T2, &st.T3, &st.err, &st.work,
+                                        &st.red, &st.redpart, &st.sp2, &st.sp2W, &st.sp2part, &w->diisF[0],
+                                        &w->diisF[1], &w->diisE[0], &w->diisE[1]})
+            b += 8.0 * (double)m->n;
+    }
+    return b;
+}
 
 extern "C" {
 
@@ -569,7 +604,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     std::memset(r, 0, sizeof(*r));
     hipStream_t s = ctx->stream;
     const bool uhf = p->unrestricted != 0;
-    ScfState st;
+    if (!ctx->scf_ws) ctx->scf_ws = new (std::nothrow) ScfWorkspace();
+    if (!ctx->scf_ws) return ctx->fail(HF_E_OOM, "SCF workspace");
+    ScfWorkspace& ws = *ctx->scf_ws;
+    ScfState& st = ws.st;
     const int n = ctx->mol.nbf;
     st.n = n;
     st.nn = (size_t)n * n;
@@ -611,15 +649,8 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     const bool mixed_tier = ctx->opt.precision == HF_PREC_MIXED;
     bool incore = false, store_pending = false;
     double eri_ms = 0;
-    hipEvent_t eri_ev[2] = {nullptr, nullptr};
-    struct EventGuard {
-        hipEvent_t* e;
-        ~EventGuard() {
-            for (int k = 0; k < 2; ++k)
-                if (e[k]) (void)hipEventDestroy(e[k]);
-        }
-    } eri_guard{eri_ev};
-    if (p->incore && n <= kIncoreMaxN) {
+    hipEvent_t* eri_ev = ws.eri_ev;
+    if (p->incore && n <= kIncoreMaxN && p->incore_max_bytes >= 0) {
         const uint64_t len = hf_pair_matrix_len(n);
         size_t free_b = 0, total_b = 0;
         HF_HIP_CHECK(ctx, hipMemGetInfo(&free_b, &total_b));
@@ -633,8 +664,8 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         if (incore) {
             if (!ctx->eri_stream) HF_HIP_CHECK(ctx, hipStreamCreateWithFlags(&ctx->eri_stream, hipStreamNonBlocking));
             hipStream_t es = ctx->eri_stream;
-            HF_HIP_CHECK(ctx, hipEventCreate(&eri_ev[0]));
-            HF_HIP_CHECK(ctx, hipEventCreate(&eri_ev[1]));
+            for (int k = 0; k < 2; ++k)
+                if (!eri_ev[k]) HF_HIP_CHECK(ctx, hipEventCreate(&eri_ev[k]));
             HF_HIP_CHECK(ctx, hipEventRecord(eri_ev[0], es));
             if (int rc = hf_eri_pairs_impl(ctx, ctx->d_G.p, len, es)) return rc;
             if (mixed_tier)
@@ -644,11 +675,27 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             store_pending = true;
         }
     }
-    if (int rc = hf_one_electron_device(ctx, st.S.p, st.T1.p, st.T2.p, s)) return rc;
+    // h = T + V + E . M with the moment integrals about the origin (HartreeFock/HartreeFockAlgorithm.cpp:38)
+    const bool field = ctx->mol.has_field();
+    if (field) HF_HIP_CHECK(ctx, st.Fb.alloc(3 * nn));
+    if (int rc = hf_one_electron_device(ctx, st.S.p, st.T1.p, st.T2.p, s, field ? st.Fb.p : nullptr)) return rc;
     hipLaunchKernelGGL(add_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.T2.p, st.h.p, nn);
+    if (field) {
+        LinComb lc{};
+        lc.m = 4;
+        lc.src[0] = st.h.p;
+        lc.c[0] = 1.0;
+        for (int k = 0; k < 3; ++k) {
+            lc.src[k + 1] = st.Fb.p + k * nn;
+            lc.c[k + 1] = ctx->mol.field[k];
+        }
+        hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(nn)), dim3(256), 0, s, lc, st.T3.p, nn);
+        HF_HIP_CHECK(ctx, hipMemcpyAsync(st.h.p, st.T3.p, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
     // S^{-1/2}, symmetrised (:48-59)
     if (int rc = inverse_sqrt(ctx, st)) return rc;
     // occupations (HartreeFock/RestrictedHartreeFock.cpp:18-32, UnrestrictedHartreeFock.cpp:20-60)
+    // ElectronsNumber() is alpha + beta when either is set, else the atoms' count (HartreeFock/Molecule.cpp:96-101)
     const int ne = ctx->mol.electrons();
     int occ[2];
     if (!uhf) {
@@ -657,7 +704,11 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         if (occ[0] > n) occ[0] = n;
         occ[1] = 0;
     } else {
-        int a = ne / 2, b = ne - a;   // Molecule::Init split; UHF then takes alpha=ne/2, beta=rest
+        int a = ne / 2, b = ne - a;   // UHF takes alpha = floor(N/2), beta = the rest, unless they are set
+        if (ctx->mol.alpha_set > 0 || ctx->mol.beta_set > 0) {
+            a = ctx->mol.alpha_set;
+            b = ctx->mol.beta_set;
+        }
         if (a + b > 2 * n) a = b = n;
         if (a > n) { b += a - n; a = n; }
         if (b > n) b = n;
@@ -671,8 +722,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     HF_HIP_CHECK(ctx, hipMemsetAsync(st.D.p, 0, 2 * nn * sizeof(double), s));
     Diis diis[2];
     for (int k = 0; k < nsp; ++k) {
-        HF_HIP_CHECK(ctx, diis[k].F.alloc(Diis::kKeep * nn));
-        HF_HIP_CHECK(ctx, diis[k].E.alloc(Diis::kKeep * nn));
+        HF_HIP_CHECK(ctx, ws.diisF[k].alloc(Diis::kKeep * nn));
+        HF_HIP_CHECK(ctx, ws.diisE[k].alloc(Diis::kKeep * nn));
+        diis[k].F = ws.diisF[k].p;
+        diis[k].E = ws.diisE[k].p;
         diis[k].nn = nn;
     }
     const double etolD = p->energy_tolerance > 0 ? p->energy_tolerance : 1e-11;
@@ -694,16 +747,9 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     const int first_full = incore_mixed ? 2 : 1;
     constexpr double kIncoreFp32MaxDelta = 1e-3;   // largest max|D - Dprev| contracted with the FP32 store
     // J/K build time from events on the SCF stream, read at the step's own synchronisation
-    struct EventPair {
-        hipEvent_t e[2] = {nullptr, nullptr};
-        ~EventPair() {
-            for (hipEvent_t x : e)
-                if (x) (void)hipEventDestroy(x);
-        }
-    } jk_events;
-    HF_HIP_CHECK(ctx, hipEventCreate(&jk_events.e[0]));
-    HF_HIP_CHECK(ctx, hipEventCreate(&jk_events.e[1]));
-    hipEvent_t* jk_ev = jk_events.e;
+    hipEvent_t* jk_ev = ws.jk_ev;
+    for (int k = 0; k < 2; ++k)
+        if (!jk_ev[k]) HF_HIP_CHECK(ctx, hipEventCreate(&jk_ev[k]));
     bool jk_pending = false;
     const int rebuild_every = p->full_rebuild_every > 0 ? p->full_rebuild_every : 8;
     // mixed-precision tier: quartets below the context's fp32_threshold in FP32 until fp64_phase
@@ -898,7 +944,9 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     };
     int iter = 0;
     bool converged = false, cancelled = false;
-    auto stop = [&]() { return p->terminate && *p->terminate; };
+    // the caller's cancellation flag (HartreeFockAlgorithm::terminate, an atomic in the reference,
+    // HartreeFock/HartreeFockAlgorithm.h:57), read with an atomic load once per iteration
+    auto stop = [&]() { return p->terminate && __atomic_load_n(p->terminate, __ATOMIC_ACQUIRE) != 0; };
     // Convergence is accepted only on a step whose J/K build was full (not incremental) and, in
     // the mixed-precision tier, all-FP64: the tier switches to FP64 once the criteria are met
     // within a factor 100 (or met outright) and then needs final_fp64_iterations FP64 steps.
@@ -914,10 +962,14 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         }
         return !(mixed && (last_fp32 || fp64_steps < final_fp64));
     };
+    // a non-finite energy or density change (coincident nuclei, a corrupted density) fails the SCF at
+    // once; the reference would iterate to maxIterations on NaN
+    auto finite = [&]() { return std::isfinite(energy) && std::isfinite(rms); };
     for (; iter < p->max_iterations; ++iter) {
         if (int rc = step(iter, rms)) return rc;
         cur = energy;
         trace_line(iter, rms);
+        if (!finite()) return ctx->fail(HF_E_NUMERIC, "SCF energy or density is not finite");
         const double de = std::fabs(prev - cur), etol = useDIIS ? etolD : etolN;
         if (accept(de <= etol && rms < dtol && lastErr < itol,
                    de <= 100 * etol && rms < 100 * dtol && lastErr < 100 * itol)) {
@@ -931,6 +983,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         useDIIS = false;
         for (int i = 0; i < p->normal_iter_after_diis; ++i) {
             if (int rc = step(iter + i, rms)) return rc;
+            if (!finite()) return ctx->fail(HF_E_NUMERIC, "SCF energy or density is not finite");
             cur = energy;
             prev = cur;
             if (stop()) { cancelled = true; break; }
@@ -943,6 +996,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             if (int rc = step(iter, rms)) return rc;
             cur = energy;
             trace_line(iter, rms);
+            if (!finite()) return ctx->fail(HF_E_NUMERIC, "SCF energy or density is not finite");
             const double de = std::fabs(prev - cur);
             if (accept(de <= etolN && rms < dtol, de <= 100 * etolN && rms < 100 * dtol)) {
                 converged = true;

@@ -2,13 +2,15 @@
 //
 // `slots` worker threads each own a context (stream, side streams, device memory).  Requests wait
 // in a FIFO queue; the head request is admitted by the first free worker once its estimated HBM
-// footprint fits in the budget together with the running requests (no overtaking, so a large
-// request is not starved).  Admission happens whenever a worker or memory frees up, so requests
-// that arrive while others run join them, and the class kernels of all running requests overlap
+// footprint fits in the budget together with what the other slots hold (no overtaking, so a large
+// request is not starved); a request larger than the whole budget is rejected at submission.
+// Admission happens whenever a worker or memory frees up, so requests that arrive while others run
+// join them, and the class kernels of all running requests overlap
 // on the device through their contexts' streams.  Results go to a completion queue read by
 // hf_serve_poll.  All times are steady-clock milliseconds since the server was created.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -29,6 +31,7 @@ struct Request {
     int unrestricted = 0;
     int nbf = 0;
     double footprint = 0;
+    double store_bytes = 0;   // the in-core store's share of the footprint (0: none)
     double arrival_ms = 0;
 };
 
@@ -43,13 +46,32 @@ struct hf_server {
     std::deque<hf_serve_result> done;
     std::vector<std::thread> workers;
     std::vector<hf_ctx*> ctxs;
-    double in_use = 0, budget = 0;
+    // Device bytes charged to each slot against the budget: while it runs a request, the larger of that
+    // request's footprint and what the slot already held; while idle, what its context keeps (SCF matrices,
+    // J/K scratch, an in-core store) for the next request.
+    std::vector<double> charged;
+    std::vector<char> busy;
+    double budget = 0, keep_store = 0;
     uint64_t next_id = 1;
     int pending = 0;
     bool stop = false;
 
     double now_ms() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    double charged_total() const {
+        double t = 0;
+        for (double c : charged) t += c;
+        return t;
+    }
+    // The head request fits on slot w: the other slots' charges plus this slot's during the request.
+    bool fits(int w, double fp) const { return charged_total() - charged[w] + std::max(charged[w], fp) <= budget; }
+    // No idle slot can take the head request because idle slots hold memory: those slots release their stores.
+    bool head_blocked() const {
+        if (queue.empty()) return false;
+        for (size_t v = 0; v < charged.size(); ++v)
+            if (!busy[v] && fits((int)v, queue.front().footprint)) return false;
+        return true;
     }
 
     void run(int w) {
@@ -58,13 +80,32 @@ struct hf_server {
             Request rq;
             {
                 std::unique_lock<std::mutex> lock(mu);
+                bool release = false;
                 cv_work.wait(lock, [&] {
-                    return stop || (!queue.empty() && (in_use == 0 || in_use + queue.front().footprint <= budget));
+                    if (stop) return true;
+                    if (!queue.empty() && fits(w, queue.front().footprint)) return true;
+                    release = head_blocked() && (ctx->d_G.n + ctx->d_G32.n > 0 || ctx->scf_ws);
+                    return release;
                 });
                 if (stop) return;
+                if (release) {
+                    // free this idle slot's in-core store (then its SCF matrices) so that the queue head can
+                    // be admitted somewhere
+                    lock.unlock();
+                    if (ctx->d_G.n + ctx->d_G32.n > 0)
+                        hf_release_incore(ctx, 0);
+                    else
+                        hf_scf_workspace_free(ctx);
+                    const double held = hf_ctx_held_bytes(ctx);
+                    lock.lock();
+                    charged[w] = held;
+                    cv_work.notify_all();
+                    continue;
+                }
                 rq = std::move(queue.front());
                 queue.pop_front();
-                in_use += rq.footprint;
+                charged[w] = std::max(charged[w], rq.footprint);
+                busy[w] = 1;
             }
             hf_serve_result r{};
             r.id = rq.id;
@@ -78,21 +119,25 @@ struct hf_server {
                 hf_scf_params p = opt.scf;
                 p.unrestricted = rq.unrestricted;
                 p.terminate = nullptr;
+                // the in-core store only within the allowance the admission counted (never from the
+                // device's free memory, which the other slots are using at the same time)
+                if (p.incore) p.incore_max_bytes = rq.store_bytes > 0 ? rq.store_bytes : -1.0;
                 hf_scf_result sr{};
                 rc = hf_scf(ctx, &p, &sr, nullptr);
-                // the in-core store is counted in the request's footprint only while it runs; a slot
-                // keeps stores up to 16 GiB for its next request (freeing synchronises the whole device,
-                // which stalls every other slot), and frees larger ones
-                hf_release_incore(ctx, (size_t)16 << 30);
+                // the slot keeps its store for its next request up to keep_store bytes (freeing synchronises
+                // the whole device, which stalls every other slot); what it keeps stays charged
+                hf_release_incore(ctx, (size_t)keep_store);
                 r.energy = sr.energy;
                 r.iterations = sr.iterations;
                 r.converged = sr.converged;
             }
             r.status = rc;
+            const double held = hf_ctx_held_bytes(ctx);
             r.done_ms = now_ms();
             {
                 std::lock_guard<std::mutex> lock(mu);
-                in_use -= rq.footprint;
+                charged[w] = held;
+                busy[w] = 0;
                 done.push_back(r);
             }
             cv_done.notify_all();
@@ -142,6 +187,9 @@ int hf_serve_create(const hf_serve_options* o, hf_server** out) {
         }
         s->ctxs.push_back(c);
     }
+    s->charged.assign(o->slots, 0.0);
+    s->busy.assign(o->slots, 0);
+    s->keep_store = std::min(16.0 * (1 << 30), s->budget / o->slots);
     for (int w = 0; w < o->slots; ++w) s->workers.emplace_back([s, w] { s->run(w); });
     *out = s;
     return HF_OK;
@@ -161,8 +209,12 @@ int hf_serve_submit(hf_server* s, const char* basis_file, int natoms, const int*
     if (!hfhost::load_molecule(basis_file, natoms, Z, xyz, m, err)) return HF_E_IO;
     rq.nbf = m.nbf;
     rq.footprint = hf_serve_footprint(m.nbf, rq.unrestricted);
-    if (s->opt.scf.incore && m.nbf <= 512)   // the in-core ERI store (and its FP32 copy in the mixed tier)
-        rq.footprint += (double)hf_pair_matrix_len(m.nbf) * (s->opt.ctx.precision == HF_PREC_MIXED ? 12.0 : 8.0);
+    if (s->opt.scf.incore && m.nbf <= 512) {   // the in-core ERI store (and its FP32 copy in the mixed tier)
+        rq.store_bytes = (double)hf_pair_matrix_len(m.nbf) * (s->opt.ctx.precision == HF_PREC_MIXED ? 12.0 : 8.0);
+        if (s->opt.scf.incore_max_bytes > 0 && rq.store_bytes > s->opt.scf.incore_max_bytes) rq.store_bytes = 0;
+        rq.footprint += rq.store_bytes;
+    }
+    if (rq.footprint > s->budget) return HF_E_OOM;   // could never be admitted: rejected, never queued
     {
         std::lock_guard<std::mutex> lock(s->mu);
         rq.id = s->next_id++;

@@ -32,6 +32,7 @@ extern "C" {
 #define HF_E_IO (-6)
 #define HF_E_STATE (-7)
 #define HF_E_UNSUPPORTED (-8)
+#define HF_E_NUMERIC (-9)      /* a non-finite energy or density: the request failed, nothing else is touched */
 
 #define HF_PREC_FP64 0
 #define HF_PREC_MIXED 1
@@ -87,9 +88,23 @@ int hf_set_basis(hf_ctx* ctx, const hf_basis_soa* basis);
 int hf_set_molecule(hf_ctx* ctx, const char* basis_file, int natoms, const int* Z, const double* xyz);
 int hf_nbf(const hf_ctx* ctx);
 int hf_nshell(const hf_ctx* ctx);
-/* Electrons as Molecule::Init (HartreeFock/Molecule.cpp:127-140): alpha = N/2, beta = N - alpha. */
+/* Electrons as Molecule::Init (HartreeFock/Molecule.cpp:127-140): alpha = N/2, beta = N - alpha, or the
+ * counts given to hf_set_electrons; returns N. */
 int hf_electrons(const hf_ctx* ctx, int* alpha, int* beta);
 double hf_nuclear_repulsion(const hf_ctx* ctx);   /* Molecule::NuclearRepulsionEnergy */
+/* Molecule::alphaElectrons / betaElectrons (HartreeFock/Molecule.h:16-17) of the installed molecule, as
+ * RestrictedHartreeFock::Init (N = alpha + beta, N/2 doubly occupied; HartreeFock/RestrictedHartreeFock.cpp:23-31)
+ * and UnrestrictedHartreeFock::Init (alpha and beta occupied; HartreeFock/UnrestrictedHartreeFock.cpp:29-58) read
+ * them: ions and high-spin states.  Both 0 (the default): the neutral count, split as Molecule::Init.  Reset by
+ * hf_set_molecule / hf_set_basis. */
+int hf_set_electrons(hf_ctx* ctx, int alpha, int beta);
+/* Molecule::ElectricField (HartreeFock/Molecule.h:19), atomic units: the SCF's core Hamiltonian becomes
+ * h = T + V + E . <mu|r|nu> with the moment integrals about the origin (HartreeFock/HartreeFockAlgorithm.cpp:38).
+ * Reset by hf_set_molecule / hf_set_basis. */
+int hf_set_electric_field(hf_ctx* ctx, const double* field3);
+/* Molecule::NuclearElectricFieldEnergy (HartreeFock/Molecule.cpp:86-94), -sum_A Z_A R_A . E; as in the
+ * reference it is not part of the SCF's total energy (HartreeFock/RestrictedHartreeFock.cpp:248). */
+double hf_nuclear_electric_field_energy(const hf_ctx* ctx);
 
 /* Packed unique ERIs at index GetElectronElectronIndex(i,j,k,l)
  * (HartreeFock/IntegralsRepository.h:151-162).  Replaces
@@ -149,7 +164,9 @@ typedef struct hf_scf_params {
     double energy_tolerance;       /* 0: the reference's (1e-11 with DIIS, 1e-13 without) */
     double density_tolerance;      /* 0: the reference's 1e-10 (Frobenius norm of the change) */
     double diis_tolerance;         /* 0: the reference's 1e-10 */
-    volatile int* terminate;       /* optional cancellation flag, polled once per iteration */
+    volatile int* terminate;       /* optional cancellation flag (another thread sets it non-zero), read
+                                      with an atomic load once per iteration; the SCF then returns
+                                      HF_E_CANCELLED with converged = 0 */
     int density_method;            /* 0: spectral-projection purification (SP2), falling back to
                                       diagonalisation for a step it cannot resolve; 1: diagonalise
                                       every step (the reference's route).  Same fixed point. */
@@ -161,7 +178,8 @@ typedef struct hf_scf_params {
                                       end before convergence is accepted (default 3) */
     int incore;                    /* 1: evaluate the ERIs once per SCF into HBM (hf_eri_pairs_device)
                                       and build J/K from the store when it fits; 0 (default): direct */
-    double incore_max_bytes;       /* largest store the SCF allocates (0: half the free device memory) */
+    double incore_max_bytes;       /* largest store the SCF allocates (0: half the free device memory;
+                                      negative: none, the SCF stays direct) */
     int mp2;                       /* 1: after the SCF, the MP2 correlation energy as the reference's
                                       CalculateMp2Energy (HartreeFock/RestrictedHartreeFock.cpp:251-288,
                                       HartreeFock/UnrestrictedHartreeFock.cpp:295-375) from the in-core
@@ -247,7 +265,9 @@ typedef struct hf_serve_result {
 
 void hf_serve_default_options(hf_serve_options* o);
 int hf_serve_create(const hf_serve_options* o, hf_server** out);
-/* Queue one molecule (atoms in bohr); never blocks on the GPU. */
+/* Queue one molecule (atoms in bohr); never blocks on the GPU.  A request whose footprint
+ * (hf_serve_footprint, plus its in-core store) exceeds the server's hbm_budget is rejected here with
+ * HF_E_OOM: it could never be admitted. */
 int hf_serve_submit(hf_server* s, const char* basis_file, int natoms, const int* Z, const double* xyz,
                     int unrestricted, uint64_t tag, uint64_t* id);
 /* Up to max finished results, waiting up to wait_ms for the first one; returns the count. */
@@ -255,7 +275,11 @@ int hf_serve_poll(hf_server* s, hf_serve_result* out, int max, int wait_ms);
 /* Requests submitted and not yet returned by hf_serve_poll. */
 int hf_serve_pending(hf_server* s);
 /* Estimated device bytes of an SCF on n basis functions (the admission rule); a server whose SCF
- * parameters ask for the in-core store adds hf_pair_matrix_len(n) doubles (x 1.5 in the mixed tier). */
+ * parameters ask for the in-core store adds hf_pair_matrix_len(n) doubles (x 1.5 in the mixed tier), and
+ * the request's SCF takes the store only within that allowance.  A slot keeps its device memory (SCF
+ * matrices, a store up to min(16 GiB, hbm_budget / slots)) for its next request; those bytes stay
+ * charged to the budget while the slot is idle, and idle slots release their stores when the request at
+ * the head of the queue would otherwise not fit. */
 double hf_serve_footprint(int nbf, int unrestricted);
 /* Waits for the running requests, drops the queued ones, frees everything. */
 int hf_serve_destroy(hf_server* s);

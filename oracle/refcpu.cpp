@@ -246,6 +246,7 @@ struct ref_mol {
     std::vector<Atom> atoms;
     std::vector<const CGO*> bfs;
     unsigned alphaE = 0, betaE = 0;
+    double field[3] = {0, 0, 0};                     /* Molecule::ElectricField, HartreeFock/Molecule.h:19 */
     unsigned maxL = 0;
     std::map<double, std::vector<double>> boys;     /* HartreeFock/IntegralsRepository.h:168 */
     typedef std::tuple<unsigned, unsigned, unsigned, unsigned, unsigned, unsigned, unsigned, unsigned,
@@ -775,6 +776,20 @@ int ref_mol_electrons(const ref_mol* m, int* a, int* b) {
     return (int)(m->alphaE + m->betaE);
 }
 
+/* Molecule::alphaElectrons / betaElectrons set by the caller before the SCF (HartreeFock/Molecule.h:16-17;
+   RHF reads alpha + beta, HartreeFock/Molecule.cpp:96-101, UHF each, HartreeFock/UnrestrictedHartreeFock.cpp:33-37) */
+int ref_mol_set_electrons(ref_mol* m, int a, int b) {
+    if (a < 0 || b < 0) return -1;
+    m->alphaE = (unsigned)a;
+    m->betaE = (unsigned)b;
+    return 0;
+}
+
+int ref_mol_set_field(ref_mol* m, const double* E) {
+    for (int k = 0; k < 3; ++k) m->field[k] = E[k];
+    return 0;
+}
+
 /* HartreeFock/Molecule.cpp:75-84 */
 double ref_mol_nuclear_repulsion(const ref_mol* m) {
     double e = 0;
@@ -986,6 +1001,13 @@ int ref_scf(ref_mol* m, const ref_scf_params* p, ref_scf_result* r, double* D_ou
     std::vector<double> S(nn), T(nn), Vn(nn), h(nn);
     ref_one_electron(m, S.data(), T.data(), Vn.data());
     for (size_t i = 0; i < nn; ++i) h[i] = T[i] + Vn[i];
+    if (m->field[0] != 0 || m->field[1] != 0 || m->field[2] != 0) {
+        /* h = T + V + Ex Mx + Ey My + Ez Mz (HartreeFock/HartreeFockAlgorithm.cpp:38) */
+        std::vector<double> M(3 * nn);
+        ref_moment(m, M.data(), M.data() + nn, M.data() + 2 * nn);
+        for (size_t i = 0; i < nn; ++i)
+            h[i] += m->field[0] * M[i] + m->field[1] * M[nn + i] + m->field[2] * M[2 * nn + i];
+    }
     const double Enuc = ref_mol_nuclear_repulsion(m);
     std::vector<double> eri(ref_packed_len(n));
     ref_eri_packed(m, eri.data(), eri.size());

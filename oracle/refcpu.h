@@ -31,6 +31,10 @@ int ref_mol_nbf(const ref_mol* m);
 int ref_mol_max_l(const ref_mol* m);
 int ref_mol_electrons(const ref_mol* m, int* alpha, int* beta);
 double ref_mol_nuclear_repulsion(const ref_mol* m);
+/* Molecule::alphaElectrons / betaElectrons (HartreeFock/Molecule.h:16-17): ions, high-spin states. */
+int ref_mol_set_electrons(ref_mol* m, int alpha, int beta);
+/* Molecule::ElectricField (HartreeFock/Molecule.h:19): h = T + V + E . M in the SCF. */
+int ref_mol_set_field(ref_mol* m, const double* E);
 /* Per basis function: L, (l,m,n), atom index, shell index (reference IDs). */
 int ref_mol_bf_info(const ref_mol* m, int* L, int* lmn, int* atom, int* shell);
 

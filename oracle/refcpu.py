@@ -42,6 +42,8 @@ def lib():
         L.ref_mol_electrons.argtypes = [ctypes.c_void_p, ip, ip]
         L.ref_mol_nuclear_repulsion.argtypes = [ctypes.c_void_p]
         L.ref_mol_nuclear_repulsion.restype = ctypes.c_double
+        L.ref_mol_set_electrons.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.ref_mol_set_field.argtypes = [ctypes.c_void_p, dp]
         L.ref_mol_bf_info.argtypes = [ctypes.c_void_p, ip, ip, ip, ip]
         L.ref_set_exact_transfer.argtypes = [ctypes.c_int]
         L.ref_boys.argtypes = [ctypes.c_int, ctypes.c_double, dp]
@@ -101,6 +103,18 @@ class Molecule:
 
     def nuclear_repulsion(self):
         return lib().ref_mol_nuclear_repulsion(self.h)
+
+    def set_electrons(self, alpha, beta):
+        """Molecule::alphaElectrons / betaElectrons (ions, high-spin states)."""
+        if lib().ref_mol_set_electrons(self.h, int(alpha), int(beta)) != 0:
+            raise ValueError("bad electron counts")
+        return self
+
+    def set_field(self, E):
+        """Molecule::ElectricField (au): the SCF's h gains E . M."""
+        e = np.ascontiguousarray(E, dtype=np.float64)
+        lib().ref_mol_set_field(self.h, _dp(e))
+        return self
 
     def bf_info(self):
         n = self.nbf

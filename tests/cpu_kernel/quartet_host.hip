@@ -29,3 +29,23 @@ extern "C" void hfq_boys_table(double* out) {
 }
 
 extern "C" int hfq_boys_table_size() { return hfgpu::BOYS_NROW * hfgpu::BOYS_STRIDE; }
+
+// The kernels' Boys evaluation, boys_scaled<L> (FP64) and boys_scaled_f32<L> (FP32), on the host with the
+// device table: F_0..F_L(T) with pref = 1.  L <= 8 (the largest total angular momentum of an L <= 2 quartet).
+extern "C" int hfq_boys(int L, double T, const double* tab, double* F) {
+    switch (L) {
+#define HFQ_B(LL) case LL: hfgpu::boys_scaled<LL>(T, 1.0, tab, F); return 0;
+        HFQ_B(0) HFQ_B(1) HFQ_B(2) HFQ_B(3) HFQ_B(4) HFQ_B(5) HFQ_B(6) HFQ_B(7) HFQ_B(8)
+#undef HFQ_B
+    }
+    return -1;
+}
+
+extern "C" int hfq_boys_f32(int L, float T, const float* tab32, float* F) {
+    switch (L) {
+#define HFQ_B(LL) case LL: hfgpu::boys_scaled_f32<LL>(T, 1.0f, tab32, F); return 0;
+        HFQ_B(0) HFQ_B(1) HFQ_B(2) HFQ_B(3) HFQ_B(4) HFQ_B(5) HFQ_B(6) HFQ_B(7) HFQ_B(8)
+#undef HFQ_B
+    }
+    return -1;
+}

@@ -65,7 +65,9 @@ def write(path, header, blocks):
 
 def main():
     src = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/HartreeFock"
-    here = os.path.dirname(os.path.abspath(__file__))
+    # the package's data directory (this script is the recipe for those files and is not shipped)
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                        "hartreefock_amd", "basis")
     for name in FILES:
         header, blocks, _ = read_blocks(os.path.join(src, name))
         write(os.path.join(here, name), header, blocks)

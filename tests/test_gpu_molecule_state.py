@@ -1,0 +1,200 @@
+"""GPU tests of the molecule state the reference's Molecule carries beyond atoms and basis: electron counts
+(ions, high-spin states), the external electric field, and molecules given as shell arrays (hf_set_basis),
+plus the SCF's failure paths (cancellation, iteration cap, non-finite input).  Every energy is checked against
+the CPU oracle or a closed relation."""
+import ctypes
+import threading
+
+import numpy as np
+import pytest
+
+from conftest import basis
+from hartreefock_amd import geometry as g
+from oracle import refcpu
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-8
+KW = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+
+
+def _print_case(name):
+    print("case", name, flush=True)
+
+
+def _shells_from_nwchem(path, atoms):
+    """Shell arrays for hf_set_basis from an NWChem file: SP split into S then P, coefficients times the radial
+    normalisation (2a/pi)^{3/4} (4a)^{L/2} (HartreeFock/GaussianOrbital.cpp:25-30 without the per-component
+    factor, which the library applies).  A plain reader for the files shipped with the package."""
+    blocks, cur = {}, None
+    for line in open(path):
+        t = line.split()
+        if not t or t[0].startswith("#") or t[0].startswith("BASIS") or t[0].startswith("END"):
+            continue
+        if len(t) == 2 and t[0].isalpha() and t[1].isalpha():
+            z = ["H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne"].index(t[0]) + 1
+            cur = {"letters": t[1], "exps": [], "coefs": []}
+            blocks.setdefault(z, []).append(cur)
+            continue
+        v = [float(x.replace("D", "E")) for x in t]
+        cur["exps"].append(v[0])
+        cur["coefs"].append(v[1:])
+    shells = []
+    for a, (z, _) in enumerate(atoms):
+        for sh in blocks[z]:
+            for li, letter in enumerate(sh["letters"]):
+                L = "SPD".index(letter.upper())
+                ex = sh["exps"]
+                co = [c[li] * (2 * e / np.pi) ** 0.75 * (4 * e) ** (L / 2.0) for e, c in zip(ex, sh["coefs"])]
+                shells.append((L, a, list(ex), co))
+    return shells
+
+
+@pytest.mark.parametrize("bfile,atoms", [("sto3g.txt", g.water_cfg1()), ("6-31g_st_.1.nw", g.alkane_zigzag(2, seed=5))],
+                         ids=["h2o_sto3g", "ethane_631gs"])
+def test_set_basis_matches_set_molecule(gpu_engine_factory, bfile, atoms):
+    """A molecule installed from shell arrays (hf_set_basis) gives the same ERIs and SCF energy as the same
+    molecule read from its basis file (hf_set_molecule)."""
+    _print_case(bfile)
+    a = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis(bfile), atoms)
+    b = gpu_engine_factory(schwarz_threshold=0.0).set_basis(_shells_from_nwchem(basis(bfile), atoms), atoms)
+    assert a.nbf == b.nbf
+    n = a.nbf
+    m = refcpu.packed_index(n - 1, n - 1, n - 1, n - 1) + 1
+    assert np.abs(a.eri_packed()[:m] - b.eri_packed()[:m]).max() <= 1e-12     # coefficients rounded apart
+    ea, eb = a.scf(**KW).energy, b.scf(**KW).energy
+    assert abs(ea - eb) <= 1e-10, (ea, eb)
+
+
+ION_CASES = [
+    # (name, basis, atoms, alpha, beta, unrestricted)
+    ("oh_anion_rhf", "sto3g.txt", [(8, (0.0, 0.0, 0.0)), (1, (0.0, 0.0, 1.83))], 5, 5, False),
+    ("h2o_cation_uhf", "sto3g.txt", g.water_cfg1(), 5, 4, True),
+    ("ch2_triplet_uhf_sto3g", "sto3g.txt", [(6, (0.0, 0.0, 0.0)), (1, (0.0, 1.64, 1.09)), (1, (0.0, -1.64, 1.09))],
+     4, 2, True),
+    ("ch2_triplet_uhf_631gs", "6-31g_st_.1.nw", [(6, (0.0, 0.0, 0.0)), (1, (0.0, 1.64, 1.09)), (1, (0.0, -1.64, 1.09))],
+     4, 2, True),
+]
+
+
+@pytest.mark.parametrize("name,bfile,atoms,na,nb,uhf", ION_CASES, ids=[c[0] for c in ION_CASES])
+def test_electron_counts_match_oracle(gpu_engine_factory, name, bfile, atoms, na, nb, uhf):
+    """Molecule::alphaElectrons / betaElectrons set by the caller (HartreeFock/Molecule.h:16-17): an anion in
+    RHF and a cation and triplet CH2 (two bases) in UHF, against the oracle; <S^2> near S(S+1).  (Linear O2 is
+    left out: its degenerate pi orbitals at the iteration-0 Fermi level make the UHF solution depend on which
+    vectors the eigensolver returns for a degenerate pair, in the reference as anywhere.)"""
+    _print_case(name)
+    eng = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis(bfile), atoms).set_electrons(na, nb)
+    assert eng.electrons() == (na, nb)
+    # the density by diagonalisation every step, the reference's route: open shells with near-degenerate
+    # frontier orbitals have several UHF stationary points, and the two routes can reach different ones
+    r = eng.scf(unrestricted=uhf, density_method=1, **KW)
+    mol = refcpu.Molecule(basis(bfile), atoms).set_electrons(na, nb)
+    ref, _ = mol.scf(unrestricted=uhf, **KW)
+    print("%s gpu %.12f oracle %.12f  s2 %.6f / %.6f" % (name, r.energy, ref.energy, r.s2, ref.s2), flush=True)
+    assert r.converged and ref.converged
+    assert abs(r.energy - ref.energy) <= E_TOL
+    if uhf:
+        s = 0.5 * (na - nb)
+        assert abs(r.s2 - ref.s2) <= 1e-6
+        assert s * (s + 1) - 1e-9 <= r.s2 < s * (s + 1) + 0.1
+
+
+def test_electric_field_matches_oracle(gpu_engine_factory):
+    """h = T + V + E . M (HartreeFock/HartreeFockAlgorithm.cpp:38) on H2O/STO-3G in a field along each axis and
+    an oblique one: the GPU energy equals the oracle's."""
+    atoms = g.water_crawford()
+    for E in ([0.01, 0, 0], [0, -0.02, 0], [0, 0, 0.015], [0.005, 0.007, -0.01]):
+        eng = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis("sto3g.txt"), atoms).set_electric_field(E)
+        r = eng.scf(**KW)
+        ref, _ = refcpu.Molecule(basis("sto3g.txt"), atoms).set_field(E).scf(**KW)
+        e_nf = -sum(z * np.dot(x, E) for z, x in atoms)
+        print("field %s gpu %.12f oracle %.12f" % (E, r.energy, ref.energy), flush=True)
+        assert r.converged and abs(r.energy - ref.energy) <= E_TOL
+        assert abs(eng.nuclear_electric_field_energy() - e_nf) <= 1e-13
+
+
+def test_field_finite_difference_dipole(gpu_engine_factory):
+    """The reference's own check (HartreeFock/Test.cpp:626-733): the SCF dipole of H2O/STO-3G (Crawford
+    geometry; no DIIS, alpha 0.5, initGuess 0.7) equals the nuclear moment minus the central difference of the
+    energy in a field of +-0.001 au along each axis, and is 0.60352 au."""
+    atoms = g.water_crawford()
+    kw = dict(use_diis=0, alpha=0.5, init_guess=0.7, normal_iter_after_diis=0)
+    eng = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis("sto3g.txt"), atoms)
+    r, D = eng.scf(want_density=True, **kw)
+    _, mu = eng.properties(D[0])
+    d = 0.001
+    fd = np.array([z * np.asarray(x, float) for z, x in atoms]).sum(axis=0)
+    for k in range(3):
+        E = np.zeros(3)
+        E[k] = -d
+        e1 = eng.set_molecule(basis("sto3g.txt"), atoms).set_electric_field(E).scf(**kw).energy
+        E[k] = d
+        e2 = eng.set_molecule(basis("sto3g.txt"), atoms).set_electric_field(E).scf(**kw).energy
+        fd[k] += (e1 - e2) / (2 * d)
+    print("analytic %s |%.8f|  finite difference %s |%.8f|" % (mu, np.linalg.norm(mu), fd, np.linalg.norm(fd)),
+          flush=True)
+    assert np.abs(mu - fd).max() <= 2e-6
+    assert abs(np.linalg.norm(fd) - 0.60352) <= 1e-5
+
+
+# ---- failure paths (SURVEY: cancellation, iteration cap, bad input) ----
+
+def test_terminate_before_start_cancels(gpu_engine_factory):
+    """A terminate flag already set: the SCF stops after its first iteration with HF_E_CANCELLED and
+    converged = 0 (HartreeFock/HartreeFockAlgorithm.cpp:137-142 polls the flag once per iteration)."""
+    import hartreefock_amd as hf
+    eng = gpu_engine_factory().set_molecule(basis("6-31g_st_.1.nw"), g.alkane_zigzag(2, seed=5))
+    flag = ctypes.c_int(1)
+    r = eng.scf(check=False, terminate=ctypes.pointer(flag), **KW)
+    assert r.converged == 0 and r.iterations <= 1
+    with pytest.raises(hf.HFError) as ei:
+        eng.scf(terminate=ctypes.pointer(flag), **KW)
+    assert ei.value.code == -5
+
+
+def test_terminate_from_another_thread(gpu_engine_factory):
+    """The flag set by another thread while the SCF runs (no DIIS and a 1e-13 Eh criterion keep it running for
+    hundreds of iterations): the SCF returns cancelled, well before its iteration cap."""
+    eng = gpu_engine_factory().set_molecule(basis("6-31g_st_.1.nw"), g.alkane_zigzag(3, seed=5))
+    flag = ctypes.c_int(0)
+    timer = threading.Timer(0.3, lambda: setattr(flag, "value", 1))
+    timer.start()
+    try:
+        r = eng.scf(check=False, terminate=ctypes.pointer(flag), use_diis=0, alpha=0.05, init_guess=0.0,
+                    normal_iter_after_diis=0, max_iterations=100000)
+    finally:
+        timer.cancel()
+    print("cancelled after %d iterations" % r.iterations, flush=True)
+    assert flag.value == 1 and r.converged == 0 and r.iterations < 100000
+
+
+def test_iteration_cap_reports_not_converged(gpu_engine_factory):
+    import hartreefock_amd as hf
+    eng = gpu_engine_factory().set_molecule(basis("6-31g_st_.1.nw"), g.alkane_zigzag(2, seed=5))
+    r = eng.scf(check=False, max_iterations=2, **KW)
+    assert r.converged == 0 and r.iterations == 2
+    with pytest.raises(hf.HFError) as ei:
+        eng.scf(max_iterations=2, **KW)
+    assert ei.value.code == -4
+
+
+def test_non_finite_input_is_rejected(gpu_engine_factory):
+    """A NaN coordinate is refused at hf_set_molecule; coincident nuclei (an infinite nuclear repulsion) make
+    the SCF fail with HF_E_NUMERIC at its first iteration instead of iterating on NaN; the same context then
+    solves a good molecule normally."""
+    import hartreefock_amd as hf
+    eng = gpu_engine_factory()
+    with pytest.raises(hf.HFError) as ei:
+        eng.set_molecule(basis("sto3g.txt"), [(8, (0.0, 0.0, float("nan"))), (1, (0.0, 0.0, 1.8))])
+    assert ei.value.code == -1
+    eng.set_molecule(basis("sto3g.txt"), [(8, (0.0, 0.0, 0.0)), (1, (0.0, 0.0, 0.0)), (1, (0.0, 1.4, 1.0))])
+    r = eng.scf(check=False, **KW)
+    assert r.converged == 0
+    with pytest.raises(hf.HFError) as ei:
+        eng.scf(**KW)
+    assert ei.value.code == -9
+    eng.set_molecule(basis("sto3g.txt"), g.water_cfg1())
+    r = eng.scf(**KW)
+    ref, _ = refcpu.Molecule(basis("sto3g.txt"), g.water_cfg1()).scf(**KW)
+    assert abs(r.energy - ref.energy) <= E_TOL

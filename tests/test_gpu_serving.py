@@ -1,0 +1,140 @@
+"""Serving (continuous batching) at the serving workload's shape: a Poisson slice of the request mix (water
+clusters 1-12 and alkanes C1-C16 in 6-31G), each result equal to the same molecule solved alone, no request
+lost; admission limits (a request above the HBM budget is rejected, retained stores are released when the
+queue head needs the memory) and a failing request that leaves the others untouched."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import basis
+from hartreefock_amd import geometry as g
+
+pytestmark = pytest.mark.gpu
+
+SCF = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0, energy_tolerance=1e-8, density_tolerance=1e-6,
+           diis_tolerance=1e-6, max_iterations=300)
+
+
+def _drain(srv, want, timeout_s=100.0):
+    got = {}
+    t0 = time.time()
+    while len(got) < want:
+        for r in srv.poll(64, 500):
+            got[r.id] = r
+        assert time.time() - t0 < timeout_s, "results missing: %d of %d" % (len(got), want)
+    return got
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_poisson_slice_results_match_alone(gpu_engine_factory, prec):
+    """200 requests of the serving mix arriving as a Poisson process (40/s, seeded) to 8 slots with in-core
+    stores: every request comes back, and each converged energy equals the molecule's SCF run alone on one
+    context (1e-9 Eh FP64; the mixed tier's request against its own solo run in the same tier, 1e-9 Eh, and
+    against the FP64 solo energy, 1e-6 Eh)."""
+    import hartreefock_amd as hf
+    bfile = basis("6-31g.1.nw")
+    mols = g.serving_mix(200, seed=77)
+    opts = dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1)
+    srv = hf.Server(slots=8, options=opts, scf=dict(SCF, incore=1))
+    rng = np.random.default_rng(5)
+    ids = {}
+    try:
+        t0 = time.perf_counter()
+        due = 0.0
+        for i, (name, atoms) in enumerate(mols):
+            due += rng.exponential(1.0 / 40.0)
+            while time.perf_counter() - t0 < due:
+                time.sleep(0.0005)
+            ids[srv.submit(bfile, atoms, tag=i)] = i
+        got = _drain(srv, len(mols))
+    finally:
+        srv.close()
+    print("served %d requests in %.1f s" % (len(got), time.perf_counter() - t0), flush=True)
+    assert sorted(r.tag for r in got.values()) == list(range(len(mols)))
+    solo = hf.Engine(**opts)
+    solo64 = hf.Engine(exact_transfer=1)
+    bad = 0
+    for r in got.values():
+        name, atoms = mols[r.tag]
+        a = solo.set_molecule(bfile, atoms).scf(check=False, incore=1, **SCF)
+        assert r.nbf == solo.nbf
+        if not (r.converged and a.converged):
+            assert r.converged == a.converged, (name, r.converged, a.converged)
+            bad += 1
+            continue
+        assert r.status == 0
+        assert abs(r.energy - a.energy) <= 1e-9, (name, r.energy, a.energy)
+        if prec:
+            e64 = solo64.set_molecule(bfile, atoms).scf(incore=1, **SCF).energy
+            assert abs(r.energy - e64) <= 1e-6, (name, r.energy, e64)
+    print("not converged within 300 iterations (alone and served): %d" % bad, flush=True)
+    assert bad <= 4
+
+
+def test_request_above_budget_is_rejected(gpu_engine_factory):
+    """A request whose footprint exceeds the whole HBM budget is refused at submission (never admitted
+    alone); smaller requests are served."""
+    import hartreefock_amd as hf
+    bfile = basis("6-31g.1.nw")
+    small = g.water_cluster(1, seed=11)
+    big = g.alkane_zigzag(12, seed=3)
+    fp_small = hf.serve_footprint(gpu_engine_factory().set_molecule(bfile, small).nbf)
+    srv = hf.Server(slots=2, hbm_budget=1.2 * fp_small, scf=SCF)
+    try:
+        with pytest.raises(hf.HFError) as ei:
+            srv.submit(bfile, big)
+        assert ei.value.code == -2
+        rid = srv.submit(bfile, small)
+        got = _drain(srv, 1)
+    finally:
+        srv.close()
+    assert got[rid].status == 0 and got[rid].converged
+
+
+def test_idle_stores_released_for_queue_head(gpu_engine_factory):
+    """Slots keep their in-core stores between requests and those bytes stay charged to the budget.  A large
+    request that fits the budget only once the idle slots' stores are gone is still admitted (the idle slots
+    release them), and no two requests run together beyond the budget."""
+    import hartreefock_amd as hf
+    bfile = basis("6-31g.1.nw")
+    smalls = [g.water_cluster(k, seed=40 + k) for k in (3, 4, 5)]
+    big = g.alkane_zigzag(9, seed=8)
+    n_big = gpu_engine_factory().set_molecule(bfile, big).nbf
+    fp_big = hf.serve_footprint(n_big) + 8.0 * hf._lib.lib().hf_pair_matrix_len(n_big)
+    srv = hf.Server(slots=3, hbm_budget=1.2 * fp_big, scf=dict(SCF, incore=1))
+    try:
+        first = [srv.submit(bfile, a) for a in smalls]
+        got = _drain(srv, 3)
+        rid = srv.submit(bfile, big)
+        got.update(_drain(srv, 1))
+    finally:
+        srv.close()
+    for i in first + [rid]:
+        assert got[i].status == 0 and got[i].converged
+    rs = list(got.values())
+    for r in rs:
+        conc = sum(o.footprint for o in rs if o.admit_ms < r.done_ms and r.admit_ms < o.done_ms)
+        assert conc <= 1.2 * fp_big + 1.0
+
+
+def test_failing_request_leaves_others_untouched(gpu_engine_factory):
+    """One request with coincident nuclei (its energy is not finite) fails with HF_E_NUMERIC; the requests
+    around it, on the same slots, converge to their solo energies."""
+    bfile = basis("6-31g.1.nw")
+    import hartreefock_amd as hf
+    good = [g.water_cluster(2, seed=21), g.alkane_zigzag(3, seed=22), g.water_cluster(1, seed=23),
+            g.alkane_zigzag(2, seed=24)]
+    broken = [(8, (0.0, 0.0, 0.0)), (1, (0.0, 0.0, 0.0)), (1, (0.0, 1.4, 1.0))]
+    srv = hf.Server(slots=2, scf=dict(SCF, incore=1))
+    try:
+        ids = [srv.submit(bfile, a) for a in good[:2]]
+        bad = srv.submit(bfile, broken)
+        ids += [srv.submit(bfile, a) for a in good[2:]]
+        got = _drain(srv, 5)
+    finally:
+        srv.close()
+    assert got[bad].status == -9 and not got[bad].converged
+    for rid, atoms in zip(ids, good):
+        e = gpu_engine_factory().set_molecule(bfile, atoms).scf(incore=1, **SCF).energy
+        assert got[rid].status == 0 and abs(got[rid].energy - e) <= 1e-9

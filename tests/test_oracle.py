@@ -301,3 +301,93 @@ def test_uhf_mp2_reference_formula_closed_shell_limit():
     eo, ev = eps[:nocc], eps[nocc:]
     den = eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]
     assert abs(e_u - 0.5 * np.sum(ov * ov / den)) < 1e-13
+
+
+# ---------------------------------------------------------------- UHF, (dd|dd) convention, field: closed relations
+@pytest.mark.parametrize("bname,atoms", [("sto3g.txt", g.water_cfg1()), ("6-31g_st_.1.nw", g.alkane_zigzag(2, seed=5))],
+                         ids=["h2o_sto3g", "ethane_631gs"])
+def test_closed_shell_uhf_equals_rhf(bname, atoms):
+    """A closed shell in UHF without the iteration-0 asymmetry has D_alpha = D_beta = D/2, so
+    F_alpha = h + J[D_alpha + D_beta] - K[D_alpha] is the RHF Fock matrix: the energy must equal RHF's and
+    <S^2> must vanish.  This pins the alpha/beta Coulomb coupling of the UHF Fock build
+    (HartreeFock/UnrestrictedHartreeFock.cpp:228-238; dropping D_beta J changes the energy by Eh) and the
+    overlap term of <S^2> (Tr(P_a S P_b S) = N_alpha here)."""
+    kw = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+    r, _ = refcpu.Molecule(basis(bname), atoms).scf(**kw)
+    u, _ = refcpu.Molecule(basis(bname), atoms).scf(unrestricted=True, add_asymmetry=0, **kw)
+    assert r.converged and u.converged
+    assert abs(r.energy - u.energy) <= 1e-10, (r.energy, u.energy)
+    assert abs(u.s2) <= 1e-10
+
+
+def test_dddd_reference_convention_error_scales_as_ab_squared():
+    """The reference's transfer range leaves the (dd|dd) intermediates |e| = 2 at |f| = 4 at zero
+    (HartreeFock/GaussianTwoElectrons.cpp:152); their weight in the bra HRR is the square of the bra's centre
+    separation, so ref - exact for a two-centre bra must fall by 4 each time the separation halves (a range
+    starting one level off would leave a term linear in AB: ratio 2)."""
+    d = np.array([0.3, 1.2, 2.5])
+    d /= np.linalg.norm(d)
+    diffs = []
+    for R in (0.04, 0.02, 0.01):
+        m = refcpu.Molecule(basis("6-31g_st_.1.nw"), [(6, (0.0, 0.0, 0.0)), (6, tuple(R * d))])
+        L, _, atom, _ = m.bf_info()
+        dA = [i for i in range(m.nbf) if L[i] == 2 and atom[i] == 0]
+        dB = [i for i in range(m.nbf) if L[i] == 2 and atom[i] == 1]
+        worst = 0.0
+        try:
+            for i in dA:
+                for j in dB:
+                    for k in dA[:3]:
+                        for l in dA[:3]:
+                            refcpu.set_exact_transfer(True)
+                            ex = m.eri(i, j, k, l)
+                            refcpu.set_exact_transfer(False)
+                            worst = max(worst, abs(m.eri(i, j, k, l) - ex))
+        finally:
+            refcpu.set_exact_transfer(False)
+        diffs.append(worst)
+    assert diffs[2] > 0
+    for a, b in zip(diffs, diffs[1:]):
+        assert abs(a / b - 4.0) < 0.01, diffs
+
+
+def test_field_finite_difference_dipole_oracle():
+    """The reference's dipole check (HartreeFock/Test.cpp:626-733) on the oracle: H2O/STO-3G (Crawford geometry,
+    no DIIS, alpha 0.5, initGuess 0.7) in fields of +-0.001 au, h = T + V + E . M
+    (HartreeFock/HartreeFockAlgorithm.cpp:38).  The nuclear moment plus the central difference of the energy
+    equals the analytic SCF dipole within 2e-6 au, both 0.60352 au (the reference's comment quotes 0.603521
+    and 0.60352); a wrong sign of the field term flips the difference."""
+    atoms = g.water_crawford()
+    kw = dict(use_diis=0, alpha=0.5, init_guess=0.7, normal_iter_after_diis=0)
+    mol = refcpu.Molecule(basis("sto3g.txt"), atoms)
+    r, D = mol.scf(**kw)
+    mu = refcpu.dipole(D[0], mol.moment(), [a[0] for a in atoms], [a[1] for a in atoms])
+    d = 0.001
+    fd = np.array([z * np.asarray(x, float) for z, x in atoms]).sum(axis=0)
+    for k in range(3):
+        E = np.zeros(3)
+        E[k] = -d
+        e1 = refcpu.Molecule(basis("sto3g.txt"), atoms).set_field(E).scf(**kw)[0].energy
+        E[k] = d
+        e2 = refcpu.Molecule(basis("sto3g.txt"), atoms).set_field(E).scf(**kw)[0].energy
+        fd[k] += (e1 - e2) / (2 * d)
+    assert np.abs(mu - fd).max() <= 2e-6, (mu, fd)
+    assert abs(np.linalg.norm(mu) - 0.603521) <= 1e-6
+    assert abs(np.linalg.norm(fd) - 0.60352) <= 1e-5
+
+
+def test_ion_and_high_spin_electron_counts():
+    """Molecule::alphaElectrons / betaElectrons (HartreeFock/Molecule.h:16-17): O2 in STO-3G as a triplet
+    (9 alpha, 7 beta) has <S^2> just above S(S+1) = 2 and lies below the closed-shell RHF singlet; OH- in RHF
+    has 5 doubly occupied orbitals, so its energy differs from neutral OH's."""
+    kw = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+    o2 = [(8, (0.0, 0.0, 0.0)), (8, (0.0, 0.0, 2.28))]
+    t, _ = refcpu.Molecule(basis("sto3g.txt"), o2).set_electrons(9, 7).scf(unrestricted=True, **kw)
+    s, _ = refcpu.Molecule(basis("sto3g.txt"), o2).scf(**kw)
+    assert t.converged and s.converged
+    assert 2.0 <= t.s2 < 2.1 and t.energy < s.energy
+    oh = [(8, (0.0, 0.0, 0.0)), (1, (0.0, 0.0, 1.83))]
+    m = refcpu.Molecule(basis("sto3g.txt"), oh).set_electrons(5, 5)
+    assert m.electrons() == (5, 5)
+    a, _ = m.scf(**kw)
+    assert a.converged
