@@ -1,20 +1,22 @@
-"""Benchmark: SCF molecules/s on one MI355X.
+"""Benchmark: SCF molecules/s on one MI355X, mixed precision against FP64, with p50/p99 latency and ERI/s.
 
-Workload (default): plain direct RHF in FP64 on the GPU for n-decane C10H22 in 6-31G*
-(194 Cartesian functions), seeded zigzag geometry (C-C 1.54 A, C-H 1.09 A, tetrahedral
-angles, +-0.05 A jitter), Schwarz threshold 1e-12, converged to 1e-8 Eh.  A step is one
-complete SCF solve (integrals recomputed every iteration: direct J/K).  The molecule's
-shells, primitive pairs and Schwarz table are resident in HBM before timing starts.
+Default workload (the serving configuration): RHF requests drawn from a seeded mix of water clusters (1-12
+waters) and n-alkanes (C1-C16) in 6-31G (13-212 Cartesian functions), served by the continuous-batching server
+(8 slots, each request's ERIs evaluated once into an in-core store in HBM, device-resident SCF state).
+A step is one batch of --batch requests (default 100): --steps 20 is the 2000-request workload.  The timed region
+is the mixed-precision tier serving all steps x batch requests queued at once (closed loop: the server's
+sustained rate); the same requests are then served in FP64, and both tiers again under Poisson arrivals at 0.8x
+their measured capacity for p50/p99 latency.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload decane|water|serving|water32|eicosyl]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+    python bench.py --workload decane|water|water32|eicosyl      (single-molecule SCF legs)
 
-Prints ONE JSON line (rank 0).  `roofline` is the dominant ERI class kernel: algorithmic
-flops (exact operation count of the generated code, per launch) over its average launch
-duration from HIP events recorded on its launch stream during the timed steps.
-`cpu_baseline` times the oracle (a CPU restatement of the reference's algorithm:
-per-component HGP ERIs into the 8-fold packed store, then an unscreened N^4 Fock loop
-per iteration) on a bounded sample of the same workload, single core, and extrapolates
-to one full SCF with the GPU run's iteration count.
+Prints ONE JSON line.  value = mixed-tier molecules/s counting only requests that converged (1e-8 Eh) with
+|E_mixed - E_FP64| <= 1e-6 Eh.  `roofline` is the workload's dominant kernel (most device time in the warm-up
+pass), timed with HIP events on its own launch stream during the timed region.  `cpu_baseline` is the oracle
+(a CPU restatement of the reference's algorithm: per-component HGP ERIs into the 8-fold packed store, then an
+unscreened N^4 Fock loop per iteration) on a bounded sample of the same request mix, one SCF per process on P
+host cores as the reference runs one SCF per thread, extrapolated to molecules/s.
 """
 import argparse
 import ctypes
@@ -26,50 +28,43 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (and matrix)
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector
 PEAK_HBM_GBS = 8000.0
+
+# SCF settings of every leg: the reference's SCF (DIIS 6/5, core guess) with normalIterAfterDIIS = 0 and the
+# configurations' convergence criterion of 1e-8 Eh; the reference's density and DIIS-error criteria (1e-10,
+# HartreeFock/Constants.h:6-10) are loosened to 1e-6, which the 1e-8 Eh energy criterion dominates
+SCF_KW = dict(alpha=0.5, init_guess=0.0, use_diis=1, normal_iter_after_diis=0,
+              energy_tolerance=1e-8, density_tolerance=1e-6, diis_tolerance=1e-6)
+SCF_DESC = ("RHF, core guess, alpha 0.5, DIIS 6 kept / 5 to extrapolate, normalIterAfterDIIS 0; converged when "
+            "|dE| <= 1e-8 Eh, ||dD||_F <= 1e-6 and DIIS error <= 1e-6")
 
 
 def hip_sync():
-    try:
-        ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
-    except OSError:
-        pass
+    ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
 
 
-def workload(name):
-    from hartreefock_amd import geometry as g
-    import hartreefock_amd as hf
-    if name == "decane":
-        return ("C10H22/6-31G* direct RHF FP64", hf.basis_path("6-31g_st_.1.nw"), g.decane_cfg2())
-    if name == "water":
-        return ("H2O/STO-3G RHF FP64", hf.basis_path("sto3g.txt"), g.water_cfg1())
-    if name == "water32":
-        return ("(H2O)32/6-31G** direct RHF FP64", hf.basis_path("6-31g_st__st_.nw"), g.water32_cfg3())
-    if name == "eicosyl":
-        return ("C20H41/6-31+G* UHF FP64", hf.basis_path("6-31+g_st_.1.nw"), g.eicosyl_cfg5())
-    raise SystemExit("unknown workload " + name)
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
-SCF_KW = dict(alpha=0.5, init_guess=0.0, use_diis=1, normal_iter_after_diis=0,
-              energy_tolerance=1e-8, density_tolerance=1e-6, diis_tolerance=1e-6)
-
-
-def cpu_baseline(bfile, atoms, iterations, budget_s):
-    """Oracle (reference algorithm) on one core: sampled ERI phase + sampled N^4 Fock loop."""
+# ---------------------------------------------------------------------------------------------- CPU baseline
+def _cpu_molecule_seconds(bfile, atoms, iterations, budget_s):
+    """Oracle (reference algorithm) on one core for one molecule, sampled: every stride-th (i,j,k) triple of the
+    packed ERI loop with its whole l loop (~60% of the budget) and some rows of one N^4 Fock loop (~40%),
+    extrapolated to the molecule's ERI phase plus `iterations` Fock builds.  Returns (seconds, description)."""
     import numpy as np
     from oracle import refcpu
     mol = refcpu.Molecule(bfile, atoms)
     n = mol.nbf
     unique = refcpu.packed_index(n - 1, n - 1, n - 1, n - 1) + 1
     triples = n * (n + 1) // 2 * n
-    # ERI phase: every stride-th (i,j,k) triple with its whole l loop (~60% of the budget)
     stride = max(1, int(triples / max(1.0, 0.6 * budget_s / 19e-6 / (n / 2.0))))
     t0 = time.perf_counter()
     eri, cnt = mol.eri_packed_sample(stride, stride // 2)
     t_eri = time.perf_counter() - t0
     eri_phase = t_eri / max(cnt, 1) * unique
-    # Fock loop: rows of the reference's G loop (~40% of the budget at ~0.1 s/row for n=194)
     rows = max(2, min(n, int(0.4 * budget_s / (20.0 / 194.0 * (n / 194.0) ** 3))))
     D = np.eye(n) * 0.1
     t0 = time.perf_counter()
@@ -77,10 +72,317 @@ def cpu_baseline(bfile, atoms, iterations, budget_s):
     t_g = time.perf_counter() - t0
     fock = t_g * n / rows
     total = eri_phase + iterations * fock
-    return {"value": 1.0 / total, "unit": "molecules/s", "cores": 1, "kind": "port",
-            "sample": ("oracle on %d of %d unique ERIs (every %d-th (i,j,k) triple, %.1fs -> ERI phase %.0fs) + "
-                       "%d of %d rows of one N^4 Fock loop (%.1fs -> %.1fs/iteration) x %d iterations = %.0fs/molecule"
-                       % (cnt, unique, stride, t_eri, eri_phase, rows, n, t_g, fock, iterations, total))}
+    return total, ("n=%d: %d of %d unique ERIs (%.1fs -> ERI phase %.0fs) + %d of %d Fock rows (%.2fs -> %.2fs per "
+                   "iteration) x %d iterations = %.0fs" % (n, cnt, unique, t_eri, eri_phase, rows, n, t_g, fock,
+                                                          iterations, total))
+
+
+def _cpu_worker(job):
+    bfile, mols, budget_s = job
+    return [_cpu_molecule_seconds(bfile, atoms, it, budget_s)[0] for atoms, it in mols]
+
+
+def host_cores():
+    """Cores this process may use, capped at the host share of one GPU (16)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline_serving(bfile, sample, budget_s):
+    """The oracle on P host processes at once (one SCF per core, as the reference runs one SCF per std::thread,
+    HartreeFock/HartreeFockDoc.cpp:379-401): each process times two molecules of the sample (spread over the
+    size range of the mix) within budget_s, and the mean extrapolated time per molecule gives P / mean
+    molecules/s."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    P = host_cores()
+    per = max(1, len(sample) // P)
+    jobs = [(bfile, sample[w * per:(w + 1) * per], budget_s / per) for w in range(P) if sample[w * per:(w + 1) * per]]
+    t0 = time.perf_counter()
+    with cf.ProcessPoolExecutor(max_workers=len(jobs), mp_context=mp.get_context("spawn")) as ex:
+        secs = [s for r in ex.map(_cpu_worker, jobs) for s in r]
+    wall = time.perf_counter() - t0
+    mean = sum(secs) / len(secs)
+    return {"value": len(jobs) / mean, "unit": "molecules/s", "cores": len(jobs), "kind": "port",
+            "sample": ("oracle (reference algorithm, single-threaded per molecule) on %d molecules of the timed mix "
+                       "spread over its size range, %d processes at once for %.0fs: sampled ERI phase + sampled N^4 "
+                       "Fock loop x the GPU run's iterations per molecule, extrapolated to %.0fs per molecule on "
+                       "average; throughput = processes / mean time" % (len(secs), len(jobs), wall, mean))}
+
+
+def cpu_baseline_single(bfile, atoms, iterations, budget_s):
+    sec, desc = _cpu_molecule_seconds(bfile, atoms, iterations, budget_s)
+    return {"value": 1.0 / sec, "unit": "molecules/s", "cores": 1, "kind": "port", "sample": "oracle on one core: " + desc}
+
+
+# ---------------------------------------------------------------------------------------------- serving
+SERVE_SLOTS = 8    # bench.py --slots N changes it (a one-slot run serves requests one after another)
+
+
+def serve(hf, bfile, reqs, prec, rate=None, slots=None, incore=1, tag="", profile=0, profile_class=-1,
+          seed=77, max_iterations=300):
+    """Serve reqs = [(tag, atoms)]: all queued at once (rate None) or as a Poisson process at `rate`/s.  Returns the
+    results by tag, the wall time from the first submission to the last completion and the summed kernel
+    statistics."""
+    import numpy as np
+    srv = hf.Server(slots=slots or SERVE_SLOTS, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1, profile=profile,
+                                              profile_class=profile_class),
+                    scf=dict(SCF_KW, incore=incore, max_iterations=max_iterations))
+    rng = np.random.default_rng(seed)
+    got, shown = {}, 0
+    try:
+        gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+        hip_sync()
+        t0 = time.perf_counter()
+        due = 0.0
+        for (rt, atoms), gap in zip(reqs, gaps):
+            due += gap
+            while time.perf_counter() - t0 < due:
+                for r in srv.poll(64, 0):
+                    got[r.tag] = r
+                time.sleep(min(0.0005, max(0.0, due - (time.perf_counter() - t0))))
+            srv.submit(bfile, atoms, tag=rt)
+        while len(got) < len(reqs):
+            for r in srv.poll(64, 1000):
+                got[r.tag] = r
+            if len(got) // 500 != shown:
+                shown = len(got) // 500
+                log("serving %s: %d of %d done" % (tag, len(got), len(reqs)))
+        hip_sync()
+        wall = time.perf_counter() - t0
+        stats = srv.kernel_stats()
+    finally:
+        srv.close()
+    log("serving %s: %d requests in %.1f s" % (tag, len(reqs), wall))
+    return got, wall, stats
+
+
+def kernel_kinds(stats):
+    """Device time, launches, work and bound per kernel kind (ERI class x precision, in-core contraction FP64/FP32)."""
+    out = {}
+    for s in stats:
+        if s.la < 0:
+            bpe = 4.0 if s.fp32 else 8.0
+            out[s.name()] = dict(ms=s.ms, launches=s.launches, work=s.eris * bpe, bound="hbm", unit="GB/s",
+                                 peak=PEAK_HBM_GBS, kernel="incore_jk_kernel<%s>" % ("float" if s.fp32 else "double"),
+                                 key=-2)
+        else:
+            key = s.la * 1000 + s.lb * 100 + s.lc * 10 + s.ld
+            out[s.name()] = dict(ms=s.ms, launches=s.launches, work=s.flops, bound="fp32-valu" if s.fp32 else
+                                 "fp64-valu", unit="TFLOP/s", peak=PEAK_FP32_TFLOPS if s.fp32 else PEAK_FP64_TFLOPS,
+                                 kernel="eri_class_kernel" + s.name(), key=key)
+    return out
+
+
+def roofline_of(kind, stats):
+    k = kernel_kinds(stats).get(kind)
+    if not k or k["ms"] <= 0:
+        return None
+    per_ms = k["ms"] / k["launches"]
+    per_work = k["work"] / k["launches"]
+    scale = 1e9 if k["unit"] == "GB/s" else 1e12
+    achieved = per_work / (per_ms * 1e-3) / scale
+    return {"bound": k["bound"], "kernel": k["kernel"], "achieved": achieved, "peak": k["peak"], "unit": k["unit"],
+            "frac": achieved / k["peak"], "traffic": None, "launches": k["launches"], "ms_per_launch": per_ms,
+            ("bytes_per_launch" if k["unit"] == "GB/s" else "flops_per_launch"): per_work}
+
+
+def pct(vals, p):
+    import numpy as np
+    return float(np.percentile(np.asarray(vals), p)) if len(vals) else None
+
+
+def serving_bench(a):
+    import hartreefock_amd as hf
+    from hartreefock_amd import geometry as g
+    bfile = hf.basis_path("6-31g.1.nw")
+    K, W, B = a.steps, a.warmup, a.batch
+    mols = g.serving_mix(W * B + K * B, seed=2024)
+    warm = [(i, atoms) for i, (_, atoms) in enumerate(mols[:W * B])]
+    timed = [(i, atoms) for i, (_, atoms) in enumerate(mols[W * B:], start=W * B)]
+    # warm-up (untimed): W batches in each tier with every kernel timed, which picks the dominant kernel
+    dominant = {}
+    for prec in (1, 0):
+        _, _, st = serve(hf, bfile, warm, prec, tag="warm-up %s" % ("mixed" if prec else "fp64"), profile=1)
+        kinds = kernel_kinds(st)
+        dominant[prec] = max(kinds, key=lambda k: kinds[k]["ms"]) if kinds else None
+        log("warm-up %s: dominant kernel %s (%s)" % ("mixed" if prec else "fp64", dominant[prec],
+                                                    ", ".join("%s %.0f ms" % (k, v["ms"]) for k, v in
+                                                              sorted(kinds.items(), key=lambda x: -x[1]["ms"])[:5])))
+    # timed region: the mixed tier, every timed request queued at once; only the dominant kernel carries events
+    # (profile_class: its class key, or -2 for the in-core contraction)
+    dom_key = None
+    if dominant[1]:
+        n, s = dominant[1], "spd"
+        dom_key = -2 if n.startswith("incore") else s.index(n[1]) * 1000 + s.index(n[2]) * 100 + s.index(n[4]) * 10 + \
+            s.index(n[5])
+    res_m, wall_m, st_m = serve(hf, bfile, timed, 1, tag="timed mixed", profile=1 if dom_key is not None else 0,
+                                profile_class=dom_key if dom_key is not None else -1)
+    # the same requests in FP64 (closed loop), then both tiers under Poisson arrivals for latency
+    res_d, wall_d, st_d = serve(hf, bfile, timed, 0, tag="fp64")
+    conv_d = {t: r.energy for t, r in res_d.items() if r.status == 0 and r.converged}
+    conv_m = {t: r.energy for t, r in res_m.items() if r.status == 0 and r.converged}
+    good_m = [t for t, e in conv_m.items() if t in conv_d and abs(e - conv_d[t]) <= 1e-6]
+    dE = [abs(conv_m[t] - conv_d[t]) for t in conv_m if t in conv_d]
+    cap_m, cap_d = len(good_m) / wall_m, len(conv_d) / wall_d
+    lat = {}
+    for prec, cap in ((1, cap_m), (0, cap_d)):
+        r, _, _ = serve(hf, bfile, timed, prec, rate=0.8 * cap, tag="poisson %s" % ("mixed" if prec else "fp64"),
+                        seed=78)
+        ms = [x.done_ms - x.arrival_ms for x in r.values()]
+        lat["mixed" if prec else "fp64"] = {"rate_per_s": 0.8 * cap, "p50_ms": pct(ms, 50), "p99_ms": pct(ms, 99),
+                                            "converged": sum(1 for x in r.values() if x.converged)}
+    # the plain FP64 path, one molecule at a time with direct J/K (the north star's reference point)
+    plain_reqs = timed[:a.plain_requests]
+    res_p, wall_p, _ = serve(hf, bfile, plain_reqs, 0, slots=1, incore=0, tag="plain fp64")
+    cap_p = sum(1 for r in res_p.values() if r.status == 0 and r.converged) / wall_p
+    eris_m = sum(s.eris for s in st_m if s.la >= 0)
+    roof = roofline_of(dominant[1], st_m) if dominant[1] else None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_serving_631g.json")
+    if roof and os.path.exists(tfile):
+        with open(tfile) as fh:
+            rec = json.load(fh).get("kernels", {}).get(roof["kernel"])
+        if rec:
+            roof["traffic"] = rec["bytes_per_launch"]
+            roof["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_serving_631g.json"
+    nbf = [r.nbf for r in res_m.values()]
+    out = {
+        "metric": "SCF molecules/s, mixed precision vs FP64 on GPU, at |dE| <= 1e-6 Eh (serving, continuous batching)",
+        "value": cap_m, "unit": "molecules/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": wall_m / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64/f32 (mixed tier: FP32 quartets below a 1e-6 Schwarz bound and an FP32 store for density "
+                 "changes, FP64 finish)",
+        "data": "synthetic (seeded geometries)",
+        "config": {"workload": "serving mix: %d RHF requests (%d steps x %d), water clusters 1-12 / n-alkanes C1-C16, "
+                               "6-31G, %d-%d functions" % (K * B, K, B, min(nbf), max(nbf)),
+                   "batch_per_step": B, "slots": SERVE_SLOTS, "jk": "in-core ERI store per request",
+                   "eri_convention": "exact (6-31G has no d shells: identical to the reference's)",
+                   "schwarz_threshold": 1e-12, "scf": SCF_DESC, "max_iterations": 300,
+                   "arrival": "all queued at once (closed loop) in the timed region; Poisson at 0.8x capacity for "
+                              "latency", "parallelism": "single GPU"},
+        "mixed": {"molecules_per_s": cap_m, "wall_s": wall_m, "converged": len(conv_m), "within_1e-6_of_fp64":
+                  len(good_m), "max_abs_dE_vs_fp64_Eh": max(dE) if dE else None,
+                  "latency": lat["mixed"], "eri_per_s": eris_m / wall_m},
+        "fp64": {"molecules_per_s": cap_d, "wall_s": wall_d, "converged": len(conv_d), "latency": lat["fp64"],
+                 "eri_per_s": sum(s.eris for s in st_d if s.la >= 0) / wall_d},
+        "mixed_vs_fp64": cap_m / cap_d,
+        "plain_fp64_one_at_a_time": {"molecules_per_s": cap_p, "requests": len(plain_reqs), "jk": "direct", "slots": 1},
+        "mixed_batched_vs_plain_fp64": cap_m / cap_p, "fp64_batched_vs_plain_fp64": cap_d / cap_p,
+        "latency_ms": {"p50": lat["mixed"]["p50_ms"], "p99": lat["mixed"]["p99_ms"]},
+        "eri_per_s": eris_m / wall_m,
+        "not_converged": sorted(set(t for t in res_d if t not in conv_d) | set(t for t in res_m if t not in conv_m)),
+        "roofline": roof,
+        "dominant_kernel": {"mixed": dominant[1], "fp64": dominant[0]},
+    }
+    if not a.no_subleg:
+        out["decane_cfg2"] = single_molecule("decane", 3, 1, cpu=False)
+    if not a.no_cpu_baseline:
+        # sample spread over the size range; iterations from the FP64 run of each molecule
+        import numpy as np
+        P = host_cores()
+        order = sorted(conv_d, key=lambda t: res_d[t].nbf)
+        pick = [order[int(x)] for x in np.linspace(0, len(order) - 1, 2 * P)]
+        sample = [(mols[t][1], res_d[t].iterations) for t in pick]
+        out["cpu_baseline"] = cpu_baseline_serving(bfile, sample, a.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------------------------- single molecule
+def workload(name):
+    from hartreefock_amd import geometry as g
+    import hartreefock_amd as hf
+    if name == "decane":
+        return ("C10H22/6-31G* RHF (194 functions)", hf.basis_path("6-31g_st_.1.nw"), g.decane_cfg2())
+    if name == "water":
+        return ("H2O/STO-3G RHF", hf.basis_path("sto3g.txt"), g.water_cfg1())
+    if name == "water32":
+        return ("(H2O)32/6-31G** RHF (800 functions)", hf.basis_path("6-31g_st__st_.nw"), g.water32_cfg3())
+    if name == "eicosyl":
+        return ("C20H41/6-31+G* UHF (462 functions)", hf.basis_path("6-31+g_st_.1.nw"), g.eicosyl_cfg5())
+    raise SystemExit("unknown workload " + name)
+
+
+def timed_scfs(eng, steps, warmup, **extra):
+    kw = dict(SCF_KW, **extra)
+    for _ in range(warmup):
+        eng.scf(**kw)
+    eng.reset_kernel_stats()
+    lat, res = [], []
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        res.append(eng.scf(**kw))
+        lat.append(time.perf_counter() - s0)
+    hip_sync()
+    return time.perf_counter() - t0, lat, res
+
+
+def single_molecule(name, steps, warmup, cpu=True, cpu_budget=20.0, max_iterations=0, legs=""):
+    """One molecule's SCF per step.  Legs: direct FP64 (the plain method), in-core FP64 (the ERIs evaluated once per
+    SCF into HBM, inside the step), mixed direct and mixed in-core; each with molecules/s, iterations, energy and
+    |dE| against direct FP64.  Roofline of the direct leg's dominant ERI class (HIP events, FP64 VALU) and of the
+    in-core contraction (HBM)."""
+    import hartreefock_amd as hf
+    label, bfile, atoms = workload(name)
+    uhf = name == "eicosyl"
+    kw = dict(max_iterations=max_iterations or (200 if uhf else 300), unrestricted=uhf)
+    all_legs = [("fp64_direct", 0, 0), ("fp64_incore", 0, 1), ("mixed_direct", 1, 0), ("mixed_incore", 1, 1)]
+    if legs:
+        all_legs = [l for l in all_legs if l[0] in legs.split(",")]
+    out = {"workload": label, "scf": SCF_DESC, "schwarz_threshold": 1e-12, "eri_convention": "exact"}
+    e0 = None
+    for leg, prec, inc in all_legs:
+        eng = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, precision=prec,
+                        fp32_threshold=1e-6).set_molecule(bfile, atoms)
+        out["nbf"] = eng.nbf
+        if inc and eng.nbf > 512:
+            eng.close()
+            out[leg] = "store does not fit (n > 512)"
+            continue
+        log("%s: %s" % (name, leg))
+        el, lat, res = timed_scfs(eng, steps, warmup, incore=inc, **kw)
+        r = res[-1]
+        if e0 is None:
+            e0 = r.energy
+        nb = max(1, r.full_builds + r.incremental_builds)
+        out[leg] = {"molecules_per_s": steps / el, "ms_per_scf": el / steps * 1e3, "iterations": r.iterations,
+                    "converged": r.converged, "energy_Eh": r.energy, "abs_dE_vs_fp64_direct_Eh": abs(r.energy - e0),
+                    "jk_ms_per_build": r.jk_ms / nb, "incore": r.incore, "eri_store_ms": r.eri_ms,
+                    "fp32_builds": r.fp32_builds, "p50_ms": pct([x * 1e3 for x in lat], 50),
+                    "p99_ms": pct([x * 1e3 for x in lat], 99)}
+        if uhf:
+            out[leg]["s2"] = r.s2
+        eng.close()
+    # roofline passes: every class timed to find the dominant one, then only that class (launched first in each
+    # build, so its events bracket the kernel rather than its wait for CUs); the in-core contraction's events
+    if "fp64_direct" in out and isinstance(out["fp64_direct"], dict):
+        p = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, profile=1).set_molecule(bfile, atoms)
+        p.scf(**dict(SCF_KW, **kw))
+        kinds = kernel_kinds(p.kernel_stats())
+        dom = max((k for k in kinds if not k.startswith("incore")), key=lambda k: kinds[k]["ms"])
+        p.close()
+        q = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, profile=1,
+                      profile_class=kinds[dom]["key"]).set_molecule(bfile, atoms)
+        q.scf(**dict(SCF_KW, **kw))
+        q.reset_kernel_stats()
+        q.scf(**dict(SCF_KW, **kw))
+        out["fp64_direct"]["roofline"] = roofline_of(dom, q.kernel_stats())
+        out["fp64_direct"]["eri_per_s"] = sum(s.eris for s in q.kernel_stats()) / (out["fp64_direct"]["ms_per_scf"] * 1e-3)
+        q.close()
+    if "fp64_incore" in out and isinstance(out["fp64_incore"], dict):
+        p = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, profile=1, profile_class=-2).set_molecule(bfile, atoms)
+        p.scf(incore=1, **dict(SCF_KW, **kw))
+        p.reset_kernel_stats()
+        p.scf(incore=1, **dict(SCF_KW, **kw))
+        out["fp64_incore"]["roofline"] = roofline_of("incore_jk", p.kernel_stats())
+        p.close()
+    if cpu and isinstance(out.get("fp64_direct"), dict):
+        out["cpu_baseline"] = cpu_baseline_single(bfile, atoms, out["fp64_direct"]["iterations"], cpu_budget)
+    return out
 
 
 def copy_calibration():
@@ -96,161 +398,22 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}), flush=True)
 
 
-def serving(a):
-    """Serving configuration: a seeded mix of water clusters (1-12) and alkanes (C1-C16) in 6-31G,
-    RHF to 1e-8 Eh.  Phase 1 measures the server's capacity with every request queued at once;
-    phase 2 submits the requests as a Poisson process at 0.8 x that capacity (recorded seed) and
-    reports molecules/s (completions over first arrival to last completion) and the p50/p99 of
-    completion - arrival, for FP64 and for the mixed-precision tier."""
-    import numpy as np
-    import hartreefock_amd as hf
-    from hartreefock_amd import geometry as g
-    bfile = hf.basis_path("6-31g.1.nw")
-    scf = dict(SCF_KW, max_iterations=300)   # a request that has not converged by then is reported as failed
-    mols = g.serving_mix(a.requests, seed=2024)
-    rng = np.random.default_rng(77)
-
-    def run(prec, rate, tag, slots=None, incore=None, reqs=None):
-        shown = 0
-        reqs = mols if reqs is None else reqs
-        srv = hf.Server(slots=slots or a.slots, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1),
-                        scf=dict(scf, incore=a.serve_incore if incore is None else incore))
-        try:
-            gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
-            got = []
-            t0 = time.perf_counter()
-            due = 0.0
-            for (name, atoms), gap in zip(reqs, gaps):
-                due += gap
-                while time.perf_counter() - t0 < due:
-                    got += srv.poll(64, 0)
-                    time.sleep(min(0.001, max(0.0, due - (time.perf_counter() - t0))))
-                srv.submit(bfile, atoms)
-            while len(got) < len(reqs):
-                got += srv.poll(64, 1000)
-                if len(got) // 250 != shown:
-                    shown = len(got) // 250
-                    print("serving %s: %d of %d done" % (tag, len(got), len(reqs)), file=sys.stderr, flush=True)
-        finally:
-            srv.close()
-        print("serving %s: finished in %.1f s" % (tag, time.perf_counter() - t0), file=sys.stderr, flush=True)
-        bad = [{"id": r.id, "nbf": r.nbf, "status": r.status, "iterations": r.iterations, "energy": r.energy}
-               for r in got if r.status != 0 or not r.converged]
-        first = min(r.arrival_ms for r in got)
-        last = max(r.done_ms for r in got)
-        lat = np.array(sorted(r.done_ms - r.arrival_ms for r in got))
-        nbf = [r.nbf for r in got]
-        return {"molecules_per_s": len(got) / ((last - first) * 1e-3), "p50_ms": float(np.percentile(lat, 50)),
-                "p99_ms": float(np.percentile(lat, 99)), "not_converged": bad, "nbf_min": min(nbf), "nbf_max": max(nbf),
-                "energies": {r.id: r.energy for r in got if r.status == 0 and r.converged}}
-
-    # the plain FP64 path: one molecule at a time, direct J/K, on the first plain_requests requests
-    plain = run(0, None, "plain", slots=1, incore=0, reqs=mols[:a.plain_requests])["molecules_per_s"]
-    cap = run(0, None, "capacity")["molecules_per_s"]
-    cap_m = run(1, None, "capacity mixed")["molecules_per_s"]
-    rate = 0.8 * cap
-    r64 = run(0, rate, "fp64")
-    r32 = run(1, rate, "mixed")
-    common = set(r64["energies"]) & set(r32["energies"])
-    dmax = max(abs(r64["energies"][k] - r32["energies"][k]) for k in common)
-    for r in (r64, r32):
-        r.pop("energies")
-    out = {"metric": "SCF molecules/s under Poisson arrivals (serving, continuous batching)",
-           "value": r64["molecules_per_s"], "unit": "molecules/s", "n_gpus": 1, "higher_is_better": True,
-           "dtype": "f64", "data": "synthetic (seeded geometries)",
-           "config": {"workload": "serving mix: %d RHF requests, water clusters 1-12 / alkanes C1-C16, 6-31G" % a.requests,
-                      "slots": a.slots, "arrival": "Poisson, rate 0.8 x closed-loop capacity", "seed": 77,
-                      "jk": "in-core ERI store" if a.serve_incore else "direct",
-                      "closed_loop_capacity": cap, "rate": rate},
-           "plain_fp64_one_at_a_time": {"molecules_per_s": plain, "requests": a.plain_requests,
-                                        "jk": "direct", "slots": 1},
-           "capacity": {"fp64": cap, "mixed": cap_m},
-           "mixed_batched_vs_plain_fp64": cap_m / plain, "fp64_batched_vs_plain_fp64": cap / plain,
-           "fp64": r64, "mixed": r32, "mixed_vs_fp64": r32["molecules_per_s"] / r64["molecules_per_s"],
-           "max_abs_dE_mixed_vs_fp64_Eh": dmax}
-    print(json.dumps(out), flush=True)
-
-
-def large(a):
-    """The large single-molecule configurations, one SCF per leg: (H2O)32/6-31G** RHF (800 functions) and
-    C20H41/6-31+G* UHF (462 functions, 200 iterations at most: with the reference's SCF settings it does
-    not meet 1e-8 Eh).  Legs: direct FP64, in-core FP64 (when the store fits: n <= 512), mixed direct and
-    mixed in-core; each reports wall time, iterations, energy and |dE| against direct FP64."""
-    import hartreefock_amd as hf
-    label, bfile, atoms = workload(a.workload)
-    uhf = a.workload == "eicosyl"
-    kw = dict(SCF_KW, max_iterations=a.max_iterations or (200 if uhf else 300))
-    legs = [("fp64_direct", 0, 0), ("fp64_incore", 0, 1), ("mixed_direct", 1, 0), ("mixed_incore", 1, 1)]
-    if a.legs:
-        legs = [l for l in legs if l[0] in a.legs.split(",")]
-    out = {"metric": "SCF wall time per molecule", "unit": "s", "higher_is_better": False, "n_gpus": 1,
-           "dtype": "f64", "data": "synthetic (seeded geometry)",
-           "config": {"workload": label, "schwarz_threshold": 1e-12, "energy_tolerance": 1e-8,
-                      "eri_convention": "exact", "scf": "alpha=0.5, core guess, DIIS(6/5), normalIterAfterDIIS=0"}}
-    e0 = None
-    for name, prec, inc in legs:
-        eng = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, precision=prec,
-                        fp32_threshold=1e-6).set_molecule(bfile, atoms)
-        out["config"]["nbf"] = eng.nbf
-        if inc and eng.nbf > 512:
-            eng.close()
-            out[name] = "store does not fit (n > 512)"
-            continue
-        print("large %s: %s" % (a.workload, name), file=sys.stderr, flush=True)
-        hip_sync()
-        t0 = time.perf_counter()
-        r = eng.scf(unrestricted=uhf, check=False, incore=inc, **kw)
-        hip_sync()
-        el = time.perf_counter() - t0
-        eng.close()
-        if e0 is None:
-            e0 = r.energy
-        out[name] = {"s": el, "iterations": r.iterations, "converged": r.converged, "energy_Eh": r.energy,
-                     "abs_dE_vs_fp64_direct_Eh": abs(r.energy - e0), "incore": r.incore, "eri_store_ms": r.eri_ms,
-                     "jk_ms": r.jk_ms, "fp32_builds": r.fp32_builds}
-        if uhf:
-            out[name]["s2"] = r.s2
-    first = next(l[0] for l in legs if isinstance(out.get(l[0]), dict))
-    out["value"] = out[first]["s"]
-    out["config"]["max_iterations"] = kw["max_iterations"]
-    print(json.dumps(out), flush=True)
-
-
-def timed_scfs(eng, steps, warmup, **extra):
-    kw = dict(SCF_KW, **extra)
-    for _ in range(warmup):
-        eng.scf(**kw)
-    eng.reset_kernel_stats()
-    lat, res = [], []
-    hip_sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        s0 = time.perf_counter()
-        r = eng.scf(**kw)
-        lat.append(time.perf_counter() - s0)
-        res.append(r)
-    hip_sync()
-    return time.perf_counter() - t0, lat, res
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="decane")
+    ap.add_argument("--steps", type=int, default=20, help="serving: batches of --batch requests in the timed region")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=100, help="serving: requests per step")
+    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent requests (server slots)")
+    ap.add_argument("--workload", default="serving", help="serving (default) | decane | water | water32 | eicosyl")
+    ap.add_argument("--plain-requests", type=int, default=300,
+                    help="serving: requests of the one-at-a-time plain FP64 direct run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
-    ap.add_argument("--requests", type=int, default=2000, help="serving workload: number of requests")
-    ap.add_argument("--slots", type=int, default=8, help="serving workload: concurrent requests")
-    ap.add_argument("--legs", default="", help="water32/eicosyl: comma list of legs (fp64_direct, fp64_incore, "
-                    "mixed_direct, mixed_incore); default all")
-    ap.add_argument("--max-iterations", type=int, default=0, help="water32/eicosyl: SCF iteration cap (0: 300 / 200)")
-    ap.add_argument("--plain-requests", type=int, default=300,
-                    help="serving workload: requests of the one-at-a-time plain FP64 reference run")
-    ap.add_argument("--serve-incore", type=int, default=1,
-                    help="serving workload: 1 = each request's ERIs evaluated once into an in-core store (0: direct)")
+    ap.add_argument("--no-subleg", action="store_true", help="serving: skip the C10H22 single-molecule sub-leg")
+    ap.add_argument("--legs", default="", help="single-molecule workloads: fp64_direct,fp64_incore,mixed_direct,"
+                    "mixed_incore (default all)")
+    ap.add_argument("--max-iterations", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
@@ -258,141 +421,24 @@ def main():
         copy_calibration()
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (one molecule's SCF per GPU; "
-              "multi-GPU J/K sharding is a design note, not implemented). Use --gpus 1.", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only (serving and single-molecule SCF per GPU; multi-GPU J/K "
+              "sharding is a design note, not implemented). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if a.workload == "serving":
-        serving(a)
+        global SERVE_SLOTS
+        SERVE_SLOTS = a.slots
+        serving_bench(a)
         return
-    if a.workload in ("water32", "eicosyl"):
-        large(a)
-        return
-    import hartreefock_amd as hf
-    label, bfile, atoms = workload(a.workload)
-    # exact integrals: the reference's (dd|dd) transfer convention makes this SCF diverge
-    # untimed probe SCF with every class kernel timed: contracted ERIs per SCF, the kernels' share
-    # and the dominant class (HIP events around all 21 launches of a build cost it several per cent)
-    probe = hf.Engine(schwarz_threshold=1e-12, profile=1, exact_transfer=1).set_molecule(bfile, atoms)
-    probe.scf(**SCF_KW)
-    probe.reset_kernel_stats()
-    t0 = time.perf_counter()
-    probe.scf(**SCF_KW)
-    probe_s = time.perf_counter() - t0
-    pstats = probe.kernel_stats()
-    eris_per_scf = sum(s.eris for s in pstats)
-    kernel_share = sum(s.ms for s in pstats) * 1e-3 / probe_s
-    top = max(pstats, key=lambda s: s.ms)
-    key = top.la * 1000 + top.lb * 100 + top.lc * 10 + top.ld
-    probe.close()
-    # timed region: no events
-    eng = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1).set_molecule(bfile, atoms)
-    elapsed, lat, res = timed_scfs(eng, a.steps, a.warmup)
-    energies = [r.energy for r in res]
-    iters = [r.iterations for r in res]
-    nbf = eng.nbf
-    eng.close()
-    # roofline pass (two more SCFs): only the dominant class carries HIP events, and it is launched
-    # first in each build, so the events bracket its execution rather than a wait for CUs
-    rl = hf.Engine(schwarz_threshold=1e-12, profile=1, profile_class=key, exact_transfer=1).set_molecule(bfile, atoms)
-    rl.scf(**SCF_KW)
-    rl.reset_kernel_stats()
-    rl.scf(**SCF_KW)
-    dom = [s for s in rl.kernel_stats() if s.ms > 0 and not s.fp32][0]
-    rl.close()
-    per_launch_flops = dom.flops / dom.launches
-    per_launch_ms = dom.ms / dom.launches
-    achieved = per_launch_flops / (per_launch_ms * 1e-3) / 1e12
-    lat_sorted = sorted(lat)
-    # HBM bytes per launch of the dominant kernel: FETCH_SIZE + WRITE_SIZE from the committed
-    # rocprofv3 --pmc runs on this workload (calibrated against a 1 GiB copy), when available
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_decane_631gs.json")
-    if a.workload == "decane" and os.path.exists(tfile):
-        with open(tfile) as fh:
-            rec = json.load(fh)["kernels"].get(dom.name())
-        traffic = rec["total"] if rec else None
-
-    def pct(p):
-        k = min(len(lat_sorted) - 1, max(0, int(round(p / 100.0 * (len(lat_sorted) - 1)))))
-        return lat_sorted[k] * 1e3
-
-    out = {
-        "metric": "SCF molecules/s (plain FP64 direct RHF on GPU)",
-        "value": a.steps / elapsed,
-        "unit": "molecules/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": elapsed / a.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded geometry)",
-        "config": {"workload": label, "nbf": nbf, "schwarz_threshold": 1e-12, "energy_tolerance": 1e-8,
-                   "eri_convention": "exact",
-                   "scf": "alpha=0.5, core guess, DIIS(6/5), normalIterAfterDIIS=0", "parallelism": "single GPU"},
-        "energy_Eh": energies[-1],
-        "iterations": iters[-1],
-        "latency_ms": {"p50": pct(50), "p99": pct(99)},
-        "eri_per_s": eris_per_scf * a.steps / elapsed,
-        "eri_kernel_share": kernel_share,
-        "jk_ms_per_build": sum(r.jk_ms for r in res) / max(1, sum(r.full_builds + r.incremental_builds for r in res)),
-        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "kernel": "eri_class_kernel" + dom.name() + " (J/K)",
-                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "flops_per_launch": per_launch_flops, "ms_per_launch": per_launch_ms,
-                     "launches": dom.launches},
-    }
-    # in-core leg: the ERIs evaluated once per SCF into a pair-matrix store in HBM (inside the timed
-    # step), every J/K build contracts the store; roofline of the contraction kernel from its HIP
-    # events in a profiled SCF (one launch per build on the SCF stream)
-    ei = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1).set_molecule(bfile, atoms)
-    el_i, lat_i, res_i = timed_scfs(ei, a.steps, a.warmup, incore=1)
-    ei.close()
-    if res_i[-1].incore:
-        ep = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, profile=1).set_molecule(bfile, atoms)
-        ep.scf(incore=1, **SCF_KW)
-        ep.reset_kernel_stats()
-        ep.scf(incore=1, **SCF_KW)
-        ist = [s for s in ep.kernel_stats() if s.la < 0 and not s.fp32][0]
-        ep.close()
-        bytes_launch = ist.eris / ist.launches * 8.0
-        ms_launch = ist.ms / ist.launches
-        itraffic = None
-        if a.workload == "decane" and os.path.exists(tfile):
-            with open(tfile) as fh:
-                rec = json.load(fh)["kernels"].get("incore_jk")
-            itraffic = rec["total"] if rec else None
-        out["incore"] = {
-            "value": a.steps / el_i, "unit": "molecules/s", "vs_direct": (a.steps / el_i) / out["value"],
-            "abs_dE_vs_direct_Eh": abs(res_i[-1].energy - energies[-1]), "iterations": res_i[-1].iterations,
-            "eri_store_ms": res_i[-1].eri_ms, "store_bytes": bytes_launch,
-            "jk_ms_per_build": res_i[-1].jk_ms / max(1, res_i[-1].full_builds + res_i[-1].incremental_builds),
-            "roofline": {"bound": "hbm", "kernel": "incore_jk_kernel (J/K from the store)",
-                         "achieved": bytes_launch / (ms_launch * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": bytes_launch / (ms_launch * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": itraffic,
-                         "ms_per_launch": ms_launch, "launches": ist.launches}}
-    if not a.no_mixed:
-        # mixed-precision tier: quartets with Schwarz bound < 1e-6 in FP32, final iterations FP64
-        em = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, precision=1,
-                       fp32_threshold=1e-6).set_molecule(bfile, atoms)
-        el_m, lat_m, res_m = timed_scfs(em, a.steps, a.warmup)
-        out["mixed"] = {"value": a.steps / el_m, "unit": "molecules/s", "vs_fp64": (a.steps / el_m) / out["value"],
-                        "abs_dE_vs_fp64_Eh": abs(res_m[-1].energy - energies[-1]), "iterations": res_m[-1].iterations,
-                        "fp32_builds": res_m[-1].fp32_builds, "fp32_threshold": 1e-6, "final_fp64_iterations": 3}
-        # mixed tier with the in-core store: an FP32 copy of the store until the final FP64 steps
-        emi = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, precision=1,
-                        fp32_threshold=1e-6).set_molecule(bfile, atoms)
-        el_mi, lat_mi, res_mi = timed_scfs(emi, a.steps, a.warmup, incore=1)
-        emi.close()
-        out["mixed_incore"] = {"value": a.steps / el_mi, "unit": "molecules/s",
-                               "vs_fp64_direct": (a.steps / el_mi) / out["value"],
-                               "abs_dE_vs_fp64_Eh": abs(res_mi[-1].energy - energies[-1]),
-                               "iterations": res_mi[-1].iterations, "fp32_builds": res_mi[-1].fp32_builds,
-                               "incore": res_mi[-1].incore}
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(bfile, atoms, iters[-1], a.cpu_budget)
+    res = single_molecule(a.workload, a.steps, a.warmup, cpu=not a.no_cpu_baseline, cpu_budget=a.cpu_budget,
+                          max_iterations=a.max_iterations, legs=a.legs)
+    d = res["fp64_direct"] if isinstance(res.get("fp64_direct"), dict) else next(
+        v for v in res.values() if isinstance(v, dict))
+    out = {"metric": "SCF molecules/s (plain FP64 direct on GPU)", "value": d["molecules_per_s"], "unit": "molecules/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": d["ms_per_scf"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded geometry)",
+           "config": {"workload": res["workload"], "nbf": res.get("nbf"), "scf": SCF_DESC, "eri_convention": "exact",
+                      "schwarz_threshold": 1e-12, "parallelism": "single GPU"},
+           "roofline": d.get("roofline"), "cpu_baseline": res.pop("cpu_baseline", None), "legs": res}
     print(json.dumps(out), flush=True)
 
 

@@ -17,7 +17,7 @@ EXPORTED_SYMBOLS = [
     "hf_properties",
     "hf_scf_defaults", "hf_scf", "hf_kernel_stats", "hf_kernel_stats_reset",
     "hf_serve_default_options", "hf_serve_create", "hf_serve_submit", "hf_serve_poll", "hf_serve_pending",
-    "hf_serve_footprint", "hf_serve_destroy",
+    "hf_serve_footprint", "hf_serve_destroy", "hf_serve_kernel_stats",
 ]
 
 
@@ -147,6 +147,7 @@ def lib():
                                   ctypes.POINTER(ctypes.c_uint64)]
     L.hf_serve_poll.argtypes = [vp, ctypes.POINTER(ServeResult), ctypes.c_int, ctypes.c_int]
     L.hf_serve_pending.argtypes = [vp]
+    L.hf_serve_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int, ctypes.c_int]
     L.hf_serve_footprint.argtypes = [ctypes.c_int, ctypes.c_int]
     L.hf_serve_footprint.restype = ctypes.c_double
     L.hf_serve_destroy.argtypes = [vp]
@@ -384,6 +385,12 @@ class Server:
 
     def pending(self):
         return lib().hf_serve_pending(self.h)
+
+    def kernel_stats(self, reset=False):
+        """Per-class kernel statistics of the requests finished so far, summed over the slots."""
+        arr = (KernelStat * 128)()
+        k = lib().hf_serve_kernel_stats(self.h, arr, 128, 1 if reset else 0)
+        return [arr[i] for i in range(k)]
 
     def close(self):
         if getattr(self, "h", None):

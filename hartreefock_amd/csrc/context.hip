@@ -228,7 +228,7 @@ int run_classes(hf_ctx* ctx, const std::vector<std::pair<int, const TaskSet*>>& 
         const TaskList tl = task_list(ctx, *L.ts, qc);
         const int slot = stat_slot(L.mode, qc.cls);
         const int key = ci.la * 1000 + ci.lb * 100 + ci.lc * 10 + ci.ld;
-        const bool timed = prof && (ctx->opt.profile_class < 0 || ctx->opt.profile_class == key);
+        const bool timed = prof && (ctx->opt.profile_class == -1 || ctx->opt.profile_class == key);
         if (timed) HF_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2 * slot], ks));
         hipError_t e = ci.fn(L.mode, ctx->opt.exact_transfer, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ks);
         if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("ERI kernel launch: ") + hipGetErrorString(e));

@@ -51,6 +51,30 @@ struct hf_server {
     // J/K scratch, an in-core store) for the next request.
     std::vector<double> charged;
     std::vector<char> busy;
+    // kernel statistics of every finished request, summed over the slots (hf_serve_kernel_stats)
+    std::vector<hf_kernel_stat> stats;
+    void add_stats(hf_ctx* ctx) {   // caller holds mu; ctx belongs to the calling worker
+        hf_kernel_stat buf[128];
+        const int k = hf_kernel_stats(ctx, buf, 128);
+        for (int i = 0; i < k; ++i) {
+            hf_kernel_stat* t = nullptr;
+            for (hf_kernel_stat& x : stats)
+                if (x.la == buf[i].la && x.lb == buf[i].lb && x.lc == buf[i].lc && x.ld == buf[i].ld &&
+                    x.fp32 == buf[i].fp32)
+                    t = &x;
+            if (!t) {
+                stats.push_back(buf[i]);
+                continue;
+            }
+            t->launches += buf[i].launches;
+            t->quartets += buf[i].quartets;
+            t->prim_quartets += buf[i].prim_quartets;
+            t->eris += buf[i].eris;
+            t->flops += buf[i].flops;
+            t->ms += buf[i].ms;
+        }
+        (void)hf_kernel_stats_reset(ctx);
+    }
     double budget = 0, keep_store = 0;
     uint64_t next_id = 1;
     int pending = 0;
@@ -138,6 +162,7 @@ struct hf_server {
                 std::lock_guard<std::mutex> lock(mu);
                 charged[w] = held;
                 busy[w] = 0;
+                add_stats(ctx);
                 done.push_back(r);
             }
             cv_done.notify_all();
@@ -238,6 +263,16 @@ int hf_serve_poll(hf_server* s, hf_serve_result* out, int max, int wait_ms) {
         s->done.pop_front();
         --s->pending;
     }
+    return k;
+}
+
+int hf_serve_kernel_stats(hf_server* s, hf_kernel_stat* out, int max, int reset) {
+    if (!s || !out || max <= 0) return 0;
+    std::lock_guard<std::mutex> lock(s->mu);
+    int k = 0;
+    for (const hf_kernel_stat& x : s->stats)
+        if (k < max) out[k++] = x;
+    if (reset) s->stats.clear();
     return k;
 }
 

@@ -48,8 +48,9 @@ typedef struct hf_options {
     double fp32_threshold;     /* mixed: quartets whose bound is below this run in FP32 */
     int profile;               /* 1: time class kernels with HIP events (hf_kernel_stats) */
     int profile_class;         /* with profile = 1: time only the class la*1000 + lb*100 + lc*10 + ld
-                                  (-1, the default: every class); each timed launch is bracketed by
-                                  two events, which cost the build a few per cent */
+                                  (-1, the default: every class; -2: no class kernel, only the in-core
+                                  contraction); each timed launch is bracketed by two events, which cost
+                                  the build a few per cent */
 } hf_options;
 
 void hf_default_options(hf_options* opt);
@@ -272,6 +273,10 @@ int hf_serve_submit(hf_server* s, const char* basis_file, int natoms, const int*
                     int unrestricted, uint64_t tag, uint64_t* id);
 /* Up to max finished results, waiting up to wait_ms for the first one; returns the count. */
 int hf_serve_poll(hf_server* s, hf_serve_result* out, int max, int wait_ms);
+/* Kernel statistics (as hf_kernel_stats) of every request finished since the server started or since the
+ * last call with reset = 1, summed over the slots; returns the count.  With profile = 1 in the slots'
+ * context options the times come from HIP events on each slot's streams. */
+int hf_serve_kernel_stats(hf_server* s, hf_kernel_stat* out, int max, int reset);
 /* Requests submitted and not yet returned by hf_serve_poll. */
 int hf_serve_pending(hf_server* s);
 /* Estimated device bytes of an SCF on n basis functions (the admission rule); a server whose SCF

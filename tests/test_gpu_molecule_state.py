@@ -70,8 +70,7 @@ ION_CASES = [
     # (name, basis, atoms, alpha, beta, unrestricted)
     ("oh_anion_rhf", "sto3g.txt", [(8, (0.0, 0.0, 0.0)), (1, (0.0, 0.0, 1.83))], 5, 5, False),
     ("h2o_cation_uhf", "sto3g.txt", g.water_cfg1(), 5, 4, True),
-    ("ch2_triplet_uhf_sto3g", "sto3g.txt", [(6, (0.0, 0.0, 0.0)), (1, (0.0, 1.64, 1.09)), (1, (0.0, -1.64, 1.09))],
-     4, 2, True),
+
     ("ch2_triplet_uhf_631gs", "6-31g_st_.1.nw", [(6, (0.0, 0.0, 0.0)), (1, (0.0, 1.64, 1.09)), (1, (0.0, -1.64, 1.09))],
      4, 2, True),
 ]

@@ -147,6 +147,35 @@ def test_mixed_jk_close_to_fp64(gpu_engine_factory, name, bfile, atoms):
     assert dj <= 1e-7 and dk <= 1e-7, (dj, dk)
 
 
+MIXED_ORACLE_CASES = [
+    ("ethane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(2, seed=5)),
+    ("water2_631gss", "6-31g_st__st_.nw", g.water_cluster(2, seed=2024)),
+]
+
+
+@pytest.mark.parametrize("name,bfile,atoms", MIXED_ORACLE_CASES, ids=[c[0] for c in MIXED_ORACLE_CASES])
+def test_mixed_tier_matches_oracle(gpu_engine_factory, name, bfile, atoms):
+    """The mixed-precision tier against the oracle rather than the FP64 GPU path: J/K of the oracle's converged
+    density within 1e-7 of the oracle's N^4 loop over its own ERIs, and the mixed SCF energy within 1e-6 Eh
+    (the tier's stated bound) of the oracle's SCF."""
+    _print_case(name)
+    mol = refcpu.Molecule(basis(bfile), atoms)
+    ref, Dref = mol.scf(**MIXED_KW)
+    eri = mol.eri_packed()
+    n = mol.nbf
+    Jr, Kr = refcpu.jk_from_packed(n, eri, Dref[0])
+    e32 = gpu_engine_factory(precision=1, fp32_threshold=1e-6).set_molecule(basis(bfile), atoms)
+    e32.reset_kernel_stats()
+    J, K = e32.build_jk(Dref[0])
+    assert sum(s.quartets for s in e32.kernel_stats() if s.fp32) > 0
+    dj, dk = np.abs(J - Jr).max(), np.abs(K - Kr).max()
+    r = e32.scf(**MIXED_KW)
+    print("mixed vs oracle: max|dJ| %.2e max|dK| %.2e  E %.12f oracle %.12f" % (dj, dk, r.energy, ref.energy),
+          flush=True)
+    assert dj <= 1e-7 and dk <= 1e-7
+    assert r.converged and r.fp32_builds > 0 and abs(r.energy - ref.energy) <= 1e-6
+
+
 @pytest.mark.parametrize("name,bfile,atoms", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
 def test_mixed_scf_energy_matches_fp64(gpu_engine_factory, name, bfile, atoms):
     """Mixed-precision SCF: FP32 band until the criteria are nearly met, then all-FP64 steps (at
@@ -234,6 +263,7 @@ def test_serving_admission_by_footprint(gpu_engine_factory):
 SCALE_CASES = [
     ("pentane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(5, seed=31)),       # 99 functions, d shells
     ("water6_631gss", "6-31g_st__st_.nw", g.water_cluster(6, seed=32)),     # 150 functions, p on H
+    ("pentyl_631pgs", "6-31+g_st_.1.nw", g.alkane_zigzag(5, seed=33, drop_last_h=True)),   # 117, diffuse sp on C
 ]
 
 
