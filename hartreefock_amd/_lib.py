@@ -295,6 +295,16 @@ class Engine:
         self._check(lib().hf_build_jk_incore(self.h, nd, _dp(D3), _dp(J), _dp(K)))
         return J, (K[0] if D.ndim == 2 else K)
 
+    def build_jk_incore_device(self, d_G, d_D, d_J, d_K, ndens=1, stream=None):
+        """In-core J/K from the device store d_G (hf_eri_pairs_device layout); device pointers as ints."""
+        f = lib().hf_build_jk_incore_device
+        self._check(f(self.h, ndens, ctypes.c_void_p(d_G), ctypes.c_void_p(d_D), ctypes.c_void_p(d_J),
+                      ctypes.c_void_p(d_K), ctypes.c_void_p(stream or 0)))
+
+    def eri_packed_device(self, d_out, length, stream=None):
+        """Packed unique ERIs into the device buffer d_out (hf_packed_len(n) doubles)."""
+        self._check(lib().hf_eri_packed_device(self.h, ctypes.c_void_p(d_out), length, ctypes.c_void_p(stream or 0)))
+
     def eri_pairs_device(self, d_G, length, stream=None):
         """Fill the device pair matrix G (hf_pair_matrix_len(n) doubles) with the ERIs."""
         self._check(lib().hf_eri_pairs_device(self.h, ctypes.c_void_p(d_G), length, ctypes.c_void_p(stream or 0)))

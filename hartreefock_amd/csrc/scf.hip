@@ -898,6 +898,11 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                 if (int rc = gemm(ctx, false, true, n, n, occ[k], scale, st.C.p + k * nn, st.C.p + k * nn, 0.0, Dk))
                     return rc;
             }
+            // The reference's density 2 C_occ C_occ^T is symmetric to the last bit (element (i,j) and (j,i) are the
+            // same products summed in the same order); X P X from tiled matrix-core products is not, and with an
+            // ill-conditioned S (diffuse functions: large elements in X) the two triangles differ by ~1e-10.  The
+            // J/K builds read one triangle or the other, so the density is made exactly symmetric here.
+            if (occ[k] != 0) hipLaunchKernelGGL(sym_kernel, dim3(grid_for(nn)), dim3(256), 0, s, Dk, n);
         }
         // energy E = (D.h + D.F)/2 (sum over spins; D.F is twice (RHF) or once (UHF) the sum of the
         // occupied orbital energies) and the density change
