@@ -657,6 +657,22 @@ int hf_eri_pairs_impl(hf_ctx* ctx, double* d_G, uint64_t len, hipStream_t st) {
     return run_classes(ctx, MODE_PACKED, ctx->tasks_incore, sink, st, true);
 }
 
+int hf_stream_enter(hf_ctx* ctx, hipStream_t st) {
+    if (st == ctx->stream) return HF_OK;
+    for (hipEvent_t& e : ctx->order_ev)
+        if (!e) HF_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HF_HIP_CHECK(ctx, hipEventRecord(ctx->order_ev[0], ctx->stream));
+    HF_HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->order_ev[0], 0));
+    return HF_OK;
+}
+
+int hf_stream_leave(hf_ctx* ctx, hipStream_t st) {
+    if (st == ctx->stream) return HF_OK;
+    HF_HIP_CHECK(ctx, hipEventRecord(ctx->order_ev[1], st));
+    HF_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->order_ev[1], 0));
+    return HF_OK;
+}
+
 extern "C" {
 
 void hf_default_options(hf_options* o) {
@@ -707,6 +723,8 @@ int hf_destroy(hf_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->side_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->order_ev)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->incore_ev)
         if (e) (void)hipEventDestroy(e);
     for (hipStream_t s2 : ctx->side) (void)hipStreamDestroy(s2);
@@ -802,12 +820,14 @@ int hf_eri_packed_device(hf_ctx* ctx, double* d_out, uint64_t len, void* stream)
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     int rc = ensure_tasks(ctx, 0.0, ctx->tasks_all, 0.0, false);   // packed store: whole quartets
     if (rc) return rc;
+    if ((rc = hf_stream_enter(ctx, st))) return rc;
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(len)), dim3(256), 0, st, d_out, (size_t)len, 0.0);
     SinkArgs sink{};
     sink.n = n;
     sink.packed = d_out;
     sink.packed_len = len;
-    return run_classes(ctx, MODE_PACKED, ctx->tasks_all, sink, st, true);
+    if ((rc = run_classes(ctx, MODE_PACKED, ctx->tasks_all, sink, st, true))) return rc;
+    return hf_stream_leave(ctx, st);
 }
 
 int hf_eri_packed(hf_ctx* ctx, double* out, uint64_t len) {
@@ -830,7 +850,13 @@ uint64_t hf_pair_matrix_len(int nbf) {
 
 int hf_eri_pairs_device(hf_ctx* ctx, double* d_G, uint64_t len, void* stream) {
     if (!ctx) return HF_E_ARG;
-    return hf_eri_pairs_impl(ctx, d_G, len, stream ? (hipStream_t)stream : ctx->stream);
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    // the task list is built (and synchronised) on the context's stream before the caller's stream is ordered
+    if (int rc = ensure_tasks(ctx, ctx->opt.schwarz_threshold, ctx->tasks_incore, 0.0, false)) return rc;
+    if (int rc = hf_stream_enter(ctx, st)) return rc;
+    if (int rc = hf_eri_pairs_impl(ctx, d_G, len, st)) return rc;
+    return hf_stream_leave(ctx, st);
 }
 
 int hf_build_jk_incore_device(hf_ctx* ctx, int ndens, const double* d_G, const double* d_D, double* d_J, double* d_K,
@@ -838,8 +864,10 @@ int hf_build_jk_incore_device(hf_ctx* ctx, int ndens, const double* d_G, const d
     if (!ctx) return HF_E_ARG;
     if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
     if (!d_G || !d_D || !d_J || !d_K) return ctx->fail(HF_E_ARG, "null device buffer");
-    return hf_incore_jk_impl(ctx, ndens, d_G, nullptr, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream,
-                             false);
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    if (int rc = hf_stream_enter(ctx, st)) return rc;
+    if (int rc = hf_incore_jk_impl(ctx, ndens, d_G, nullptr, d_D, d_J, d_K, st, false)) return rc;
+    return hf_stream_leave(ctx, st);
 }
 
 int hf_build_jk_incore(hf_ctx* ctx, int ndens, const double* D, double* J, double* K) {
@@ -866,9 +894,20 @@ int hf_build_jk_incore(hf_ctx* ctx, int ndens, const double* D, double* J, doubl
 
 int hf_build_jk_device(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, double* d_K, void* stream) {
     if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
     if (!d_D || !d_J || !d_K) return ctx->fail(HF_E_ARG, "null device buffer");
-    return hf_jk_device_impl(ctx, ndens, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream, 0.0, false,
-                             ctx->opt.precision == HF_PREC_MIXED);
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    // task lists are built (and synchronised) on the context's stream before the caller's stream is ordered
+    TaskSet *ts = nullptr, *lo = nullptr;
+    if (ctx->opt.precision == HF_PREC_MIXED) {
+        if (int rc = mixed_tasks(ctx, ctx->opt.schwarz_threshold, ctx->opt.fp32_threshold, ts, lo)) return rc;
+    } else if (int rc = screened_tasks(ctx, ctx->opt.schwarz_threshold, ts)) {
+        return rc;
+    }
+    if (int rc = hf_stream_enter(ctx, st)) return rc;
+    if (int rc = hf_jk_device_impl(ctx, ndens, d_D, d_J, d_K, st, 0.0, false, ctx->opt.precision == HF_PREC_MIXED))
+        return rc;
+    return hf_stream_leave(ctx, st);
 }
 
 int hf_build_jk(hf_ctx* ctx, int ndens, const double* D, double* J, double* K) {

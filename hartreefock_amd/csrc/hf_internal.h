@@ -177,6 +177,9 @@ struct hf_ctx {
     // back to the caller's stream with events
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> side_ev;   // [0] fork, [1..] join per side stream
+    // A device entry point called on a stream other than the context's own: the call's work is ordered after
+    // the context stream's queued work and before any later work on it (they share the context's scratch)
+    hipEvent_t order_ev[2] = {nullptr, nullptr};
     double last_jk_ms = 0;
 
     int fail(int code, const std::string& msg) {
@@ -232,6 +235,11 @@ int hf_mp2_impl(hf_ctx* ctx, hipStream_t st, int nocc, const double* C, const do
                 bool same_spin, double& e2);
 // Mulliken charges (natoms) and dipole (3) from the device densities d_D (ndens x n x n).
 int hf_properties_impl(hf_ctx* ctx, int ndens, const double* d_D, double* charges, double* dipole, hipStream_t st);
+
+// Orders work on a caller's stream st against the context's stream (both directions; no host wait): enter before
+// the call's first launch, leave after its last.  Nothing to do when st is the context's stream.
+int hf_stream_enter(hf_ctx* ctx, hipStream_t st);
+int hf_stream_leave(hf_ctx* ctx, hipStream_t st);
 
 #define HF_HIP_CHECK(ctx, call)                                                                         \
     do {                                                                                                \
