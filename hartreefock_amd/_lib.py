@@ -14,6 +14,7 @@ EXPORTED_SYMBOLS = [
     "hf_nuclear_electric_field_energy", "hf_packed_index", "hf_packed_len",
     "hf_eri_packed", "hf_eri_packed_device", "hf_build_jk", "hf_build_jk_device", "hf_one_electron",
     "hf_pair_matrix_len", "hf_eri_pairs_device", "hf_build_jk_incore_device", "hf_build_jk_incore",
+    "hf_build_jk_incore_device_noalloc",
     "hf_properties",
     "hf_scf_defaults", "hf_scf", "hf_kernel_stats", "hf_kernel_stats_reset",
     "hf_serve_default_options", "hf_serve_create", "hf_serve_submit", "hf_serve_poll", "hf_serve_pending",
@@ -133,6 +134,7 @@ def lib():
     L.hf_eri_pairs_device.argtypes = [vp, vp, ctypes.c_uint64, vp]
     L.hf_build_jk_incore_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
     L.hf_build_jk_incore.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+    L.hf_build_jk_incore_device_noalloc.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
     L.hf_one_electron.argtypes = [vp, dp, dp, dp]
     L.hf_properties.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.hf_scf_defaults.argtypes = [ctypes.POINTER(ScfParams)]
@@ -295,9 +297,10 @@ class Engine:
         self._check(lib().hf_build_jk_incore(self.h, nd, _dp(D3), _dp(J), _dp(K)))
         return J, (K[0] if D.ndim == 2 else K)
 
-    def build_jk_incore_device(self, d_G, d_D, d_J, d_K, ndens=1, stream=None):
-        """In-core J/K from the device store d_G (hf_eri_pairs_device layout); device pointers as ints."""
-        f = lib().hf_build_jk_incore_device
+    def build_jk_incore_device(self, d_G, d_D, d_J, d_K, ndens=1, stream=None, no_alloc=False):
+        """In-core J/K from the device store d_G (hf_eri_pairs_device layout); device pointers as ints.  no_alloc:
+        the allocation-free entry point (n <= 512, scratch shared by every call on the GPU)."""
+        f = lib().hf_build_jk_incore_device_noalloc if no_alloc else lib().hf_build_jk_incore_device
         self._check(f(self.h, ndens, ctypes.c_void_p(d_G), ctypes.c_void_p(d_D), ctypes.c_void_p(d_J),
                       ctypes.c_void_p(d_K), ctypes.c_void_p(stream or 0)))
 

@@ -708,6 +708,13 @@ int hf_create(const hf_options* opt, hf_ctx** out) {
         return HF_E_HIP;
     }
     rocblas_set_stream(ctx->blas, ctx->stream);
+    // the allocation-free entry point's scratch on this device (its address, taken once per device)
+    if (hf_incore_scratch_init(ctx) != HF_OK) {
+        rocblas_destroy_handle(ctx->blas);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return HF_E_HIP;
+    }
     // pinned buffer for the SCF's small read-backs (without it they go through pageable memory)
     if (hipHostMalloc((void**)&ctx->hpin, kPinnedDoubles * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         ctx->hpin = nullptr;

@@ -215,6 +215,8 @@ int hf_eri_pairs_impl(hf_ctx* ctx, double* d_G, uint64_t len, hipStream_t st);
 // K per density, both complete (no symmetrisation needed); accumulate adds to d_J/d_K.
 int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
                       double* d_K, hipStream_t st, bool accumulate);
+// Takes the device address of this device's allocation-free scratch array (once per device; hf_create).
+int hf_incore_scratch_init(hf_ctx* ctx);
 // Frees the SCF workspace (hf_destroy).
 void hf_scf_workspace_free(hf_ctx* ctx);
 // Device bytes the context holds between calls: SCF workspace, in-core stores, J/K scratch.

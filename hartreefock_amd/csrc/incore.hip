@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <string>
 
 #include "hf_internal.h"
 #include "wave_reduce.h"
@@ -293,10 +294,19 @@ IncoreTuning& tuning() {
     return t;
 }
 
+int tuning_key(int n, int ndens, bool fp32) { return ((n + 127) / 128) * 100 + ndens * 10 + (fp32 ? 1 : 0); }
+
+// The shape the autotuner picked for this kind of build, or the default when it has not run (no timing here).
+int known_shape(int n, int ndens, bool fp32) {
+    std::lock_guard<std::mutex> lock(tuning().mu);
+    auto it = tuning().shape.find(tuning_key(n, ndens, fp32));
+    return it == tuning().shape.end() ? (int)kRows2 : it->second;
+}
+
 template <typename T>
 int tuned_shape(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double* D, const double* Dw,
                hipStream_t st) {
-    const int key = ((n + 127) / 128) * 100 + ndens * 10 + (sizeof(T) == 4 ? 1 : 0);
+    const int key = tuning_key(n, ndens, sizeof(T) == 4);
     {
         std::lock_guard<std::mutex> lock(tuning().mu);
         auto it = tuning().shape.find(key);
@@ -334,7 +344,30 @@ int tuned_shape(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double*
     return best;
 }
 
+// Scratch of the allocation-free entry point (hf_build_jk_incore_device_noalloc): the packed weighted density
+// Dw of one build, np = pair_pad_base(n) doubles, n <= 512.  One copy per GPU, for the whole process.  Host code
+// never uses the array's name as an address: hf_incore_scratch_init takes the device address of each GPU's
+// copy once, with that GPU current, and the entry point passes that address to the kernels.
+constexpr int64_t kNoAllocMaxPairs = 131584;          // pair_pad_base(512): 1.0 MiB
+__device__ __align__(256) double scratch[kNoAllocMaxPairs];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
+
+int hf_incore_scratch_init(hf_ctx* ctx) {
+    const int dev = ctx->opt.device;
+    if (dev < 0 || dev >= kMaxDevices) return ctx->fail(HF_E_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return HF_OK;
+    HF_HIP_CHECK(ctx, hipSetDevice(dev));
+    void* p = nullptr;
+    HF_HIP_CHECK(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)p;
+    return HF_OK;
+}
 
 void hf_release_incore(hf_ctx* ctx, size_t keep_bytes) {
     if (ctx->d_G.n * sizeof(double) + ctx->d_G32.n * sizeof(float) > keep_bytes) {
@@ -343,27 +376,28 @@ void hf_release_incore(hf_ctx* ctx, size_t keep_bytes) {
     }
 }
 
-int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
-                      double* d_K, hipStream_t st, bool accumulate) {
-    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
-    if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
+namespace {
+
+// One in-core J/K build with the packed weighted density in Dw (np doubles).  no_alloc: the tile shape is the one
+// already tuned (or the default), and no events are created or recorded, so nothing is allocated.
+int incore_jk_run(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
+                  double* d_K, hipStream_t st, bool accumulate, double* Dw, bool no_alloc) {
     const int n = ctx->mol.nbf;
-    if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "in-core J/K takes at most 512 basis functions");
     const int64_t np = hfgpu::pair_pad_base(n);      // padded pair positions (rows and columns of the store)
     const size_t nn = (size_t)n * n;
-    HF_HIP_CHECK(ctx, ctx->d_Dw.alloc((size_t)np));
-    hipLaunchKernelGGL(incore_dw_kernel, dim3(n), dim3(256), 0, st, d_D, n, ndens, ctx->d_Dw.p);
+    hipLaunchKernelGGL(incore_dw_kernel, dim3(n), dim3(256), 0, st, d_D, n, ndens, Dw);
     if (!accumulate) {
         hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, d_J, nn);
         hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((ndens * nn + 255) / 256)), dim3(256), 0, st, d_K,
                            ndens * nn);
     }
     // tile shape from the autotuner (first build of each kind in the process times the candidates)
-    const int shape = g32 ? tuned_shape(ctx, ndens, g32, n, np, d_D, ctx->d_Dw.p, st)
-                          : tuned_shape(ctx, ndens, G, n, np, d_D, ctx->d_Dw.p, st);
+    const int shape = no_alloc ? known_shape(n, ndens, g32 != nullptr)
+                      : g32    ? tuned_shape(ctx, ndens, g32, n, np, d_D, Dw, st)
+                               : tuned_shape(ctx, ndens, G, n, np, d_D, Dw, st);
     // statistics, and with profile = 1 events around the contraction (read at the next flush)
     const int f = g32 ? 1 : 0;
-    const bool prof = ctx->opt.profile != 0;
+    const bool prof = ctx->opt.profile != 0 && !no_alloc;
     if (prof) {
         if (ctx->incore_pending >= 0) {
             HF_HIP_CHECK(ctx, hipEventSynchronize(ctx->incore_ev[1]));
@@ -382,13 +416,41 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
     ks.flops += (double)np * (double)np * (2.0 + 8.0 * ndens);           // J, and row + column parts of K
     hipError_t e;
     if (g32)
-        e = launch_incore_shape(shape, ndens, g32, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
+        e = launch_incore_shape(shape, ndens, g32, n, np, d_D, Dw, d_J, d_K, st);
     else
-        e = launch_incore_shape(shape, ndens, G, n, np, d_D, ctx->d_Dw.p, d_J, d_K, st);
+        e = launch_incore_shape(shape, ndens, G, n, np, d_D, Dw, d_J, d_K, st);
     if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("in-core J/K: ") + hipGetErrorString(e));
     if (prof) {
         HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[1], st));
         ctx->incore_pending = f;
     }
     return HF_OK;
+}
+
+}  // namespace
+
+int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
+                      double* d_K, hipStream_t st, bool accumulate) {
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
+    const int n = ctx->mol.nbf;
+    if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "in-core J/K takes at most 512 basis functions");
+    HF_HIP_CHECK(ctx, ctx->d_Dw.alloc((size_t)hfgpu::pair_pad_base(n)));
+    return incore_jk_run(ctx, ndens, G, g32, d_D, d_J, d_K, st, accumulate, ctx->d_Dw.p, false);
+}
+
+extern "C" int hf_build_jk_incore_device_noalloc(hf_ctx* ctx, int ndens, const double* d_G, const double* d_D,
+                                                 double* d_J, double* d_K, void* stream) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!d_G || !d_D || !d_J || !d_K) return ctx->fail(HF_E_ARG, "null device buffer");
+    if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
+    const int n = ctx->mol.nbf;
+    if (hfgpu::pair_pad_base(n) > kNoAllocMaxPairs)
+        return ctx->fail(HF_E_ARG, "n = " + std::to_string(n) + " exceeds the allocation-free scratch (n <= 512)");
+    const int dev = ctx->opt.device;
+    double* Dw = (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
+    if (!Dw) return ctx->fail(HF_E_STATE, "allocation-free scratch not set up on this device");
+    return incore_jk_run(ctx, ndens, d_G, nullptr, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream, false,
+                         Dw, true);
 }

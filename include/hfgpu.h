@@ -145,6 +145,15 @@ int hf_build_jk_incore_device(hf_ctx* ctx, int ndens, const double* d_G, const d
                               double* d_K, void* stream);
 /* Host convenience: evaluates the store in the context's memory, then contracts. */
 int hf_build_jk_incore(hf_ctx* ctx, int ndens, const double* D, double* J, double* K);
+/* hf_build_jk_incore_device without any allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that forbid hipMalloc).  The packed
+ * weighted density it needs lives in a static device array of 131584 doubles (1 MiB) per GPU, so n <= 512
+ * (larger n is rejected with HF_E_ARG); the tile shape is the autotuner's if a build of this kind has already
+ * run in the process (else the default shape), and no profiling events are used.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (other streams, other contexts or threads) race and must be
+ * serialised by the caller.  Same results as hf_build_jk_incore_device. */
+int hf_build_jk_incore_device_noalloc(hf_ctx* ctx, int ndens, const double* d_G, const double* d_D, double* d_J,
+                                      double* d_K, void* stream);
 
 /* One-electron matrices S, T, V (HartreeFock/QuantumMatrix.cpp:34-155). Host buffers, n*n. */
 int hf_one_electron(hf_ctx* ctx, double* S, double* T, double* V);
