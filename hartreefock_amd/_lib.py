@@ -15,8 +15,9 @@ EXPORTED_SYMBOLS = [
     "hf_eri_packed", "hf_eri_packed_device", "hf_build_jk", "hf_build_jk_device", "hf_one_electron",
     "hf_pair_matrix_len", "hf_eri_pairs_device", "hf_build_jk_incore_device", "hf_build_jk_incore",
     "hf_build_jk_incore_device_noalloc",
-    "hf_properties",
-    "hf_scf_defaults", "hf_scf", "hf_kernel_stats", "hf_kernel_stats_reset",
+    "hf_properties", "hf_core_hamiltonian", "hf_boys",
+    "hf_scf_defaults", "hf_scf", "hf_scf_begin", "hf_scf_step", "hf_scf_end", "hf_kernel_stats",
+    "hf_kernel_stats_reset",
     "hf_serve_default_options", "hf_serve_create", "hf_serve_submit", "hf_serve_poll", "hf_serve_pending",
     "hf_serve_footprint", "hf_serve_destroy", "hf_serve_kernel_stats",
 ]
@@ -136,10 +137,15 @@ def lib():
     L.hf_build_jk_incore.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.hf_build_jk_incore_device_noalloc.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
     L.hf_one_electron.argtypes = [vp, dp, dp, dp]
+    L.hf_core_hamiltonian.argtypes = [vp, dp]
+    L.hf_boys.argtypes = [ctypes.c_int, ctypes.c_double, dp]
     L.hf_properties.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.hf_scf_defaults.argtypes = [ctypes.POINTER(ScfParams)]
     L.hf_scf_defaults.restype = None
     L.hf_scf.argtypes = [vp, ctypes.POINTER(ScfParams), ctypes.POINTER(ScfResult), dp]
+    L.hf_scf_begin.argtypes = [vp, ctypes.POINTER(ScfParams)]
+    L.hf_scf_step.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+    L.hf_scf_end.argtypes = [vp, ctypes.POINTER(ScfResult), dp]
     L.hf_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int]
     L.hf_kernel_stats_reset.argtypes = [vp]
     L.hf_serve_default_options.argtypes = [ctypes.POINTER(ServeOptions)]
@@ -322,6 +328,13 @@ class Engine:
         self._check(lib().hf_properties(self.h, nd, _dp(D), _dp(q), _dp(mu)))
         return q, mu
 
+    def core_hamiltonian(self):
+        """h = T + V + E . M, the matrix the SCF starts from (HartreeFockAlgorithm::h)."""
+        n = self.nbf
+        h = np.zeros((n, n))
+        self._check(lib().hf_core_hamiltonian(self.h, _dp(h)))
+        return h
+
     def one_electron(self):
         n = self.nbf
         S, T, V = (np.zeros((n, n)) for _ in range(3))
@@ -343,6 +356,33 @@ class Engine:
         if check:
             self._check(rc)
         return (r, D) if want_density else r
+
+    def scf_begin(self, unrestricted=False, **kw):
+        """Start an SCF driven step by step (HartreeFockAlgorithm::Step)."""
+        p = ScfParams()
+        lib().hf_scf_defaults(ctypes.byref(p))
+        p.unrestricted = 1 if unrestricted else 0
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise TypeError("unknown SCF parameter %s" % k)
+            setattr(p, k, v)
+        self._check(lib().hf_scf_begin(self.h, ctypes.byref(p)))
+        self._stepping = 2 if unrestricted else 1
+        return self
+
+    def scf_step(self, it):
+        """One Step(it): returns (energy, ||D_new - D||_F, DIIS error estimate)."""
+        e, r, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().hf_scf_step(self.h, it, ctypes.byref(e), ctypes.byref(r), ctypes.byref(d)))
+        return e.value, r.value, d.value
+
+    def scf_end(self):
+        """End a stepped SCF: (result, densities)."""
+        n = self.nbf
+        r = ScfResult()
+        D = np.zeros((self._stepping, n, n))
+        self._check(lib().hf_scf_end(self.h, ctypes.byref(r), _dp(D)))
+        return r, D
 
     def kernel_stats(self):
         arr = (KernelStat * 128)()

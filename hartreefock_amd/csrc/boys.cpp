@@ -52,3 +52,11 @@ void boys_table(std::vector<double>& tab) {
 }
 
 }  // namespace hfhost
+
+// Boys functions F_0..F_mmax(T) on the host, as IntegralsRepository::getBoysFunctions returns them
+// (HartreeFock/IntegralsRepository.cpp:326-342, HartreeFock/BoysFunctions.cpp:10-25).
+extern "C" int hf_boys(int mmax, double T, double* F) {
+    if (!F || mmax < 0 || mmax > 32 || !(T >= 0) || !std::isfinite(T)) return HF_E_ARG;
+    hfhost::boys_host(mmax, T, F);
+    return HF_OK;
+}

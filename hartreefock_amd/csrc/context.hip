@@ -253,6 +253,7 @@ int run_classes(hf_ctx* ctx, int mode, const TaskSet& ts, const SinkArgs& sink, 
 
 // Shell pairs, primitive pairs, Schwarz bounds, sorted pair lists.
 int setup_molecule(hf_ctx* ctx) {
+    hf_scf_session_free(ctx);
     const hfhost::Molecule& mol = ctx->mol;
     const int ns = (int)mol.shells.size();
     // the device buffers of the previous molecule are reused: its work must be finished
@@ -740,6 +741,7 @@ int hf_destroy(hf_ctx* ctx) {
         (void)hipStreamDestroy(ctx->eri_stream);
     }
     if (ctx->hpin) (void)hipHostFree(ctx->hpin);
+    hf_scf_session_free(ctx);
     hf_scf_workspace_free(ctx);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

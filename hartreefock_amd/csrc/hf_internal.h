@@ -127,6 +127,7 @@ struct KernelStat {
 };
 
 struct ScfWorkspace;   // the SCF's device matrices and events, kept between hf_scf calls (scf.hip)
+struct ScfRun;         // an SCF driven step by step by the caller (hf_scf_begin / hf_scf_step / hf_scf_end)
 
 struct hf_ctx {
     hf_options opt;
@@ -170,6 +171,7 @@ struct hf_ctx {
     // Device memory is kept between calls and molecules (buffers only grow): hipFree synchronises the
     // whole device, which would stall the other contexts of a server at every request.
     ScfWorkspace* scf_ws = nullptr;
+    ScfRun* scf_run = nullptr;
     std::vector<KernelStat> stats;
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_pending;         // stat slots whose last launch's events are not yet read
@@ -217,6 +219,8 @@ int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32,
                       double* d_K, hipStream_t st, bool accumulate);
 // Takes the device address of this device's allocation-free scratch array (once per device; hf_create).
 int hf_incore_scratch_init(hf_ctx* ctx);
+// Ends an SCF session the caller left open (hf_destroy, hf_scf_begin).
+void hf_scf_session_free(hf_ctx* ctx);
 // Frees the SCF workspace (hf_destroy).
 void hf_scf_workspace_free(hf_ctx* ctx);
 // Device bytes the context holds between calls: SCF workspace, in-core stores, J/K scratch.

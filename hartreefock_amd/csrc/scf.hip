@@ -574,47 +574,75 @@ double hf_ctx_held_bytes(const hf_ctx* ctx) {
     return b;
 }
 
-extern "C" {
 
-void hf_scf_defaults(hf_scf_params* p) {
-    std::memset(p, 0, sizeof(*p));
-    p->unrestricted = 0;
-    p->max_iterations = 3000;
-    p->alpha = 0.75;
-    p->init_guess = 0.75;
-    p->use_diis = 1;
-    p->max_diis_iterations = 1000;
-    p->normal_iter_after_diis = 500;
-    p->add_asymmetry = 1;
-    p->asymmetry = 0.1;
-    p->density_method = 0;
-    p->incremental_fock = 0;
-    p->full_rebuild_every = 8;
-    p->final_fp64_iterations = 3;
-    p->incore = 0;
-    p->incore_max_bytes = 0;
-    p->mp2 = 0;
-}
+// One SCF (HartreeFockAlgorithm::Init + Calculate, HartreeFock/HartreeFockAlgorithm.cpp:21-192) as a session
+// object: init() is Init, step() one RHF/UHF Step, calculate() the reference's iteration control and finish()
+// the orbital energies, <S^2>, MP2 and the result.  hf_scf runs all four; hf_scf_begin / hf_scf_step /
+// hf_scf_end let a caller drive the steps itself (HartreeFockAlgorithm::Step).
+struct ScfRun {
+    hf_ctx* ctx;
+    hf_scf_params P;
+    const hf_scf_params* p;
+    std::chrono::steady_clock::time_point t_start;
+    hipStream_t s = nullptr;
+    bool uhf = false, mixed_tier = false, incore = false, store_pending = false, field = false;
+    int n = 0, nsp = 1, ne = 0, occ[2] = {0, 0};
+    size_t nn = 0;
+    double Enuc = 0, eri_ms = 0;
+    hipEvent_t* eri_ev = nullptr;
+    hipEvent_t* jk_ev = nullptr;
+    Diis diis[2];
+    double etolD = 0, etolN = 0, dtol = 0, itol = 0;
+    double energy = 0, homo = 0, lastErr = 0, jk_ms = 0;
+    int diag_steps = 0, sp2_steps[2] = {0, 0}, sp2_total = 0;
+    bool last_diag[2] = {false, false};
+    // incremental Fock build state: J, K hold the J/K of Dprev
+    int full_builds = 0, incr_builds = 0, since_full = 0;
+    bool force_full = false, last_incremental = false;
+    double quartets = 0, last_dmax = 0, last_quartets = 0;
+    bool incore_mixed = false, incremental = false;
+    int first_full = 1, rebuild_every = 8;
+    static constexpr double kIncoreFp32MaxDelta = 1e-3;   // largest max|D - Dprev| contracted with the FP32 store
+    bool jk_pending = false;
+    // mixed-precision tier: quartets below the context's fp32_threshold in FP32 until fp64_phase
+    bool mixed = false, fp64_phase = true, last_fp32 = false;
+    int final_fp64 = 3, fp64_steps = 0, fp32_builds = 0;
+    bool useDIIS = true;
+    std::vector<double> eps_h;
+    // iteration state of calculate() (or of a caller's steps), read by finish()
+    int iter = 0;
+    bool converged = false, cancelled = false;
+    double cur = 0;
 
-int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out) {
-    if (!ctx) return HF_E_ARG;
-    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
-    if (!p || !r) return ctx->fail(HF_E_ARG, "null params/result");
-    const auto t_start = std::chrono::steady_clock::now();
-    std::memset(r, 0, sizeof(*r));
-    hipStream_t s = ctx->stream;
-    const bool uhf = p->unrestricted != 0;
+    bool external = false;   // steps driven by the caller (hf_scf_step): no convergence verdict of its own
+
+    ScfRun(hf_ctx* c, const hf_scf_params& params) : ctx(c), P(params), p(&P) {}
+    double quartet_count() const {
+        double q = 0;
+        for (const auto& ks : ctx->stats) q += (double)ks.quartets;
+        return q;
+    }
+    int init();
+    int step(int iter, double& rms);
+    int calculate();
+    int finish(hf_scf_result* r, double* D_out);
+};
+
+int ScfRun::init() {
+    t_start = std::chrono::steady_clock::now();
+    s = ctx->stream;
+    uhf = p->unrestricted != 0;
     if (!ctx->scf_ws) ctx->scf_ws = new (std::nothrow) ScfWorkspace();
     if (!ctx->scf_ws) return ctx->fail(HF_E_OOM, "SCF workspace");
     ScfWorkspace& ws = *ctx->scf_ws;
     ScfState& st = ws.st;
-    const int n = ctx->mol.nbf;
+    n = ctx->mol.nbf;
     st.n = n;
     st.nn = (size_t)n * n;
-    const size_t nn = st.nn;
+    nn = st.nn;
     // ---- Init (HartreeFock/HartreeFockAlgorithm.cpp:21-101) ----
-    const double Enuc = hfhost::nuclear_repulsion(ctx->mol);
-    const int nsp = uhf ? 2 : 1;
+    Enuc = hfhost::nuclear_repulsion(ctx->mol);
+    nsp = uhf ? 2 : 1;
     HF_HIP_CHECK(ctx, st.S.alloc(nn));
     HF_HIP_CHECK(ctx, st.h.alloc(nn));
     HF_HIP_CHECK(ctx, st.X.alloc(nn));
@@ -646,10 +674,10 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     // mixed-precision tier contracts an FP32 copy until its final FP64 phase.  The store is evaluated
     // on its own stream, overlapping the one-electron integrals, S^-1/2 and iteration 0 (which needs
     // no J/K); the first J/K build waits for it.
-    const bool mixed_tier = ctx->opt.precision == HF_PREC_MIXED;
-    bool incore = false, store_pending = false;
-    double eri_ms = 0;
-    hipEvent_t* eri_ev = ws.eri_ev;
+    mixed_tier = ctx->opt.precision == HF_PREC_MIXED;
+    incore = false;
+    store_pending = false;
+    eri_ev = ws.eri_ev;
     if (p->incore && n <= kIncoreMaxN && p->incore_max_bytes >= 0) {
         const uint64_t len = hf_pair_matrix_len(n);
         size_t free_b = 0, total_b = 0;
@@ -676,7 +704,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         }
     }
     // h = T + V + E . M with the moment integrals about the origin (HartreeFock/HartreeFockAlgorithm.cpp:38)
-    const bool field = ctx->mol.has_field();
+    field = ctx->mol.has_field();
     if (field) HF_HIP_CHECK(ctx, st.Fb.alloc(3 * nn));
     if (int rc = hf_one_electron_device(ctx, st.S.p, st.T1.p, st.T2.p, s, field ? st.Fb.p : nullptr)) return rc;
     hipLaunchKernelGGL(add_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.T2.p, st.h.p, nn);
@@ -696,8 +724,7 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     if (int rc = inverse_sqrt(ctx, st)) return rc;
     // occupations (HartreeFock/RestrictedHartreeFock.cpp:18-32, UnrestrictedHartreeFock.cpp:20-60)
     // ElectronsNumber() is alpha + beta when either is set, else the atoms' count (HartreeFock/Molecule.cpp:96-101)
-    const int ne = ctx->mol.electrons();
-    int occ[2];
+    ne = ctx->mol.electrons();
     if (!uhf) {
         occ[0] = ne / 2;
         if (!occ[0]) occ[0] = 1;
@@ -720,7 +747,6 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
     hipLaunchKernelGGL(guess_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.S.p, st.h.p, p->init_guess, n,
                        uhf && p->add_asymmetry && n > 1 ? p->asymmetry : 0.0, st.F0.p, st.F0.p + nn);
     HF_HIP_CHECK(ctx, hipMemsetAsync(st.D.p, 0, 2 * nn * sizeof(double), s));
-    Diis diis[2];
     for (int k = 0; k < nsp; ++k) {
         HF_HIP_CHECK(ctx, ws.diisF[k].alloc(Diis::kKeep * nn));
         HF_HIP_CHECK(ctx, ws.diisE[k].alloc(Diis::kKeep * nn));
@@ -728,45 +754,36 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         diis[k].E = ws.diisE[k].p;
         diis[k].nn = nn;
     }
-    const double etolD = p->energy_tolerance > 0 ? p->energy_tolerance : 1e-11;
-    const double etolN = p->energy_tolerance > 0 ? p->energy_tolerance : 1e-13;
-    const double dtol = p->density_tolerance > 0 ? p->density_tolerance : 1e-10;
-    const double itol = p->diis_tolerance > 0 ? p->diis_tolerance : 1e-10;
-    double energy = 0, homo = 0, lastErr = 0, jk_ms = 0;
-    int diag_steps = 0, sp2_steps[2] = {0, 0}, sp2_total = 0;
-    bool last_diag[2] = {false, false};
+    etolD = p->energy_tolerance > 0 ? p->energy_tolerance : 1e-11;
+    etolN = p->energy_tolerance > 0 ? p->energy_tolerance : 1e-13;
+    dtol = p->density_tolerance > 0 ? p->density_tolerance : 1e-10;
+    itol = p->diis_tolerance > 0 ? p->diis_tolerance : 1e-10;
     // incremental Fock build state: J, K hold the J/K of Dprev
-    int full_builds = 0, incr_builds = 0, since_full = 0;
-    bool force_full = false, last_incremental = false;
-    double quartets = 0, last_dmax = 0, last_quartets = 0;
     // In-core mixed tier: the first two builds are full FP64 (the density changes most there), then
     // the J/K of each density change comes from the FP32 store while max|D - Dprev| <= 1e-3 (its
     // error scales with |D - Dprev|), FP64 otherwise; full rebuilds and the final phase are FP64.
-    const bool incore_mixed = incore && mixed_tier;
-    const bool incremental = p->incremental_fock != 0 || incore_mixed;
-    const int first_full = incore_mixed ? 2 : 1;
-    constexpr double kIncoreFp32MaxDelta = 1e-3;   // largest max|D - Dprev| contracted with the FP32 store
+    incore_mixed = incore && mixed_tier;
+    incremental = p->incremental_fock != 0 || incore_mixed;
+    first_full = incore_mixed ? 2 : 1;
     // J/K build time from events on the SCF stream, read at the step's own synchronisation
-    hipEvent_t* jk_ev = ws.jk_ev;
+    jk_ev = ws.jk_ev;
     for (int k = 0; k < 2; ++k)
         if (!jk_ev[k]) HF_HIP_CHECK(ctx, hipEventCreate(&jk_ev[k]));
-    bool jk_pending = false;
-    const int rebuild_every = p->full_rebuild_every > 0 ? p->full_rebuild_every : 8;
+    rebuild_every = p->full_rebuild_every > 0 ? p->full_rebuild_every : 8;
     // mixed-precision tier: quartets below the context's fp32_threshold in FP32 until fp64_phase
-    const bool mixed = ctx->opt.precision == HF_PREC_MIXED;
-    const int final_fp64 = p->final_fp64_iterations > 0 ? p->final_fp64_iterations : 3;
-    bool fp64_phase = !mixed, last_fp32 = false;
-    int fp64_steps = 0, fp32_builds = 0;
-    auto quartet_count = [&]() {
-        double q = 0;
-        for (const auto& ks : ctx->stats) q += (double)ks.quartets;
-        return q;
-    };
-    bool useDIIS = p->use_diis != 0;
-    std::vector<double> eps_h(2 * (size_t)n);
+    mixed = ctx->opt.precision == HF_PREC_MIXED;
+    final_fp64 = p->final_fp64_iterations > 0 ? p->final_fp64_iterations : 3;
+    fp64_phase = !mixed;
+    useDIIS = p->use_diis != 0;
+    eps_h.assign(2 * (size_t)n, 0.0);
+    return HF_OK;
 
-    // ---- one Step (RHF::Step / UHF::Step) ----
-    auto step = [&](int iter, double& rms) -> int {
+}
+
+int ScfRun::step(int iter, double& rms) {
+        ScfWorkspace& ws = *ctx->scf_ws;
+        ScfState& st = ws.st;
+        (void)ws;
         double* F[2] = {st.Fa.p, st.Fa.p + nn};
         if (iter == 0) {
             for (int k = 0; k < nsp; ++k)
@@ -933,22 +950,21 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
                            nsp * nn, used ? 1 : 0);
         HF_HIP_CHECK(ctx, hipGetLastError());
         return HF_OK;
-    };
+}
 
+int ScfRun::calculate() {
     // ---- Calculate (HartreeFock/HartreeFockAlgorithm.cpp:105-192) ----
     // HFGPU_TRACE=1 prints one line per iteration to stderr (the reference's per-step TRACE,
     // HartreeFock/RestrictedHartreeFock.cpp:149).
     const char* tenv = std::getenv("HFGPU_TRACE");
     const bool trace = tenv && tenv[0] == '1';
-    double cur = 0, prev = std::numeric_limits<double>::infinity(), rms = 0;
+    double prev = std::numeric_limits<double>::infinity(), rms = 0;
     auto trace_line = [&](int it, double rmsv) {
         if (trace)
             std::fprintf(stderr, "hf_scf %s iter %4d  E % .12f  dE % .3e  |dD| %.3e  diis %.3e  jk %s max|dD| %.2e quartets %.3e\n",
                          uhf ? "UHF" : "RHF", it, energy, energy - prev, rmsv, lastErr,
                          last_incremental ? "incr" : "full", last_dmax, last_quartets);
     };
-    int iter = 0;
-    bool converged = false, cancelled = false;
     // the caller's cancellation flag (HartreeFockAlgorithm::terminate, an atomic in the reference,
     // HartreeFock/HartreeFockAlgorithm.h:57), read with an atomic load once per iteration
     auto stop = [&]() { return p->terminate && __atomic_load_n(p->terminate, __ATOMIC_ACQUIRE) != 0; };
@@ -1011,6 +1027,11 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
             prev = cur;
         }
     }
+    return HF_OK;
+}
+
+int ScfRun::finish(hf_scf_result* r, double* D_out) {
+    ScfState& st = ctx->scf_ws->st;
     // orbital energies (HOMO) of the last Fock matrices: eigenvalues only
     for (int k = 0; k < nsp; ++k) {
         if (last_diag[k]) continue;
@@ -1098,8 +1119,120 @@ int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out)
         HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
     }
     r->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    if (external) return HF_OK;
     if (cancelled) return ctx->fail(HF_E_CANCELLED, "SCF cancelled");
     return converged ? HF_OK : ctx->fail(HF_E_NOT_CONVERGED, "SCF did not converge");
+}
+
+void hf_scf_session_free(hf_ctx* ctx) {
+    delete ctx->scf_run;
+    ctx->scf_run = nullptr;
+}
+
+extern "C" {
+
+int hf_core_hamiltonian(hf_ctx* ctx, double* h_out) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!h_out) return ctx->fail(HF_E_ARG, "null output");
+    const int n = ctx->mol.nbf;
+    const size_t nn = (size_t)n * n;
+    hipStream_t s = ctx->stream;
+    DevBuf<double> w;
+    HF_HIP_CHECK(ctx, w.alloc(6 * nn));
+    // h = T + V + E . M (HartreeFock/HartreeFockAlgorithm.cpp:38), the matrix hf_scf starts from
+    if (int rc = hf_one_electron_device(ctx, w.p, w.p + nn, w.p + 2 * nn, s, w.p + 3 * nn)) return rc;
+    LinComb lc{};
+    lc.m = 5;
+    lc.src[0] = w.p + nn;
+    lc.c[0] = 1.0;
+    lc.src[1] = w.p + 2 * nn;
+    lc.c[1] = 1.0;
+    for (int k = 0; k < 3; ++k) {
+        lc.src[k + 2] = w.p + (3 + k) * nn;
+        lc.c[k + 2] = ctx->mol.field[k];
+    }
+    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(nn)), dim3(256), 0, s, lc, w.p, nn);
+    HF_HIP_CHECK(ctx, hipGetLastError());
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(h_out, w.p, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    return HF_OK;
+}
+
+void hf_scf_defaults(hf_scf_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->unrestricted = 0;
+    p->max_iterations = 3000;
+    p->alpha = 0.75;
+    p->init_guess = 0.75;
+    p->use_diis = 1;
+    p->max_diis_iterations = 1000;
+    p->normal_iter_after_diis = 500;
+    p->add_asymmetry = 1;
+    p->asymmetry = 0.1;
+    p->density_method = 0;
+    p->incremental_fock = 0;
+    p->full_rebuild_every = 8;
+    p->final_fp64_iterations = 3;
+    p->incore = 0;
+    p->incore_max_bytes = 0;
+    p->mp2 = 0;
+}
+
+extern "C" int hf_scf_begin(hf_ctx* ctx, const hf_scf_params* p) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!p) return ctx->fail(HF_E_ARG, "null params");
+    hf_scf_session_free(ctx);
+    ctx->scf_run = new (std::nothrow) ScfRun(ctx, *p);
+    if (!ctx->scf_run) return ctx->fail(HF_E_OOM, "SCF session");
+    ctx->scf_run->external = true;
+    ctx->scf_run->fp64_phase = true;       // a caller's own steps are all FP64
+    if (int rc = ctx->scf_run->init()) {
+        hf_scf_session_free(ctx);
+        return rc;
+    }
+    return HF_OK;
+}
+
+extern "C" int hf_scf_step(hf_ctx* ctx, int iter, double* energy, double* rms, double* diis_error) {
+    if (!ctx) return HF_E_ARG;
+    ScfRun* run = ctx->scf_run;
+    if (!run) return ctx->fail(HF_E_STATE, "no SCF session (hf_scf_begin)");
+    if (iter < 0) return ctx->fail(HF_E_ARG, "negative iteration");
+    double d = 0;
+    if (int rc = run->step(iter, d)) return rc;
+    run->cur = run->energy;
+    run->iter = iter + 1;
+    if (energy) *energy = run->energy;
+    if (rms) *rms = d;
+    if (diis_error) *diis_error = run->lastErr;
+    if (!std::isfinite(run->energy) || !std::isfinite(d))
+        return ctx->fail(HF_E_NUMERIC, "SCF energy or density is not finite");
+    return HF_OK;
+}
+
+extern "C" int hf_scf_end(hf_ctx* ctx, hf_scf_result* r, double* D_out) {
+    if (!ctx) return HF_E_ARG;
+    ScfRun* run = ctx->scf_run;
+    if (!run) return ctx->fail(HF_E_STATE, "no SCF session (hf_scf_begin)");
+    hf_scf_result tmp;
+    if (!r) r = &tmp;
+    std::memset(r, 0, sizeof(*r));
+    const int rc = run->finish(r, D_out);
+    hf_scf_session_free(ctx);
+    return rc;
+}
+
+int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out) {
+    if (!ctx) return HF_E_ARG;
+    if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+    if (!p || !r) return ctx->fail(HF_E_ARG, "null params/result");
+    std::memset(r, 0, sizeof(*r));
+    ScfRun run(ctx, *p);
+    if (int rc = run.init()) return rc;
+    if (int rc = run.calculate()) return rc;
+    return run.finish(r, D_out);
 }
 
 }  // extern "C"

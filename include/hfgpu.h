@@ -157,6 +157,12 @@ int hf_build_jk_incore_device_noalloc(hf_ctx* ctx, int ndens, const double* d_G,
 
 /* One-electron matrices S, T, V (HartreeFock/QuantumMatrix.cpp:34-155). Host buffers, n*n. */
 int hf_one_electron(hf_ctx* ctx, double* S, double* T, double* V);
+/* The core Hamiltonian the SCF starts from, h = T + V + E . M (the public HartreeFockAlgorithm::h,
+ * HartreeFock/HartreeFockAlgorithm.h:47, set at :38), n*n host buffer. */
+int hf_core_hamiltonian(hf_ctx* ctx, double* h);
+/* Boys functions F_0..F_mmax(T) (mmax <= 32), as IntegralsRepository::getBoysFunctions / BoysFunctions
+ * (HartreeFock/IntegralsRepository.cpp:326-342, HartreeFock/BoysFunctions.cpp:10-25); computed on the host. */
+int hf_boys(int mmax, double T, double* F);
 
 /* SCF, replacing RestrictedHartreeFock / UnrestrictedHartreeFock Init() + Calculate()
  * (HartreeFock/HartreeFockAlgorithm.cpp:21-192).  Field meanings and defaults follow the
@@ -218,6 +224,19 @@ typedef struct hf_scf_result {
 void hf_scf_defaults(hf_scf_params* p);
 /* D_out (optional): n*n (RHF total density) or 2*n*n (UHF alpha then beta). */
 int hf_scf(hf_ctx* ctx, const hf_scf_params* p, hf_scf_result* r, double* D_out);
+
+/* The same SCF driven one step at a time by the caller, for HartreeFockAlgorithm::Step(int)
+ * (HartreeFock/HartreeFockAlgorithm.h:77; RestrictedHartreeFock::Step, HartreeFock/RestrictedHartreeFock.cpp:61-166,
+ * UnrestrictedHartreeFock::Step, HartreeFock/UnrestrictedHartreeFock.cpp:87-191).  hf_scf_begin does Init (one-electron
+ * integrals, S^-1/2, iteration-0 guess, the in-core store when asked); each hf_scf_step(iter) is one Step: the Fock
+ * build of the current density (iter 0: the guess), DIIS while p->use_diis and iter < max_diis_iterations, the new
+ * density and the energy, returning the energy, ||D_new - D||_F (Step's return value) and the DIIS error estimate;
+ * hf_scf_end gives the result (orbital energies, <S^2>, the density; converged = 0: convergence is the caller's
+ * call) and ends the session.  The steps are all FP64 (the mixed tier's switching belongs to hf_scf's own loop).
+ * One session per context; hf_set_molecule / hf_set_basis / hf_destroy end an open one. */
+int hf_scf_begin(hf_ctx* ctx, const hf_scf_params* p);
+int hf_scf_step(hf_ctx* ctx, int iter, double* energy, double* rms, double* diis_error);
+int hf_scf_end(hf_ctx* ctx, hf_scf_result* r, double* D_out);
 
 /* Mulliken charges (natoms) and dipole moment (3, au, about the origin) of the densities D (host,
  * ndens x n x n: the RHF total density, or UHF alpha then beta), replacing

@@ -459,8 +459,10 @@ def test_mp2_matches_oracle(gpu_engine_factory, name, bfile, atoms, uhf, incore)
 
 
 def test_cpp_dropin_runs(tmp_path):
-    """The reference-style C++ caller (drop-in header over the C ABI) runs an RHF, its dipole, a Mulliken
-    charge and MP2 on the GPU, and its energy matches the oracle's."""
+    """The reference-style C++ callers (tests/cpp/dropin_example.cpp, over include/hartreefock_amd.hpp and the C ABI)
+    against the oracle: atoms taken from Chemistry::Basis::atoms, RHF Init/Calculate; the public h; ERIs by index and
+    by orbital pointers; getBoysFunctions; the SCF driven by Step(iter); an H2 bond scan moving atoms with
+    SetCenterForShells; LoadXYZ with a UHF cation in an electric field."""
     import os
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -471,15 +473,35 @@ def test_cpp_dropin_runs(tmp_path):
     assert subprocess.run(cmd, capture_output=True, text=True).returncode == 0
     env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(root, "hartreefock_amd") + ":" +
                os.environ.get("LD_LIBRARY_PATH", ""))
-    p = subprocess.run([str(exe), basis("sto3g.txt")], capture_output=True, text=True, env=env, timeout=100)
+    p = subprocess.run([str(exe), basis("sto3g.txt"), str(tmp_path / "water.xyz")], capture_output=True, text=True,
+                       env=env, timeout=100)
     print(p.stdout, p.stderr, flush=True)
     assert p.returncode == 0
     vals = dict(line.split(" = ", 1) for line in p.stdout.splitlines() if " = " in line)
-    e = float(vals["E"].split()[0])
-    ref, _ = refcpu.Molecule(basis("sto3g.txt"), g.water_cfg1()).scf(alpha=0.5, init_guess=0.0,
-                                                                     normal_iter_after_diis=0)
-    assert abs(e - ref.energy) <= 1e-8
+    kw = dict(alpha=0.5, init_guess=0.0, normal_iter_after_diis=0)
+    wat = [(1, (0.0, 1.43233673, -0.96104039)), (1, (0.0, -1.43233673, -0.96104039)), (8, (0.0, 0.0, 0.24026010))]
+    mol = refcpu.Molecule(basis("sto3g.txt"), wat)
+    ref, _ = mol.scf(**kw)
+    assert int(vals["nbf"]) == 7 and vals["electrons"].split() == ["5", "5"]
+    assert abs(float(vals["E"]) - ref.energy) <= 1e-8 and vals["converged"] == "1"
+    assert abs(float(vals["E_steps"]) - ref.energy) <= 1e-8
+    S, T, V = mol.one_electron()
+    assert abs(float(vals["h00"]) - (T + V)[0, 0]) <= 1e-12
+    refcpu.set_exact_transfer(False)
+    e = mol.eri(2, 3, 5, 6)
+    assert abs(float(vals["eri_int"]) - e) <= 1e-12 and abs(float(vals["eri_orb"]) - e) <= 1e-12
+    assert int(vals["boys_n"]) == 6 and abs(float(vals["boys0"]) - refcpu.boys(5, 2.5)[0]) <= 1e-14
     assert float(vals["E(MP2)"]) < 0
+    for r in (1.2, 1.4, 1.6):
+        er, conv, _ = vals["scan %.1f" % r].split()
+        h2, _ = refcpu.Molecule(basis("sto3g.txt"), [(1, (-r / 2, 0.0, 0.0)), (1, (r / 2, 0.0, 0.0))]).scf(
+            use_diis=0, **kw)
+        assert conv == "1" and abs(float(er) - h2.energy) <= 1e-8
+    cat = refcpu.Molecule(basis("sto3g.txt"), g.water_cfg1()).set_electrons(5, 4).set_field([0.0, 0.0, 0.01])
+    u, _ = cat.scf(unrestricted=True, **kw)
+    assert vals["loaded"] == "1" and vals["uhf_converged"] == "1"
+    assert abs(float(vals["E_uhf_field"]) - u.energy) <= 1e-8
+    assert abs(float(vals["E_field_nuc"]) - (-sum(z * x[2] * 0.01 for z, x in g.water_cfg1()))) <= 1e-13
 
 
 @pytest.mark.parametrize("prec,incr", [(0, 1), (1, 0)], ids=["fp64_incremental", "mixed"])
