@@ -685,6 +685,7 @@ void hf_default_options(hf_options* o) {
     o->fp32_threshold = 1e-6;
     o->profile = 0;
     o->profile_class = -1;
+    o->incore_shape = -1;
 }
 
 int hf_create(const hf_options* opt, hf_ctx** out) {

@@ -70,208 +70,344 @@ struct PairOf<float> {
     using V = float2;
 };
 
-// One workgroup per padded row index R of the store (R = pair_pad_index(i, j); padding rows return at
-// once).  M2 = ceil(n / 128) column chunks: lane L owns columns 2L + 128 m and 2L + 1 + 128 m and reads
-// both with one aligned 16-byte (FP64) or 8-byte (FP32) load.
-template <int NDENS, int M2, int ROWS, bool PREFETCH, typename T>
+// Element loads of CPL consecutive columns: one aligned 16-byte (FP64) or 8-byte (FP32) load for a column pair,
+// a plain load for one column.
+template <typename T, int CPL>
+struct ColsOf;
+template <typename T>
+struct ColsOf<T, 1> {
+    using V = T;
+    __device__ static double get(const V& v, int) { return (double)v; }
+};
+template <typename T>
+struct ColsOf<T, 2> {
+    using V = typename PairOf<T>::V;
+    __device__ static double get(const V& v, int h) { return h ? (double)v.y : (double)v.x; }
+};
+template <int CPL>
+struct DwOf;
+template <>
+struct DwOf<1> {
+    using V = double;
+    __device__ static double get(const V& v, int) { return v; }
+};
+template <>
+struct DwOf<2> {
+    using V = double2;
+    __device__ static double get(const V& v, int h) { return h ? v.y : v.x; }
+};
+
+// Buffer-resource word 3 of gfx9-family raw buffers (32-bit data format, no swizzle, no index stride).
+constexpr int kBufferFlags = 0x00020000;
+
+// A raw buffer load of 4, 8 or 16 bytes at byte offset off; zero where off is beyond the buffer's range.
+template <typename V>
+__device__ __forceinline__ V buffer_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    if constexpr (sizeof(V) == 16)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+    else if constexpr (sizeof(V) == 8)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    else
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+// Blocks of NR store rows: bra pairs (i, j0 + s), s < NR, j0 = NR t, of one bra index i.  Row i holds
+// ceil((i + 1) / NR) blocks, so blocks_before(i) = sum_{m=1..i} ceil(m / NR) = NR q (q + 1) / 2 + r (q + 1)
+// with q = i / NR, r = i % NR.
+__host__ __device__ inline int64_t blocks_before(int i, int nr) {
+    const int64_t q = i / nr, r = i % nr;
+    return nr * q * (q + 1) / 2 + r * (q + 1);
+}
+
+// One workgroup per block of NR store rows P_s = pair_pad_index(i, j_s) with one bra index i.  A wavefront
+// covers a row k of the pair triangle in chunks of W = 64 CPL columns: lane L owns columns CPL L + W m + h
+// (h < CPL, m < MC = ceil(n / W)) and reads a chunk's CPL columns of each of the NR rows with one load each
+// (two columns per lane halve the load instructions; one column per lane wastes fewer lanes on the short
+// rows, row k having k + 1 elements).  The J weights Dw of a chunk are loaded once for the NR rows, and
+// the rows share K_i: its row sums (sum_l B_s[k][l] D_{j_s}[l] over s) and its column part are one
+// accumulator each, so a row k of the triangle needs NDENS (NR + 1) row sums instead of 2 NR NDENS.
+template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
 __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t npad,
                                                         const double* __restrict__ D, const double* __restrict__ Dw,
                                                         double* __restrict__ J, double* __restrict__ K) {
     extern __shared__ double lds[];
-    const int64_t R = blockIdx.x;
-    int i = (int)sqrt(2.0 * (double)R) - 1;
+    constexpr int W = 64 * CPL;
+    const int64_t blk = blockIdx.x;
+    int i = (int)sqrt(2.0 * NR * (double)blk) - 1;
     if (i < 0) i = 0;
-    while (hfgpu::pair_pad_base(i + 1) <= R) ++i;
-    while (i > 0 && hfgpu::pair_pad_base(i) > R) --i;
-    const int j = (int)(R - hfgpu::pair_pad_base(i));
-    if (j > i) return;                                      // padding row: all zero (whole block)
-    double* xj = lds;               // D_d[j][:]
-    double* xi = xj + NDENS * n;    // D_d[i][:]
-    double* rI = xi + NDENS * n;    // additions to K_d[i][:]
-    double* rJ = rI + NDENS * n;    // additions to K_d[j][:]
-    double* red = rJ + NDENS * n;   // per-wavefront partials of J_ij
+    while (blocks_before(i + 1, NR) <= blk) ++i;
+    while (i > 0 && blocks_before(i, NR) > blk) --i;
+    const int j0 = (int)(blk - blocks_before(i, NR)) * NR;
+    const int64_t P0 = hfgpu::pair_pad_base(i) + j0;     // row of s = 0; rows s with j0 + s > i are empty
+    const int nrow = i - j0 + 1 < NR ? i - j0 + 1 : NR;
+    double* xj = lds;                      // D_d[j_s][:]      [NR][NDENS][n]
+    double* xi = xj + NR * NDENS * n;      // D_d[i][:]        [NDENS][n]
+    double* rI = xi + NDENS * n;           // additions to K_d[i][:]
+    double* rJ = rI + NDENS * n;           // additions to K_d[j_s][:]   [NR][NDENS][n]
+    double* gdg = rJ + NR * NDENS * n;     // diagonal elements B_s[c][c]  [NR][n]
+    double* red = gdg + NR * n;            // per-wavefront partials of J_{i j_s}  [NR][4]
     const size_t nn = (size_t)n * n;
+    // B_s[c][c] for the columns c = threadIdx.x + 256 q (n <= 512), read now and used in the epilogue
+    double gd[NR][2];
+#pragma unroll
+    for (int s = 0; s < NR; ++s)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int c = threadIdx.x + 256 * q;
+            gd[s][q] = s < nrow && c < n ? (double)G[(P0 + s) * npad + hfgpu::pair_pad_base(c) + c] : 0.0;
+        }
     for (int t = threadIdx.x; t < NDENS * n; t += blockDim.x) {
         const int d = t / n, c = t - d * n;
-        xj[t] = D[d * nn + (size_t)j * n + c];
         xi[t] = D[d * nn + (size_t)i * n + c];
+#pragma unroll
+        for (int s = 0; s < NR; ++s) xj[s * NDENS * n + t] = s < nrow ? D[d * nn + (size_t)(j0 + s) * n + c] : 0.0;
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double xjl[NDENS][M2][2], xil[NDENS][M2][2], cI[NDENS][M2][2], cJ[NDENS][M2][2];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform: row bases in SGPRs
+    double xjl[NR][NDENS][MC][CPL], xil[NDENS][MC][CPL], cI[NDENS][MC][CPL], cJ[NR][NDENS][MC][CPL];
 #pragma unroll
     for (int d = 0; d < NDENS; ++d)
 #pragma unroll
-        for (int m = 0; m < M2; ++m)
+        for (int m = 0; m < MC; ++m)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int l = 2 * lane + 128 * m + h;
-                xjl[d][m][h] = l < n ? xj[d * n + l] : 0.0;
+            for (int h = 0; h < CPL; ++h) {
+                const int l = CPL * lane + W * m + h;
                 xil[d][m][h] = l < n ? xi[d * n + l] : 0.0;
                 cI[d][m][h] = 0.0;
-                cJ[d][m][h] = 0.0;
+#pragma unroll
+                for (int s = 0; s < NR; ++s) {
+                    xjl[s][d][m][h] = l < n ? xj[(s * NDENS + d) * n + l] : 0.0;
+                    cJ[s][d][m][h] = 0.0;
+                }
             }
-    double jacc = 0.0;
-    // byte offsets from the block-uniform row base (a padded row is at most ~1 MB)
-    const char* __restrict__ row0 = (const char*)(G + R * npad);
+    constexpr int JN = NR == 1 ? 1 : NR == 2 ? 2 : 4;   // J partials, padded to a power of two
+    static_assert(NR <= 4, "at most four store rows per block");
+    double jacc[JN];
+#pragma unroll
+    for (int s = 0; s < JN; ++s) jacc[s] = 0.0;
     const char* __restrict__ dwb = (const char*)Dw;
-    constexpr int NV = 2 * ROWS * NDENS;           // ROWS rows x (K_i, K_j) x densities
+    constexpr int NA = NR + 1;                     // row sums of one row k and density: K_i, then K_{j_s}
+    constexpr int NV = ROWS * NA * NDENS;
+    static_assert((NV & (NV - 1)) == 0, "row sums are reduced by recursive halving");
     constexpr int SH = 6 - log2c<NV>();
-    using PV = typename PairOf<T>::V;
-    // The loads of a row group: every column chunk of its ROWS rows k0 + 4r, issued together.  Addresses
-    // are clamped into the row (rows past the last one read row n - 1), so every load is in bounds;
-    // elements outside the row are masked where they are used.
-    auto load = [&](int k0, PV (&g)[ROWS][M2], double2 (&dd)[ROWS][M2]) {
+    using PV = typename ColsOf<T, CPL>::V;
+    using DV = typename DwOf<CPL>::V;
+    // The loads of a row group: every column chunk of its ROWS rows k0 + 4r in each of the NR store rows,
+    // and of Dw, issued together, as buffer loads whose range is row k (its padded, even length; zero for
+    // rows past the last one and for empty store rows).  Loads beyond the range return zero, so elements
+    // outside the row need no masks, and no load leaves it.
+    auto load = [&](int k0, PV (&g)[ROWS][NR][MC], DV (&dd)[ROWS][MC]) {
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
             const int k = k0 + 4 * r;
             const int kk = k < n ? k : n - 1;
-            const int b = (int)hfgpu::pair_pad_base(kk);
-            const int kev = kk & ~1;               // last even column of the row: (kev, kev + 1) is in the row
+            const int64_t b = hfgpu::pair_pad_base(kk);
+            const int len = k < n ? (int)(hfgpu::pair_pad_base(kk + 1) - b) : 0;
+            const auto rd = __builtin_amdgcn_make_buffer_rsrc((void*)(dwb + b * 8), (short)0, len * 8, kBufferFlags);
 #pragma unroll
-            for (int m = 0; m < M2; ++m) {
-                const int l0 = 2 * lane + 128 * m;
-                const unsigned at = (unsigned)(b + (l0 <= kev ? l0 : kev));
-                g[r][m] = *(const PV*)(row0 + at * (unsigned)sizeof(T));
-                dd[r][m] = *(const double2*)(dwb + at * 8u);
+            for (int m = 0; m < MC; ++m) dd[r][m] = buffer_load<DV>(rd, (unsigned)(CPL * lane + W * m) * 8u);
+#pragma unroll
+            for (int s = 0; s < NR; ++s) {
+                const auto rg = __builtin_amdgcn_make_buffer_rsrc(
+                    (void*)(G + (P0 + s) * npad + b), (short)0, s < nrow ? len * (int)sizeof(T) : 0, kBufferFlags);
+#pragma unroll
+                for (int m = 0; m < MC; ++m)
+                    g[r][s][m] = buffer_load<PV>(rg, (unsigned)(CPL * lane + W * m) * (unsigned)sizeof(T));
             }
         }
     };
-    PV ga[ROWS][M2];
-    double2 da[ROWS][M2];
-    load(w, ga, da);
-    for (int k0 = w; k0 < n; k0 += 4 * ROWS) {
-        // PREFETCH: the next group's loads go out before this group's arithmetic and reduction
-        PV gb[ROWS][M2];
-        double2 db[ROWS][M2];
-        if constexpr (PREFETCH) load(k0 + 4 * ROWS, gb, db);
+    // The arithmetic and the row-sum reduction of the group of rows k0 + 4 r held in g, dd.
+    auto group = [&](int k0, const PV (&g)[ROWS][NR][MC], const DV (&dd)[ROWS][MC]) {
         double V[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) V[v] = 0.0;
-        int kr[ROWS];
-        double xkj[ROWS][NDENS], xki[ROWS][NDENS];
+        double xkj[ROWS][NR][NDENS], xki[ROWS][NDENS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
             const int k = k0 + 4 * r;
-            kr[r] = k < n ? k : -1;                // -1: past the last row (no element)
             const int kk = k < n ? k : n - 1;
 #pragma unroll
             for (int d = 0; d < NDENS; ++d) {
-                xkj[r][d] = xj[d * n + kk];
                 xki[r][d] = xi[d * n + kk];
+#pragma unroll
+                for (int s = 0; s < NR; ++s) xkj[r][s][d] = xj[(s * NDENS + d) * n + kk];
             }
         }
         const int klast = k0 + 4 * (ROWS - 1) < n ? k0 + 4 * (ROWS - 1) : n - 1;
 #pragma unroll
-        for (int m = 0; m < M2; ++m) {
-            if (128 * m > klast) break;            // uniform across the wavefront
-            const int l0 = 2 * lane + 128 * m;
+        for (int m = 0; m < MC; ++m) {
+            if (W * m > klast) break;              // uniform across the wavefront
 #pragma unroll
             for (int r = 0; r < ROWS; ++r)
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const double graw = h ? (double)ga[r][m].y : (double)ga[r][m].x;
-                    const double g = l0 + h <= kr[r] ? graw : 0.0;
-                    jacc = fma(g, h ? da[r][m].y : da[r][m].x, jacc);
+                for (int h = 0; h < CPL; ++h) {
+                    const double dw = DwOf<CPL>::get(dd[r][m], h);
 #pragma unroll
-                    for (int d = 0; d < NDENS; ++d) {
-                        V[(2 * r) * NDENS + d] = fma(g, xjl[d][m][h], V[(2 * r) * NDENS + d]);
-                        V[(2 * r + 1) * NDENS + d] = fma(g, xil[d][m][h], V[(2 * r + 1) * NDENS + d]);
-                    }
-                    const double gc = l0 + h < kr[r] ? g : 0.0;    // the diagonal element has no column part
+                    for (int s = 0; s < NR; ++s) {
+                        // zero outside row k; the diagonal element l = k enters both the row and the column
+                        // part, and the epilogue takes one of the two out
+                        const double e = ColsOf<T, CPL>::get(g[r][s][m], h);
+                        jacc[s] = fma(e, dw, jacc[s]);
 #pragma unroll
-                    for (int d = 0; d < NDENS; ++d) {
-                        cI[d][m][h] = fma(gc, xkj[r][d], cI[d][m][h]);
-                        cJ[d][m][h] = fma(gc, xki[r][d], cJ[d][m][h]);
+                        for (int d = 0; d < NDENS; ++d) {
+                            V[(r * NA) * NDENS + d] = fma(e, xjl[s][d][m][h], V[(r * NA) * NDENS + d]);
+                            V[(r * NA + 1 + s) * NDENS + d] = fma(e, xil[d][m][h], V[(r * NA + 1 + s) * NDENS + d]);
+                            cI[d][m][h] = fma(e, xkj[r][s][d], cI[d][m][h]);
+                            cJ[s][d][m][h] = fma(e, xki[r][d], cJ[s][d][m][h]);
+                        }
                     }
                 }
         }
         Halving<NV, 32>::run(V, lane);
         if ((lane & ((1 << SH) - 1)) == 0) {
             const int idx = lane >> SH;
-            const int r = idx / (2 * NDENS), a = (idx / NDENS) & 1, d = idx % NDENS;
+            const int r = idx / (NA * NDENS), a = (idx / NDENS) % NA, d = idx % NDENS;
             const int k = k0 + 4 * r;
-            if (k < n) (a ? rJ : rI)[d * n + k] = V[0];   // row k belongs to this wavefront alone
+            // row k belongs to this wavefront alone
+            if (k < n) (a == 0 ? rI + d * n : rJ + ((a - 1) * NDENS + d) * n)[k] = V[0];
         }
-        if constexpr (PREFETCH) {
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r)
-#pragma unroll
-                for (int m = 0; m < M2; ++m) {
-                    ga[r][m] = gb[r][m];
-                    da[r][m] = db[r][m];
-                }
-        } else if (k0 + 4 * ROWS < n) {
-            load(k0 + 4 * ROWS, ga, da);
+    };
+    PV ga[ROWS][NR][MC];
+    DV da[ROWS][MC];
+    load(w, ga, da);
+    if constexpr (PREFETCH) {
+        // Two register buffers in turn: the next group's loads go out before this group's arithmetic and
+        // reduction, into the buffer the previous group has finished with (no copies between buffers, so
+        // nothing waits for the next group's data before its own arithmetic).
+        PV gb[ROWS][NR][MC];
+        DV db[ROWS][MC];
+        for (int k0 = w; k0 < n;) {
+            if (k0 + 4 * ROWS < n) load(k0 + 4 * ROWS, gb, db);
+            group(k0, ga, da);
+            k0 += 4 * ROWS;
+            if (k0 >= n) break;
+            if (k0 + 4 * ROWS < n) load(k0 + 4 * ROWS, ga, da);
+            group(k0, gb, db);
+            k0 += 4 * ROWS;
+        }
+    } else {
+        for (int k0 = w; k0 < n; k0 += 4 * ROWS) {
+            group(k0, ga, da);
+            if (k0 + 4 * ROWS < n) load(k0 + 4 * ROWS, ga, da);
         }
     }
-    // J_ij: wavefront sums, then the block's
+    // J_{i j_s}: wavefront sums, then the block's
+    Halving<JN, 32>::run(jacc, lane);               // lane L then holds the sum of jacc[L >> (6 - log2 JN)]
     {
-        double jv[1] = {jacc};
-        Halving<1, 32>::run(jv, lane);
-        jacc = jv[0];
+        constexpr int JS = 6 - log2c<JN>();
+        if ((lane & ((1 << JS) - 1)) == 0 && (lane >> JS) < NR) red[(lane >> JS) * 4 + w] = jacc[0];
     }
-    if (lane == 0) red[w] = jacc;
+#pragma unroll
+    for (int s = 0; s < NR; ++s)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            if (threadIdx.x + 256 * q < n) gdg[s * n + threadIdx.x + 256 * q] = gd[s][q];
     __syncthreads();
     // column parts of every wavefront
 #pragma unroll
     for (int d = 0; d < NDENS; ++d)
 #pragma unroll
-        for (int m = 0; m < M2; ++m)
+        for (int m = 0; m < MC; ++m)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int l = 2 * lane + 128 * m + h;
+            for (int h = 0; h < CPL; ++h) {
+                const int l = CPL * lane + W * m + h;
                 if (l < n) {
                     atomicAdd(&rI[d * n + l], cI[d][m][h]);
-                    atomicAdd(&rJ[d * n + l], cJ[d][m][h]);
+#pragma unroll
+                    for (int s = 0; s < NR; ++s) atomicAdd(&rJ[(s * NDENS + d) * n + l], cJ[s][d][m][h]);
                 }
             }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double v = (red[0] + red[1]) + (red[2] + red[3]);
+    if (threadIdx.x < nrow) {
+        const int s = threadIdx.x, j = j0 + s;
+        const double v = (red[s * 4] + red[s * 4 + 1]) + (red[s * 4 + 2] + red[s * 4 + 3]);
         J[(size_t)i * n + j] += v;
         if (i != j) J[(size_t)j * n + i] += v;
     }
+    // the diagonal element B_s[c][c] was added to K row c by both parts: one of them comes out here
     for (int t = threadIdx.x; t < NDENS * n; t += blockDim.x) {
         const int d = t / n, c = t - d * n;
-        atomicAdd(&K[d * nn + (size_t)i * n + c], rI[t]);
-        if (i != j) atomicAdd(&K[d * nn + (size_t)j * n + c], rJ[t]);
+        double vi = rI[t];
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {
+            if (s >= nrow) break;
+            const int j = j0 + s;
+            vi = fma(-gdg[s * n + c], xj[s * NDENS * n + t], vi);
+            if (i != j) atomicAdd(&K[d * nn + (size_t)j * n + c], fma(-gdg[s * n + c], xi[t], rJ[s * NDENS * n + t]));
+        }
+        atomicAdd(&K[d * nn + (size_t)i * n + c], vi);
     }
 }
 
-template <int NDENS, int ROWS, bool PREFETCH, typename T>
+// The kernel for MC column chunks, instantiated only where the in-core size limit (n <= kIncoreMaxN) and the
+// register bounds of shape_fits can reach it.
+template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
+hipError_t launch_incore_mc(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
+                            hipStream_t st) {
+    if constexpr (64 * CPL * (MC - 1) < kIncoreMaxN && (CPL == 2 || NDENS == 1 || MC <= 4) &&
+                  (NR == 1 || NDENS == 1 || 64 * CPL * (MC - 1) < 256)) {
+        const size_t lds = ((2 * NR + 2) * (size_t)NDENS * n + (size_t)NR * n + 4 * NR) * sizeof(double);
+        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MC, CPL, ROWS, PREFETCH, NR, T>),
+                           dim3((unsigned)blocks_before(n, NR)), dim3(256), lds, st, G, n, npad, D, Dw, J, K);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+template <int NDENS, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
 hipError_t launch_incore(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
                          hipStream_t st) {
-    const size_t lds = (4 * (size_t)NDENS * n + 4) * sizeof(double);
-    const dim3 grid((unsigned)npad), block(256);
-    switch ((n + 127) / 128) {
-#define HF_INCORE_CASE(MM)                                                                                        \
-    case MM:                                                                                                      \
-        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MM, ROWS, PREFETCH, T>), grid, block, lds, st, G, n, npad, D, \
-                           Dw, J, K);                                                                             \
-        break;
-        HF_INCORE_CASE(1)
-        HF_INCORE_CASE(2)
-        HF_INCORE_CASE(3)
-        HF_INCORE_CASE(4)
-#undef HF_INCORE_CASE
-        default:
-            return hipErrorInvalidValue;
+    switch ((n + 64 * CPL - 1) / (64 * CPL)) {
+        case 1: return launch_incore_mc<NDENS, 1, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        case 2: return launch_incore_mc<NDENS, 2, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        case 3: return launch_incore_mc<NDENS, 3, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        case 4: return launch_incore_mc<NDENS, 4, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        case 5: return launch_incore_mc<NDENS, 5, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        case 6: return launch_incore_mc<NDENS, 6, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        case 7: return launch_incore_mc<NDENS, 7, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        case 8: return launch_incore_mc<NDENS, 8, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
-// Tile shapes of the contraction: rows of one wavefront whose loads are issued together and whose
-// row sums are reduced together, and whether the next group's loads are issued before the current
-// group's arithmetic (more bytes in flight per wavefront for more registers, so fewer wavefronts).
-enum IncoreShape { kRows2 = 0, kRows4 = 1, kRows2Prefetch = 2, kIncoreShapes = 3 };
+// Tile shapes of the contraction: columns per lane, rows k of the pair triangle whose loads are issued
+// together and whose row sums are reduced together, whether the next group's loads are issued before the
+// current group's arithmetic (two register buffers in turn: more bytes in flight per wavefront for more
+// registers, so fewer wavefronts), and store rows per block (NR).
+struct ShapeDesc {
+    int cpl, rows, prefetch, nr;
+};
+constexpr ShapeDesc kShapes[] = {
+    {2, 2, 0, 1}, {2, 2, 1, 1}, {2, 1, 1, 1}, {1, 2, 0, 1}, {1, 2, 1, 1}, {2, 1, 0, 3}, {2, 1, 1, 3}, {2, 2, 0, 3},
+};
+constexpr int kIncoreShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+
+// Whether a shape has an instance for n and the density count: one column per lane needs ceil(n / 64)
+// column registers per density and array (at most 8, n <= 512, with one density; 4, n <= 256, with two),
+// and three store rows per block with two densities are kept to n <= 256 for registers.
+inline bool shape_fits(int shape, int n, int ndens) {
+    const ShapeDesc& d = kShapes[shape];
+    if (d.cpl == 1 && n > (ndens == 1 ? 512 : 256)) return false;
+    if (d.nr > 1 && ndens == 2 && n > 256) return false;
+    return true;
+}
 
 template <int NDENS, typename T>
 hipError_t launch_incore_nd(int shape, const T* G, int n, int64_t np, const double* D, const double* Dw, double* J,
                             double* K, hipStream_t st) {
-    if (shape == kRows4) return launch_incore<NDENS, 4, false>(G, n, np, D, Dw, J, K, st);
-    if (shape == kRows2Prefetch) return launch_incore<NDENS, 2, true>(G, n, np, D, Dw, J, K, st);
-    return launch_incore<NDENS, 2, false>(G, n, np, D, Dw, J, K, st);
+    if (!shape_fits(shape, n, NDENS)) shape = 0;
+    switch (shape) {
+        case 1: return launch_incore<NDENS, 2, 2, true, 1>(G, n, np, D, Dw, J, K, st);
+        case 2: return launch_incore<NDENS, 2, 1, true, 1>(G, n, np, D, Dw, J, K, st);
+        case 3: return launch_incore<NDENS, 1, 2, false, 1>(G, n, np, D, Dw, J, K, st);
+        case 4: return launch_incore<NDENS, 1, 2, true, 1>(G, n, np, D, Dw, J, K, st);
+        case 5: return launch_incore<NDENS, 2, 1, false, 3>(G, n, np, D, Dw, J, K, st);
+        case 6: return launch_incore<NDENS, 2, 1, true, 3>(G, n, np, D, Dw, J, K, st);
+        case 7: return launch_incore<NDENS, 2, 2, false, 3>(G, n, np, D, Dw, J, K, st);
+        default: return launch_incore<NDENS, 2, 2, false, 1>(G, n, np, D, Dw, J, K, st);
+    }
 }
 
 template <typename T>
@@ -281,7 +417,7 @@ hipError_t launch_incore_shape(int shape, int ndens, const T* G, int n, int64_t 
                       : launch_incore_nd<2>(shape, G, n, np, D, Dw, J, K, st);
 }
 
-// Autotuner: the tile shape is picked per (column chunks M2 = ceil(n/128), densities, store
+// Autotuner: the tile shape is picked per (64-column chunks ceil(n/64), densities, store
 // precision) by timing every candidate on the caller's own store, density and stream, once per
 // process (the table is shared by all contexts; a server's slots tune each key once).  Candidates
 // write to scratch J/K, so the build's outputs are untouched.
@@ -294,13 +430,13 @@ IncoreTuning& tuning() {
     return t;
 }
 
-int tuning_key(int n, int ndens, bool fp32) { return ((n + 127) / 128) * 100 + ndens * 10 + (fp32 ? 1 : 0); }
+int tuning_key(int n, int ndens, bool fp32) { return ((n + 63) / 64) * 100 + ndens * 10 + (fp32 ? 1 : 0); }
 
 // The shape the autotuner picked for this kind of build, or the default when it has not run (no timing here).
 int known_shape(int n, int ndens, bool fp32) {
     std::lock_guard<std::mutex> lock(tuning().mu);
     auto it = tuning().shape.find(tuning_key(n, ndens, fp32));
-    return it == tuning().shape.end() ? (int)kRows2 : it->second;
+    return it == tuning().shape.end() ? 0 : it->second;
 }
 
 template <typename T>
@@ -315,12 +451,13 @@ int tuned_shape(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double*
     const size_t nn = (size_t)n * n;
     DevBuf<double> sJ, sK;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    int best = kRows2;
+    int best = 0;
     float best_ms = 0;
     bool ok = sJ.alloc(nn) == hipSuccess && sK.alloc(2 * nn) == hipSuccess &&
               hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess;
     for (int shape = 0; shape < kIncoreShapes; ++shape) {
         if (!ok) break;
+        if (!shape_fits(shape, n, ndens)) continue;
         float t = 0;
         for (int rep = 0; rep < 3 && ok; ++rep) {          // first run warms the caches and the code
             ok = hipEventRecord(ev[0], st) == hipSuccess &&
@@ -338,7 +475,7 @@ int tuned_shape(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double*
     for (hipEvent_t e : ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipGetLastError();
-    if (!ok) return kRows2;                                // tuning failed: the default, not recorded
+    if (!ok) return 0;                                     // tuning failed: the default, not recorded
     std::lock_guard<std::mutex> lock(tuning().mu);
     tuning().shape[key] = best;
     return best;
@@ -392,7 +529,11 @@ int incore_jk_run(hf_ctx* ctx, int ndens, const double* G, const float* g32, con
                            ndens * nn);
     }
     // tile shape from the autotuner (first build of each kind in the process times the candidates)
-    const int shape = no_alloc ? known_shape(n, ndens, g32 != nullptr)
+    const int forced = ctx->opt.incore_shape;
+    if (forced >= kIncoreShapes) return ctx->fail(HF_E_ARG, "incore_shape must be -1 or 0.." +
+                                                             std::to_string(kIncoreShapes - 1));
+    const int shape = forced >= 0 ? forced
+                      : no_alloc  ? known_shape(n, ndens, g32 != nullptr)
                       : g32    ? tuned_shape(ctx, ndens, g32, n, np, d_D, Dw, st)
                                : tuned_shape(ctx, ndens, G, n, np, d_D, Dw, st);
     // statistics, and with profile = 1 events around the contraction (read at the next flush)

@@ -51,6 +51,14 @@ typedef struct hf_options {
                                   (-1, the default: every class; -2: no class kernel, only the in-core
                                   contraction); each timed launch is bracketed by two events, which cost
                                   the build a few per cent */
+    int incore_shape;          /* in-core contraction tile shape: -1 (the default) lets the autotuner time
+                                  the candidates once per process and size; 0..7 forces one, for tests of
+                                  every shape and for measurements.  (columns per lane, rows per group,
+                                  two register buffers in turn, store rows per block): 0 (2,2,no,1),
+                                  1 (2,2,yes,1), 2 (2,1,yes,1), 3 (1,2,no,1), 4 (1,2,yes,1), 5 (2,1,no,3),
+                                  6 (2,1,yes,3), 7 (2,2,no,3).  Shapes 3 and 4 need n <= 512 with one
+                                  density and n <= 256 with two, shapes 5..7 n <= 256 with two densities
+                                  (else shape 0 runs); other values fail with HF_E_ARG */
 } hf_options;
 
 void hf_default_options(hf_options* opt);

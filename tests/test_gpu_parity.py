@@ -361,6 +361,70 @@ def test_incore_jk_at_scale_matches_packed_contraction(gpu_engine_factory, name,
     assert np.abs(K - Kd).max() <= 1e-11 * max(1.0, np.abs(Kd).max())
 
 
+NSHAPES = 8        # tile shapes of the in-core contraction (hf_options.incore_shape 0..7)
+SHAPE_CASES = [
+    ("water4_631g", "6-31g.1.nw", g.water_cluster(4, seed=41)),            # 52 functions: one 64-column chunk
+    ("pentane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(5, seed=31)),       # 99: two chunks of 64, one of 128
+    ("water6_631gss", "6-31g_st__st_.nw", g.water_cluster(6, seed=32)),     # 150: three of 64, two of 128
+    ("water24_631g", "6-31g.1.nw", g.water_cluster(24, seed=42)),           # 312: five of 64, three of 128
+]
+
+
+@pytest.mark.parametrize("name,bfile,atoms", SHAPE_CASES, ids=[c[0] for c in SHAPE_CASES])
+def test_incore_every_shape_matches_direct(gpu_engine_factory, name, bfile, atoms):
+    """Every tile shape of the in-core contraction forced in turn (one or two columns per lane, one or two
+    rows per group, two register buffers in turn, one or three store rows per block), for one and two densities, on one pair-matrix store: against the direct J/K
+    build of the same densities at 1e-11 relative, and the shapes against each other; the autotuner may
+    pick any of them for a size."""
+    import hartreefock_amd as hf
+    from devmem import DevArray, sync
+    _print_case(name)
+    ref = gpu_engine_factory(schwarz_threshold=0.0).set_molecule(basis(bfile), atoms)
+    n = ref.nbf
+    lg = hf.lib().hf_pair_matrix_len(n)
+    dG = DevArray(lg, fill=0.0)
+    ref.eri_pairs_device(dG.ptr, lg)
+    rng = np.random.default_rng(23 + n)
+    X = rng.normal(size=(2, n, n)) * 0.1
+    D2 = X + X.transpose(0, 2, 1)
+    cases = []
+    for nd in (1, 2):
+        D = D2[:nd]
+        Jd, Kd = ref.build_jk(D if nd == 2 else D[0])
+        cases.append((nd, DevArray.of(D), Jd, Kd.reshape(nd, n, n)))
+    sync()
+    first = {}
+    for shape in range(NSHAPES):
+        eng = gpu_engine_factory(schwarz_threshold=0.0, incore_shape=shape).set_molecule(basis(bfile), atoms)
+        for nd, dD, Jd, Kd in cases:
+            dJ, dK = DevArray((n, n), fill=0.0), DevArray((nd, n, n), fill=0.0)
+            eng.build_jk_incore_device(dG.ptr, dD.ptr, dJ.ptr, dK.ptr, ndens=nd)
+            sync()
+            J, K = dJ.get(), dK.get()
+            dJ.free(); dK.free()
+            ej = np.abs(J - Jd).max() / max(1.0, np.abs(Jd).max())
+            ek = np.abs(K - Kd).max() / max(1.0, np.abs(Kd).max())
+            print("n %d shape %d ndens %d: rel |dJ| %.2e |dK| %.2e" % (n, shape, nd, ej, ek), flush=True)
+            assert ej <= 1e-11 and ek <= 1e-11, (shape, nd)
+            if nd in first:
+                J0, K0 = first[nd]
+                assert np.abs(J - J0).max() <= 1e-12 * max(1.0, np.abs(J0).max()), (shape, nd)
+                assert np.abs(K - K0).max() <= 1e-12 * max(1.0, np.abs(K0).max()), (shape, nd)
+            else:
+                first[nd] = (J, K)
+        eng.close()
+    # the shape count is pinned: one past the last shape is refused
+    eng = gpu_engine_factory(schwarz_threshold=0.0, incore_shape=NSHAPES).set_molecule(basis(bfile), atoms)
+    dJ, dK = DevArray((n, n), fill=0.0), DevArray((1, n, n), fill=0.0)
+    with pytest.raises(hf.HFError):
+        eng.build_jk_incore_device(dG.ptr, cases[0][1].ptr, dJ.ptr, dK.ptr, ndens=1)
+    eng.close()
+    for c in cases:
+        c[1].free()
+    dG.free()
+    ref.close()
+
+
 INCORE_SCF_CASES = [
     ("h2o_crawford_sto3g", "sto3g.txt", g.water_crawford(), False, dict(alpha=0.5, init_guess=0.7)),
     ("ethane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(2, seed=5), False, dict(alpha=0.5, init_guess=0.0)),
