@@ -33,7 +33,7 @@ class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("schwarz_threshold", ctypes.c_double),
                 ("prim_threshold", ctypes.c_double), ("exact_transfer", ctypes.c_int),
                 ("precision", ctypes.c_int), ("fp32_threshold", ctypes.c_double), ("profile", ctypes.c_int),
-                ("profile_class", ctypes.c_int), ("incore_shape", ctypes.c_int)]
+                ("profile_class", ctypes.c_int), ("incore_shape", ctypes.c_int), ("deterministic", ctypes.c_int)]
 
 
 class BasisSoa(ctypes.Structure):

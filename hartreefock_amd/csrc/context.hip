@@ -17,7 +17,12 @@ __global__ void fill_kernel(double* __restrict__ p, size_t n, double v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// out = in + in^T for `count` n x n matrices
+// Deterministic mode: the fixed-point accumulator pairs fx[2 i], fx[2 i + 1] folded into out[i]
+__global__ void fx_finalize_kernel(const long long* __restrict__ fx, double* __restrict__ out, size_t len) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = fx_value(fx[2 * i], fx[2 * i + 1]);
+}
+
 // out = in + in^T (per matrix), or out += in + in^T
 __global__ void symmetrize_kernel(const double* __restrict__ in, double* __restrict__ out, int n, int count,
                                   int accumulate) {
@@ -634,7 +639,18 @@ int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, do
     sink.J = ctx->d_Jp.p;
     sink.K = ctx->d_Kp.p;
     sink.boys32 = ctx->d_boys32.p;
+    const size_t nfx = (1 + (size_t)ndens) * nn;   // J' then K' elements
+    if (ctx->opt.deterministic) {
+        HF_HIP_CHECK(ctx, ctx->d_fx.alloc(2 * 3 * nn));
+        hipLaunchKernelGGL(fill_kernel, dim3(grid_for(2 * nfx)), dim3(256), 0, st, (double*)ctx->d_fx.p, 2 * nfx, 0.0);
+        sink.fx = ctx->d_fx.p;
+    }
     if (int rc = run_classes(ctx, jobs, sink, st, true)) return rc;
+    if (ctx->opt.deterministic) {
+        hipLaunchKernelGGL(fx_finalize_kernel, dim3(grid_for(nn)), dim3(256), 0, st, ctx->d_fx.p, ctx->d_Jp.p, nn);
+        hipLaunchKernelGGL(fx_finalize_kernel, dim3(grid_for(ndens * nn)), dim3(256), 0, st, ctx->d_fx.p + 2 * nn,
+                           ctx->d_Kp.p, ndens * nn);
+    }
     hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(nn)), dim3(256), 0, st, ctx->d_Jp.p, d_J, n, 1,
                        accumulate ? 1 : 0);
     hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for(ndens * nn)), dim3(256), 0, st, ctx->d_Kp.p, d_K, n, ndens,
@@ -686,6 +702,7 @@ void hf_default_options(hf_options* o) {
     o->profile = 0;
     o->profile_class = -1;
     o->incore_shape = -1;
+    o->deterministic = 0;
 }
 
 int hf_create(const hf_options* opt, hf_ctx** out) {

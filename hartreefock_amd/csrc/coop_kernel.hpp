@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
                         for (int k = 0; k < sink.ndens; ++k) dsum += sink.D[k * nn + (size_t)(c0 + ic) * n + d0 + id];
                         acc += v(((ia * NB + ib) * NC + ic) * ND + id) * dsum;
                     }
-                unsafeAtomicAdd(&sink.J[(size_t)(a0 + ia) * n + b0 + ib], w2 * acc);
+                sink_add_J(sink, (size_t)(a0 + ia) * n + b0 + ib, w2 * acc);
             } else {
                 const int rr = r - NA * NB, ic = rr / ND, id = rr % ND;
                 for (int ia = 0; ia < NA; ++ia)
@@ -215,13 +215,12 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
                         for (int k = 0; k < sink.ndens; ++k) dsum += sink.D[k * nn + (size_t)(a0 + ia) * n + b0 + ib];
                         acc += v(((ia * NB + ib) * NC + ic) * ND + id) * dsum;
                     }
-                unsafeAtomicAdd(&sink.J[(size_t)(c0 + ic) * n + d0 + id], w2 * acc);
+                sink_add_J(sink, (size_t)(c0 + ic) * n + d0 + id, w2 * acc);
             }
         }
         // Exchange per density: K'_ac, K'_ad, K'_bc, K'_bd
         for (int k = 0; k < sink.ndens; ++k) {
             const double* __restrict__ Dm = sink.D + k * nn;
-            double* __restrict__ Km = sink.K + k * nn;
             constexpr int R1 = NA * NC, R2 = R1 + NA * ND, R3 = R2 + NB * NC, R4 = R3 + NB * ND;
             for (int r = lane; r < R4; r += 64) {
                 double acc = 0;
@@ -230,25 +229,25 @@ __global__ void __launch_bounds__(64) eri_coop_kernel(TaskList tl, ShellsDev sh,
                     for (int ib = 0; ib < NB; ++ib)
                         for (int id = 0; id < ND; ++id)
                             acc += v(((ia * NB + ib) * NC + ic) * ND + id) * Dm[(size_t)(b0 + ib) * n + d0 + id];
-                    unsafeAtomicAdd(&Km[(size_t)(a0 + ia) * n + c0 + ic], w * acc);
+                    sink_add_K(sink, k, (size_t)(a0 + ia) * n + c0 + ic, w * acc);
                 } else if (r < R2) {
                     const int rr = r - R1, ia = rr / ND, id = rr % ND;
                     for (int ib = 0; ib < NB; ++ib)
                         for (int ic = 0; ic < NC; ++ic)
                             acc += v(((ia * NB + ib) * NC + ic) * ND + id) * Dm[(size_t)(b0 + ib) * n + c0 + ic];
-                    unsafeAtomicAdd(&Km[(size_t)(a0 + ia) * n + d0 + id], w * acc);
+                    sink_add_K(sink, k, (size_t)(a0 + ia) * n + d0 + id, w * acc);
                 } else if (r < R3) {
                     const int rr = r - R2, ib = rr / NC, ic = rr % NC;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int id = 0; id < ND; ++id)
                             acc += v(((ia * NB + ib) * NC + ic) * ND + id) * Dm[(size_t)(a0 + ia) * n + d0 + id];
-                    unsafeAtomicAdd(&Km[(size_t)(b0 + ib) * n + c0 + ic], w * acc);
+                    sink_add_K(sink, k, (size_t)(b0 + ib) * n + c0 + ic, w * acc);
                 } else {
                     const int rr = r - R3, ib = rr / ND, id = rr % ND;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int ic = 0; ic < NC; ++ic)
                             acc += v(((ia * NB + ib) * NC + ic) * ND + id) * Dm[(size_t)(a0 + ia) * n + c0 + ic];
-                    unsafeAtomicAdd(&Km[(size_t)(b0 + ib) * n + d0 + id], w * acc);
+                    sink_add_K(sink, k, (size_t)(b0 + ib) * n + d0 + id, w * acc);
                 }
             }
         }

@@ -190,7 +190,7 @@ __device__ __forceinline__ void sink_jk(const V* __restrict__ v, int a0, int b0,
                 for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
                     for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dcd[ic * ND + id];
-                unsafeAtomicAdd(&s.J[(size_t)(a0 + ia) * n + b0 + ib], w2 * acc);
+                sink_add_J(s, (size_t)(a0 + ia) * n + b0 + ib, w2 * acc);
             }
 #pragma unroll
         for (int ic = 0; ic < NC; ++ic)
@@ -201,13 +201,12 @@ __device__ __forceinline__ void sink_jk(const V* __restrict__ v, int a0, int b0,
                 for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
                     for (int ib = 0; ib < NB; ++ib) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dab[ia * NB + ib];
-                unsafeAtomicAdd(&s.J[(size_t)(c0 + ic) * n + d0 + id], w2 * acc);
+                sink_add_J(s, (size_t)(c0 + ic) * n + d0 + id, w2 * acc);
             }
     }
     // Exchange, per density: K'_ac += w (ab|cd) D_bd, K'_ad += w D_bc, K'_bc += w D_ad, K'_bd += w D_ac
     for (int k = 0; k < s.ndens; ++k) {
         const double* __restrict__ D = s.D + k * nn;
-        double* __restrict__ K = s.K + k * nn;
         double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
@@ -232,7 +231,7 @@ __device__ __forceinline__ void sink_jk(const V* __restrict__ v, int a0, int b0,
                 for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
                     for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dbd[ib * ND + id];
-                unsafeAtomicAdd(&K[(size_t)(a0 + ia) * n + c0 + ic], w * acc);
+                sink_add_K(s, k, (size_t)(a0 + ia) * n + c0 + ic, w * acc);
             }
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia)
@@ -243,7 +242,7 @@ __device__ __forceinline__ void sink_jk(const V* __restrict__ v, int a0, int b0,
                 for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
                     for (int ic = 0; ic < NC; ++ic) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dbc[ib * NC + ic];
-                unsafeAtomicAdd(&K[(size_t)(a0 + ia) * n + d0 + id], w * acc);
+                sink_add_K(s, k, (size_t)(a0 + ia) * n + d0 + id, w * acc);
             }
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib)
@@ -254,7 +253,7 @@ __device__ __forceinline__ void sink_jk(const V* __restrict__ v, int a0, int b0,
                 for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
                     for (int id = 0; id < ND; ++id) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dad[ia * ND + id];
-                unsafeAtomicAdd(&K[(size_t)(b0 + ib) * n + c0 + ic], w * acc);
+                sink_add_K(s, k, (size_t)(b0 + ib) * n + c0 + ic, w * acc);
             }
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib)
@@ -265,7 +264,7 @@ __device__ __forceinline__ void sink_jk(const V* __restrict__ v, int a0, int b0,
                 for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
                     for (int ic = 0; ic < NC; ++ic) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dac[ia * NC + ic];
-                unsafeAtomicAdd(&K[(size_t)(b0 + ib) * n + d0 + id], w * acc);
+                sink_add_K(s, k, (size_t)(b0 + ib) * n + d0 + id, w * acc);
             }
     }
 }
@@ -320,7 +319,7 @@ __device__ __forceinline__ void sink_jk_rows(const V* __restrict__ v, double* __
                 for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
                     for (int ib = 0; ib < NB; ++ib) acc += v[((ia * NB + ib) * NC + ic) * ND + id] * Dab[ia * NB + ib];
-                unsafeAtomicAdd(&s.J[(size_t)(c0 + ic) * n + d0 + id], w2 * acc);
+                sink_add_J(s, (size_t)(c0 + ic) * n + d0 + id, w2 * acc);
             }
     }
     for (int k = 0; k < s.ndens; ++k) {
@@ -705,7 +704,8 @@ inline hipError_t launch_eri(const TaskList& tl, const ShellsDev& sh, const Prim
         // up to 4 bra pairs per block, within 40 KB of LDS (a block of 256 tasks rarely spans more)
         const size_t slot_bytes = (size_t)sink.ndens * (C::NA + C::NB) * sink.n * sizeof(double);
         const size_t cap = 40 * 1024;
-        s.lds_slots = (int)std::min<size_t>(4, cap / slot_bytes);
+        // (none in deterministic mode: the slots' LDS atomics add in no fixed order)
+        s.lds_slots = sink.fx ? 0 : (int)std::min<size_t>(4, cap / slot_bytes);
         lds = s.lds_slots * slot_bytes;
         if (boys_in_lds<C>()) lds += kBoysLdsBytes;    // (the FP32 slice needs less)
     }

@@ -174,6 +174,40 @@ struct SinkArgs {
     double* Qout;            // MODE_DIAG output, one per bra pair
     int lds_slots;           // MODE_JK: bra pairs per block whose exchange rows are staged in LDS
     const float* boys32;     // MODE_JK32: FP32 copy of the Boys table
+    long long* fx;           // deterministic mode: fixed-point accumulators (see fx_add) of J' (n x n) then
+                             // K' (ndens x n x n), two per element; null: FP64 atomics into J and K
 };
+
+#if defined(__HIP__)   // (the header is also read by host-only sources)
+// Deterministic accumulation.  A contribution v is added as the 80-fractional-bit fixed-point number
+// round(v 2^80), split over two 64-bit integers: fx[2 idx] += floor(v 2^40) and fx[2 idx + 1] += the
+// remaining fraction times 2^40 (in [0, 2^40]).  Integer addition is associative, so the sums do not
+// depend on the order in which threads add, and a build gives the same bits every time.  |v| < 2^23 per
+// contribution; a sum of up to 2^23 contributions per element keeps the low word in range.  fx_value
+// folds the two words back into a double.
+__device__ __forceinline__ void fx_add(long long* fx, size_t idx, double v) {
+    const double t = v * 0x1p40;
+    const double h = floor(t);
+    atomicAdd((unsigned long long*)fx + 2 * idx, (unsigned long long)(long long)h);
+    atomicAdd((unsigned long long*)fx + 2 * idx + 1, (unsigned long long)(long long)rint((t - h) * 0x1p40));
+}
+__device__ __forceinline__ double fx_value(long long hi, long long lo) {
+    const long long carry = lo >> 40;                  // lo = carry 2^40 + r, 0 <= r < 2^40
+    return (double)(hi + carry) * 0x1p-40 + (double)(lo - carry * (1LL << 40)) * 0x1p-80;
+}
+__device__ __forceinline__ void sink_add_J(const SinkArgs& s, size_t idx, double v) {
+    if (s.fx)
+        fx_add(s.fx, idx, v);
+    else
+        unsafeAtomicAdd(&s.J[idx], v);
+}
+__device__ __forceinline__ void sink_add_K(const SinkArgs& s, int k, size_t idx, double v) {
+    const size_t nn = (size_t)s.n * s.n;
+    if (s.fx)
+        fx_add(s.fx, (size_t)(k + 1) * nn + idx, v);
+    else
+        unsafeAtomicAdd(&s.K[(size_t)k * nn + idx], v);
+}
+#endif
 
 }  // namespace hfgpu

@@ -160,6 +160,7 @@ struct hf_ctx {
     int incore_pending = -1;               // stat index whose last launch's events are not yet read
     // J/K scratch
     DevBuf<double> d_Jp, d_Kp, d_D, d_J, d_K;
+    DevBuf<long long> d_fx;          // deterministic mode: fixed-point J'/K' (direct) or K (in-core) accumulators
     // one-electron kernel inputs: per-shell L/bf/prim0/nprim, and alpha|coef|Z|atom xyz
     DevBuf<int> d_oe_L;
     DevBuf<double> d_oe_prim;

@@ -129,7 +129,8 @@ __host__ __device__ inline int64_t blocks_before(int i, int nr) {
 template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
 __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t npad,
                                                         const double* __restrict__ D, const double* __restrict__ Dw,
-                                                        double* __restrict__ J, double* __restrict__ K) {
+                                                        double* __restrict__ J, double* __restrict__ K,
+                                                        long long* __restrict__ fxK) {
     extern __shared__ double lds[];
     constexpr int W = 64 * CPL;
     const int64_t blk = blockIdx.x;
@@ -304,21 +305,26 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
         for (int q = 0; q < 2; ++q)
             if (threadIdx.x + 256 * q < n) gdg[s * n + threadIdx.x + 256 * q] = gd[s][q];
     __syncthreads();
-    // column parts of every wavefront
+    // column parts of every wavefront (deterministic mode: the wavefronts in turn, so that every element
+    // receives their parts in one order)
+    for (int turn = 0; turn < (fxK ? 4 : 1); ++turn) {
+        if (!fxK || w == turn) {
 #pragma unroll
-    for (int d = 0; d < NDENS; ++d)
+            for (int d = 0; d < NDENS; ++d)
 #pragma unroll
-        for (int m = 0; m < MC; ++m)
+                for (int m = 0; m < MC; ++m)
 #pragma unroll
-            for (int h = 0; h < CPL; ++h) {
-                const int l = CPL * lane + W * m + h;
-                if (l < n) {
-                    atomicAdd(&rI[d * n + l], cI[d][m][h]);
+                    for (int h = 0; h < CPL; ++h) {
+                        const int l = CPL * lane + W * m + h;
+                        if (l < n) {
+                            atomicAdd(&rI[d * n + l], cI[d][m][h]);
 #pragma unroll
-                    for (int s = 0; s < NR; ++s) atomicAdd(&rJ[(s * NDENS + d) * n + l], cJ[s][d][m][h]);
-                }
-            }
-    __syncthreads();
+                            for (int s = 0; s < NR; ++s) atomicAdd(&rJ[(s * NDENS + d) * n + l], cJ[s][d][m][h]);
+                        }
+                    }
+        }
+        __syncthreads();
+    }
     if (threadIdx.x < nrow) {
         const int s = threadIdx.x, j = j0 + s;
         const double v = (red[s * 4] + red[s * 4 + 1]) + (red[s * 4 + 2] + red[s * 4 + 3]);
@@ -334,9 +340,27 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
             if (s >= nrow) break;
             const int j = j0 + s;
             vi = fma(-gdg[s * n + c], xj[s * NDENS * n + t], vi);
-            if (i != j) atomicAdd(&K[d * nn + (size_t)j * n + c], fma(-gdg[s * n + c], xi[t], rJ[s * NDENS * n + t]));
+            if (i != j) {
+                const double vj = fma(-gdg[s * n + c], xi[t], rJ[s * NDENS * n + t]);
+                if (fxK)
+                    hfgpu::fx_add(fxK, d * nn + (size_t)j * n + c, vj);
+                else
+                    atomicAdd(&K[d * nn + (size_t)j * n + c], vj);
+            }
         }
-        atomicAdd(&K[d * nn + (size_t)i * n + c], vi);
+        if (fxK)
+            hfgpu::fx_add(fxK, d * nn + (size_t)i * n + c, vi);
+        else
+            atomicAdd(&K[d * nn + (size_t)i * n + c], vi);
+    }
+}
+
+// Deterministic mode: K (+)= the fixed-point sums
+__global__ void incore_fx_finalize_kernel(const long long* __restrict__ fx, double* __restrict__ K, size_t len,
+                                          int accumulate) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = hfgpu::fx_value(fx[2 * i], fx[2 * i + 1]);
+        K[i] = accumulate ? K[i] + v : v;
     }
 }
 
@@ -344,12 +368,12 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
 // register bounds of shape_fits can reach it.
 template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
 hipError_t launch_incore_mc(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
-                            hipStream_t st) {
+                            long long* fxK, hipStream_t st) {
     if constexpr (64 * CPL * (MC - 1) < kIncoreMaxN && (CPL == 2 || NDENS == 1 || MC <= 4) &&
                   (NR == 1 || NDENS == 1 || 64 * CPL * (MC - 1) < 256)) {
         const size_t lds = ((2 * NR + 2) * (size_t)NDENS * n + (size_t)NR * n + 4 * NR) * sizeof(double);
         hipLaunchKernelGGL((incore_jk_kernel<NDENS, MC, CPL, ROWS, PREFETCH, NR, T>),
-                           dim3((unsigned)blocks_before(n, NR)), dim3(256), lds, st, G, n, npad, D, Dw, J, K);
+                           dim3((unsigned)blocks_before(n, NR)), dim3(256), lds, st, G, n, npad, D, Dw, J, K, fxK);
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;
@@ -358,16 +382,16 @@ hipError_t launch_incore_mc(const T* G, int n, int64_t npad, const double* D, co
 
 template <int NDENS, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
 hipError_t launch_incore(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
-                         hipStream_t st) {
+                         long long* fxK, hipStream_t st) {
     switch ((n + 64 * CPL - 1) / (64 * CPL)) {
-        case 1: return launch_incore_mc<NDENS, 1, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
-        case 2: return launch_incore_mc<NDENS, 2, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
-        case 3: return launch_incore_mc<NDENS, 3, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
-        case 4: return launch_incore_mc<NDENS, 4, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
-        case 5: return launch_incore_mc<NDENS, 5, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
-        case 6: return launch_incore_mc<NDENS, 6, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
-        case 7: return launch_incore_mc<NDENS, 7, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
-        case 8: return launch_incore_mc<NDENS, 8, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, st);
+        case 1: return launch_incore_mc<NDENS, 1, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
+        case 2: return launch_incore_mc<NDENS, 2, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
+        case 3: return launch_incore_mc<NDENS, 3, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
+        case 4: return launch_incore_mc<NDENS, 4, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
+        case 5: return launch_incore_mc<NDENS, 5, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
+        case 6: return launch_incore_mc<NDENS, 6, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
+        case 7: return launch_incore_mc<NDENS, 7, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
+        case 8: return launch_incore_mc<NDENS, 8, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -396,25 +420,25 @@ inline bool shape_fits(int shape, int n, int ndens) {
 
 template <int NDENS, typename T>
 hipError_t launch_incore_nd(int shape, const T* G, int n, int64_t np, const double* D, const double* Dw, double* J,
-                            double* K, hipStream_t st) {
+                            double* K, long long* fxK, hipStream_t st) {
     if (!shape_fits(shape, n, NDENS)) shape = 0;
     switch (shape) {
-        case 1: return launch_incore<NDENS, 2, 2, true, 1>(G, n, np, D, Dw, J, K, st);
-        case 2: return launch_incore<NDENS, 2, 1, true, 1>(G, n, np, D, Dw, J, K, st);
-        case 3: return launch_incore<NDENS, 1, 2, false, 1>(G, n, np, D, Dw, J, K, st);
-        case 4: return launch_incore<NDENS, 1, 2, true, 1>(G, n, np, D, Dw, J, K, st);
-        case 5: return launch_incore<NDENS, 2, 1, false, 3>(G, n, np, D, Dw, J, K, st);
-        case 6: return launch_incore<NDENS, 2, 1, true, 3>(G, n, np, D, Dw, J, K, st);
-        case 7: return launch_incore<NDENS, 2, 2, false, 3>(G, n, np, D, Dw, J, K, st);
-        default: return launch_incore<NDENS, 2, 2, false, 1>(G, n, np, D, Dw, J, K, st);
+        case 1: return launch_incore<NDENS, 2, 2, true, 1>(G, n, np, D, Dw, J, K, fxK, st);
+        case 2: return launch_incore<NDENS, 2, 1, true, 1>(G, n, np, D, Dw, J, K, fxK, st);
+        case 3: return launch_incore<NDENS, 1, 2, false, 1>(G, n, np, D, Dw, J, K, fxK, st);
+        case 4: return launch_incore<NDENS, 1, 2, true, 1>(G, n, np, D, Dw, J, K, fxK, st);
+        case 5: return launch_incore<NDENS, 2, 1, false, 3>(G, n, np, D, Dw, J, K, fxK, st);
+        case 6: return launch_incore<NDENS, 2, 1, true, 3>(G, n, np, D, Dw, J, K, fxK, st);
+        case 7: return launch_incore<NDENS, 2, 2, false, 3>(G, n, np, D, Dw, J, K, fxK, st);
+        default: return launch_incore<NDENS, 2, 2, false, 1>(G, n, np, D, Dw, J, K, fxK, st);
     }
 }
 
 template <typename T>
 hipError_t launch_incore_shape(int shape, int ndens, const T* G, int n, int64_t np, const double* D,
-                               const double* Dw, double* J, double* K, hipStream_t st) {
-    return ndens == 1 ? launch_incore_nd<1>(shape, G, n, np, D, Dw, J, K, st)
-                      : launch_incore_nd<2>(shape, G, n, np, D, Dw, J, K, st);
+                               const double* Dw, double* J, double* K, long long* fxK, hipStream_t st) {
+    return ndens == 1 ? launch_incore_nd<1>(shape, G, n, np, D, Dw, J, K, fxK, st)
+                      : launch_incore_nd<2>(shape, G, n, np, D, Dw, J, K, fxK, st);
 }
 
 // Autotuner: the tile shape is picked per (64-column chunks ceil(n/64), densities, store
@@ -461,7 +485,7 @@ int tuned_shape(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double*
         float t = 0;
         for (int rep = 0; rep < 3 && ok; ++rep) {          // first run warms the caches and the code
             ok = hipEventRecord(ev[0], st) == hipSuccess &&
-                 launch_incore_shape(shape, ndens, G, n, np, D, Dw, sJ.p, sK.p, st) == hipSuccess &&
+                 launch_incore_shape(shape, ndens, G, n, np, D, Dw, sJ.p, sK.p, nullptr, st) == hipSuccess &&
                  hipEventRecord(ev[1], st) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess;
             float ms = 0;
             if (ok) ok = hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess;
@@ -555,12 +579,26 @@ int incore_jk_run(hf_ctx* ctx, int ndens, const double* G, const float* g32, con
     ks.launches += 1;
     ks.eris += (double)np * (double)np;                                  // elements streamed
     ks.flops += (double)np * (double)np * (2.0 + 8.0 * ndens);           // J, and row + column parts of K
+    // deterministic mode: K accumulates in fixed point (fx_add) and is folded in after the contraction
+    long long* fxK = nullptr;
+    if (ctx->opt.deterministic) {
+        if (no_alloc)
+            return ctx->fail(HF_E_UNSUPPORTED, "deterministic mode needs an accumulator the allocation-free entry "
+                                               "point does not have");
+        HF_HIP_CHECK(ctx, ctx->d_fx.alloc(2 * 3 * nn));
+        hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((2 * ndens * nn + 255) / 256)), dim3(256), 0, st,
+                           (double*)ctx->d_fx.p, 2 * ndens * nn);
+        fxK = ctx->d_fx.p;
+    }
     hipError_t e;
     if (g32)
-        e = launch_incore_shape(shape, ndens, g32, n, np, d_D, Dw, d_J, d_K, st);
+        e = launch_incore_shape(shape, ndens, g32, n, np, d_D, Dw, d_J, d_K, fxK, st);
     else
-        e = launch_incore_shape(shape, ndens, G, n, np, d_D, Dw, d_J, d_K, st);
+        e = launch_incore_shape(shape, ndens, G, n, np, d_D, Dw, d_J, d_K, fxK, st);
     if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("in-core J/K: ") + hipGetErrorString(e));
+    if (fxK)
+        hipLaunchKernelGGL(incore_fx_finalize_kernel, dim3((unsigned)std::min<size_t>((ndens * nn + 255) / 256, 4096)),
+                           dim3(256), 0, st, fxK, d_K, ndens * nn, accumulate ? 1 : 0);
     if (prof) {
         HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[1], st));
         ctx->incore_pending = f;

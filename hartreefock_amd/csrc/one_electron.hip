@@ -82,6 +82,7 @@ __host__ __device__ constexpr double dfact(int n) { return n > 1 ? n * dfact(n -
 
 struct OneEArgs {
     int nshell, natoms, n;
+    int atom0;               // nucleus of blockIdx.y = 0 in the launches after the first
     const double* sxyz;
     const int* sL;
     const int* sbf;
@@ -106,7 +107,7 @@ template <int LA, int LB, bool FIRST>
 __global__ void __launch_bounds__(128) one_electron_kernel(OneEArgs a, const int2* __restrict__ pairs, int npairs) {
     constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2, L = LA + LB, JM = FIRST ? LB + 2 : LB;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = FIRST ? 0 : blockIdx.y + 1;           // nucleus
+    const int c = FIRST ? 0 : blockIdx.y + a.atom0;     // nucleus
     if (t >= npairs) return;
     constexpr bool first = FIRST;
     const int sa = pairs[t].x, sb = pairs[t].y;
@@ -278,6 +279,7 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
         ctx->oe_nprim = al.size();
     }
     OneEArgs a;
+    a.atom0 = 1;
     a.nshell = ns;
     a.natoms = (int)mol.Z.size();
     a.n = mol.nbf;
@@ -321,6 +323,9 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
         ctx->oe_valid = true;
     }
     const size_t nn = (size_t)mol.nbf * mol.nbf;
+    // Deterministic mode: one launch per nucleus, in order, so that every V element receives its nuclei's
+    // terms in the same order each time (within one launch each shell pair's elements have one writer).
+    const bool det = ctx->opt.deterministic != 0;
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, st, dV, nn);
     for (int c = 0; c < kOeClasses && a.natoms > 0; ++c) {
         const int off = ctx->oe_class_off[c], np = ctx->oe_class_off[c + 1] - off;
@@ -331,7 +336,14 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
 #define HF_OE_LAUNCH(C, LA, LB)                                                                              \
     case C:                                                                                                  \
         hipLaunchKernelGGL((one_electron_kernel<LA, LB, true>), g1, dim3(128), 0, st, a, pl, np);           \
-        if (a.natoms > 1) hipLaunchKernelGGL((one_electron_kernel<LA, LB, false>), gr, dim3(128), 0, st, a, pl, np); \
+        if (a.natoms > 1 && !det) {                                                                          \
+            a.atom0 = 1;                                                                                     \
+            hipLaunchKernelGGL((one_electron_kernel<LA, LB, false>), gr, dim3(128), 0, st, a, pl, np);       \
+        }                                                                                                    \
+        for (int at = 1; det && at < a.natoms; ++at) {                                                       \
+            a.atom0 = at;                                                                                    \
+            hipLaunchKernelGGL((one_electron_kernel<LA, LB, false>), g1, dim3(128), 0, st, a, pl, np);       \
+        }                                                                                                    \
         break;
             HF_OE_LAUNCH(0, 0, 0)
             HF_OE_LAUNCH(1, 1, 0)

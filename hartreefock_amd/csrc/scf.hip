@@ -562,7 +562,7 @@ void hf_scf_workspace_free(hf_ctx* ctx) {
 
 double hf_ctx_held_bytes(const hf_ctx* ctx) {
     double b = (double)ctx->d_G.n * 8.0 + (double)ctx->d_G32.n * 4.0 + (double)ctx->d_Dw.n * 8.0 +
-               8.0 * (double)(ctx->d_Jp.n + ctx->d_Kp.n + ctx->d_D.n + ctx->d_J.n + ctx->d_K.n);
+               8.0 * (double)(ctx->d_Jp.n + ctx->d_Kp.n + ctx->d_D.n + ctx->d_J.n + ctx->d_K.n + ctx->d_fx.n);
     if (const ScfWorkspace* w = ctx->scf_ws) {
         const ScfState& st = w->st;
         for (const DevBuf<double>* m : {&st.S, &st.h, &st.X, &st.F0, &st.Fa, &st.Fb, &st.J, &st.K, &st.D, &st.Dn,

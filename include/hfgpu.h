@@ -59,6 +59,12 @@ typedef struct hf_options {
                                   6 (2,1,yes,3), 7 (2,2,no,3).  Shapes 3 and 4 need n <= 512 with one
                                   density and n <= 256 with two, shapes 5..7 n <= 256 with two densities
                                   (else shape 0 runs); other values fail with HF_E_ARG */
+    int deterministic;         /* 1: J/K builds and the nuclear-attraction matrix accumulate in a fixed-point
+                                  form whose sums do not depend on the order of the GPU's additions (two
+                                  64-bit integers per element, 2^-80 resolution), so repeated runs give
+                                  bit-identical energies; the FP64 atomics' order varies from run to run.
+                                  Slower (two integer atomics per contribution, no LDS staging of exchange
+                                  rows); not available to hf_build_jk_incore_device_noalloc */
 } hf_options;
 
 void hf_default_options(hf_options* opt);
