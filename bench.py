@@ -123,13 +123,14 @@ SERVE_SLOTS = 8    # bench.py --slots N changes it (a one-slot run serves reques
 
 
 def serve(hf, bfile, reqs, prec, rate=None, slots=None, incore=1, tag="", profile=0, profile_class=-1,
-          seed=77, max_iterations=300):
+          seed=77, max_iterations=300, deterministic=0):
     """Serve reqs = [(tag, atoms)]: all queued at once (rate None) or as a Poisson process at `rate`/s.  Returns the
     results by tag, the wall time from the first submission to the last completion and the summed kernel
     statistics."""
     import numpy as np
-    srv = hf.Server(slots=slots or SERVE_SLOTS, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1, profile=profile,
-                                              profile_class=profile_class),
+    srv = hf.Server(slots=slots or SERVE_SLOTS, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1,
+                                                             profile=profile, profile_class=profile_class,
+                                                             deterministic=deterministic),
                     scf=dict(SCF_KW, incore=incore, max_iterations=max_iterations))
     rng = np.random.default_rng(seed)
     got, shown = {}, 0
@@ -225,8 +226,13 @@ def serving_bench(a):
     res_d, wall_d, st_d = serve(hf, bfile, timed, 0, tag="fp64")
     conv_d = {t: r.energy for t, r in res_d.items() if r.status == 0 and r.converged}
     conv_m = {t: r.energy for t, r in res_m.items() if r.status == 0 and r.converged}
-    good_m = [t for t, e in conv_m.items() if t in conv_d and abs(e - conv_d[t]) <= 1e-6]
-    dE = [abs(conv_m[t] - conv_d[t]) for t in conv_m if t in conv_d]
+    # the |dE| reference: the same requests in FP64 deterministic mode (fixed-point J/K and V sums, whose value
+    # does not depend on the order of the GPU's additions); untimed
+    res_r, wall_r, _ = serve(hf, bfile, timed, 0, tag="fp64 deterministic reference", deterministic=1)
+    conv_r = {t: r.energy for t, r in res_r.items() if r.status == 0 and r.converged}
+    good_m = [t for t, e in conv_m.items() if t in conv_r and abs(e - conv_r[t]) <= 1e-6]
+    dE = [abs(conv_m[t] - conv_r[t]) for t in conv_m if t in conv_r]
+    dE_d = [abs(conv_d[t] - conv_r[t]) for t in conv_d if t in conv_r]
     cap_m, cap_d = len(good_m) / wall_m, len(conv_d) / wall_d
     lat = {}
     for prec, cap in ((1, cap_m), (0, cap_d)):
@@ -265,9 +271,12 @@ def serving_bench(a):
                               "latency", "parallelism": "single GPU"},
         "mixed": {"molecules_per_s": cap_m, "wall_s": wall_m, "converged": len(conv_m), "within_1e-6_of_fp64":
                   len(good_m), "max_abs_dE_vs_fp64_Eh": max(dE) if dE else None,
+                  "dE_reference": "FP64 deterministic mode, same requests (%d converged)" % len(conv_r),
                   "latency": lat["mixed"], "eri_per_s": eris_m / wall_m},
         "fp64": {"molecules_per_s": cap_d, "wall_s": wall_d, "converged": len(conv_d), "latency": lat["fp64"],
+                 "max_abs_dE_vs_fp64_deterministic_Eh": max(dE_d) if dE_d else None,
                  "eri_per_s": sum(s.eris for s in st_d if s.la >= 0) / wall_d},
+        "fp64_deterministic_reference": {"wall_s": wall_r, "converged": len(conv_r)},
         "mixed_vs_fp64": cap_m / cap_d,
         "plain_fp64_one_at_a_time": {"molecules_per_s": cap_p, "requests": len(plain_reqs), "jk": "direct", "slots": 1},
         "mixed_batched_vs_plain_fp64": cap_m / cap_p, "fp64_batched_vs_plain_fp64": cap_d / cap_p,
@@ -324,7 +333,7 @@ def timed_scfs(eng, steps, warmup, **extra):
 def single_molecule(name, steps, warmup, cpu=True, cpu_budget=20.0, max_iterations=0, legs=""):
     """One molecule's SCF per step.  Legs: direct FP64 (the plain method), in-core FP64 (the ERIs evaluated once per
     SCF into HBM, inside the step), mixed direct and mixed in-core; each with molecules/s, iterations, energy and
-    |dE| against direct FP64.  Roofline of the direct leg's dominant ERI class (HIP events, FP64 VALU) and of the
+    |dE| against direct FP64 and against the deterministic-mode FP64 direct reference (one untimed SCF).  Roofline of the direct leg's dominant ERI class (HIP events, FP64 VALU) and of the
     in-core contraction (HBM)."""
     import hartreefock_amd as hf
     label, bfile, atoms = workload(name)
@@ -334,6 +343,18 @@ def single_molecule(name, steps, warmup, cpu=True, cpu_budget=20.0, max_iteratio
     if legs:
         all_legs = [l for l in all_legs if l[0] in legs.split(",")]
     out = {"workload": label, "scf": SCF_DESC, "schwarz_threshold": 1e-12, "eri_convention": "exact"}
+    # the |dE| reference: FP64 direct in deterministic mode (order-independent fixed-point J/K and V sums), so
+    # the reference itself does not move with the order of the GPU's atomic additions
+    ref = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, deterministic=1).set_molecule(bfile, atoms)
+    log("%s: fp64 deterministic reference" % name)
+    t0 = time.perf_counter()
+    rr = ref.scf(**dict(SCF_KW, incore=0, **kw))
+    out["fp64_direct_deterministic"] = {"energy_Eh": rr.energy, "iterations": rr.iterations,
+                                        "converged": rr.converged, "ms_per_scf": (time.perf_counter() - t0) * 1e3}
+    if uhf:
+        out["fp64_direct_deterministic"]["s2"] = rr.s2
+    ref.close()
+    e_ref = rr.energy
     e0 = None
     for leg, prec, inc in all_legs:
         eng = hf.Engine(schwarz_threshold=1e-12, exact_transfer=1, precision=prec,
@@ -351,6 +372,7 @@ def single_molecule(name, steps, warmup, cpu=True, cpu_budget=20.0, max_iteratio
         nb = max(1, r.full_builds + r.incremental_builds)
         out[leg] = {"molecules_per_s": steps / el, "ms_per_scf": el / steps * 1e3, "iterations": r.iterations,
                     "converged": r.converged, "energy_Eh": r.energy, "abs_dE_vs_fp64_direct_Eh": abs(r.energy - e0),
+                    "abs_dE_vs_fp64_deterministic_Eh": abs(r.energy - e_ref),
                     "jk_ms_per_build": r.jk_ms / nb, "incore": r.incore, "eri_store_ms": r.eri_ms,
                     "fp32_builds": r.fp32_builds, "p50_ms": pct([x * 1e3 for x in lat], 50),
                     "p99_ms": pct([x * 1e3 for x in lat], 99)}
@@ -385,6 +407,46 @@ def single_molecule(name, steps, warmup, cpu=True, cpu_budget=20.0, max_iteratio
     return out
 
 
+def incore_shape_sweep():
+    """In-core contraction: every tile shape (hf_options.incore_shape) at sizes across the serving range and C10H22,
+    one and two densities; HIP-event time per launch and the store bytes over it (GB/s), printed as one JSON line."""
+    import numpy as np
+    import hartreefock_amd as hf
+    from hartreefock_amd import geometry as g
+    cases = [("(H2O)4/6-31G", "6-31g.1.nw", g.water_cluster(4, seed=1)),
+             ("C6H14/6-31G", "6-31g.1.nw", g.alkane_zigzag(6, seed=1)),
+             ("(H2O)8/6-31G", "6-31g.1.nw", g.water_cluster(8, seed=1)),
+             ("C11H24/6-31G", "6-31g.1.nw", g.alkane_zigzag(11, seed=1)),
+             ("C10H22/6-31G*", "6-31g_st_.1.nw", g.decane_cfg2()),
+             ("C16H34/6-31G", "6-31g.1.nw", g.alkane_zigzag(16, seed=1)),
+             ("(H2O)24/6-31G", "6-31g.1.nw", g.water_cluster(24, seed=42))]
+    rows = []
+    for label, b, atoms in cases:
+        for shape in range(8):
+            eng = hf.Engine(schwarz_threshold=1e-12, profile=1, profile_class=-2,
+                            incore_shape=shape).set_molecule(hf.basis_path(b), atoms)
+            n = eng.nbf
+            store = hf.lib().hf_pair_matrix_len(n) * 8.0
+            rng = np.random.default_rng(1)
+            X = rng.normal(size=(2, n, n)) * 0.1
+            D2 = X + X.transpose(0, 2, 1)
+            for nd in (1, 2):
+                D = D2 if nd == 2 else D2[0]
+                for _ in range(2):
+                    eng.build_jk_incore(D)
+                eng.reset_kernel_stats()
+                for _ in range(10):
+                    eng.build_jk_incore(D)
+                st = [x for x in eng.kernel_stats() if x.la < 0 and not x.fp32][0]
+                ms = st.ms / st.launches
+                rows.append({"molecule": label, "nbf": n, "densities": nd, "shape": shape, "us_per_launch": ms * 1e3,
+                             "store_bytes": store, "GB_per_s": store / (ms * 1e-3) / 1e9})
+            eng.close()
+            log("%s shape %d: %s" % (label, shape, ", ".join("nd %d %.0f GB/s" % (r["densities"], r["GB_per_s"])
+                                                            for r in rows[-2:])))
+    print(json.dumps({"incore_shape_sweep": rows, "peak_GB_per_s": PEAK_HBM_GBS}), flush=True)
+
+
 def copy_calibration():
     """A plain device copy of known size: 1 GiB read + 1 GiB written per launch, three launches.
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this gives the counters' scale for HBM bytes."""
@@ -405,7 +467,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=100, help="serving: requests per step")
     ap.add_argument("--slots", type=int, default=8, help="serving: concurrent requests (server slots)")
-    ap.add_argument("--workload", default="serving", help="serving (default) | decane | water | water32 | eicosyl")
+    ap.add_argument("--workload", default="serving", help="serving (default) | decane | water | water32 | eicosyl | "
+                    "incore-shapes (in-core contraction per tile shape and size)")
     ap.add_argument("--plain-requests", type=int, default=300,
                     help="serving: requests of the one-at-a-time plain FP64 direct run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU work for the baseline sample")
@@ -424,6 +487,9 @@ def main():
         print("bench.py: this build runs on one GPU only (serving and single-molecule SCF per GPU; multi-GPU J/K "
               "sharding is a design note, not implemented). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "incore-shapes":
+        incore_shape_sweep()
+        return
     if a.workload == "serving":
         global SERVE_SLOTS
         SERVE_SLOTS = a.slots
@@ -432,7 +498,7 @@ def main():
     res = single_molecule(a.workload, a.steps, a.warmup, cpu=not a.no_cpu_baseline, cpu_budget=a.cpu_budget,
                           max_iterations=a.max_iterations, legs=a.legs)
     d = res["fp64_direct"] if isinstance(res.get("fp64_direct"), dict) else next(
-        v for v in res.values() if isinstance(v, dict))
+        v for v in res.values() if isinstance(v, dict) and "molecules_per_s" in v)
     out = {"metric": "SCF molecules/s (plain FP64 direct on GPU)", "value": d["molecules_per_s"], "unit": "molecules/s",
            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": d["ms_per_scf"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded geometry)",

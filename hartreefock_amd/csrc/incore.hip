@@ -148,15 +148,6 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
     double* gdg = rJ + NR * NDENS * n;     // diagonal elements B_s[c][c]  [NR][n]
     double* red = gdg + NR * n;            // per-wavefront partials of J_{i j_s}  [NR][4]
     const size_t nn = (size_t)n * n;
-    // B_s[c][c] for the columns c = threadIdx.x + 256 q (n <= 512), read now and used in the epilogue
-    double gd[NR][2];
-#pragma unroll
-    for (int s = 0; s < NR; ++s)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int c = threadIdx.x + 256 * q;
-            gd[s][q] = s < nrow && c < n ? (double)G[(P0 + s) * npad + hfgpu::pair_pad_base(c) + c] : 0.0;
-        }
     for (int t = threadIdx.x; t < NDENS * n; t += blockDim.x) {
         const int d = t / n, c = t - d * n;
         xi[t] = D[d * nn + (size_t)i * n + c];
@@ -260,6 +251,27 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
                     }
                 }
         }
+        // the diagonal elements B_s[k][k] of this group's rows, for the epilogue: the lane that loaded column k
+        // keeps it (the chunk and pair half are uniform across the wavefront)
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int k = k0 + 4 * r;
+            if (k < n && lane == (k % W) / CPL) {
+                const int mk = k / W, hk = k % CPL;
+#pragma unroll
+                for (int s = 0; s < NR; ++s) {
+                    double dv = 0.0;
+#pragma unroll
+                    for (int m = 0; m < MC; ++m)
+#pragma unroll
+                        for (int h = 0; h < CPL; ++h) {   // constant indices: selects, no indexed register access
+                            const double x = ColsOf<T, CPL>::get(g[r][s][m], h);
+                            dv = (m == mk && h == hk) ? x : dv;
+                        }
+                    gdg[s * n + k] = dv;
+                }
+            }
+        }
         Halving<NV, 32>::run(V, lane);
         if ((lane & ((1 << SH) - 1)) == 0) {
             const int idx = lane >> SH;
@@ -299,11 +311,6 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
         constexpr int JS = 6 - log2c<JN>();
         if ((lane & ((1 << JS) - 1)) == 0 && (lane >> JS) < NR) red[(lane >> JS) * 4 + w] = jacc[0];
     }
-#pragma unroll
-    for (int s = 0; s < NR; ++s)
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-            if (threadIdx.x + 256 * q < n) gdg[s * n + threadIdx.x + 256 * q] = gd[s][q];
     __syncthreads();
     // column parts of every wavefront (deterministic mode: the wavefronts in turn, so that every element
     // receives their parts in one order)
