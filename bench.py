@@ -247,13 +247,7 @@ def serving_bench(a):
     cap_p = sum(1 for r in res_p.values() if r.status == 0 and r.converged) / wall_p
     eris_m = sum(s.eris for s in st_m if s.la >= 0)
     roof = roofline_of(dominant[1], st_m) if dominant[1] else None
-    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_serving_631g.json")
-    if roof and os.path.exists(tfile):
-        with open(tfile) as fh:
-            rec = json.load(fh).get("kernels", {}).get(roof["kernel"])
-        if rec:
-            roof["traffic"] = rec["bytes_per_launch"]
-            roof["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_serving_631g.json"
+    roof = with_traffic(roof, "hbm_traffic_serving_631g.json")
     nbf = [r.nbf for r in res_m.values()]
     out = {
         "metric": "SCF molecules/s, mixed precision vs FP64 on GPU, at |dE| <= 1e-6 Eh (serving, continuous batching)",
@@ -400,7 +394,8 @@ def single_molecule(name, steps, warmup, cpu=True, cpu_budget=20.0, max_iteratio
         p.scf(incore=1, **dict(SCF_KW, **kw))
         p.reset_kernel_stats()
         p.scf(incore=1, **dict(SCF_KW, **kw))
-        out["fp64_incore"]["roofline"] = roofline_of("incore_jk", p.kernel_stats())
+        out["fp64_incore"]["roofline"] = with_traffic(roofline_of("incore_jk", p.kernel_stats()),
+                                                      "hbm_traffic_incore_decane_631gs.json")
         p.close()
     if cpu and isinstance(out.get("fp64_direct"), dict):
         out["cpu_baseline"] = cpu_baseline_single(bfile, atoms, out["fp64_direct"]["iterations"], cpu_budget)
@@ -447,6 +442,78 @@ def incore_shape_sweep():
     print(json.dumps({"incore_shape_sweep": rows, "peak_GB_per_s": PEAK_HBM_GBS}), flush=True)
 
 
+def _pair_pad_base(k):
+    return k * (k + 2) // 2 if k % 2 == 0 else (k + 1) * (k + 1) // 2
+
+
+def _blocks_before(i, nr):
+    q, r = divmod(i, nr)
+    return nr * q * (q + 1) // 2 + r * (q + 1)
+
+
+def _counter_rows(root, counter):
+    import csv
+    import glob
+    rows = []
+    for f in sorted(glob.glob(os.path.join(root, "**", "*counter_collection*.csv"), recursive=True)):
+        with open(f) as fh:
+            rows += [r for r in csv.DictReader(fh) if r.get("Counter_Name") == counter]
+    if not rows:
+        raise SystemExit("no %s rows under %s" % (counter, root))
+    return rows
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
+    """HBM traffic of the in-core contraction from rocprofv3 --pmc runs (FETCH_SIZE and WRITE_SIZE in KiB per
+    dispatch), scaled by what the same counters read on the plain 1 GiB device copy of --copy-calibration, over the
+    kernel's algorithmic bytes per dispatch: the store streamed once, np^2 elements of 8 (FP64) or 4 (FP32) bytes,
+    with n recovered from the dispatch's grid (blocks of NR store rows, NR the sixth template argument)."""
+    import re
+    gib_kib = float(1 << 20)
+    scale = {}
+    for counter, d in (("FETCH_SIZE", copy_fetch_dir), ("WRITE_SIZE", copy_write_dir)):
+        v = [float(r["Counter_Value"]) for r in _counter_rows(d, counter) if "copyBuffer" in r["Kernel_Name"]]
+        scale[counter] = gib_kib / (sum(v) / len(v))
+    out = {"copy_scale": scale, "kernels": {}}
+    for counter, d, key in (("FETCH_SIZE", fetch_dir, "fetch"), ("WRITE_SIZE", write_dir, "write")):
+        for r in _counter_rows(d, counter):
+            m = re.search(r"incore_jk_kernel<(\d+), (\d+), (\d+), (\d+), (true|false), (\d+), (double|float)>",
+                          r["Kernel_Name"])
+            if not m:
+                continue
+            nr, t = int(m.group(6)), m.group(7)
+            blocks = int(r["Grid_Size"]) // 256
+            n = next(x for x in range(1, 513) if _blocks_before(x, nr) >= blocks)
+            alg = _pair_pad_base(n) ** 2 * (8 if t == "double" else 4)
+            k = out["kernels"].setdefault("incore_jk_kernel<%s>" % t, {"fetch_bytes": 0.0, "fetch_alg": 0.0,
+                                                                        "write_bytes": 0.0, "write_alg": 0.0,
+                                                                        "dispatches": 0})
+            k[key + "_bytes"] += float(r["Counter_Value"]) * 1024.0 * scale[counter]
+            k[key + "_alg"] += alg
+            if key == "fetch":
+                k["dispatches"] += 1
+    for k in out["kernels"].values():
+        k["fetch_ratio"] = k["fetch_bytes"] / k["fetch_alg"] if k["fetch_alg"] else None
+        k["write_ratio"] = k["write_bytes"] / k["write_alg"] if k["write_alg"] else None
+        k["traffic_ratio"] = (k["fetch_ratio"] or 0.0) + (k["write_ratio"] or 0.0)
+        k["mean_alg_bytes_per_dispatch"] = k["fetch_alg"] / max(1, k["dispatches"])
+    print(json.dumps(out), flush=True)
+
+
+def with_traffic(roof, fname):
+    """roofline.traffic = the measured traffic-to-algorithmic-bytes ratio of the kernel (profiles/fname) times this
+    run's algorithmic bytes per launch."""
+    path = os.path.join(ROOT, "profiles", fname)
+    if roof and roof.get("unit") == "GB/s" and os.path.exists(path):
+        with open(path) as fh:
+            rec = json.load(fh).get("kernels", {}).get(roof["kernel"])
+        if rec and rec.get("traffic_ratio"):
+            roof["traffic"] = rec["traffic_ratio"] * roof["bytes_per_launch"]
+            roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the kernel's algorithmic bytes "
+                                      "(ratio %.3f), profiles/%s" % (rec["traffic_ratio"], fname))
+    return roof
+
+
 def copy_calibration():
     """A plain device copy of known size: 1 GiB read + 1 GiB written per launch, three launches.
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this gives the counters' scale for HBM bytes."""
@@ -477,9 +544,15 @@ def main():
     ap.add_argument("--legs", default="", help="single-molecule workloads: fp64_direct,fp64_incore,mixed_direct,"
                     "mixed_incore (default all)")
     ap.add_argument("--max-iterations", type=int, default=0)
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="only summarise rocprofv3 --pmc output directories: in-core contraction HBM traffic against "
+                         "its algorithmic bytes, scaled by the copy calibration")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
+    if a.pmc_summary:
+        pmc_summary(*a.pmc_summary)
+        return
     if a.copy_calibration:
         copy_calibration()
         return
