@@ -500,6 +500,36 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
     print(json.dumps(out), flush=True)
 
 
+def trace_summary(root):
+    """Per-kernel resources and durations from a rocprofv3 --kernel-trace directory (kernel_trace.csv): calls, mean
+    duration, VGPRs (arch + accumulation), SGPRs, LDS and scratch per workgroup/lane, workgroup size, and the
+    wavefronts per SIMD that the VGPR count allows (512 VGPRs per SIMD lane, granule 8, at most 8)."""
+    import csv
+    import glob
+    agg = {}
+    for f in sorted(glob.glob(os.path.join(root, "**", "*kernel_trace.csv"), recursive=True)):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                name = r["Kernel_Name"]
+                k = agg.setdefault(name, {"calls": 0, "ns": 0.0, "vgpr": int(r.get("VGPR_Count", 0) or 0),
+                                          "agpr": int(r.get("Accum_VGPR_Count", 0) or 0),
+                                          "sgpr": int(r.get("SGPR_Count", 0) or 0),
+                                          "lds": int(r.get("LDS_Block_Size", 0) or 0),
+                                          "scratch": int(r.get("Scratch_Size", 0) or 0),
+                                          "wg": int(r.get("Workgroup_Size", 0) or 0), "grid_max": 0})
+                k["calls"] += 1
+                k["ns"] += float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+                k["grid_max"] = max(k["grid_max"], int(r.get("Grid_Size", 0) or 0))
+    rows = []
+    for name, k in agg.items():
+        regs = max(8, -(-(k["vgpr"] + k["agpr"]) // 8) * 8)
+        rows.append(dict(kernel=name, calls=k["calls"], mean_us=k["ns"] / k["calls"] / 1e3, vgpr=k["vgpr"],
+                         agpr=k["agpr"], sgpr=k["sgpr"], lds_bytes=k["lds"], scratch_bytes_per_lane=k["scratch"],
+                         workgroup=k["wg"], max_grid=k["grid_max"], waves_per_simd=min(8, 512 // regs)))
+    rows.sort(key=lambda x: -x["calls"] * x["mean_us"])
+    print(json.dumps({"kernels": rows}), flush=True)
+
+
 def with_traffic(roof, fname):
     """roofline.traffic = the measured traffic-to-algorithmic-bytes ratio of the kernel (profiles/fname) times this
     run's algorithmic bytes per launch."""
@@ -547,9 +577,14 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="only summarise rocprofv3 --pmc output directories: in-core contraction HBM traffic against "
                          "its algorithmic bytes, scaled by the copy calibration")
+    ap.add_argument("--trace-summary", metavar="TRACE_DIR",
+                    help="only summarise a rocprofv3 --kernel-trace directory: per-kernel resources and durations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
+    if a.trace_summary:
+        trace_summary(a.trace_summary)
+        return
     if a.pmc_summary:
         pmc_summary(*a.pmc_summary)
         return

@@ -11,6 +11,7 @@ import argparse
 import concurrent.futures as cf
 import glob
 import os
+import re
 import subprocess
 import sys
 import time
@@ -108,14 +109,54 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
     return LIB
 
 
+def _usage_of(src):
+    """Device compile of one source with the compiler's kernel-resource remarks: per kernel VGPRs, AGPRs, SGPRs,
+    scratch bytes per lane, LDS bytes per block and the occupancy (waves per SIMD) the compiler derives."""
+    cmd = [HIPCC] + COMMON + HIP_FLAGS + ["-x", "hip", "--offload-device-only", "-c", src, "-o", os.devnull,
+                                          "-Rpass-analysis=kernel-resource-usage"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    rows, cur = [], None
+    for line in p.stderr.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            cur = {"kernel": m.group(1), "file": os.path.basename(src)}
+            rows.append(cur)
+            continue
+        m = re.search(r"remark:\s+(TotalSGPRs|VGPRs Spill|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|"
+                      r"Occupancy \[waves/SIMD\]|LDS Size \[bytes/block\]): (\d+)", line)
+        if m and cur is not None:
+            key = {"TotalSGPRs": "SGPRs", "VGPRs Spill": "Spill"}.get(m.group(1), m.group(1).split(" ")[0])
+            cur[key] = int(m.group(2))
+    return rows
+
+
+def resource_usage(jobs=None):
+    """Print the kernel-resource table of every device source (after build(), which generates the class sources)."""
+    srcs = sorted(glob.glob(os.path.join(GEN, "k_*.hip"))) + \
+        [os.path.join(CSRC, n) for n in ("incore.hip", "dense.hip", "scf.hip", "one_electron.hip", "post.hip")]
+    with cf.ThreadPoolExecutor(max_workers=jobs or min(16, os.cpu_count() or 4)) as ex:
+        rows = [r for rs in ex.map(_usage_of, srcs) for r in rs]
+    print("# static LDS only (the J/K kernels' exchange-row slots and Boys table are dynamic LDS set at launch)")
+    print("%-18s %5s %5s %5s %6s %8s %7s %6s  %s" % ("file", "VGPR", "AGPR", "SGPR", "spill", "scratch", "LDS", "waves",
+                                                    "kernel"))
+    for r in rows:
+        print("%-18s %5s %5s %5s %6s %8s %7s %6s  %s" % (r["file"], r.get("VGPRs"), r.get("AGPRs"), r.get("SGPRs"),
+                                                       r.get("Spill"), r.get("ScratchSize"), r.get("LDS"),
+                                                       r.get("Occupancy"), r["kernel"]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("--force", action="store_true")
+    ap.add_argument("--resource-usage", action="store_true",
+                    help="after building, print every kernel's registers, scratch, LDS and occupancy")
     a = ap.parse_args()
     t0 = time.time()
     build(jobs=a.j, force=a.force, verbose=True)
     print("built %s in %.1fs" % (LIB, time.time() - t0))
+    if a.resource_usage:
+        resource_usage(a.j)
 
 
 if __name__ == "__main__":
