@@ -560,12 +560,17 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
     const int rows_per_slot = sink.ndens * (NA + NB);
     const int slot_len = rows_per_slot * n;
     __shared__ int s_pe;
-    const int2 me = tl.tasks[(size_t)blockIdx.x * blockDim.x + threadIdx.x];
-    const bool active = me.x >= 0;
-    const int pb = tl.tasks[(size_t)blockIdx.x * blockDim.x].x;     // a block's first task is never padding
+    // this workgroup's task blocks: tasks_per_block consecutive 256-task blocks (the last group may be shorter)
+    const int tpb = sink.tasks_per_block > 1 ? sink.tasks_per_block : 1;
+    const int64_t blk0 = (int64_t)blockIdx.x * tpb;
+    const int nblk = (int)(tl.nblocks - blk0 < tpb ? tl.nblocks - blk0 : tpb);
+    const int pb = tl.tasks[(size_t)blk0 * blockDim.x].x;     // a block's first task is never padding
     if (threadIdx.x == 0) s_pe = pb;
     __syncthreads();
-    if (active && me.x > pb) atomicMax(&s_pe, me.x);
+    for (int r = 0; r < nblk; ++r) {
+        const int x = tl.tasks[(size_t)(blk0 + r) * blockDim.x + threadIdx.x].x;
+        if (x > pb) atomicMax(&s_pe, x);
+    }
     __syncthreads();
     const int pe = s_pe;
     const int nslot = pe - pb + 1 < sink.lds_slots ? pe - pb + 1 : sink.lds_slots;
@@ -585,55 +590,60 @@ __device__ __forceinline__ void eri_jk_block(const TaskList& tl, const ShellsDev
     }
     __syncthreads();
 
-    int key = -1, a0 = 0, b0 = 0;
-    double jab[NAB];
+    for (int r = 0; r < nblk; ++r) {
+        const int2 me = tl.tasks[(size_t)(blk0 + r) * blockDim.x + threadIdx.x];
+        const bool active = me.x >= 0;
+        int key = -1, a0 = 0, b0 = 0;
+        double jab[NAB];
 #pragma unroll
-    for (int i = 0; i < NAB; ++i) jab[i] = 0.0;
-    if (active) {
-        const int p = me.x, q = me.y & kTaskKetMask, sl = me.y >> kTaskKetBits;
-        const PairRec bp = tl.bra.rec[p], kp = tl.ket.rec[q];
-        const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;
-        a0 = bp.a0;
-        b0 = bp.b0;
-        const int c0 = kp.a0, d0 = kp.b0;
-        R out[C::NOUT];
-        if constexpr (kLdsBoys)
-            quartet_values<C, R, true>(sh, pp, (const R*)btab, bp, kp, out, sl, tl.slice_prims);
-        else
-            quartet_values<C, R>(sh, pp, boys, bp, kp, out, sl, tl.slice_prims);
-        double w = 1.0;
-        if (A == B) w *= 0.5;
-        if (Cs == Ds) w *= 0.5;
-        if (tl.same_class && p == q) w *= 0.5;
-        if (p - pb < nslot) {
-            key = p - pb;
-            sink_jk_rows<C, R>(out, rows + (size_t)key * slot_len, a0, b0, c0, d0, w, sink, jab);
-        } else {
-            sink_jk<C, R>(out, a0, b0, c0, d0, w, sink);
+        for (int i = 0; i < NAB; ++i) jab[i] = 0.0;
+        if (active) {
+            const int p = me.x, q = me.y & kTaskKetMask, sl = me.y >> kTaskKetBits;
+            const PairRec bp = tl.bra.rec[p], kp = tl.ket.rec[q];
+            const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;
+            a0 = bp.a0;
+            b0 = bp.b0;
+            const int c0 = kp.a0, d0 = kp.b0;
+            R out[C::NOUT];
+            if constexpr (kLdsBoys)
+                quartet_values<C, R, true>(sh, pp, (const R*)btab, bp, kp, out, sl, tl.slice_prims);
+            else
+                quartet_values<C, R>(sh, pp, boys, bp, kp, out, sl, tl.slice_prims);
+            double w = 1.0;
+            if (A == B) w *= 0.5;
+            if (Cs == Ds) w *= 0.5;
+            if (tl.same_class && p == q) w *= 0.5;
+            // (a later block of the group may start a new segment, whose bra pairs can precede pb)
+            if (p >= pb && p - pb < nslot) {
+                key = p - pb;
+                sink_jk_rows<C, R>(out, rows + (size_t)key * slot_len, a0, b0, c0, d0, w, sink, jab);
+            } else {
+                sink_jk<C, R>(out, a0, b0, c0, d0, w, sink);
+            }
         }
-    }
-    // J'_ab: lanes without a slot hold zeros; lane 0 is active whenever any lane is
-    const int key0 = __shfl(key, 0);
-    if (__all(key == key0 || key < 0)) {
-        // the NAB sums by recursive halving in registers (padded to a power of two); sum v ends in lane
-        // v << SH, and those lanes add their sums to J' in one atomic instruction
-        static_assert(NAB <= 64, "one value per lane at most");
-        constexpr int CNT = NAB <= 1 ? 1 : NAB <= 2 ? 2 : NAB <= 4 ? 4 : NAB <= 8 ? 8 : NAB <= 16 ? 16 : NAB <= 32 ? 32 : 64;
-        constexpr int SH = 6 - log2c<CNT>();
-        double V[CNT];
+        // J'_ab: lanes without a slot hold zeros; lane 0 is active whenever any lane is
+        const int key0 = __shfl(key, 0);
+        if (__all(key == key0 || key < 0)) {
+            // the NAB sums by recursive halving in registers (padded to a power of two); sum v ends in lane
+            // v << SH, and those lanes add their sums to J' in one atomic instruction
+            static_assert(NAB <= 64, "one value per lane at most");
+            constexpr int CNT = NAB <= 1 ? 1 : NAB <= 2 ? 2 : NAB <= 4 ? 4 : NAB <= 8 ? 8 : NAB <= 16 ? 16 : NAB <= 32 ? 32 : 64;
+            constexpr int SH = 6 - log2c<CNT>();
+            double V[CNT];
 #pragma unroll
-        for (int i = 0; i < CNT; ++i) V[i] = i < NAB ? jab[i] : 0.0;
-        const int lane = threadIdx.x & 63;
-        Halving<CNT, 32>::run(V, lane);
-        const int v = lane >> SH;
-        const int ra0 = __builtin_amdgcn_readlane(a0, 0), rb0 = __builtin_amdgcn_readlane(b0, 0);
-        if (key0 >= 0 && (lane & ((1 << SH) - 1)) == 0 && v < NAB)
-            unsafeAtomicAdd(&sink.J[(size_t)(ra0 + v / NB) * n + rb0 + v % NB], V[0]);
-    } else if (key >= 0) {
+            for (int i = 0; i < CNT; ++i) V[i] = i < NAB ? jab[i] : 0.0;
+            const int lane = threadIdx.x & 63;
+            Halving<CNT, 32>::run(V, lane);
+            const int v = lane >> SH;
+            const int ra0 = __builtin_amdgcn_readlane(a0, 0), rb0 = __builtin_amdgcn_readlane(b0, 0);
+            if (key0 >= 0 && (lane & ((1 << SH) - 1)) == 0 && v < NAB)
+                unsafeAtomicAdd(&sink.J[(size_t)(ra0 + v / NB) * n + rb0 + v % NB], V[0]);
+        } else if (key >= 0) {
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
+            for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
-            for (int ib = 0; ib < NB; ++ib) unsafeAtomicAdd(&sink.J[(size_t)(a0 + ia) * n + b0 + ib], jab[ia * NB + ib]);
+                for (int ib = 0; ib < NB; ++ib) unsafeAtomicAdd(&sink.J[(size_t)(a0 + ia) * n + b0 + ib], jab[ia * NB + ib]);
+        }
     }
     __syncthreads();
     // flush: row by row, consecutive threads on consecutive columns
@@ -696,11 +706,19 @@ template <int LA, int LB, int LC, int LD, int EXACT, int MODE>
 inline hipError_t launch_eri(const TaskList& tl, const ShellsDev& sh, const PrimPair* pp, const double* boys,
                              const SinkArgs& sink, hipStream_t st) {
     if (tl.ntasks <= 0) return hipSuccess;
-    const int64_t blocks = MODE == MODE_DIAG ? (tl.ntasks + 255) / 256 : tl.nblocks;
+    int64_t blocks = MODE == MODE_DIAG ? (tl.ntasks + 255) / 256 : tl.nblocks;
     SinkArgs s = sink;
     size_t lds = 0;
     if (MODE == MODE_JK || MODE == MODE_JK32) {
         using C = EriClass<LA, LB, LC, LD, 0>;
+        // A block flushes its exchange rows (ndens (NA + NB) rows of n columns) with one coalesced atomic per
+        // element; where that is more than 8 atomics per quartet (long rows: large n, two densities) a workgroup
+        // digests four 256-task blocks before its flush, when that still leaves >= 1024 workgroups.  (For short
+        // rows the fewer, longer workgroups overlap less with the build's other classes and cost more than the
+        // flushes they save.)
+        const int64_t flush = (int64_t)sink.ndens * (C::NA + C::NB) * sink.n;
+        s.tasks_per_block = (flush > 8 * 256 && blocks >= 4096) ? 4 : 1;
+        blocks = (blocks + s.tasks_per_block - 1) / s.tasks_per_block;
         // up to 4 bra pairs per block, within 40 KB of LDS (a block of 256 tasks rarely spans more)
         const size_t slot_bytes = (size_t)sink.ndens * (C::NA + C::NB) * sink.n * sizeof(double);
         const size_t cap = 40 * 1024;

@@ -174,6 +174,7 @@ struct SinkArgs {
     double* Qout;            // MODE_DIAG output, one per bra pair
     int lds_slots;           // MODE_JK: bra pairs per block whose exchange rows are staged in LDS
     const float* boys32;     // MODE_JK32: FP32 copy of the Boys table
+    int tasks_per_block;     // MODE_JK, MODE_JK32: consecutive 256-task blocks per workgroup (set at launch)
     long long* fx;           // deterministic mode: fixed-point accumulators (see fx_add) of J' (n x n) then
                              // K' (ndens x n x n), two per element; null: FP64 atomics into J and K
 };
