@@ -2,7 +2,8 @@
 
 Default workload (the serving configuration): RHF requests drawn from a seeded mix of water clusters (1-12
 waters) and n-alkanes (C1-C16) in 6-31G (13-212 Cartesian functions), served by the continuous-batching server
-(8 slots, each request's ERIs evaluated once into an in-core store in HBM, device-resident SCF state).
+(32 slots stepped in lockstep — each phase of an SCF iteration one launch over every admitted request — each
+request's ERIs evaluated once into an in-core store in HBM, device-resident SCF state).
 A step is one batch of --batch requests (default 100): --steps 20 is the 2000-request workload.  The timed region
 is the mixed-precision tier serving all steps x batch requests queued at once (closed loop: the server's
 sustained rate); the same requests are then served in FP64, and both tiers again under Poisson arrivals at 0.8x
@@ -119,19 +120,22 @@ def cpu_baseline_single(bfile, atoms, iterations, budget_s):
 
 
 # ---------------------------------------------------------------------------------------------- serving
-SERVE_SLOTS = 8    # bench.py --slots N changes it (a one-slot run serves requests one after another)
+SERVE_SLOTS = 32       # bench.py --slots N changes it
+SERVE_LOCKSTEP = 1     # bench.py --lockstep 0: one worker thread per slot instead of the lockstep batch
+SLOT_MODE_SLOTS = 8    # slots of the slot-thread comparison leg (its measured saturation point is 4-8)
 
 
 def serve(hf, bfile, reqs, prec, rate=None, slots=None, incore=1, tag="", profile=0, profile_class=-1,
-          seed=77, max_iterations=300, deterministic=0):
+          seed=77, max_iterations=300, deterministic=0, lockstep=None):
     """Serve reqs = [(tag, atoms)]: all queued at once (rate None) or as a Poisson process at `rate`/s.  Returns the
     results by tag, the wall time from the first submission to the last completion and the summed kernel
     statistics."""
     import numpy as np
+    ls = SERVE_LOCKSTEP if lockstep is None else lockstep
     srv = hf.Server(slots=slots or SERVE_SLOTS, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1,
                                                              profile=profile, profile_class=profile_class,
                                                              deterministic=deterministic),
-                    scf=dict(SCF_KW, incore=incore, max_iterations=max_iterations))
+                    scf=dict(SCF_KW, incore=incore, max_iterations=max_iterations), lockstep=bool(ls))
     rng = np.random.default_rng(seed)
     got, shown = {}, 0
     try:
@@ -241,9 +245,16 @@ def serving_bench(a):
         ms = [x.done_ms - x.arrival_ms for x in r.values()]
         lat["mixed" if prec else "fp64"] = {"rate_per_s": 0.8 * cap, "p50_ms": pct(ms, 50), "p99_ms": pct(ms, 99),
                                             "converged": sum(1 for x in r.values() if x.converged)}
+    # the slot-thread server (one worker thread and context per slot, each request's SCF alone), FP64, same requests
+    slot_leg = None
+    if SERVE_LOCKSTEP and not a.no_slot_leg:
+        res_s, wall_s, _ = serve(hf, bfile, timed, 0, slots=SLOT_MODE_SLOTS, lockstep=0, tag="fp64 slot threads")
+        conv_s = sum(1 for r in res_s.values() if r.status == 0 and r.converged)
+        slot_leg = {"molecules_per_s": conv_s / wall_s, "wall_s": wall_s, "converged": conv_s,
+                    "slots": SLOT_MODE_SLOTS}
     # the plain FP64 path, one molecule at a time with direct J/K (the north star's reference point)
     plain_reqs = timed[:a.plain_requests]
-    res_p, wall_p, _ = serve(hf, bfile, plain_reqs, 0, slots=1, incore=0, tag="plain fp64")
+    res_p, wall_p, _ = serve(hf, bfile, plain_reqs, 0, slots=1, incore=0, tag="plain fp64", lockstep=0)
     cap_p = sum(1 for r in res_p.values() if r.status == 0 and r.converged) / wall_p
     eris_m = sum(s.eris for s in st_m if s.la >= 0)
     roof = roofline_of(dominant[1], st_m) if dominant[1] else None
@@ -258,7 +269,10 @@ def serving_bench(a):
         "data": "synthetic (seeded geometries)",
         "config": {"workload": "serving mix: %d RHF requests (%d steps x %d), water clusters 1-12 / n-alkanes C1-C16, "
                                "6-31G, %d-%d functions" % (K * B, K, B, min(nbf), max(nbf)),
-                   "batch_per_step": B, "slots": SERVE_SLOTS, "jk": "in-core ERI store per request",
+                   "batch_per_step": B, "slots": SERVE_SLOTS,
+                   "server": "lockstep: every admitted request's SCF steps advance together, one launch per phase "
+                             "for all of them" if SERVE_LOCKSTEP else "slot threads: one worker thread per slot",
+                   "jk": "in-core ERI store per request",
                    "eri_convention": "exact (6-31G has no d shells: identical to the reference's)",
                    "schwarz_threshold": 1e-12, "scf": SCF_DESC, "max_iterations": 300,
                    "arrival": "all queued at once (closed loop) in the timed region; Poisson at 0.8x capacity for "
@@ -273,6 +287,7 @@ def serving_bench(a):
         "fp64_deterministic_reference": {"wall_s": wall_r, "converged": len(conv_r)},
         "mixed_vs_fp64": cap_m / cap_d,
         "plain_fp64_one_at_a_time": {"molecules_per_s": cap_p, "requests": len(plain_reqs), "jk": "direct", "slots": 1},
+        "fp64_slot_threads": slot_leg,
         "mixed_batched_vs_plain_fp64": cap_m / cap_p, "fp64_batched_vs_plain_fp64": cap_d / cap_p,
         "latency_ms": {"p50": lat["mixed"]["p50_ms"], "p99": lat["mixed"]["p99_ms"]},
         "eri_per_s": eris_m / wall_m,
@@ -563,7 +578,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20, help="serving: batches of --batch requests in the timed region")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=100, help="serving: requests per step")
-    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent requests (server slots)")
+    ap.add_argument("--slots", type=int, default=32, help="serving: concurrent requests (server slots)")
+    ap.add_argument("--lockstep", type=int, default=1, help="serving: 1 lockstep batch server (default), 0 slot threads")
+    ap.add_argument("--no-slot-leg", action="store_true", help="serving: skip the FP64 slot-thread comparison leg")
     ap.add_argument("--workload", default="serving", help="serving (default) | decane | water | water32 | eicosyl | "
                     "incore-shapes (in-core contraction per tile shape and size)")
     ap.add_argument("--plain-requests", type=int, default=300,
@@ -599,8 +616,9 @@ def main():
         incore_shape_sweep()
         return
     if a.workload == "serving":
-        global SERVE_SLOTS
+        global SERVE_SLOTS, SERVE_LOCKSTEP
         SERVE_SLOTS = a.slots
+        SERVE_LOCKSTEP = a.lockstep
         serving_bench(a)
         return
     res = single_molecule(a.workload, a.steps, a.warmup, cpu=not a.no_cpu_baseline, cpu_budget=a.cpu_budget,

@@ -5,10 +5,11 @@ Python binding of the C ABI in include/hfgpu.h (libhfgpu.so, built in-tree by
 HIP library, and importing the binding raises if libhfgpu.so is missing.
 """
 from ._lib import (Engine, HFError, KernelStat, Options, ScfParams, ScfResult, Server, ServeResult, lib,
-                   library_path, packed_index, packed_len, serve_footprint, EXPORTED_SYMBOLS)
+                   library_path, packed_index, packed_len, scf_batch, serve_footprint, EXPORTED_SYMBOLS)
 
 __all__ = ["Engine", "HFError", "KernelStat", "Options", "ScfParams", "ScfResult", "Server", "ServeResult", "lib",
-           "library_path", "packed_index", "packed_len", "serve_footprint", "EXPORTED_SYMBOLS", "basis_path"]
+           "library_path", "packed_index", "packed_len", "scf_batch", "serve_footprint", "EXPORTED_SYMBOLS",
+           "basis_path"]
 
 
 def basis_path(name):

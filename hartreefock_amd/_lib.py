@@ -16,7 +16,7 @@ EXPORTED_SYMBOLS = [
     "hf_pair_matrix_len", "hf_eri_pairs_device", "hf_build_jk_incore_device", "hf_build_jk_incore",
     "hf_build_jk_incore_device_noalloc",
     "hf_properties", "hf_core_hamiltonian", "hf_boys",
-    "hf_scf_defaults", "hf_scf", "hf_scf_begin", "hf_scf_step", "hf_scf_end", "hf_kernel_stats",
+    "hf_scf_defaults", "hf_scf", "hf_scf_begin", "hf_scf_step", "hf_scf_end", "hf_scf_batch", "hf_kernel_stats",
     "hf_kernel_stats_reset",
     "hf_serve_default_options", "hf_serve_create", "hf_serve_submit", "hf_serve_poll", "hf_serve_pending",
     "hf_serve_footprint", "hf_serve_destroy", "hf_serve_kernel_stats",
@@ -68,7 +68,8 @@ class ScfResult(ctypes.Structure):
 
 
 class ServeOptions(ctypes.Structure):
-    _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_double), ("ctx", Options), ("scf", ScfParams)]
+    _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_double), ("ctx", Options), ("scf", ScfParams),
+                ("lockstep", ctypes.c_int), ("prep_threads", ctypes.c_int)]
 
 
 class ServeResult(ctypes.Structure):
@@ -146,6 +147,8 @@ def lib():
     L.hf_scf_begin.argtypes = [vp, ctypes.POINTER(ScfParams)]
     L.hf_scf_step.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.hf_scf_end.argtypes = [vp, ctypes.POINTER(ScfResult), dp]
+    L.hf_scf_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(ScfParams),
+                               ctypes.POINTER(ScfResult), ip]
     L.hf_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStat), ctypes.c_int]
     L.hf_kernel_stats_reset.argtypes = [vp]
     L.hf_serve_default_options.argtypes = [ctypes.POINTER(ServeOptions)]
@@ -393,19 +396,50 @@ class Engine:
         self._check(lib().hf_kernel_stats_reset(self.h))
 
 
+def scf_batch(engines, unrestricted=False, **kw):
+    """Lockstep SCF of several molecules (hf_scf_batch): engines is a list of Engine objects, each with its
+    molecule set; unrestricted is one flag or one per engine; kw are SCF parameters for every molecule.
+    Returns [(status, ScfResult)] in the engines' order (status 0 = converged)."""
+    count = len(engines)
+    flags = unrestricted if isinstance(unrestricted, (list, tuple)) else [unrestricted] * count
+    params = (ScfParams * count)()
+    for i in range(count):
+        lib().hf_scf_defaults(ctypes.byref(params[i]))
+        params[i].unrestricted = 1 if flags[i] else 0
+        for k, v in kw.items():
+            if not hasattr(params[i], k):
+                raise TypeError("unknown SCF parameter %s" % k)
+            setattr(params[i], k, v)
+    hs = (ctypes.c_void_p * count)(*[e.h for e in engines])
+    res = (ScfResult * count)()
+    st = (ctypes.c_int * count)()
+    lib().hf_scf_batch(hs, count, params, res, st)
+    return [(st[i], res[i]) for i in range(count)]
+
+
 class Server:
     """Continuous-batching SCF server on one GPU (hf_serve_* in include/hfgpu.h).
 
-    Server(slots=8, hbm_budget=0, options={...context options...}, scf={...SCF parameters...})
+    Server(slots=32, hbm_budget=0, options={...context options...}, scf={...SCF parameters...}, lockstep=True,
+           prep_threads=8): lockstep steps the admitted requests together (one launch per phase for all the
+           requests of a batch; True: the library's default of two batch threads, an int: that many);
+           lockstep=False runs one worker thread per slot.
     submit(basis_file, atoms, unrestricted=False, tag=0) -> id;  poll(max, wait_ms) -> [ServeResult]
     """
 
-    def __init__(self, slots=8, hbm_budget=0.0, options=None, scf=None):
+    def __init__(self, slots=None, hbm_budget=0.0, options=None, scf=None, lockstep=True, prep_threads=None):
         L = lib()
         o = ServeOptions()
         L.hf_serve_default_options(ctypes.byref(o))
-        o.slots = slots
+        if slots is not None:
+            o.slots = slots
         o.hbm_budget = hbm_budget
+        if lockstep is False or lockstep == 0:
+            o.lockstep = 0
+        elif lockstep is not True:
+            o.lockstep = int(lockstep)
+        if prep_threads is not None:
+            o.prep_threads = prep_threads
         for k, v in (options or {}).items():
             if not hasattr(o.ctx, k):
                 raise TypeError("unknown option %s" % k)

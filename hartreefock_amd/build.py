@@ -57,7 +57,7 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
              os.path.join(CSRC, "coop_kernel.hpp"), os.path.join(GEN, "eri_classes.inc")]
     kdeps += sorted(glob.glob(os.path.join(GEN, "coop_*.inc")))
     hdeps = [os.path.join(CSRC, "hf_device.h"), os.path.join(CSRC, "wave_reduce.h"), os.path.join(CSRC, "hf_internal.h"),
-             os.path.join(CSRC, "gershgorin.h"),
+             os.path.join(CSRC, "gershgorin.h"), os.path.join(CSRC, "scf_run.h"), os.path.join(CSRC, "dense_tile.h"),
              os.path.join(GEN, "dispatch.inc"), os.path.join(ROOT, "include", "hfgpu.h")]
     for stale in glob.glob(os.path.join(OBJ, "k_????.hip.o")):
         os.remove(stale)                 # objects of the former one-file-per-class layout
@@ -66,7 +66,7 @@ def build(jobs=None, force=False, verbose=False, oracle=True):
         units.append((src, True, kdeps))
     for src in sorted(glob.glob(os.path.join(GEN, "d_*.hip"))):
         units.append((src, True, [os.path.join(CSRC, "hf_device.h")]))
-    for name in ("context.hip", "scf.hip", "one_electron.hip", "dense.hip", "incore.hip", "post.hip"):
+    for name in ("context.hip", "scf.hip", "batch.hip", "one_electron.hip", "dense.hip", "incore.hip", "post.hip"):
         units.append((os.path.join(CSRC, name), True, hdeps))
     for name in ("basis.cpp", "boys.cpp", "serve.cpp"):
         units.append((os.path.join(CSRC, name), False, hdeps))

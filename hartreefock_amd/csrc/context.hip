@@ -41,6 +41,45 @@ int grid_for(size_t n) {
     return (int)std::min<size_t>(std::max<size_t>(b, 1), 8192);
 }
 
+}  // namespace
+
+static int stage_reserve(hf_ctx* ctx, size_t need) {
+    if (ctx->stage_used + need <= ctx->stage_cap) return HF_OK;
+    // the queued copies still read the staged bytes: wait for them, then reuse (or grow) the buffer
+    HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stage_used = 0;
+    if (need > ctx->stage_cap) {
+        const size_t cap = std::max<size_t>(need, std::max<size_t>(2 * ctx->stage_cap, (size_t)4 << 20));
+        if (ctx->stage) (void)hipHostFree(ctx->stage);
+        ctx->stage = nullptr;
+        ctx->stage_cap = 0;
+        HF_HIP_CHECK(ctx, hipHostMalloc((void**)&ctx->stage, cap, hipHostMallocDefault));
+        ctx->stage_cap = cap;
+    }
+    return HF_OK;
+}
+
+int hf_upload(hf_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return HF_OK;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (int rc = stage_reserve(ctx, need)) return rc;
+    char* p = ctx->stage + ctx->stage_used;
+    std::memcpy(p, src, bytes);
+    HF_HIP_CHECK(ctx, hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    ctx->stage_used += need;
+    return HF_OK;
+}
+
+void* hf_stage_host(hf_ctx* ctx, size_t bytes) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (stage_reserve(ctx, need)) return nullptr;
+    void* p = ctx->stage + ctx->stage_used;
+    ctx->stage_used += need;
+    return p;
+}
+
+namespace {
+
 int pair_class_index(int la, int lb) {
     for (int i = 0; i < kNumPairClasses; ++i)
         if (kPairClassL[i][0] == la && kPairClassL[i][1] == lb) return i;
@@ -55,11 +94,11 @@ int upload_pairs(hf_ctx* ctx, PairClassHost& pc) {
     HF_HIP_CHECK(ctx, pc.d_nprim.alloc(np));
     HF_HIP_CHECK(ctx, pc.d_Q.alloc(np));
     if (!np) return HF_OK;
-    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_sa.p, pc.sa.data(), np * sizeof(int), hipMemcpyHostToDevice));
-    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_sb.p, pc.sb.data(), np * sizeof(int), hipMemcpyHostToDevice));
-    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_prim0.p, pc.prim0.data(), np * sizeof(int), hipMemcpyHostToDevice));
-    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_nprim.p, pc.nprim.data(), np * sizeof(int), hipMemcpyHostToDevice));
-    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_Q.p, pc.Q.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    if (int rc = hf_upload(ctx, pc.d_sa.p, pc.sa.data(), np * sizeof(int))) return rc;
+    if (int rc = hf_upload(ctx, pc.d_sb.p, pc.sb.data(), np * sizeof(int))) return rc;
+    if (int rc = hf_upload(ctx, pc.d_prim0.p, pc.prim0.data(), np * sizeof(int))) return rc;
+    if (int rc = hf_upload(ctx, pc.d_nprim.p, pc.nprim.data(), np * sizeof(int))) return rc;
+    if (int rc = hf_upload(ctx, pc.d_Q.p, pc.Q.data(), np * sizeof(double))) return rc;
     std::vector<PairRec> rec(np);
     for (size_t i = 0; i < np; ++i) {
         rec[i].sa = pc.sa[i];
@@ -71,10 +110,10 @@ int upload_pairs(hf_ctx* ctx, PairClassHost& pc) {
         rec[i].pad0 = rec[i].pad1 = 0;
     }
     HF_HIP_CHECK(ctx, pc.d_rec.alloc(np));
-    HF_HIP_CHECK(ctx, hipMemcpy(pc.d_rec.p, rec.data(), np * sizeof(PairRec), hipMemcpyHostToDevice));
+    if (int rc = hf_upload(ctx, pc.d_rec.p, rec.data(), np * sizeof(PairRec))) return rc;
     if (!pc.group.empty()) {
         HF_HIP_CHECK(ctx, pc.d_group.alloc(pc.group.size()));
-        HF_HIP_CHECK(ctx, hipMemcpy(pc.d_group.p, pc.group.data(), pc.group.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (int rc = hf_upload(ctx, pc.d_group.p, pc.group.data(), pc.group.size() * sizeof(int))) return rc;
     }
     return HF_OK;
 }
@@ -265,6 +304,7 @@ int setup_molecule(hf_ctx* ctx) {
     HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->eri_stream) HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->eri_stream));
     for (hipStream_t s2 : ctx->side) HF_HIP_CHECK(ctx, hipStreamSynchronize(s2));
+    ctx->stage_used = 0;                 // no staged upload is queued any more
     if (mol.maxL > kLmax) return ctx->fail(HF_E_UNSUPPORTED, "basis has L > 2 shells; this build supports s, p, d");
     std::vector<double> sxyz(3 * ns);
     std::vector<int> sbf(ns);
@@ -274,8 +314,8 @@ int setup_molecule(hf_ctx* ctx) {
     }
     HF_HIP_CHECK(ctx, ctx->d_shell_xyz.alloc(sxyz.size()));
     HF_HIP_CHECK(ctx, ctx->d_shell_bf.alloc(sbf.size()));
-    HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_shell_xyz.p, sxyz.data(), sxyz.size() * sizeof(double), hipMemcpyHostToDevice));
-    HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_shell_bf.p, sbf.data(), sbf.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (int rc = hf_upload(ctx, ctx->d_shell_xyz.p, sxyz.data(), sxyz.size() * sizeof(double))) return rc;
+    if (int rc = hf_upload(ctx, ctx->d_shell_bf.p, sbf.data(), sbf.size() * sizeof(int))) return rc;
     if (!ctx->d_boys.p) {
         std::vector<double> tab;
         hfhost::boys_table(tab);
@@ -332,17 +372,21 @@ int setup_molecule(hf_ctx* ctx) {
         }
     HF_HIP_CHECK(ctx, ctx->d_pp.alloc(std::max<size_t>(pp.size(), 1)));
     if (!pp.empty())
-        HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_pp.p, pp.data(), pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice));
+        if (int rc = hf_upload(ctx, ctx->d_pp.p, pp.data(), pp.size() * sizeof(PrimPair))) return rc;
     for (auto& pc : ctx->pcls) {
         int rc = upload_pairs(ctx, pc);
         if (rc) return rc;
     }
-    // Schwarz bounds from the diagonal quartets (ab|ab), always with exact integrals
+    // Schwarz bounds from the diagonal quartets (ab|ab), always with exact integrals: every pair class's kernel,
+    // then one read-back of all the bounds
     {
-        size_t maxp = 1;
-        for (auto& pc : ctx->pcls) maxp = std::max<size_t>(maxp, pc.sa.size());
-        HF_HIP_CHECK(ctx, ctx->d_Qdiag.alloc(maxp));
-        double* dQ = ctx->d_Qdiag.p;
+        size_t tot = 0;
+        std::vector<size_t> off(ctx->pcls.size(), 0);
+        for (size_t c = 0; c < ctx->pcls.size(); ++c) {
+            off[c] = tot;
+            tot += ctx->pcls[c].sa.size();
+        }
+        HF_HIP_CHECK(ctx, ctx->d_Qdiag.alloc(std::max<size_t>(tot, 1)));
         ShellsDev sh{ctx->d_shell_xyz.p, ctx->d_shell_bf.p};
         for (int c = 0; c < kNumClasses; ++c) {
             const ClassInfo& ci = kClasses[c];
@@ -355,12 +399,18 @@ int setup_molecule(hf_ctx* ctx) {
             tl.ntasks = pc.npairs();
             tl.same_class = 1;
             SinkArgs sink{};
-            sink.Qout = dQ;
+            sink.Qout = ctx->d_Qdiag.p + off[pc_i];
             hipError_t e = ci.fn(MODE_DIAG, 1, tl, sh, ctx->d_pp.p, ctx->d_boys.p, sink, ctx->stream);
             if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("Schwarz kernel: ") + hipGetErrorString(e));
-            HF_HIP_CHECK(ctx, hipMemcpyAsync(pc.Q.data(), dQ, pc.Q.size() * sizeof(double), hipMemcpyDeviceToHost,
+        }
+        if (tot) {
+            double* hq = (double*)hf_stage_host(ctx, tot * sizeof(double));
+            if (!hq) return ctx->fail(HF_E_OOM, "pinned staging buffer");
+            HF_HIP_CHECK(ctx, hipMemcpyAsync(hq, ctx->d_Qdiag.p, tot * sizeof(double), hipMemcpyDeviceToHost,
                                              ctx->stream));
             HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+            for (size_t c = 0; c < ctx->pcls.size(); ++c)
+                std::copy(hq + off[c], hq + off[c] + ctx->pcls[c].Q.size(), ctx->pcls[c].Q.begin());
         }
     }
     // sort each class into contraction groups (primitive-pair count, descending), each by Q, descending
@@ -391,6 +441,33 @@ int setup_molecule(hf_ctx* ctx) {
         pc.group.push_back((int)np);
         int rc = upload_pairs(ctx, pc);
         if (rc) return rc;
+    }
+    // per padded pair position, the Schwarz factor of its shell pair rounded up to FP32 (the in-core contraction
+    // leaves out the loads of store elements the screening zeroed)
+    {
+        std::vector<double> qs((size_t)ns * ns, 0.0);
+        for (const PairClassHost& pc : ctx->pcls)
+            for (size_t q = 0; q < pc.sa.size(); ++q)
+                qs[(size_t)pc.sa[q] * ns + pc.sb[q]] = qs[(size_t)pc.sb[q] * ns + pc.sa[q]] = pc.Q[q];
+        const int n = mol.nbf;
+        std::vector<int> shell_of(n, 0);
+        for (int sh = 0; sh < ns; ++sh)
+            for (int c = 0; c < hfhost::ncart(mol.shells[sh].L); ++c) shell_of[mol.shells[sh].bf + c] = sh;
+        const int64_t np = pair_pad_base(n);
+        std::vector<float> qf((size_t)np, 0.0f);
+        double qmax = 0;
+        for (int k = 0; k < n; ++k)
+            for (int l = 0; l <= k; ++l) {
+                const double q = qs[(size_t)shell_of[k] * ns + shell_of[l]];
+                float f = (float)q;
+                if ((double)f < q) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+                qf[(size_t)(pair_pad_base(k) + l)] = f;
+                qmax = std::max(qmax, (double)f);
+            }
+        HF_HIP_CHECK(ctx, ctx->d_qf.alloc(std::max<size_t>(qf.size(), 1)));
+        if (int rc = hf_upload(ctx, ctx->d_qf.p, qf.data(), qf.size() * sizeof(float))) return rc;
+        ctx->qf_max = qmax;
+        ctx->qf_valid = true;
     }
     // task sets of the previous molecule: marked stale (rebuilt on first use), device buffers kept
     auto stale = [](TaskSet& ts) {
@@ -581,15 +658,13 @@ int hf_build_tasks(hf_ctx* ctx, double thr, TaskSet& ts, double upper, bool slic
     HF_HIP_CHECK(ctx, ts.d_kbeg.alloc(std::max<size_t>(kbeg_all.size(), 1)));
     HF_HIP_CHECK(ctx, ts.d_seg.alloc(std::max<size_t>(seg_all.size(), 1)));
     if (!prefix_all.empty())
-        HF_HIP_CHECK(ctx, hipMemcpy(ts.d_prefix.p, prefix_all.data(), prefix_all.size() * sizeof(int64_t),
-                                    hipMemcpyHostToDevice));
+        if (int rc = hf_upload(ctx, ts.d_prefix.p, prefix_all.data(), prefix_all.size() * sizeof(int64_t))) return rc;
     if (!koff_all.empty())
-        HF_HIP_CHECK(ctx, hipMemcpy(ts.d_koff.p, koff_all.data(), koff_all.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (int rc = hf_upload(ctx, ts.d_koff.p, koff_all.data(), koff_all.size() * sizeof(int))) return rc;
     if (!kbeg_all.empty())
-        HF_HIP_CHECK(ctx, hipMemcpy(ts.d_kbeg.p, kbeg_all.data(), kbeg_all.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (int rc = hf_upload(ctx, ts.d_kbeg.p, kbeg_all.data(), kbeg_all.size() * sizeof(int))) return rc;
     if (!seg_all.empty())
-        HF_HIP_CHECK(ctx, hipMemcpy(ts.d_seg.p, seg_all.data(), seg_all.size() * sizeof(TaskSeg),
-                                    hipMemcpyHostToDevice));
+        if (int rc = hf_upload(ctx, ts.d_seg.p, seg_all.data(), seg_all.size() * sizeof(TaskSeg))) return rc;
     // expand every class's tasks once (launch-order (bra, ket) pairs)
     size_t total = 0;
     for (QuartetClassTasks& qc : ts.classes) {
@@ -759,6 +834,7 @@ int hf_destroy(hf_ctx* ctx) {
         (void)hipStreamDestroy(ctx->eri_stream);
     }
     if (ctx->hpin) (void)hipHostFree(ctx->hpin);
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
     hf_scf_session_free(ctx);
     hf_scf_workspace_free(ctx);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
@@ -893,7 +969,7 @@ int hf_build_jk_incore_device(hf_ctx* ctx, int ndens, const double* d_G, const d
     if (!d_G || !d_D || !d_J || !d_K) return ctx->fail(HF_E_ARG, "null device buffer");
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     if (int rc = hf_stream_enter(ctx, st)) return rc;
-    if (int rc = hf_incore_jk_impl(ctx, ndens, d_G, nullptr, d_D, d_J, d_K, st, false)) return rc;
+    if (int rc = hf_incore_jk_impl(ctx, ndens, d_G, nullptr, d_D, d_J, d_K, st, false, false)) return rc;
     return hf_stream_leave(ctx, st);
 }
 
@@ -911,7 +987,8 @@ int hf_build_jk_incore(hf_ctx* ctx, int ndens, const double* D, double* J, doubl
     HF_HIP_CHECK(ctx, ctx->d_J.alloc(nn));
     HF_HIP_CHECK(ctx, ctx->d_K.alloc(2 * nn));
     HF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->d_D.p, D, ndens * nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    int rc = hf_incore_jk_impl(ctx, ndens, ctx->d_G.p, nullptr, ctx->d_D.p, ctx->d_J.p, ctx->d_K.p, ctx->stream, false);
+    int rc = hf_incore_jk_impl(ctx, ndens, ctx->d_G.p, nullptr, ctx->d_D.p, ctx->d_J.p, ctx->d_K.p, ctx->stream, false,
+                               true);
     if (rc) return rc;
     HF_HIP_CHECK(ctx, hipMemcpyAsync(J, ctx->d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HF_HIP_CHECK(ctx, hipMemcpyAsync(K, ctx->d_K.p, ndens * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -151,8 +151,17 @@ struct hf_ctx {
     DevBuf<double> d_G;
     DevBuf<float> d_G32;
     DevBuf<double> d_Dw;                   // packed weighted total density of an in-core J/K build
+    // per padded pair position: the Schwarz factor of its shell pair, rounded up to FP32 (0 for padding), and the
+    // largest; a store element (P, Q) whose product of the two is below the threshold was screened out (zero)
+    DevBuf<float> d_qf;
+    double qf_max = 0;
+    bool qf_valid = false;
     hipStream_t eri_stream = nullptr;      // the SCF evaluates the in-core store here, beside its own stream
     double* hpin = nullptr;                // kPinnedDoubles of pinned host memory for small read-backs
+    // pinned staging of host-to-device uploads (hf_upload): bytes [0, stage_used) belong to copies queued on the
+    // context's stream
+    char* stage = nullptr;
+    size_t stage_cap = 0, stage_used = 0;
     // in-core contraction statistics ([0] FP64 store, [1] FP32 store): launches, elements read
     // (eris), algorithmic flops, and with profile = 1 HIP-event time
     KernelStat incore_stat[2];
@@ -196,14 +205,7 @@ struct hf_ctx {
 // column-major as BLAS.
 int hf_dgemm_small(hf_ctx* ctx, hipStream_t st, bool ta, bool tb, int m, int n, int k, double alpha, const double* A,
                    int lda, const double* B, int ldb, double beta, double* C, int ldc);
-// SP2 purification of the n x n orthogonalised Fock matrix Fp to the projector onto its nocc lowest
-// eigenvectors, one fused kernel per step; X0, X1, W0, W1 are n x n work matrices, hist holds
-// 8 + 2 (max_steps + 1) doubles, part 2 ceil(n/16) (max_steps + 1).  result points at the projector
-// (one of X0, X1), or is null when the stopping rule was not met within max_steps.  hint: the step
-// count of a previous, similar call (0: none), used only to decide when to poll the stop flag.
-int hf_sp2_fused(hf_ctx* ctx, hipStream_t st, int n, int nocc, const double* Fp, double* X0, double* X1, double* W0,
-                 double* W1, double* hist, double* part, int max_steps, int hint, double*& result, int& steps);
-constexpr int kSmallDense = 384;   // the matrix-core kernels above serve n <= this
+constexpr int kSmallDense = 384;   // the matrix-core product kernel and the batched SCF kernels serve n <= this
 
 // J/K of the densities d_D (ndens x n x n) on stream st, screened at Schwarz product thr (0: the
 // context's threshold); accumulate adds to d_J/d_K instead of overwriting them; fp32 evaluates
@@ -216,8 +218,26 @@ int hf_jk_device_impl(hf_ctx* ctx, int ndens, const double* d_D, double* d_J, do
 int hf_eri_pairs_impl(hf_ctx* ctx, double* d_G, uint64_t len, hipStream_t st);
 // J/K from in-core ERIs (G FP64, or FP32 when g32 is set) on stream st: J of the summed densities,
 // K per density, both complete (no symmetrisation needed); accumulate adds to d_J/d_K.
+// own_store: G is this context's store, evaluated at its Schwarz threshold, so the loads of elements that screening
+// zeroed can be left out (same results, fewer bytes); false for a store the caller brings.
 int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
-                      double* d_K, hipStream_t st, bool accumulate);
+                      double* d_K, hipStream_t st, bool accumulate, bool own_store);
+// One molecule's in-core build in hf_incore_jk_batch (arguments as hf_incore_jk_impl).
+struct IncoreJob {
+    hf_ctx* ctx;
+    int ndens;
+    const double* G;
+    const float* g32;
+    const double* D;
+    double* J;
+    double* K;
+    bool accumulate;
+    bool own_store;
+};
+// The in-core builds of several molecules on stream st: one launch packs every weighted density and clears the
+// outputs, then each molecule's contraction (its tile shape and register layout depend on its n).  Same results
+// as hf_incore_jk_impl per molecule.
+int hf_incore_jk_batch(const std::vector<IncoreJob>& jobs, hipStream_t st);
 // Takes the device address of this device's allocation-free scratch array (once per device; hf_create).
 int hf_incore_scratch_init(hf_ctx* ctx);
 // Ends an SCF session the caller left open (hf_destroy, hf_scf_begin).
@@ -247,6 +267,15 @@ int hf_properties_impl(hf_ctx* ctx, int ndens, const double* d_D, double* charge
 // the call's first launch, leave after its last.  Nothing to do when st is the context's stream.
 int hf_stream_enter(hf_ctx* ctx, hipStream_t st);
 int hf_stream_leave(hf_ctx* ctx, hipStream_t st);
+
+// Host-to-device upload of bytes from src to dst, queued on the context's stream without waiting for it: the
+// bytes are staged in the context's pinned buffer (the copy engine reads it directly; a synchronous hipMemcpy
+// from pageable memory would wait for the copy, behind whatever the device is running).  src may be reused at
+// once; the staging space is recycled after a wait for the stream when it runs out.
+int hf_upload(hf_ctx* ctx, void* dst, const void* src, size_t bytes);
+// Pinned host space for a device-to-host copy on the context's stream (valid until the next hf_upload or
+// hf_stage_host after a wait for that copy).
+void* hf_stage_host(hf_ctx* ctx, size_t bytes);
 
 #define HF_HIP_CHECK(ctx, call)                                                                         \
     do {                                                                                                \

@@ -119,6 +119,18 @@ __host__ __device__ inline int64_t blocks_before(int i, int nr) {
     return nr * q * (q + 1) / 2 + r * (q + 1);
 }
 
+// Store rows the contraction can leave out: the store is screened (a shell quartet whose Schwarz product is below the
+// threshold was never evaluated, so its elements are zero), so a row P is zero whenever qf_P qmax < thr, qf being an
+// upper bound (rounded up to FP32) of the Schwarz factor of the position's shell pair, qmax the largest and thr the
+// store's threshold less a margin.  Such a row is given an empty buffer range, whose loads return the same zeros
+// without an HBM access, and a block of such rows returns at once, so the results are those of the full stream.
+// qf null: every row.
+struct StoreSkip {
+    const float* qf;      // per padded pair position (0 for padding)
+    double thr;
+    double qmax;          // largest qf
+};
+
 // One workgroup per block of NR store rows P_s = pair_pad_index(i, j_s) with one bra index i.  A wavefront
 // covers a row k of the pair triangle in chunks of W = 64 CPL columns: lane L owns columns CPL L + W m + h
 // (h < CPL, m < MC = ceil(n / W)) and reads a chunk's CPL columns of each of the NR rows with one load each
@@ -130,7 +142,7 @@ template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename 
 __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t npad,
                                                         const double* __restrict__ D, const double* __restrict__ Dw,
                                                         double* __restrict__ J, double* __restrict__ K,
-                                                        long long* __restrict__ fxK) {
+                                                        long long* __restrict__ fxK, StoreSkip sk) {
     extern __shared__ double lds[];
     constexpr int W = 64 * CPL;
     const int64_t blk = blockIdx.x;
@@ -141,6 +153,18 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
     const int j0 = (int)(blk - blocks_before(i, NR)) * NR;
     const int64_t P0 = hfgpu::pair_pad_base(i) + j0;     // row of s = 0; rows s with j0 + s > i are empty
     const int nrow = i - j0 + 1 < NR ? i - j0 + 1 : NR;
+    // store rows the screening zeroed entirely (qf_P qmax below the threshold) are not read; a block whose rows are
+    // all zero has nothing to add
+    bool live[NR];
+    {
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {
+            live[s] = s < nrow && (!sk.qf || (double)sk.qf[P0 + s] * sk.qmax >= sk.thr);
+            any = any || live[s];
+        }
+        if (!any) return;
+    }
     double* xj = lds;                      // D_d[j_s][:]      [NR][NDENS][n]
     double* xi = xj + NR * NDENS * n;      // D_d[i][:]        [NDENS][n]
     double* rI = xi + NDENS * n;           // additions to K_d[i][:]
@@ -201,8 +225,10 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
             for (int m = 0; m < MC; ++m) dd[r][m] = buffer_load<DV>(rd, (unsigned)(CPL * lane + W * m) * 8u);
 #pragma unroll
             for (int s = 0; s < NR; ++s) {
+                // an empty range for a row the store's screening zeroed: its loads return zero without a fetch
                 const auto rg = __builtin_amdgcn_make_buffer_rsrc(
-                    (void*)(G + (P0 + s) * npad + b), (short)0, s < nrow ? len * (int)sizeof(T) : 0, kBufferFlags);
+                    (void*)(G + (P0 + s) * npad + b), (short)0, s < nrow && live[s] ? len * (int)sizeof(T) : 0,
+                    kBufferFlags);
 #pragma unroll
                 for (int m = 0; m < MC; ++m)
                     g[r][s][m] = buffer_load<PV>(rg, (unsigned)(CPL * lane + W * m) * (unsigned)sizeof(T));
@@ -375,12 +401,12 @@ __global__ void incore_fx_finalize_kernel(const long long* __restrict__ fx, doub
 // register bounds of shape_fits can reach it.
 template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
 hipError_t launch_incore_mc(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
-                            long long* fxK, hipStream_t st) {
+                            long long* fxK, hipStream_t st, const StoreSkip& sk) {
     if constexpr (64 * CPL * (MC - 1) < kIncoreMaxN && (CPL == 2 || NDENS == 1 || MC <= 4) &&
                   (NR == 1 || NDENS == 1 || 64 * CPL * (MC - 1) < 256)) {
         const size_t lds = ((2 * NR + 2) * (size_t)NDENS * n + (size_t)NR * n + 4 * NR) * sizeof(double);
         hipLaunchKernelGGL((incore_jk_kernel<NDENS, MC, CPL, ROWS, PREFETCH, NR, T>),
-                           dim3((unsigned)blocks_before(n, NR)), dim3(256), lds, st, G, n, npad, D, Dw, J, K, fxK);
+                           dim3((unsigned)blocks_before(n, NR)), dim3(256), lds, st, G, n, npad, D, Dw, J, K, fxK, sk);
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;
@@ -389,16 +415,16 @@ hipError_t launch_incore_mc(const T* G, int n, int64_t npad, const double* D, co
 
 template <int NDENS, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
 hipError_t launch_incore(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
-                         long long* fxK, hipStream_t st) {
+                         long long* fxK, hipStream_t st, const StoreSkip& sk) {
     switch ((n + 64 * CPL - 1) / (64 * CPL)) {
-        case 1: return launch_incore_mc<NDENS, 1, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
-        case 2: return launch_incore_mc<NDENS, 2, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
-        case 3: return launch_incore_mc<NDENS, 3, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
-        case 4: return launch_incore_mc<NDENS, 4, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
-        case 5: return launch_incore_mc<NDENS, 5, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
-        case 6: return launch_incore_mc<NDENS, 6, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
-        case 7: return launch_incore_mc<NDENS, 7, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
-        case 8: return launch_incore_mc<NDENS, 8, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st);
+        case 1: return launch_incore_mc<NDENS, 1, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
+        case 2: return launch_incore_mc<NDENS, 2, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
+        case 3: return launch_incore_mc<NDENS, 3, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
+        case 4: return launch_incore_mc<NDENS, 4, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
+        case 5: return launch_incore_mc<NDENS, 5, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
+        case 6: return launch_incore_mc<NDENS, 6, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
+        case 7: return launch_incore_mc<NDENS, 7, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
+        case 8: return launch_incore_mc<NDENS, 8, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
         default: return hipErrorInvalidValue;
     }
 }
@@ -427,25 +453,26 @@ inline bool shape_fits(int shape, int n, int ndens) {
 
 template <int NDENS, typename T>
 hipError_t launch_incore_nd(int shape, const T* G, int n, int64_t np, const double* D, const double* Dw, double* J,
-                            double* K, long long* fxK, hipStream_t st) {
+                            double* K, long long* fxK, hipStream_t st, const StoreSkip& sk) {
     if (!shape_fits(shape, n, NDENS)) shape = 0;
     switch (shape) {
-        case 1: return launch_incore<NDENS, 2, 2, true, 1>(G, n, np, D, Dw, J, K, fxK, st);
-        case 2: return launch_incore<NDENS, 2, 1, true, 1>(G, n, np, D, Dw, J, K, fxK, st);
-        case 3: return launch_incore<NDENS, 1, 2, false, 1>(G, n, np, D, Dw, J, K, fxK, st);
-        case 4: return launch_incore<NDENS, 1, 2, true, 1>(G, n, np, D, Dw, J, K, fxK, st);
-        case 5: return launch_incore<NDENS, 2, 1, false, 3>(G, n, np, D, Dw, J, K, fxK, st);
-        case 6: return launch_incore<NDENS, 2, 1, true, 3>(G, n, np, D, Dw, J, K, fxK, st);
-        case 7: return launch_incore<NDENS, 2, 2, false, 3>(G, n, np, D, Dw, J, K, fxK, st);
-        default: return launch_incore<NDENS, 2, 2, false, 1>(G, n, np, D, Dw, J, K, fxK, st);
+        case 1: return launch_incore<NDENS, 2, 2, true, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
+        case 2: return launch_incore<NDENS, 2, 1, true, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
+        case 3: return launch_incore<NDENS, 1, 2, false, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
+        case 4: return launch_incore<NDENS, 1, 2, true, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
+        case 5: return launch_incore<NDENS, 2, 1, false, 3>(G, n, np, D, Dw, J, K, fxK, st, sk);
+        case 6: return launch_incore<NDENS, 2, 1, true, 3>(G, n, np, D, Dw, J, K, fxK, st, sk);
+        case 7: return launch_incore<NDENS, 2, 2, false, 3>(G, n, np, D, Dw, J, K, fxK, st, sk);
+        default: return launch_incore<NDENS, 2, 2, false, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
     }
 }
 
 template <typename T>
 hipError_t launch_incore_shape(int shape, int ndens, const T* G, int n, int64_t np, const double* D,
-                               const double* Dw, double* J, double* K, long long* fxK, hipStream_t st) {
-    return ndens == 1 ? launch_incore_nd<1>(shape, G, n, np, D, Dw, J, K, fxK, st)
-                      : launch_incore_nd<2>(shape, G, n, np, D, Dw, J, K, fxK, st);
+                               const double* Dw, double* J, double* K, long long* fxK, hipStream_t st,
+                               const StoreSkip& sk) {
+    return ndens == 1 ? launch_incore_nd<1>(shape, G, n, np, D, Dw, J, K, fxK, st, sk)
+                      : launch_incore_nd<2>(shape, G, n, np, D, Dw, J, K, fxK, st, sk);
 }
 
 // Autotuner: the tile shape is picked per (64-column chunks ceil(n/64), densities, store
@@ -472,7 +499,7 @@ int known_shape(int n, int ndens, bool fp32) {
 
 template <typename T>
 int tuned_shape(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double* D, const double* Dw,
-               hipStream_t st) {
+               hipStream_t st, const StoreSkip& sk) {
     const int key = tuning_key(n, ndens, sizeof(T) == 4);
     {
         std::lock_guard<std::mutex> lock(tuning().mu);
@@ -492,7 +519,7 @@ int tuned_shape(hf_ctx*, int ndens, const T* G, int n, int64_t np, const double*
         float t = 0;
         for (int rep = 0; rep < 3 && ok; ++rep) {          // first run warms the caches and the code
             ok = hipEventRecord(ev[0], st) == hipSuccess &&
-                 launch_incore_shape(shape, ndens, G, n, np, D, Dw, sJ.p, sK.p, nullptr, st) == hipSuccess &&
+                 launch_incore_shape(shape, ndens, G, n, np, D, Dw, sJ.p, sK.p, nullptr, st, sk) == hipSuccess &&
                  hipEventRecord(ev[1], st) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess;
             float ms = 0;
             if (ok) ok = hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess;
@@ -546,27 +573,22 @@ void hf_release_incore(hf_ctx* ctx, size_t keep_bytes) {
 
 namespace {
 
-// One in-core J/K build with the packed weighted density in Dw (np doubles).  no_alloc: the tile shape is the one
-// already tuned (or the default), and no events are created or recorded, so nothing is allocated.
-int incore_jk_run(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
-                  double* d_K, hipStream_t st, bool accumulate, double* Dw, bool no_alloc) {
+// The contraction of one in-core J/K build, after its packed weighted density is in Dw (np doubles) and, unless
+// accumulate, J and K are zero.  no_alloc: the tile shape is the one already tuned (or the default), and no events
+// are created or recorded, so nothing is allocated.
+int incore_contract(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
+                    double* d_K, hipStream_t st, bool accumulate, double* Dw, bool no_alloc, const StoreSkip& sk) {
     const int n = ctx->mol.nbf;
     const int64_t np = hfgpu::pair_pad_base(n);      // padded pair positions (rows and columns of the store)
     const size_t nn = (size_t)n * n;
-    hipLaunchKernelGGL(incore_dw_kernel, dim3(n), dim3(256), 0, st, d_D, n, ndens, Dw);
-    if (!accumulate) {
-        hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, d_J, nn);
-        hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((ndens * nn + 255) / 256)), dim3(256), 0, st, d_K,
-                           ndens * nn);
-    }
     // tile shape from the autotuner (first build of each kind in the process times the candidates)
     const int forced = ctx->opt.incore_shape;
     if (forced >= kIncoreShapes) return ctx->fail(HF_E_ARG, "incore_shape must be -1 or 0.." +
                                                              std::to_string(kIncoreShapes - 1));
     const int shape = forced >= 0 ? forced
                       : no_alloc  ? known_shape(n, ndens, g32 != nullptr)
-                      : g32    ? tuned_shape(ctx, ndens, g32, n, np, d_D, Dw, st)
-                               : tuned_shape(ctx, ndens, G, n, np, d_D, Dw, st);
+                      : g32    ? tuned_shape(ctx, ndens, g32, n, np, d_D, Dw, st, sk)
+                               : tuned_shape(ctx, ndens, G, n, np, d_D, Dw, st, sk);
     // statistics, and with profile = 1 events around the contraction (read at the next flush)
     const int f = g32 ? 1 : 0;
     const bool prof = ctx->opt.profile != 0 && !no_alloc;
@@ -599,9 +621,9 @@ int incore_jk_run(hf_ctx* ctx, int ndens, const double* G, const float* g32, con
     }
     hipError_t e;
     if (g32)
-        e = launch_incore_shape(shape, ndens, g32, n, np, d_D, Dw, d_J, d_K, fxK, st);
+        e = launch_incore_shape(shape, ndens, g32, n, np, d_D, Dw, d_J, d_K, fxK, st, sk);
     else
-        e = launch_incore_shape(shape, ndens, G, n, np, d_D, Dw, d_J, d_K, fxK, st);
+        e = launch_incore_shape(shape, ndens, G, n, np, d_D, Dw, d_J, d_K, fxK, st, sk);
     if (e != hipSuccess) return ctx->fail(HF_E_HIP, std::string("in-core J/K: ") + hipGetErrorString(e));
     if (fxK)
         hipLaunchKernelGGL(incore_fx_finalize_kernel, dim3((unsigned)std::min<size_t>((ndens * nn + 255) / 256, 4096)),
@@ -613,16 +635,81 @@ int incore_jk_run(hf_ctx* ctx, int ndens, const double* G, const float* g32, con
     return HF_OK;
 }
 
+
+// One in-core J/K build: the packed weighted density, zero outputs (unless accumulate), the contraction.
+int incore_jk_run(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
+                  double* d_K, hipStream_t st, bool accumulate, double* Dw, bool no_alloc, const StoreSkip& sk) {
+    const int n = ctx->mol.nbf;
+    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(incore_dw_kernel, dim3(n), dim3(256), 0, st, d_D, n, ndens, Dw);
+    if (!accumulate) {
+        hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, d_J, nn);
+        hipLaunchKernelGGL(incore_fill_kernel, dim3((unsigned)((ndens * nn + 255) / 256)), dim3(256), 0, st, d_K,
+                           ndens * nn);
+    }
+    return incore_contract(ctx, ndens, G, g32, d_D, d_J, d_K, st, accumulate, Dw, no_alloc, sk);
+}
+
+// The skips of a store this context evaluated (at its Schwarz threshold), or none.
+StoreSkip store_skip(const hf_ctx* ctx, bool own_store) {
+    StoreSkip sk{nullptr, 0.0, 0.0};
+    if (own_store && ctx->qf_valid && ctx->opt.schwarz_threshold > 0) {
+        sk.qf = ctx->d_qf.p;
+        // the task lists keep a quartet when Q_ket >= thr / Q_bra; the margin covers the rounding of that division
+        sk.thr = ctx->opt.schwarz_threshold * (1.0 - 1e-6);
+        sk.qmax = ctx->qf_max;
+    }
+    return sk;
+}
+
+// The set-up launches of several in-core builds in one: per job (blockIdx.y) the packed weighted density and,
+// unless accumulate, zero J and K (the same values incore_dw_kernel and incore_fill_kernel write).
+constexpr int kPrepJobs = 32;
+struct IncorePrep {
+    int count;
+    int n[kPrepJobs], ndens[kPrepJobs], clear[kPrepJobs];
+    const double* D[kPrepJobs];
+    double* Dw[kPrepJobs];
+    double* J[kPrepJobs];
+    double* K[kPrepJobs];
+};
+
+__global__ void __launch_bounds__(256) incore_prep_batch_kernel(IncorePrep g) {
+    const int j = blockIdx.y, n = g.n[j], ndens = g.ndens[j];
+    const size_t nn = (size_t)n * n;
+    const double* __restrict__ D = g.D[j];
+    double* __restrict__ Dw = g.Dw[j];
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int64_t b = hfgpu::pair_pad_base(k), len = hfgpu::pair_pad_base(k + 1) - b;
+        for (int l = threadIdx.x; l < len; l += blockDim.x) {
+            double v = 0.0;
+            if (l <= k)
+                for (int d = 0; d < ndens; ++d) v += D[d * nn + (size_t)k * n + l];
+            Dw[b + l] = (l == k ? 1.0 : 2.0) * v;
+        }
+    }
+    if (g.clear[j]) {
+        double* __restrict__ J = g.J[j];
+        double* __restrict__ K = g.K[j];
+        for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x)
+            J[idx] = 0.0;
+        for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < ndens * nn;
+             idx += (size_t)gridDim.x * blockDim.x)
+            K[idx] = 0.0;
+    }
+}
+
 }  // namespace
 
 int hf_incore_jk_impl(hf_ctx* ctx, int ndens, const double* G, const float* g32, const double* d_D, double* d_J,
-                      double* d_K, hipStream_t st, bool accumulate) {
+                      double* d_K, hipStream_t st, bool accumulate, bool own_store) {
     if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
     if (ndens < 1 || ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
     const int n = ctx->mol.nbf;
     if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "in-core J/K takes at most 512 basis functions");
     HF_HIP_CHECK(ctx, ctx->d_Dw.alloc((size_t)hfgpu::pair_pad_base(n)));
-    return incore_jk_run(ctx, ndens, G, g32, d_D, d_J, d_K, st, accumulate, ctx->d_Dw.p, false);
+    return incore_jk_run(ctx, ndens, G, g32, d_D, d_J, d_K, st, accumulate, ctx->d_Dw.p, false,
+                         store_skip(ctx, own_store));
 }
 
 extern "C" int hf_build_jk_incore_device_noalloc(hf_ctx* ctx, int ndens, const double* d_G, const double* d_D,
@@ -638,5 +725,39 @@ extern "C" int hf_build_jk_incore_device_noalloc(hf_ctx* ctx, int ndens, const d
     double* Dw = (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
     if (!Dw) return ctx->fail(HF_E_STATE, "allocation-free scratch not set up on this device");
     return incore_jk_run(ctx, ndens, d_G, nullptr, d_D, d_J, d_K, stream ? (hipStream_t)stream : ctx->stream, false,
-                         Dw, true);
+                         Dw, true, store_skip(ctx, false));
+}
+
+int hf_incore_jk_batch(const std::vector<IncoreJob>& jobs, hipStream_t st) {
+    if (jobs.empty()) return HF_OK;
+    for (const IncoreJob& q : jobs) {
+        hf_ctx* ctx = q.ctx;
+        if (!ctx->have_mol) return ctx->fail(HF_E_STATE, "no molecule set");
+        if (q.ndens < 1 || q.ndens > 2) return ctx->fail(HF_E_ARG, "ndens must be 1 or 2");
+        if (ctx->mol.nbf > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "in-core J/K takes at most 512 basis functions");
+        HF_HIP_CHECK(ctx, ctx->d_Dw.alloc((size_t)hfgpu::pair_pad_base(ctx->mol.nbf)));
+    }
+    for (size_t j0 = 0; j0 < jobs.size(); j0 += kPrepJobs) {
+        IncorePrep g{};
+        g.count = (int)std::min<size_t>(kPrepJobs, jobs.size() - j0);
+        int nmax = 1;
+        for (int j = 0; j < g.count; ++j) {
+            const IncoreJob& q = jobs[j0 + j];
+            g.n[j] = q.ctx->mol.nbf;
+            g.ndens[j] = q.ndens;
+            g.clear[j] = q.accumulate ? 0 : 1;
+            g.D[j] = q.D;
+            g.Dw[j] = q.ctx->d_Dw.p;
+            g.J[j] = q.J;
+            g.K[j] = q.K;
+            nmax = std::max(nmax, g.n[j]);
+        }
+        hipLaunchKernelGGL(incore_prep_batch_kernel, dim3(nmax, g.count), dim3(256), 0, st, g);
+    }
+    HF_HIP_CHECK(jobs[0].ctx, hipGetLastError());
+    for (const IncoreJob& q : jobs)
+        if (int rc = incore_contract(q.ctx, q.ndens, q.G, q.g32, q.D, q.J, q.K, st, q.accumulate, q.ctx->d_Dw.p, false,
+                                     store_skip(q.ctx, q.own_store)))
+            return rc;
+    return HF_OK;
 }

@@ -251,6 +251,7 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
     const hfhost::Molecule& mol = ctx->mol;
     const int ns = (int)mol.shells.size();
     const bool rebuild = !ctx->oe_valid;   // inputs of the current molecule not uploaded yet
+    bool uploaded = false;
     if (rebuild) {
         std::vector<int> L(ns), bf(ns), p0(ns), np(ns);
         std::vector<double> al, co;
@@ -266,16 +267,15 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
         HF_HIP_CHECK(ctx, ctx->d_oe_L.alloc(4 * (size_t)ns));
         HF_HIP_CHECK(ctx, ctx->d_oe_prim.alloc(2 * al.size() + Z.size() + mol.xyz.size()));
         int* ip = ctx->d_oe_L.p;
-        HF_HIP_CHECK(ctx, hipMemcpy(ip, L.data(), ns * sizeof(int), hipMemcpyHostToDevice));
-        HF_HIP_CHECK(ctx, hipMemcpy(ip + ns, bf.data(), ns * sizeof(int), hipMemcpyHostToDevice));
-        HF_HIP_CHECK(ctx, hipMemcpy(ip + 2 * ns, p0.data(), ns * sizeof(int), hipMemcpyHostToDevice));
-        HF_HIP_CHECK(ctx, hipMemcpy(ip + 3 * ns, np.data(), ns * sizeof(int), hipMemcpyHostToDevice));
-        double* dp = ctx->d_oe_prim.p;
-        HF_HIP_CHECK(ctx, hipMemcpy(dp, al.data(), al.size() * sizeof(double), hipMemcpyHostToDevice));
-        HF_HIP_CHECK(ctx, hipMemcpy(dp + al.size(), co.data(), co.size() * sizeof(double), hipMemcpyHostToDevice));
-        HF_HIP_CHECK(ctx, hipMemcpy(dp + 2 * al.size(), Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice));
-        HF_HIP_CHECK(ctx, hipMemcpy(dp + 2 * al.size() + Z.size(), mol.xyz.data(), mol.xyz.size() * sizeof(double),
-                                    hipMemcpyHostToDevice));
+        std::vector<int> ints;
+        for (const std::vector<int>* v : {&L, &bf, &p0, &np}) ints.insert(ints.end(), v->begin(), v->end());
+        std::vector<double> dbl;
+        for (const std::vector<double>* v : {(const std::vector<double>*)&al, (const std::vector<double>*)&co,
+                                             (const std::vector<double>*)&Z, &mol.xyz})
+            dbl.insert(dbl.end(), v->begin(), v->end());
+        if (int rc = hf_upload(ctx, ip, ints.data(), ints.size() * sizeof(int))) return rc;
+        if (int rc = hf_upload(ctx, ctx->d_oe_prim.p, dbl.data(), dbl.size() * sizeof(double))) return rc;
+        uploaded = true;
         ctx->oe_nprim = al.size();
     }
     OneEArgs a;
@@ -318,10 +318,14 @@ int hf_one_electron_device(hf_ctx* ctx, double* dS, double* dT, double* dV, hipS
         }
         ctx->oe_class_off[kOeClasses] = (int)all.size();
         HF_HIP_CHECK(ctx, ctx->d_oe_pairs.alloc(std::max<size_t>(all.size(), 1)));
-        if (!all.empty())
-            HF_HIP_CHECK(ctx, hipMemcpy(ctx->d_oe_pairs.p, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+        if (!all.empty()) {
+            if (int rc = hf_upload(ctx, ctx->d_oe_pairs.p, all.data(), all.size() * sizeof(int2))) return rc;
+            uploaded = true;
+        }
         ctx->oe_valid = true;
     }
+    // the uploads are queued on the context's stream
+    if (uploaded && st != ctx->stream) HF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const size_t nn = (size_t)mol.nbf * mol.nbf;
     // Deterministic mode: one launch per nucleus, in order, so that every V element receives its nuclei's
     // terms in the same order each time (within one launch each shell pair's elements have one writer).

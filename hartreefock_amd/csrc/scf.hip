@@ -26,20 +26,10 @@
 #include <limits>
 
 #include "gershgorin.h"
-#include "hf_internal.h"
+#include "scf_run.h"
 #include "wave_reduce.h"
 
 namespace {
-
-__global__ void fock_kernel(const double* __restrict__ h, const double* __restrict__ J, const double* __restrict__ K,
-                            double sk, double* __restrict__ F, int n, int symmetrize) {
-    const size_t nn = (size_t)n * n;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = idx / n, j = idx % n, t = j * n + i;
-        const double a = h[idx] + J[idx] - sk * K[idx];
-        F[idx] = symmetrize ? 0.5 * (a + (h[t] + J[t] - sk * K[t])) : a;
-    }
-}
 
 __global__ void add_kernel(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c,
                            size_t nn) {
@@ -61,14 +51,6 @@ __global__ void guess_kernel(const double* __restrict__ S, const double* __restr
         const double f = g > 0 ? g * S[idx] * (h[i * n + i] + h[j * n + j]) * 0.5 : h[idx];
         Fb[idx] = f;
         Fa[idx] = (idx == 1 || idx == (size_t)n) ? f + asym : f;
-    }
-}
-
-__global__ void antisym_kernel(const double* __restrict__ Y, double* __restrict__ E, int n) {
-    const size_t nn = (size_t)n * n;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = idx % n, j = idx / n;   // column-major (i row, j column)
-        E[idx] = Y[idx] - Y[i * n + j];
     }
 }
 
@@ -125,11 +107,6 @@ __global__ void lincomb_kernel(LinComb lc, double* __restrict__ out, size_t nn) 
     }
 }
 
-__global__ void mix_kernel(const double* __restrict__ Dn, double* __restrict__ D, double a, size_t nn, int replace) {
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x)
-        D[idx] = replace ? Dn[idx] : a * Dn[idx] + (1.0 - a) * D[idx];
-}
-
 // Deterministic reductions: out[r] = sum_i x_r[i] * y_r[i]  or  sum_i (x_r[i]-y_r[i])^2, r < m <= 8.
 // kDotBlocks blocks each reduce one contiguous chunk of every job into part[r * kDotBlocks + b]; one
 // wavefront then sums the partials of each job in a fixed order.
@@ -182,25 +159,6 @@ void launch_dots(hipStream_t s, const DotJobs& jobs, size_t nn, double* part, do
     hipLaunchKernelGGL(dots_final_kernel, dim3(1), dim3(64), 0, s, part, jobs.m, out);
 }
 
-// out = D - Dp; *mx = max |D - Dp|
-__global__ void __launch_bounds__(1024) delta_kernel(const double* __restrict__ D, const double* __restrict__ Dp,
-                                                     double* __restrict__ out, size_t m, double* __restrict__ mx) {
-    __shared__ double part[1024];
-    double v = 0;
-    for (size_t i = threadIdx.x; i < m; i += blockDim.x) {
-        const double d = D[i] - Dp[i];
-        out[i] = d;
-        v = fmax(v, fabs(d));
-    }
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *mx = part[0];
-}
-
 int grid_for(size_t n) {
     size_t b = (n + 255) / 256;
     return (int)std::min<size_t>(std::max<size_t>(b, 1), 4096);
@@ -216,7 +174,6 @@ int grid_for(size_t n) {
 // e_k = trace(X - X^2) is small, stop as soon as it no longer decreases (e_k >= e_{k-2}).
 // State (device, 8 doubles): lo, hi, done, e_{k-1}, e_{k-2}, steps, use_square, apply.
 enum { SP2_LO, SP2_HI, SP2_DONE, SP2_E1, SP2_E2, SP2_STEPS, SP2_SQUARE, SP2_APPLY };
-constexpr int kSp2MaxSteps = 100;
 
 __device__ double block_sum_256(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -290,83 +247,6 @@ __global__ void sp2_update_kernel(double* __restrict__ X, const double* __restri
         X[idx] = sq ? X2[idx] : 2.0 * X[idx] - X2[idx];
 }
 
-// Column-pivoted Householder QR solve of the (m x m) DIIS system (row-major A).
-std::vector<double> qr_solve(int m, std::vector<double> A, std::vector<double> b) {
-    std::vector<int> perm(m);
-    for (int i = 0; i < m; ++i) perm[i] = i;
-    double maxnorm = 0;
-    for (int j = 0; j < m; ++j) {
-        double s = 0;
-        for (int i = 0; i < m; ++i) s += A[i * m + j] * A[i * m + j];
-        maxnorm = std::max(maxnorm, std::sqrt(s));
-    }
-    int rank = m;
-    for (int k = 0; k < m; ++k) {
-        int piv = k;
-        double best = -1;
-        for (int j = k; j < m; ++j) {
-            double s = 0;
-            for (int i = k; i < m; ++i) s += A[i * m + j] * A[i * m + j];
-            if (s > best) { best = s; piv = j; }
-        }
-        if (piv != k) {
-            for (int i = 0; i < m; ++i) std::swap(A[i * m + k], A[i * m + piv]);
-            std::swap(perm[k], perm[piv]);
-        }
-        double nrm = 0;
-        for (int i = k; i < m; ++i) nrm += A[i * m + k] * A[i * m + k];
-        nrm = std::sqrt(nrm);
-        if (nrm <= std::numeric_limits<double>::epsilon() * maxnorm * m) { rank = k; break; }
-        const double alpha = A[k * m + k] > 0 ? -nrm : nrm;
-        std::vector<double> v(m, 0.0);
-        for (int i = k; i < m; ++i) v[i] = A[i * m + k];
-        v[k] -= alpha;
-        double vv = 0;
-        for (int i = k; i < m; ++i) vv += v[i] * v[i];
-        if (vv == 0) continue;
-        for (int j = k; j < m; ++j) {
-            double s = 0;
-            for (int i = k; i < m; ++i) s += v[i] * A[i * m + j];
-            s *= 2 / vv;
-            for (int i = k; i < m; ++i) A[i * m + j] -= s * v[i];
-        }
-        double s = 0;
-        for (int i = k; i < m; ++i) s += v[i] * b[i];
-        s *= 2 / vv;
-        for (int i = k; i < m; ++i) b[i] -= s * v[i];
-    }
-    std::vector<double> y(m, 0.0), x(m, 0.0);
-    for (int i = rank - 1; i >= 0; --i) {
-        double s = b[i];
-        for (int j = i + 1; j < rank; ++j) s -= A[i * m + j] * y[j];
-        y[i] = s / A[i * m + i];
-    }
-    for (int i = 0; i < m; ++i) x[perm[i]] = y[i];
-    return x;
-}
-
-// DIIS history of one spin: device slots for F and e, host inner products.
-struct Diis {
-    static constexpr int kKeep = 6, kFirst = 5;
-    double* F = nullptr;          // kKeep * nn each (SCF workspace)
-    double* E = nullptr;
-    std::deque<int> order;        // slot indices, oldest first
-    double B[kKeep][kKeep] = {};  // B[slot][slot]
-    size_t nn = 0;
-    int next_slot() {
-        if ((int)order.size() < kKeep) return (int)order.size();
-        return order.front();
-    }
-};
-
-struct ScfState {
-    int n = 0;
-    size_t nn = 0;
-    DevBuf<double> S, h, X, F0, Fa, Fb, J, K, D, Dn, Dprev, dD, C, eps, T1, T2, T3, err, work, red, redpart, sp2, sp2W,
-        sp2part;
-    DevBuf<rocblas_int> info;
-};
-
 #define BL(call)                                                                                        \
     do {                                                                                                \
         rocblas_status s_ = (call);                                                                     \
@@ -374,36 +254,10 @@ struct ScfState {
             return ctx->fail(HF_E_HIP, std::string(#call) + ": rocblas status " + std::to_string((int)s_)); \
     } while (0)
 
+// Products of Init and finish on the context's stream.
 int gemm(hf_ctx* ctx, bool ta, bool tb, int m, int n, int k, double alpha, const double* A, const double* B,
          double beta, double* C) {
-    if (std::max(m, std::max(n, k)) <= kSmallDense)
-        return hf_dgemm_small(ctx, ctx->stream, ta, tb, m, n, k, alpha, A, ta ? k : m, B, tb ? n : k, beta, C, m);
-    BL(rocblas_dgemm(ctx->blas, ta ? rocblas_operation_transpose : rocblas_operation_none,
-                     tb ? rocblas_operation_transpose : rocblas_operation_none, m, n, k, &alpha, A, ta ? k : m, B,
-                     tb ? n : k, &beta, C, m));
-    return HF_OK;
-}
-
-// Symmetric eigenproblem, in place: A -> eigenvectors (column-major), w ascending.
-// HFGPU_EIGEN selects the rocSOLVER routine: syevd (divide and conquer, default), syevj (Jacobi),
-// syevdj, syev (QR iteration).  All FP64.
-int syevd(hf_ctx* ctx, ScfState& st, double* A, double* w) {
-    static const char* env = std::getenv("HFGPU_EIGEN");
-    static const int alg = !env ? 0 : (!std::strcmp(env, "syevj") ? 1 : !std::strcmp(env, "syevdj") ? 2
-                                                                     : !std::strcmp(env, "syev") ? 3 : 0);
-    if (alg == 1) {
-        BL(rocsolver_dsyevj(ctx->blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, st.n, A,
-                            st.n, 0.0, st.red.p + 8, 100, st.info.p + 1, w, st.info.p));
-    } else if (alg == 2) {
-        BL(rocsolver_dsyevdj(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.info.p));
-    } else if (alg == 3) {
-        BL(rocsolver_dsyev(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.work.p,
-                           st.info.p));
-    } else {
-        BL(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.work.p,
-                            st.info.p));
-    }
-    return HF_OK;
+    return scf_gemm(ctx, ctx->stream, ta, tb, m, n, k, alpha, A, B, beta, C);
 }
 
 // S^{-1/2} into st.X.  The reference diagonalises S (HF/HartreeFockAlgorithm.cpp:48-59); the
@@ -455,7 +309,7 @@ int inverse_sqrt(hf_ctx* ctx, ScfState& st) {
     } else {
         // S^{-1/2} = U s^{-1/2} U^T
         HF_HIP_CHECK(ctx, hipMemcpyAsync(st.T1.p, st.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-        if (int rc = syevd(ctx, st, st.T1.p, st.eps.p)) return rc;
+        if (int rc = scf_syevd(ctx, st, ctx->stream, st.T1.p, st.eps.p)) return rc;
         hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T1.p, st.eps.p, st.T2.p, n);
         if (int rc = gemm(ctx, false, true, n, n, n, 1.0, st.T2.p, st.T1.p, 0.0, st.X.p)) return rc;
     }
@@ -467,15 +321,17 @@ int inverse_sqrt(hf_ctx* ctx, ScfState& st) {
 // Aufbau projector of the orthogonalised Fock matrix Fp (not modified) onto its nocc lowest
 // eigenvectors, by SP2 steps, into P; W is workspace.  ok = false when the stopping rule was not
 // met within kMaxSteps (a vanishing HOMO-LUMO gap leaves fractional eigenvalues).
-int sp2_projector(hf_ctx* ctx, ScfState& st, const double* Fp, int nocc, double* P, double* W, bool& ok) {
-    hipStream_t s = ctx->stream;
+}  // namespace
+
+int scf_sp2_projector(hf_ctx* ctx, ScfState& st, hipStream_t s, const double* Fp, int nocc, double* P, double* W,
+                      bool& ok) {
     const int n = st.n;
     constexpr int kMaxSteps = 100, kFirstCheck = 16, kEvery = 4;
     hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(1024), 0, s, Fp, n, st.sp2.p);
     hipLaunchKernelGGL(sp2_init_kernel, dim3(grid_for(st.nn)), dim3(256), 0, s, Fp, P, n, st.sp2.p);
     ok = false;
     for (int k = 1; k <= kMaxSteps; ++k) {
-        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, P, P, 0.0, W)) return rc;
+        if (int rc = scf_gemm(ctx, s, false, false, n, n, n, 1.0, P, P, 0.0, W)) return rc;
         hipLaunchKernelGGL(sp2_decide_kernel, dim3(1), dim3(256), 0, s, P, W, n, (double)nocc, st.sp2.p);
         hipLaunchKernelGGL(sp2_update_kernel, dim3(grid_for(st.nn)), dim3(256), 0, s, P, W, st.nn, st.sp2.p);
         if ((k >= kFirstCheck && (k - kFirstCheck) % kEvery == 0) || k == kMaxSteps) {
@@ -491,69 +347,38 @@ int sp2_projector(hf_ctx* ctx, ScfState& st, const double* Fp, int nocc, double*
     return HF_OK;
 }
 
-// Add (F, e) to a DIIS history; returns true when an extrapolation is possible.
-int diis_add(hf_ctx* ctx, ScfState& st, Diis& d, const double* Fsrc, const double* esrc, bool& ready) {
-    hipStream_t s = ctx->stream;
-    const int slot = d.next_slot();
-    if ((int)d.order.size() == Diis::kKeep) d.order.pop_front();
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(d.F + slot * st.nn, Fsrc, st.nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(d.E + slot * st.nn, esrc, st.nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-    d.order.push_back(slot);
-    DotJobs jobs{};
-    jobs.m = (int)d.order.size();
-    for (int r = 0; r < jobs.m; ++r) {
-        jobs.x[r] = d.E + slot * st.nn;
-        jobs.y[r] = d.E + d.order[r] * st.nn;
-        jobs.diff[r] = 0;
-    }
-    launch_dots(s, jobs, st.nn, st.redpart.p, st.red.p);
-    double h[8];
-    if (int rc = hf_readback(ctx, s, h, st.red.p, jobs.m)) return rc;
-    for (int r = 0; r < jobs.m; ++r) d.B[slot][d.order[r]] = d.B[d.order[r]][slot] = h[r];
-    ready = (int)d.order.size() >= Diis::kFirst;
+// Symmetric eigenproblem, in place: A -> eigenvectors (column-major), w ascending, on stream st.
+// HFGPU_EIGEN selects the rocSOLVER routine: syevd (divide and conquer, default), syevj (Jacobi),
+// syevdj, syev (QR iteration).  All FP64.
+int scf_syevd(hf_ctx* ctx, ScfState& st, hipStream_t stream, double* A, double* w) {
+    static const char* env = std::getenv("HFGPU_EIGEN");
+    static const int alg = !env ? 0 : (!std::strcmp(env, "syevj") ? 1 : !std::strcmp(env, "syevdj") ? 2
+                                                                     : !std::strcmp(env, "syev") ? 3 : 0);
+    if (stream != ctx->stream) rocblas_set_stream(ctx->blas, stream);
+    rocblas_status s;
+    if (alg == 1)
+        s = rocsolver_dsyevj(ctx->blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, st.n, A,
+                             st.n, 0.0, st.red.p + 8, 100, st.info.p + 1, w, st.info.p);
+    else if (alg == 2)
+        s = rocsolver_dsyevdj(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.info.p);
+    else if (alg == 3)
+        s = rocsolver_dsyev(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.work.p,
+                            st.info.p);
+    else
+        s = rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, st.n, A, st.n, w, st.work.p,
+                             st.info.p);
+    if (stream != ctx->stream) rocblas_set_stream(ctx->blas, ctx->stream);
+    if (s != rocblas_status_success) return ctx->fail(HF_E_HIP, "rocsolver eigensolver: status " + std::to_string((int)s));
     return HF_OK;
 }
 
-// Extrapolate F = sum c_i F_i; returns the error estimate sqrt(B_last,last + B_prev,prev).
-int diis_estimate(hf_ctx* ctx, ScfState& st, Diis& d, double* Fout, double& est) {
-    const int m = (int)d.order.size(), mp = m + 1;
-    std::vector<double> B((size_t)mp * mp, 0.0), rhs(mp, 0.0);
-    est = 0;
-    for (int i = 0; i < m; ++i) {
-        for (int j = 0; j < m; ++j) B[i * mp + j] = d.B[d.order[i]][d.order[j]];
-        if (i == m - 1 || i == m - 2) est += B[i * mp + i];
-        B[m * mp + i] = B[i * mp + m] = 1;
-    }
-    est = std::sqrt(est);
-    rhs[m] = 1;
-    std::vector<double> c = qr_solve(mp, B, rhs);
-    LinComb lc{};
-    lc.m = m;
-    for (int i = 0; i < m; ++i) {
-        lc.src[i] = d.F + d.order[i] * st.nn;
-        lc.c[i] = c[i];
-    }
-    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(st.nn)), dim3(256), 0, ctx->stream, lc, Fout, st.nn);
-    HF_HIP_CHECK(ctx, hipGetLastError());
-    return HF_OK;
+ScfWorkspace::~ScfWorkspace() {
+    for (hipEvent_t e : jk_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : eri_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ready_ev) (void)hipEventDestroy(ready_ev);
 }
-
-}  // namespace
-
-// Device matrices and events of hf_scf, kept in the context between calls (buffers only grow), so a
-// context that solves one molecule after another (a serving slot) allocates and frees nothing per SCF.
-struct ScfWorkspace {
-    ScfState st;
-    DevBuf<double> diisF[2], diisE[2];
-    hipEvent_t jk_ev[2] = {nullptr, nullptr};
-    hipEvent_t eri_ev[2] = {nullptr, nullptr};
-    ~ScfWorkspace() {
-        for (hipEvent_t e : jk_ev)
-            if (e) (void)hipEventDestroy(e);
-        for (hipEvent_t e : eri_ev)
-            if (e) (void)hipEventDestroy(e);
-    }
-};
 
 void hf_scf_workspace_free(hf_ctx* ctx) {
     delete ctx->scf_ws;
@@ -567,66 +392,13 @@ double hf_ctx_held_bytes(const hf_ctx* ctx) {
         const ScfState& st = w->st;
         for (const DevBuf<double>* m : {&st.S, &st.h, &st.X, &st.F0, &st.Fa, &st.Fb, &st.J, &st.K, &st.D, &st.Dn,
                                         &st.Dprev, &st.dD, &st.C, &st.eps, &st.T1, &st.T2, &st.T3, &st.err, &st.work,
-                                        &st.red, &st.redpart, &st.sp2, &st.sp2W, &st.sp2part, &w->diisF[0],
+                                        &st.red, &st.redpart, &st.sp2, &st.sp2W, &w->diisF[0],
                                         &w->diisF[1], &w->diisE[0], &w->diisE[1]})
             b += 8.0 * (double)m->n;
     }
     return b;
 }
 
-
-// One SCF (HartreeFockAlgorithm::Init + Calculate, HartreeFock/HartreeFockAlgorithm.cpp:21-192) as a session
-// object: init() is Init, step() one RHF/UHF Step, calculate() the reference's iteration control and finish()
-// the orbital energies, <S^2>, MP2 and the result.  hf_scf runs all four; hf_scf_begin / hf_scf_step /
-// hf_scf_end let a caller drive the steps itself (HartreeFockAlgorithm::Step).
-struct ScfRun {
-    hf_ctx* ctx;
-    hf_scf_params P;
-    const hf_scf_params* p;
-    std::chrono::steady_clock::time_point t_start;
-    hipStream_t s = nullptr;
-    bool uhf = false, mixed_tier = false, incore = false, store_pending = false, field = false;
-    int n = 0, nsp = 1, ne = 0, occ[2] = {0, 0};
-    size_t nn = 0;
-    double Enuc = 0, eri_ms = 0;
-    hipEvent_t* eri_ev = nullptr;
-    hipEvent_t* jk_ev = nullptr;
-    Diis diis[2];
-    double etolD = 0, etolN = 0, dtol = 0, itol = 0;
-    double energy = 0, homo = 0, lastErr = 0, jk_ms = 0;
-    int diag_steps = 0, sp2_steps[2] = {0, 0}, sp2_total = 0;
-    bool last_diag[2] = {false, false};
-    // incremental Fock build state: J, K hold the J/K of Dprev
-    int full_builds = 0, incr_builds = 0, since_full = 0;
-    bool force_full = false, last_incremental = false;
-    double quartets = 0, last_dmax = 0, last_quartets = 0;
-    bool incore_mixed = false, incremental = false;
-    int first_full = 1, rebuild_every = 8;
-    static constexpr double kIncoreFp32MaxDelta = 1e-3;   // largest max|D - Dprev| contracted with the FP32 store
-    bool jk_pending = false;
-    // mixed-precision tier: quartets below the context's fp32_threshold in FP32 until fp64_phase
-    bool mixed = false, fp64_phase = true, last_fp32 = false;
-    int final_fp64 = 3, fp64_steps = 0, fp32_builds = 0;
-    bool useDIIS = true;
-    std::vector<double> eps_h;
-    // iteration state of calculate() (or of a caller's steps), read by finish()
-    int iter = 0;
-    bool converged = false, cancelled = false;
-    double cur = 0;
-
-    bool external = false;   // steps driven by the caller (hf_scf_step): no convergence verdict of its own
-
-    ScfRun(hf_ctx* c, const hf_scf_params& params) : ctx(c), P(params), p(&P) {}
-    double quartet_count() const {
-        double q = 0;
-        for (const auto& ks : ctx->stats) q += (double)ks.quartets;
-        return q;
-    }
-    int init();
-    int step(int iter, double& rms);
-    int calculate();
-    int finish(hf_scf_result* r, double* D_out);
-};
 
 int ScfRun::init() {
     t_start = std::chrono::steady_clock::now();
@@ -660,10 +432,7 @@ int ScfRun::init() {
     HF_HIP_CHECK(ctx, st.T2.alloc(nn));
     HF_HIP_CHECK(ctx, st.T3.alloc(nn));
     HF_HIP_CHECK(ctx, st.sp2.alloc(8 + 2 * (kSp2MaxSteps + 1)));
-    if (n <= kSmallDense) {
-        HF_HIP_CHECK(ctx, st.sp2W.alloc(2 * nn));
-        HF_HIP_CHECK(ctx, st.sp2part.alloc(2 * (size_t)((n + 15) / 16) * (kSp2MaxSteps + 1)));
-    }
+    if (n <= kSmallDense) HF_HIP_CHECK(ctx, st.sp2W.alloc(2 * nn));
     HF_HIP_CHECK(ctx, st.err.alloc(nn));
     HF_HIP_CHECK(ctx, st.work.alloc(n));
     HF_HIP_CHECK(ctx, st.red.alloc(16));
@@ -773,204 +542,106 @@ int ScfRun::init() {
     // mixed-precision tier: quartets below the context's fp32_threshold in FP32 until fp64_phase
     mixed = ctx->opt.precision == HF_PREC_MIXED;
     final_fp64 = p->final_fp64_iterations > 0 ? p->final_fp64_iterations : 3;
-    fp64_phase = !mixed;
+    fp64_phase = !mixed || external;      // a caller's own steps (hf_scf_step) are all FP64
     useDIIS = p->use_diis != 0;
     eps_h.assign(2 * (size_t)n, 0.0);
-    return HF_OK;
-
-}
-
-int ScfRun::step(int iter, double& rms) {
-        ScfWorkspace& ws = *ctx->scf_ws;
-        ScfState& st = ws.st;
-        (void)ws;
-        double* F[2] = {st.Fa.p, st.Fa.p + nn};
-        if (iter == 0) {
-            for (int k = 0; k < nsp; ++k)
-                HF_HIP_CHECK(ctx, hipMemcpyAsync(F[k], st.F0.p + k * nn, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-        } else {
-            HF_HIP_CHECK(ctx, hipEventRecord(jk_ev[0], s));
-            const double q0 = quartet_count();
-            const bool full = !incremental || force_full || full_builds < first_full || since_full >= rebuild_every;
-            bool used_fp32 = !fp64_phase;
-            if (incore && store_pending) {   // the store (own stream) must be complete
-                HF_HIP_CHECK(ctx, hipStreamWaitEvent(s, eri_ev[1], 0));
-                store_pending = false;
-            }
-            if (full && incore) {
-                used_fp32 = !fp64_phase && !incore_mixed;
-                if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, used_fp32 ? ctx->d_G32.p : nullptr, st.D.p, st.J.p,
-                                               st.K.p, s, false))
-                    return rc;
-                ++full_builds;
-                since_full = 0;
-                force_full = false;
-            } else if (full) {
-                if (int rc = hf_jk_device_impl(ctx, nsp, st.D.p, st.J.p, st.K.p, s, 0.0, false, !fp64_phase)) return rc;
-                ++full_builds;
-                since_full = 0;
-                force_full = false;
-            } else {
-                // J/K are linear in D: add the J/K of D - Dprev, screened at thr / max|D - Dprev|
-                hipLaunchKernelGGL(delta_kernel, dim3(1), dim3(1024), 0, s, st.D.p, st.Dprev.p, st.dD.p, nsp * nn,
-                                   st.red.p + 15);
-                double dmax = 0;
-                // the in-core FP64 contraction reads the whole store whatever max|dD| is; the mixed tier
-                // needs max|dD| to decide whether the FP32 copy is safe
-                if (!incore || (incore_mixed && used_fp32)) {
-                    if (int rc = hf_readback(ctx, s, &dmax, st.red.p + 15, 1)) return rc;
-                    last_dmax = dmax;
-                }
-                // FP32 store only for small density changes: its rounding enters J/K scaled by |dD|, and
-                // near-linearly-dependent (diffuse) bases keep |dD| large for many iterations
-                if (incore_mixed && used_fp32 && !(dmax <= kIncoreFp32MaxDelta)) used_fp32 = false;
-                if (incore) {
-                    if (int rc = hf_incore_jk_impl(ctx, nsp, ctx->d_G.p, used_fp32 ? ctx->d_G32.p : nullptr, st.dD.p,
-                                                   st.J.p, st.K.p, s, true))
-                        return rc;
-                } else if (dmax > 0) {
-                    const double thr = ctx->opt.schwarz_threshold / std::min(1.0, dmax);
-                    if (int rc = hf_jk_device_impl(ctx, nsp, st.dD.p, st.J.p, st.K.p, s, thr, true, !fp64_phase))
-                        return rc;
-                }
-                ++incr_builds;
-                ++since_full;
-            }
-            last_incremental = !full;
-            last_fp32 = used_fp32;
-            if (last_fp32) {
-                ++fp32_builds;
-                fp64_steps = 0;
-            } else {
-                ++fp64_steps;
-            }
-            if (incremental)
-                HF_HIP_CHECK(ctx, hipMemcpyAsync(st.Dprev.p, st.D.p, nsp * nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HF_HIP_CHECK(ctx, hipEventRecord(jk_ev[1], s));
-            jk_pending = true;
-            last_quartets = quartet_count() - q0;
-            quartets += last_quartets;
-            for (int k = 0; k < nsp; ++k)
-                hipLaunchKernelGGL(fock_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.h.p, st.J.p, st.K.p + k * nn,
-                                   uhf ? 1.0 : 0.5, F[k], n, uhf ? 0 : 1);
-        }
-        // DIISStep: e = S D F - F D S
-        bool used = false;
-        if (useDIIS && iter && iter < p->max_diis_iterations) {
-            bool ready[2] = {false, false};
-            for (int k = 0; k < nsp; ++k) {
-                if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.S.p, st.D.p + k * nn, 0.0, st.T1.p)) return rc;
-                if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.T1.p, F[k], 0.0, st.T2.p)) return rc;
-                hipLaunchKernelGGL(antisym_kernel, dim3(grid_for(nn)), dim3(256), 0, s, st.T2.p, st.err.p, n);
-                if (int rc = diis_add(ctx, st, diis[k], F[k], st.err.p, ready[k])) return rc;
-            }
-            if (ready[nsp - 1]) {
-                double e0 = 0, e1 = 0;
-                if (int rc = diis_estimate(ctx, st, diis[nsp - 1], F[nsp - 1], e1)) return rc;
-                if (uhf)
-                    if (int rc = diis_estimate(ctx, st, diis[0], F[0], e0)) return rc;
-                lastErr = e1 + e0;
-                used = true;
-            }
-        } else
-            lastErr = 0;
-        // orthogonalise; new density from the SP2 projector of F' or, failing that, from the
-        // eigenvectors of F' (back-transformed)
-        for (int k = 0; k < nsp; ++k) {
-            const double scale = uhf ? 1.0 : 2.0;
-            double* Dk = st.Dn.p + k * nn;
-            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, F[k], st.X.p, 0.0, st.T1.p)) return rc;
-            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
-            bool ok = false;
-            if (occ[k] == 0) {
-                HF_HIP_CHECK(ctx, hipMemsetAsync(Dk, 0, nn * sizeof(double), s));
-                ok = true;
-            } else if (p->density_method == 0) {
-                const double* P = nullptr;
-                if (n <= kSmallDense) {
-                    double* res = nullptr;
-                    int steps = 0;
-                    if (int rc = hf_sp2_fused(ctx, s, n, occ[k], st.T2.p, st.T1.p, st.T3.p, st.sp2W.p, st.sp2W.p + nn,
-                                              st.sp2.p, st.sp2part.p, kSp2MaxSteps, sp2_steps[k], res, steps))
-                        return rc;
-                    ok = res != nullptr;
-                    sp2_steps[k] = ok ? steps : 0;
-                    sp2_total += ok ? steps + 1 : 0;
-                    P = res;
-                } else {
-                    if (int rc = sp2_projector(ctx, st, st.T2.p, occ[k], st.T1.p, st.T3.p, ok)) return rc;
-                    P = st.T1.p;
-                }
-                if (ok) {
-                    double* W = P == st.T1.p ? st.T3.p : st.T1.p;
-                    if (int rc = gemm(ctx, false, false, n, n, n, 1.0, P, st.X.p, 0.0, W)) return rc;
-                    if (int rc = gemm(ctx, false, false, n, n, n, scale, st.X.p, W, 0.0, Dk)) return rc;
-                }
-            }
-            last_diag[k] = !ok;
-            if (!ok) {
-                ++diag_steps;
-                if (int rc = syevd(ctx, st, st.T2.p, st.eps.p + k * n)) return rc;
-                if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T2.p, 0.0, st.C.p + k * nn)) return rc;
-                if (int rc = gemm(ctx, false, true, n, n, occ[k], scale, st.C.p + k * nn, st.C.p + k * nn, 0.0, Dk))
-                    return rc;
-            }
-            // The reference's density 2 C_occ C_occ^T is symmetric to the last bit (element (i,j) and (j,i) are the
-            // same products summed in the same order); X P X from tiled matrix-core products is not, and with an
-            // ill-conditioned S (diffuse functions: large elements in X) the two triangles differ by ~1e-10.  The
-            // J/K builds read one triangle or the other, so the density is made exactly symmetric here.
-            if (occ[k] != 0) hipLaunchKernelGGL(sym_kernel, dim3(grid_for(nn)), dim3(256), 0, s, Dk, n);
-        }
-        // energy E = (D.h + D.F)/2 (sum over spins; D.F is twice (RHF) or once (UHF) the sum of the
-        // occupied orbital energies) and the density change
-        DotJobs jobs{};
-        jobs.m = 0;
-        for (int k = 0; k < nsp; ++k) {
-            jobs.x[jobs.m] = st.Dn.p + k * nn; jobs.y[jobs.m] = st.h.p; jobs.diff[jobs.m++] = 0;
-            jobs.x[jobs.m] = st.Dn.p + k * nn; jobs.y[jobs.m] = F[k]; jobs.diff[jobs.m++] = 0;
-            jobs.x[jobs.m] = st.Dn.p + k * nn; jobs.y[jobs.m] = st.D.p + k * nn; jobs.diff[jobs.m++] = 1;
-        }
-        launch_dots(s, jobs, nn, st.redpart.p, st.red.p);
-        double red[8];
-        if (int rc = hf_readback(ctx, s, red, st.red.p, jobs.m)) return rc;
-        if (jk_pending) {
-            float ms = 0;
-            HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, jk_ev[0], jk_ev[1]));
-            jk_ms += ms;
-            jk_pending = false;
-        }
-        double e = 0, rms2 = 0;
-        for (int k = 0; k < nsp; ++k) {
-            e += 0.5 * (red[3 * k] + red[3 * k + 1]);
-            rms2 += red[3 * k + 2];
-        }
-        energy = e + Enuc;
-        rms = std::sqrt(rms2);
-        hipLaunchKernelGGL(mix_kernel, dim3(grid_for(nsp * nn)), dim3(256), 0, s, st.Dn.p, st.D.p, p->alpha,
-                           nsp * nn, used ? 1 : 0);
-        HF_HIP_CHECK(ctx, hipGetLastError());
-        return HF_OK;
-}
-
-int ScfRun::calculate() {
-    // ---- Calculate (HartreeFock/HartreeFockAlgorithm.cpp:105-192) ----
     // HFGPU_TRACE=1 prints one line per iteration to stderr (the reference's per-step TRACE,
-    // HartreeFock/RestrictedHartreeFock.cpp:149).
+    // HartreeFock/RestrictedHartreeFock.cpp:149)
     const char* tenv = std::getenv("HFGPU_TRACE");
-    const bool trace = tenv && tenv[0] == '1';
-    double prev = std::numeric_limits<double>::infinity(), rms = 0;
-    auto trace_line = [&](int it, double rmsv) {
-        if (trace)
-            std::fprintf(stderr, "hf_scf %s iter %4d  E % .12f  dE % .3e  |dD| %.3e  diis %.3e  jk %s max|dD| %.2e quartets %.3e\n",
-                         uhf ? "UHF" : "RHF", it, energy, energy - prev, rmsv, lastErr,
-                         last_incremental ? "incr" : "full", last_dmax, last_quartets);
-    };
-    // the caller's cancellation flag (HartreeFockAlgorithm::terminate, an atomic in the reference,
-    // HartreeFock/HartreeFockAlgorithm.h:57), read with an atomic load once per iteration
+    trace = tenv && tenv[0] == '1';
+    // the end of Init on this stream, for a batch whose steps run on another stream
+    if (!ws.ready_ev) HF_HIP_CHECK(ctx, hipEventCreateWithFlags(&ws.ready_ev, hipEventDisableTiming));
+    HF_HIP_CHECK(ctx, hipEventRecord(ws.ready_ev, s));
+    return HF_OK;
+}
+
+// J/K of the current density for the coming Fock matrix (RestrictedHartreeFock::InitFockMatrix,
+// HartreeFock/RestrictedHartreeFock.cpp:189-202; UnrestrictedHartreeFock::InitFockMatrices,
+// HartreeFock/UnrestrictedHartreeFock.cpp:228-238): a full build, or with incremental builds (the in-core mixed
+// tier, or incremental_fock) the J/K of D - Dprev added to the previous ones.
+void ScfRun::jk_plan(JkPlan& pl) {
+    ScfState& ss = ctx->scf_ws->st;
+    pl = JkPlan{};
+    pl.q0 = quartet_count();
+    const bool full = !incremental || force_full || full_builds < first_full || since_full >= rebuild_every;
+    bool used_fp32 = !fp64_phase;
+    if (incore && store_pending) {   // the store (own stream) must be complete
+        pl.wait_store = true;
+        store_pending = false;
+    }
+    pl.kind = incore ? JkPlan::kIncore : JkPlan::kDirect;
+    pl.src = ss.D.p;
+    if (full) {
+        if (incore) {
+            used_fp32 = !fp64_phase && !incore_mixed;
+            pl.g32 = used_fp32 ? ctx->d_G32.p : nullptr;
+        } else {
+            pl.fp32 = !fp64_phase;
+        }
+        ++full_builds;
+        since_full = 0;
+        force_full = false;
+    } else {
+        // J/K are linear in D: add the J/K of D - Dprev, screened at thr / max|D - Dprev|.  max|D - Dprev| is
+        // known from the previous step's energy phase (no read-back here).
+        pl.sub = true;
+        pl.src = ss.dD.p;
+        pl.accumulate = true;
+        const double dmax = dmax_next >= 0 ? dmax_next : 0.0;
+        if (!incore || (incore_mixed && used_fp32)) last_dmax = dmax;
+        // FP32 store only for small density changes: its rounding enters J/K scaled by |dD|, and
+        // near-linearly-dependent (diffuse) bases keep |dD| large for many iterations
+        if (incore_mixed && used_fp32 && !(dmax <= kIncoreFp32MaxDelta)) used_fp32 = false;
+        if (incore) {
+            pl.g32 = used_fp32 ? ctx->d_G32.p : nullptr;
+        } else if (dmax > 0) {
+            pl.thr = ctx->opt.schwarz_threshold / std::min(1.0, dmax);
+            pl.fp32 = !fp64_phase;
+        } else {
+            pl.kind = JkPlan::kNone;     // no density change: J/K stay
+        }
+        ++incr_builds;
+        ++since_full;
+    }
+    pl.copy_prev = incremental;
+    last_incremental = !full;
+    last_fp32 = used_fp32;
+    if (last_fp32) {
+        ++fp32_builds;
+        fp64_steps = 0;
+    } else {
+        ++fp64_steps;
+    }
+}
+
+void ScfRun::jk_done(const JkPlan& pl) {
+    last_quartets = quartet_count() - pl.q0;
+    quartets += last_quartets;
+}
+
+int ScfRun::step(int it, double& r) {
+    step_iter = it;
+    std::vector<ScfRun*> one{this};
+    if (int rc = scf_step_phases(one, s, ctx->scf_ws->bs)) return rc;
+    if (status != HF_OK) return status;
+    r = rms;
+    return HF_OK;
+}
+
+// ---- Calculate (HartreeFock/HartreeFockAlgorithm.cpp:105-192) as a state machine, one step at a time ----
+// kPhDiis: the DIIS loop until convergence, cancellation or maxIterations; kPhAfterDiis: normalIterAfterDIIS
+// steps without DIIS after a DIIS convergence; kPhPlain: the plain loop when DIIS did not converge.
+// Convergence is accepted only on a step whose J/K build was full (not incremental) and, in the
+// mixed-precision tier, all-FP64: the tier switches to FP64 once the criteria are met within a factor 100
+// (or met outright) and then needs final_fp64_iterations FP64 steps.
+void ScfRun::begin() {
+    prev = std::numeric_limits<double>::infinity();
+    ph_i = 0;
+    ph = iter < p->max_iterations ? kPhDiis : kPhDone;
+}
+
+int ScfRun::after_step() {
     auto stop = [&]() { return p->terminate && __atomic_load_n(p->terminate, __ATOMIC_ACQUIRE) != 0; };
-    // Convergence is accepted only on a step whose J/K build was full (not incremental) and, in
-    // the mixed-precision tier, all-FP64: the tier switches to FP64 once the criteria are met
-    // within a factor 100 (or met outright) and then needs final_fp64_iterations FP64 steps.
     auto accept = [&](bool met, bool nearly) {
         if (mixed && !fp64_phase && (nearly || met)) {
             fp64_phase = true;
@@ -983,49 +654,87 @@ int ScfRun::calculate() {
         }
         return !(mixed && (last_fp32 || fp64_steps < final_fp64));
     };
+    auto trace_line = [&]() {
+        if (trace)
+            std::fprintf(stderr, "hf_scf %s iter %4d  E % .12f  dE % .3e  |dD| %.3e  diis %.3e  jk %s max|dD| %.2e quartets %.3e\n",
+                         uhf ? "UHF" : "RHF", step_iter, energy, energy - prev, rms, lastErr,
+                         last_incremental ? "incr" : "full", last_dmax, last_quartets);
+    };
     // a non-finite energy or density change (coincident nuclei, a corrupted density) fails the SCF at
     // once; the reference would iterate to maxIterations on NaN
-    auto finite = [&]() { return std::isfinite(energy) && std::isfinite(rms); };
-    for (; iter < p->max_iterations; ++iter) {
-        if (int rc = step(iter, rms)) return rc;
-        cur = energy;
-        trace_line(iter, rms);
-        if (!finite()) return ctx->fail(HF_E_NUMERIC, "SCF energy or density is not finite");
-        const double de = std::fabs(prev - cur), etol = useDIIS ? etolD : etolN;
-        if (accept(de <= etol && rms < dtol && lastErr < itol,
-                   de <= 100 * etol && rms < 100 * dtol && lastErr < 100 * itol)) {
-            converged = true;
+    const bool finite = std::isfinite(energy) && std::isfinite(rms);
+    cur = energy;
+    // after the DIIS loop: normalIterAfterDIIS plain steps when DIIS converged, else the plain loop
+    auto after_diis_loop = [&]() {
+        if (!cancelled && useDIIS && iter < p->max_diis_iterations && converged && p->normal_iter_after_diis) {
+            useDIIS = false;
+            ph = kPhAfterDiis;
+            ph_i = 0;
+            return;
+        }
+        ph = (!cancelled && !converged && iter < p->max_iterations) ? kPhPlain : kPhDone;
+    };
+    switch (ph) {
+        case kPhDiis: {
+            trace_line();
+            if (!finite) break;
+            const double de = std::fabs(prev - cur), etol = useDIIS ? etolD : etolN;
+            if (accept(de <= etol && rms < dtol && lastErr < itol,
+                       de <= 100 * etol && rms < 100 * dtol && lastErr < 100 * itol)) {
+                converged = true;
+                after_diis_loop();
+            } else if (stop()) {
+                cancelled = true;
+                after_diis_loop();
+            } else {
+                prev = cur;
+                if (++iter >= p->max_iterations) after_diis_loop();
+            }
             break;
         }
-        if (stop()) { cancelled = true; break; }
-        prev = cur;
-    }
-    if (!cancelled && useDIIS && iter < p->max_diis_iterations && converged && p->normal_iter_after_diis) {
-        useDIIS = false;
-        for (int i = 0; i < p->normal_iter_after_diis; ++i) {
-            if (int rc = step(iter + i, rms)) return rc;
-            if (!finite()) return ctx->fail(HF_E_NUMERIC, "SCF energy or density is not finite");
-            cur = energy;
+        case kPhAfterDiis: {
+            if (!finite) break;
             prev = cur;
-            if (stop()) { cancelled = true; break; }
+            const bool c = stop();
+            if (c) cancelled = true;
+            if (c || ++ph_i >= p->normal_iter_after_diis) {
+                iter += p->normal_iter_after_diis;
+                useDIIS = p->use_diis != 0;
+                ph = (!cancelled && !converged && iter < p->max_iterations) ? kPhPlain : kPhDone;
+            }
+            break;
         }
-        iter += p->normal_iter_after_diis;
-        useDIIS = p->use_diis != 0;
-    }
-    if (!cancelled && !converged) {
-        for (; iter < p->max_iterations; ++iter) {
-            if (int rc = step(iter, rms)) return rc;
-            cur = energy;
-            trace_line(iter, rms);
-            if (!finite()) return ctx->fail(HF_E_NUMERIC, "SCF energy or density is not finite");
+        case kPhPlain: {
+            trace_line();
+            if (!finite) break;
             const double de = std::fabs(prev - cur);
             if (accept(de <= etolN && rms < dtol, de <= 100 * etolN && rms < 100 * dtol)) {
                 converged = true;
-                break;
+                ph = kPhDone;
+            } else if (stop()) {
+                cancelled = true;
+                ph = kPhDone;
+            } else {
+                prev = cur;
+                if (++iter >= p->max_iterations) ph = kPhDone;
             }
-            if (stop()) { cancelled = true; break; }
-            prev = cur;
+            break;
         }
+        default: break;
+    }
+    if (!finite) {
+        ph = kPhDone;
+        return ctx->fail(HF_E_NUMERIC, "SCF energy or density is not finite");
+    }
+    return HF_OK;
+}
+
+int ScfRun::calculate() {
+    begin();
+    while (!done()) {
+        double r = 0;
+        if (int rc = step(next_iter(), r)) return rc;
+        if (int rc = after_step()) return rc;
     }
     return HF_OK;
 }
@@ -1033,26 +742,28 @@ int ScfRun::calculate() {
 int ScfRun::finish(hf_scf_result* r, double* D_out) {
     ScfState& st = ctx->scf_ws->st;
     // orbital energies (HOMO) of the last Fock matrices: eigenvalues only
-    for (int k = 0; k < nsp; ++k) {
-        if (last_diag[k]) continue;
-        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.Fa.p + k * nn, st.X.p, 0.0, st.T1.p)) return rc;
-        if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
-        BL(rocsolver_dsyevd(ctx->blas, rocblas_evect_none, rocblas_fill_upper, n, st.T2.p, n, st.eps.p + k * n,
-                            st.work.p, st.info.p));
+    if (orbitals) {
+        for (int k = 0; k < nsp; ++k) {
+            if (last_diag[k]) continue;
+            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.Fa.p + k * nn, st.X.p, 0.0, st.T1.p)) return rc;
+            if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
+            BL(rocsolver_dsyevd(ctx->blas, rocblas_evect_none, rocblas_fill_upper, n, st.T2.p, n, st.eps.p + k * n,
+                                st.work.p, st.info.p));
+        }
+        HF_HIP_CHECK(ctx, hipMemcpyAsync(eps_h.data(), st.eps.p, nsp * n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        if (!uhf)
+            homo = eps_h[occ[0] - 1];
+        else
+            homo = std::max(occ[0] ? eps_h[occ[0] - 1] : 0.0, occ[1] ? eps_h[n + occ[1] - 1] : 0.0);
     }
-    HF_HIP_CHECK(ctx, hipMemcpyAsync(eps_h.data(), st.eps.p, nsp * n * sizeof(double), hipMemcpyDeviceToHost, s));
-    HF_HIP_CHECK(ctx, hipStreamSynchronize(s));
-    if (!uhf)
-        homo = eps_h[occ[0] - 1];
-    else
-        homo = std::max(occ[0] ? eps_h[occ[0] - 1] : 0.0, occ[1] ? eps_h[n + occ[1] - 1] : 0.0);
     // MP2 from the orbitals of the last Fock matrices (the reference's C and eigenvalues after
     // Calculate(), used by CalculateMp2Energy) and the in-core ERI store
-    if (p->mp2 && !cancelled) {
+    if (orbitals && p->mp2 && !cancelled) {
         for (int k = 0; k < nsp; ++k) {
             if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.Fa.p + k * nn, st.X.p, 0.0, st.T1.p)) return rc;
             if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T1.p, 0.0, st.T2.p)) return rc;
-            if (int rc = syevd(ctx, st, st.T2.p, st.eps.p + k * n)) return rc;
+            if (int rc = scf_syevd(ctx, st, ctx->stream, st.T2.p, st.eps.p + k * n)) return rc;
             if (int rc = gemm(ctx, false, false, n, n, n, 1.0, st.X.p, st.T2.p, 0.0, st.C.p + k * nn)) return rc;
         }
         if (incore && store_pending) {
@@ -1097,7 +808,7 @@ int ScfRun::finish(hf_scf_result* r, double* D_out) {
         HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, eri_ev[0], eri_ev[1]));
         r->eri_ms = ms;
     }
-    if (uhf) {
+    if (uhf && orbitals) {
         // <S^2> = ((Na-Nb)/2)^2 + (Na+Nb)/2 - sum_ij ((C_a,occ)^T S C_b,occ)_ij^2
         //       = ((Na-Nb)/2)^2 + (Na+Nb)/2 - Tr(P_a S P_b S), P = C_occ C_occ^T the last new densities
         double ov = 0;

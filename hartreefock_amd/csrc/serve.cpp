@@ -1,16 +1,26 @@
 // Serving loop: continuous batching of SCF requests on one GPU (declared in include/hfgpu.h).
 //
-// `slots` worker threads each own a context (stream, side streams, device memory).  Requests wait
-// in a FIFO queue; the head request is admitted by the first free worker once its estimated HBM
-// footprint fits in the budget together with what the other slots hold (no overtaking, so a large
-// request is not starved); a request larger than the whole budget is rejected at submission.
-// Admission happens whenever a worker or memory frees up, so requests that arrive while others run
-// join them, and the class kernels of all running requests overlap
-// on the device through their contexts' streams.  Results go to a completion queue read by
-// hf_serve_poll.  All times are steady-clock milliseconds since the server was created.
+// Lockstep mode (the default, lockstep = k >= 1): `slots` contexts hold the admitted requests; `prep_threads`
+// threads prepare each admitted request on its slot's own stream (molecule set-up, one-electron integrals,
+// S^-1/2, the in-core store), and k batch threads, each with its own stream, step their prepared requests
+// together: each phase of an SCF iteration is one launch over all the requests of a batch (scf_step_phases),
+// so a batch of B requests costs the launches and host round trips of one, and the batches fill each other's
+// round trips on the device.  A request joins the smallest batch at its next iteration boundary and leaves it
+// when its SCF ends.
+//
+// Slot mode (lockstep = 0): `slots` worker threads each own a context (stream, side streams, device memory)
+// and run one request's SCF alone at a time, the slots' kernels overlapping on the device.
+//
+// Both modes: requests wait in a FIFO queue; the head request is admitted to a free slot once its estimated HBM
+// footprint fits in the budget together with what the other slots hold (no overtaking, so a large request is
+// not starved); a request larger than the whole budget is rejected at submission.  Admission happens whenever a
+// slot or memory frees up, so requests that arrive while others run join them.  Results go to a completion queue
+// read by hf_serve_poll.  All times are steady-clock milliseconds since the server was created.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -19,7 +29,9 @@
 #include <thread>
 #include <vector>
 
-#include "hf_internal.h"
+#include <memory>
+
+#include "scf_run.h"
 
 namespace {
 
@@ -79,6 +91,20 @@ struct hf_server {
     uint64_t next_id = 1;
     int pending = 0;
     bool stop = false;
+
+    // ---- lockstep mode ----
+    std::vector<std::unique_ptr<ScfRun>> runs;   // per slot: the request's SCF session
+    std::vector<hf_serve_result> slot_res;       // per slot: the result being filled
+    std::vector<hipEvent_t> slot_ev;             // per slot: end of its last step on the batch stream
+    std::vector<char> slot_ev_pending;
+    std::deque<int> ready;                       // prepared slots waiting to join the batch
+    int preparing = 0;                           // slots being prepared
+    std::condition_variable cv_batch;
+    // one stream, step scratch and active count per batch thread (lockstep = the number of batch threads); a
+    // prepared request joins the batch with the fewest requests
+    std::vector<hipStream_t> bstreams;
+    std::vector<std::unique_ptr<BatchScratch>> bss;
+    std::vector<int> nactive;
 
     double now_ms() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -169,13 +195,236 @@ struct hf_server {
             cv_work.notify_all();
         }
     }
+
+    // A request leaves its slot: result to the completion queue, the slot's memory charged as what it keeps.
+    void complete(int w, hf_serve_result& r) {
+        r.done_ms = now_ms();
+        const double held = hf_ctx_held_bytes(ctxs[w]);
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            charged[w] = held;
+            busy[w] = 0;
+            add_stats(ctxs[w]);
+            done.push_back(r);
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+
+    // Preparation thread: admits the queue head to a free slot whose memory fits it (or frees an idle slot's
+    // memory when only that blocks the head), installs the molecule and runs Init on the slot's stream, then
+    // hands the slot to the batch thread.
+    void prep() {
+        const char* tenv = std::getenv("HFGPU_SERVE_TRACE");
+        const bool prep_trace = tenv && tenv[0] == '2';
+        for (;;) {
+            int w = -1;
+            bool release = false;
+            Request rq;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                cv_work.wait(lock, [&] {
+                    if (stop) return true;
+                    if (queue.empty()) return false;
+                    for (size_t v = 0; v < charged.size(); ++v)
+                        if (!busy[v] && fits((int)v, queue.front().footprint)) {
+                            w = (int)v;
+                            return true;
+                        }
+                    if (head_blocked())
+                        for (size_t v = 0; v < charged.size(); ++v)
+                            if (!busy[v] && (ctxs[v]->d_G.n + ctxs[v]->d_G32.n > 0 || ctxs[v]->scf_ws)) {
+                                w = (int)v;
+                                release = true;
+                                return true;
+                            }
+                    return false;
+                });
+                if (stop) return;
+                busy[w] = 1;
+                if (!release) {
+                    rq = std::move(queue.front());
+                    queue.pop_front();
+                    charged[w] = std::max(charged[w], rq.footprint);
+                    ++preparing;
+                }
+            }
+            hf_ctx* ctx = ctxs[w];
+            if (slot_ev_pending[w]) {   // the slot's previous request may still have work queued on the batch stream
+                (void)hipEventSynchronize(slot_ev[w]);
+                slot_ev_pending[w] = 0;
+            }
+            if (release) {
+                runs[w].reset();
+                if (ctx->d_G.n + ctx->d_G32.n > 0)
+                    hf_release_incore(ctx, 0);
+                else
+                    hf_scf_workspace_free(ctx);
+                const double held = hf_ctx_held_bytes(ctx);
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    charged[w] = held;
+                    busy[w] = 0;
+                }
+                cv_work.notify_all();
+                continue;
+            }
+            // a store above what a slot keeps, from the slot's previous request
+            hf_release_incore(ctx, (size_t)keep_store);
+            hf_serve_result& r = slot_res[w];
+            r = hf_serve_result{};
+            r.id = rq.id;
+            r.tag = rq.tag;
+            r.nbf = rq.nbf;
+            r.footprint = rq.footprint;
+            r.arrival_ms = rq.arrival_ms;
+            r.admit_ms = now_ms();
+            runs[w].reset();
+            int rc = hf_set_molecule(ctx, rq.basis.c_str(), (int)rq.Z.size(), rq.Z.data(), rq.xyz.data());
+            if (rc == HF_OK) {
+                hf_scf_params p = opt.scf;
+                p.unrestricted = rq.unrestricted;
+                p.terminate = nullptr;
+                // the in-core store only within the allowance the admission counted
+                if (p.incore) p.incore_max_bytes = rq.store_bytes > 0 ? rq.store_bytes : -1.0;
+                runs[w].reset(new (std::nothrow) ScfRun(ctx, p));
+                if (!runs[w]) rc = HF_E_OOM;
+                else {
+                    runs[w]->orbitals = false;   // a served result is the energy and the convergence verdict
+                    rc = runs[w]->init();
+                }
+                if (prep_trace)
+                    std::fprintf(stderr, "[serve] prep slot %d n=%d: %.1f ms (admitted %.0f ms after arrival)\n", w,
+                                 rq.nbf, now_ms() - r.admit_ms, r.admit_ms - r.arrival_ms);
+            }
+            if (rc != HF_OK) {
+                r.status = rc;
+                runs[w].reset();
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    --preparing;
+                }
+                cv_batch.notify_one();
+                complete(w, r);
+                continue;
+            }
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                --preparing;
+                ready.push_back(w);
+            }
+            cv_batch.notify_one();
+        }
+    }
+
+    // Batch thread: steps every prepared request together until its SCF ends.  HFGPU_SERVE_TRACE=1 prints, about
+    // once a second, the batch steps taken, the mean batch size and the time spent in steps and in waiting.
+    void batch_loop(int b) {
+        hipStream_t bstream = bstreams[b];
+        BatchScratch& bs = *bss[b];
+        std::vector<int> active;
+        std::vector<ScfRun*> ms;
+        const char* tenv = std::getenv("HFGPU_SERVE_TRACE");
+        const bool trace = tenv && (tenv[0] == '1' || tenv[0] == '2');
+        double t_last = now_ms(), t_step = 0, t_wait = 0;
+        long steps = 0, members = 0;
+        for (;;) {
+            if (trace && now_ms() - t_last > 1000.0) {
+                int q = 0, prep_n = 0, rd = 0;
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    q = (int)queue.size();
+                    prep_n = preparing;
+                    rd = (int)ready.size();
+                }
+                std::fprintf(stderr, "[serve] t=%.0f ms: %ld steps, mean batch %.1f, step %.0f ms, idle %.0f ms; "
+                             "queue %d, preparing %d, ready %d\n", now_ms(), steps, steps ? (double)members / steps : 0.0,
+                             t_step, t_wait, q, prep_n, rd);
+                t_last = now_ms();
+                steps = members = 0;
+                t_step = t_wait = 0;
+            }
+            const double tw0 = now_ms();
+            std::vector<int> joining;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                if (active.empty())
+                    cv_batch.wait_for(lock, std::chrono::milliseconds(trace ? 200 : 100000),
+                                      [&] { return !ready.empty() || (stop && preparing == 0); });
+                if (active.empty() && ready.empty() && stop && preparing == 0) return;
+                // take ready requests while this batch is not larger than the smallest other one
+                while (!ready.empty()) {
+                    int others = 1 << 30;
+                    for (size_t c = 0; c < nactive.size(); ++c)
+                        if ((int)c != b) others = std::min(others, nactive[c]);
+                    if (nactive.size() > 1 && nactive[b] > others && !active.empty()) break;
+                    joining.push_back(ready.front());
+                    ready.pop_front();
+                    ++nactive[b];
+                }
+                if (!ready.empty()) cv_batch.notify_one();
+            }
+            for (int w : joining) {
+                ScfRun* m = runs[w].get();
+                (void)hipStreamWaitEvent(bstream, ctxs[w]->scf_ws->ready_ev, 0);
+                m->begin();
+                if (m->done()) {
+                    finish_slot(w, HF_OK, b);
+                    continue;
+                }
+                active.push_back(w);
+            }
+            t_wait += now_ms() - tw0;
+            if (active.empty()) continue;
+            const double ts0 = now_ms();
+            ms.clear();
+            for (int w : active) {
+                ScfRun* m = runs[w].get();
+                m->step_iter = m->next_iter();
+                ms.push_back(m);
+            }
+            const int rc = scf_step_phases(ms, bstream, bs);
+            ++steps;
+            members += (long)ms.size();
+            t_step += now_ms() - ts0;
+            std::vector<int> keep;
+            for (int w : active) {
+                ScfRun* m = runs[w].get();
+                int st = rc != HF_OK ? rc : m->status;
+                if (st == HF_OK) st = m->after_step();
+                if (st != HF_OK || m->done()) {
+                    finish_slot(w, st, b);
+                    continue;
+                }
+                keep.push_back(w);
+            }
+            active.swap(keep);
+        }
+    }
+
+    void finish_slot(int w, int st, int b) {
+        ScfRun* m = runs[w].get();
+        hf_serve_result& r = slot_res[w];
+        r.energy = m->cur;
+        r.iterations = m->iter + (m->converged ? 1 : 0);
+        r.converged = m->converged ? 1 : 0;
+        r.status = st != HF_OK ? st : m->cancelled ? HF_E_CANCELLED : m->converged ? HF_OK : HF_E_NOT_CONVERGED;
+        if (hipEventRecord(slot_ev[w], bstreams[b]) == hipSuccess) slot_ev_pending[w] = 1;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            --nactive[b];
+        }
+        complete(w, r);
+    }
 };
 
 extern "C" {
 
 void hf_serve_default_options(hf_serve_options* o) {
-    o->slots = 8;
+    o->slots = 32;
     o->hbm_budget = 0;
+    o->lockstep = 2;
+    o->prep_threads = 8;
     hf_default_options(&o->ctx);
     hf_scf_defaults(&o->scf);
 }
@@ -215,7 +464,39 @@ int hf_serve_create(const hf_serve_options* o, hf_server** out) {
     s->charged.assign(o->slots, 0.0);
     s->busy.assign(o->slots, 0);
     s->keep_store = std::min(16.0 * (1 << 30), s->budget / o->slots);
-    for (int w = 0; w < o->slots; ++w) s->workers.emplace_back([s, w] { s->run(w); });
+    if (o->lockstep) {
+        s->runs.resize(o->slots);
+        s->slot_res.resize(o->slots);
+        s->slot_ev.assign(o->slots, nullptr);
+        s->slot_ev_pending.assign(o->slots, 0);
+        const int nb = std::max(1, std::min(o->lockstep, o->slots));
+        bool ok = true;
+        for (int b = 0; b < nb && ok; ++b) {
+            hipStream_t st = nullptr;
+            ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+            if (!ok) break;
+            s->bstreams.push_back(st);
+            s->bss.emplace_back(new BatchScratch());
+            ok = s->bss.back()->reserve(s->ctxs[0], 2 * o->slots) == HF_OK;
+        }
+        s->nactive.assign(nb, 0);
+        for (int w = 0; w < o->slots && ok; ++w)
+            ok = hipEventCreateWithFlags(&s->slot_ev[w], hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            for (hipEvent_t e : s->slot_ev)
+                if (e) (void)hipEventDestroy(e);
+            s->bss.clear();
+            for (hipStream_t st : s->bstreams) (void)hipStreamDestroy(st);
+            for (hf_ctx* x : s->ctxs) hf_destroy(x);
+            delete s;
+            return HF_E_HIP;
+        }
+        const int np = std::max(1, o->prep_threads);
+        for (int t = 0; t < np; ++t) s->workers.emplace_back([s] { s->prep(); });
+        for (int b = 0; b < nb; ++b) s->workers.emplace_back([s, b] { s->batch_loop(b); });
+    } else {
+        for (int w = 0; w < o->slots; ++w) s->workers.emplace_back([s, w] { s->run(w); });
+    }
     *out = s;
     return HF_OK;
 }
@@ -290,7 +571,13 @@ int hf_serve_destroy(hf_server* s) {
         s->queue.clear();
     }
     s->cv_work.notify_all();
+    s->cv_batch.notify_all();
     for (std::thread& t : s->workers) t.join();
+    s->runs.clear();
+    for (hipEvent_t e : s->slot_ev)
+        if (e) (void)hipEventDestroy(e);
+    s->bss.clear();
+    for (hipStream_t st : s->bstreams) (void)hipStreamDestroy(st);
     for (hf_ctx* c : s->ctxs) hf_destroy(c);
     delete s;
     return HF_OK;

@@ -252,6 +252,19 @@ int hf_scf_begin(hf_ctx* ctx, const hf_scf_params* p);
 int hf_scf_step(hf_ctx* ctx, int iter, double* energy, double* rms, double* diis_error);
 int hf_scf_end(hf_ctx* ctx, hf_scf_result* r, double* D_out);
 
+/* Lockstep SCF of several molecules, each installed in its own context (count contexts, all different), with
+ * its own parameters (params[i]): no counterpart in the reference's library, which runs one SCF per std::thread
+ * (HartreeFock/HartreeFockDoc.cpp:379-401, HartreeFockThread.cpp:79-121); this replaces that loop of threads for
+ * the molecules of one GPU.  Every molecule runs hf_scf's Init, steps and iteration control, but the steps of all
+ * the molecules still iterating advance together on ctxs[0]'s stream, and each phase of an iteration (Fock
+ * assembly, DIIS products and inner products, orthogonalisation, SP2 purification, density products, energy
+ * reductions, density update) is one launch over all of them, with one host round trip per phase for the whole
+ * batch; the J/K builds stay per molecule.  A molecule leaves the batch when its SCF ends (its result then comes
+ * from its own context, as hf_scf's).  results[i] and status[i] (HF_OK, HF_E_NOT_CONVERGED, ...) per molecule;
+ * returns HF_OK when every status is HF_OK, else the first non-OK status.  Each result equals the molecule's
+ * hf_scf result to the FP64 atomics' rounding (in deterministic mode, bit for bit). */
+int hf_scf_batch(hf_ctx* const* ctxs, int count, const hf_scf_params* params, hf_scf_result* results, int* status);
+
 /* Mulliken charges (natoms) and dipole moment (3, au, about the origin) of the densities D (host,
  * ndens x n x n: the RHF total density, or UHF alpha then beta), replacing
  * RestrictedHartreeFock::CalculateAtomicCharge / GetMoment (HartreeFock/RestrictedHartreeFock.cpp:291-335)
@@ -277,8 +290,9 @@ int hf_kernel_stats_reset(hf_ctx* ctx);
 
 /* ---- Serving: continuous batching of SCF requests on one GPU ----
  * No counterpart in the reference's library (its GUI runs one SCF per std::thread,
- * HartreeFock/HartreeFockDoc.cpp:379-401).  A server owns `slots` worker threads, each with its own
- * context (stream, device memory, SCF state kept on the device between requests).  A request is
+ * HartreeFock/HartreeFockDoc.cpp:379-401).  A server owns `slots` contexts (stream, device memory, SCF
+ * state kept on the device between requests), stepped together in lockstep (see lockstep) or each by its
+ * own worker thread.  A request is
  * admitted to a free slot, first come first served, when its estimated HBM footprint fits in
  * hbm_budget together with the requests already running; requests that arrive while others run
  * are admitted as slots and memory free up, and the kernels of concurrent requests overlap on the
@@ -286,10 +300,20 @@ int hf_kernel_stats_reset(hf_ctx* ctx);
 typedef struct hf_server hf_server;
 
 typedef struct hf_serve_options {
-    int slots;                     /* concurrent requests (worker threads / contexts), default 8 */
+    int slots;                     /* concurrent requests (contexts), default 32 */
     double hbm_budget;             /* bytes for admitted requests; 0 = 80% of the device's free memory */
     hf_options ctx;                /* context options of every slot (precision, thresholds, ...) */
     hf_scf_params scf;             /* SCF parameters of every request (unrestricted is per request) */
+    int lockstep;                  /* k >= 1 (default 2): k batch threads, each with its own stream, step the
+                                      admitted requests together: the SCF steps of a batch's requests advance
+                                      together, each phase of an iteration one launch over all of them (as
+                                      hf_scf_batch), and a prepared request joins the smallest batch at its next
+                                      iteration boundary and leaves when its SCF ends (two batches fill each
+                                      other's host round trips); prep_threads threads prepare newly admitted
+                                      requests (molecule set-up, one-electron integrals, S^-1/2, in-core store)
+                                      on their slots' streams.  0: one worker thread per slot, each request's SCF
+                                      run alone (hf_scf) on its slot's stream, the slots' kernels overlapping */
+    int prep_threads;              /* lockstep: preparation threads, default 8 */
 } hf_serve_options;
 
 typedef struct hf_serve_result {

@@ -26,17 +26,20 @@ def _drain(srv, want, timeout_s=100.0):
     return got
 
 
+@pytest.mark.parametrize("lockstep", [1, 0], ids=["lockstep", "slots"])
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
-def test_poisson_slice_results_match_alone(gpu_engine_factory, prec):
-    """200 requests of the serving mix arriving as a Poisson process (40/s, seeded) to 8 slots with in-core
-    stores: every request comes back, and each converged energy equals the molecule's SCF run alone on one
-    context (1e-9 Eh FP64; the mixed tier's request against its own solo run in the same tier, 1e-9 Eh, and
-    against the FP64 solo energy, 1e-6 Eh)."""
+def test_poisson_slice_results_match_alone(gpu_engine_factory, prec, lockstep):
+    """200 requests of the serving mix arriving as a Poisson process (40/s, seeded) with in-core stores, to a
+    lockstep server of 32 slots (every admitted request stepped together) or to 8 slot threads: every request
+    comes back, and each converged energy equals the molecule's SCF run alone on one context (1e-9 Eh FP64;
+    the mixed tier's request against its own solo run in the same tier, 1e-9 Eh, and against the FP64 solo
+    energy, 1e-6 Eh)."""
     import hartreefock_amd as hf
+    print("case poisson precision=%d lockstep=%d" % (prec, lockstep), flush=True)
     bfile = basis("6-31g.1.nw")
     mols = g.serving_mix(200, seed=77)
     opts = dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1)
-    srv = hf.Server(slots=8, options=opts, scf=dict(SCF, incore=1))
+    srv = hf.Server(slots=32 if lockstep else 8, options=opts, scf=dict(SCF, incore=1), lockstep=bool(lockstep))
     rng = np.random.default_rng(5)
     ids = {}
     try:
