@@ -121,48 +121,65 @@ def cpu_baseline_single(bfile, atoms, iterations, budget_s):
 
 # ---------------------------------------------------------------------------------------------- serving
 SERVE_SLOTS = 32       # bench.py --slots N changes it
-SERVE_LOCKSTEP = 1     # bench.py --lockstep 0: one worker thread per slot instead of the lockstep batch
+SERVE_LOCKSTEP = 2     # batch threads of the lockstep server (bench.py --lockstep k); 0: one worker thread per slot
 SLOT_MODE_SLOTS = 8    # slots of the slot-thread comparison leg (its measured saturation point is 4-8)
 
 
-def serve(hf, bfile, reqs, prec, rate=None, slots=None, incore=1, tag="", profile=0, profile_class=-1,
-          seed=77, max_iterations=300, deterministic=0, lockstep=None):
-    """Serve reqs = [(tag, atoms)]: all queued at once (rate None) or as a Poisson process at `rate`/s.  Returns the
-    results by tag, the wall time from the first submission to the last completion and the summed kernel
-    statistics."""
+def _pass(srv, bfile, reqs, rate, seed, tag):
+    """One pass of reqs through a running server: all queued at once (rate None) or as a Poisson process at rate/s.
+    Returns the results by tag and the wall time from the first submission to the last completion."""
     import numpy as np
+    rng = np.random.default_rng(seed)
+    got, shown = {}, 0
+    gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+    hip_sync()
+    t0 = time.perf_counter()
+    due = 0.0
+    for (rt, atoms), gap in zip(reqs, gaps):
+        due += gap
+        while time.perf_counter() - t0 < due:
+            for r in srv.poll(64, 0):
+                got[r.tag] = r
+            time.sleep(min(0.0005, max(0.0, due - (time.perf_counter() - t0))))
+        srv.submit(bfile, atoms, tag=rt)
+    while len(got) < len(reqs):
+        for r in srv.poll(64, 1000):
+            got[r.tag] = r
+        if len(got) // 500 != shown:
+            shown = len(got) // 500
+            log("serving %s: %d of %d done" % (tag, len(got), len(reqs)))
+    hip_sync()
+    wall = time.perf_counter() - t0
+    log("serving %s: %d requests in %.1f s" % (tag, len(reqs), wall))
+    return got, wall
+
+
+def serve(hf, bfile, reqs, prec, rate=None, slots=None, incore=1, tag="", profile=0, profile_class=-1,
+          seed=77, max_iterations=300, deterministic=0, lockstep=None, warm=None, poisson_factor=None):
+    """Serve reqs = [(tag, atoms)] on a new server: first the warm-up requests `warm` (untimed: a server's slots
+    reach their working sizes), then reqs all queued at once (rate None) or as a Poisson process at `rate`/s.  With
+    poisson_factor, reqs run once more on the same server as a Poisson process at that fraction of the rate just
+    measured (converged requests / wall time).  Returns the results by tag, the wall time from the first
+    submission to the last completion, the kernel statistics of that pass and the Poisson pass's results (or
+    None)."""
     ls = SERVE_LOCKSTEP if lockstep is None else lockstep
     srv = hf.Server(slots=slots or SERVE_SLOTS, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1,
                                                              profile=profile, profile_class=profile_class,
                                                              deterministic=deterministic),
-                    scf=dict(SCF_KW, incore=incore, max_iterations=max_iterations), lockstep=bool(ls))
-    rng = np.random.default_rng(seed)
-    got, shown = {}, 0
+                    scf=dict(SCF_KW, incore=incore, max_iterations=max_iterations), lockstep=ls if ls else False)
     try:
-        gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
-        hip_sync()
-        t0 = time.perf_counter()
-        due = 0.0
-        for (rt, atoms), gap in zip(reqs, gaps):
-            due += gap
-            while time.perf_counter() - t0 < due:
-                for r in srv.poll(64, 0):
-                    got[r.tag] = r
-                time.sleep(min(0.0005, max(0.0, due - (time.perf_counter() - t0))))
-            srv.submit(bfile, atoms, tag=rt)
-        while len(got) < len(reqs):
-            for r in srv.poll(64, 1000):
-                got[r.tag] = r
-            if len(got) // 500 != shown:
-                shown = len(got) // 500
-                log("serving %s: %d of %d done" % (tag, len(got), len(reqs)))
-        hip_sync()
-        wall = time.perf_counter() - t0
+        if warm:
+            _pass(srv, bfile, warm, None, seed, tag + " (warm-up)")
+            srv.kernel_stats(reset=True)
+        got, wall = _pass(srv, bfile, reqs, rate, seed, tag)
         stats = srv.kernel_stats()
+        extra = None
+        if poisson_factor:
+            conv = sum(1 for r in got.values() if r.status == 0 and r.converged)
+            extra = _pass(srv, bfile, reqs, poisson_factor * conv / wall, seed + 1, tag + " poisson")
     finally:
         srv.close()
-    log("serving %s: %d requests in %.1f s" % (tag, len(reqs), wall))
-    return got, wall, stats
+    return got, wall, stats, extra
 
 
 def kernel_kinds(stats):
@@ -208,53 +225,56 @@ def serving_bench(a):
     mols = g.serving_mix(W * B + K * B, seed=2024)
     warm = [(i, atoms) for i, (_, atoms) in enumerate(mols[:W * B])]
     timed = [(i, atoms) for i, (_, atoms) in enumerate(mols[W * B:], start=W * B)]
-    # warm-up (untimed): W batches in each tier with every kernel timed, which picks the dominant kernel
+    # dominant kernel (untimed): one batch per tier with every kernel timed
     dominant = {}
     for prec in (1, 0):
-        _, _, st = serve(hf, bfile, warm, prec, tag="warm-up %s" % ("mixed" if prec else "fp64"), profile=1)
+        _, _, st, _ = serve(hf, bfile, warm[:B], prec, tag="kernel pick %s" % ("mixed" if prec else "fp64"), profile=1)
         kinds = kernel_kinds(st)
         dominant[prec] = max(kinds, key=lambda k: kinds[k]["ms"]) if kinds else None
-        log("warm-up %s: dominant kernel %s (%s)" % ("mixed" if prec else "fp64", dominant[prec],
-                                                    ", ".join("%s %.0f ms" % (k, v["ms"]) for k, v in
-                                                              sorted(kinds.items(), key=lambda x: -x[1]["ms"])[:5])))
-    # timed region: the mixed tier, every timed request queued at once; only the dominant kernel carries events
-    # (profile_class: its class key, or -2 for the in-core contraction)
+        log("kernel pick %s: dominant kernel %s (%s)" % ("mixed" if prec else "fp64", dominant[prec],
+                                                        ", ".join("%s %.0f ms" % (k, v["ms"]) for k, v in
+                                                                  sorted(kinds.items(), key=lambda x: -x[1]["ms"])[:5])))
+    # timed region: the mixed tier on a server warmed up with W batches, every timed request queued at once; only the
+    # dominant kernel carries events (profile_class: its class key, or -2 for the in-core contraction); then the same
+    # requests under Poisson arrivals at 0.8x the measured capacity, on the same server, for latency
     dom_key = None
     if dominant[1]:
-        n, s = dominant[1], "spd"
-        dom_key = -2 if n.startswith("incore") else s.index(n[1]) * 1000 + s.index(n[2]) * 100 + s.index(n[4]) * 10 + \
-            s.index(n[5])
-    res_m, wall_m, st_m = serve(hf, bfile, timed, 1, tag="timed mixed", profile=1 if dom_key is not None else 0,
-                                profile_class=dom_key if dom_key is not None else -1)
-    # the same requests in FP64 (closed loop), then both tiers under Poisson arrivals for latency
-    res_d, wall_d, st_d = serve(hf, bfile, timed, 0, tag="fp64")
+        nm, sp = dominant[1], "spd"
+        dom_key = -2 if nm.startswith("incore") else sp.index(nm[1]) * 1000 + sp.index(nm[2]) * 100 + \
+            sp.index(nm[4]) * 10 + sp.index(nm[5])
+    res_m, wall_m, st_m, pm = serve(hf, bfile, timed, 1, tag="timed mixed", profile=1 if dom_key is not None else 0,
+                                    profile_class=dom_key if dom_key is not None else -1, warm=warm,
+                                    poisson_factor=0.8)
+    # the same requests in FP64 (closed loop, then Poisson), on a server warmed up the same way
+    res_d, wall_d, st_d, pd = serve(hf, bfile, timed, 0, tag="fp64", warm=warm, poisson_factor=0.8)
     conv_d = {t: r.energy for t, r in res_d.items() if r.status == 0 and r.converged}
     conv_m = {t: r.energy for t, r in res_m.items() if r.status == 0 and r.converged}
     # the |dE| reference: the same requests in FP64 deterministic mode (fixed-point J/K and V sums, whose value
     # does not depend on the order of the GPU's additions); untimed
-    res_r, wall_r, _ = serve(hf, bfile, timed, 0, tag="fp64 deterministic reference", deterministic=1)
+    res_r, wall_r, _, _ = serve(hf, bfile, timed, 0, tag="fp64 deterministic reference", deterministic=1)
     conv_r = {t: r.energy for t, r in res_r.items() if r.status == 0 and r.converged}
     good_m = [t for t, e in conv_m.items() if t in conv_r and abs(e - conv_r[t]) <= 1e-6]
     dE = [abs(conv_m[t] - conv_r[t]) for t in conv_m if t in conv_r]
     dE_d = [abs(conv_d[t] - conv_r[t]) for t in conv_d if t in conv_r]
     cap_m, cap_d = len(good_m) / wall_m, len(conv_d) / wall_d
     lat = {}
-    for prec, cap in ((1, cap_m), (0, cap_d)):
-        r, _, _ = serve(hf, bfile, timed, prec, rate=0.8 * cap, tag="poisson %s" % ("mixed" if prec else "fp64"),
-                        seed=78)
+    for name, (r, w) in (("mixed", pm), ("fp64", pd)):
         ms = [x.done_ms - x.arrival_ms for x in r.values()]
-        lat["mixed" if prec else "fp64"] = {"rate_per_s": 0.8 * cap, "p50_ms": pct(ms, 50), "p99_ms": pct(ms, 99),
-                                            "converged": sum(1 for x in r.values() if x.converged)}
+        conv = sum(1 for x in r.values() if x.status == 0 and x.converged)
+        lat[name] = {"rate_per_s": 0.8 * (cap_m if name == "mixed" else cap_d), "p50_ms": pct(ms, 50),
+                     "p99_ms": pct(ms, 99), "converged": conv, "molecules_per_s": conv / w}
     # the slot-thread server (one worker thread and context per slot, each request's SCF alone), FP64, same requests
     slot_leg = None
     if SERVE_LOCKSTEP and not a.no_slot_leg:
-        res_s, wall_s, _ = serve(hf, bfile, timed, 0, slots=SLOT_MODE_SLOTS, lockstep=0, tag="fp64 slot threads")
+        res_s, wall_s, _, _ = serve(hf, bfile, timed, 0, slots=SLOT_MODE_SLOTS, lockstep=0, tag="fp64 slot threads",
+                                    warm=warm[:B])
         conv_s = sum(1 for r in res_s.values() if r.status == 0 and r.converged)
         slot_leg = {"molecules_per_s": conv_s / wall_s, "wall_s": wall_s, "converged": conv_s,
                     "slots": SLOT_MODE_SLOTS}
     # the plain FP64 path, one molecule at a time with direct J/K (the north star's reference point)
     plain_reqs = timed[:a.plain_requests]
-    res_p, wall_p, _ = serve(hf, bfile, plain_reqs, 0, slots=1, incore=0, tag="plain fp64", lockstep=0)
+    res_p, wall_p, _, _ = serve(hf, bfile, plain_reqs, 0, slots=1, incore=0, tag="plain fp64", lockstep=0,
+                                warm=warm[:10])
     cap_p = sum(1 for r in res_p.values() if r.status == 0 and r.converged) / wall_p
     eris_m = sum(s.eris for s in st_m if s.la >= 0)
     roof = roofline_of(dominant[1], st_m) if dominant[1] else None
@@ -270,13 +290,15 @@ def serving_bench(a):
         "config": {"workload": "serving mix: %d RHF requests (%d steps x %d), water clusters 1-12 / n-alkanes C1-C16, "
                                "6-31G, %d-%d functions" % (K * B, K, B, min(nbf), max(nbf)),
                    "batch_per_step": B, "slots": SERVE_SLOTS,
-                   "server": "lockstep: every admitted request's SCF steps advance together, one launch per phase "
-                             "for all of them" if SERVE_LOCKSTEP else "slot threads: one worker thread per slot",
+                   "server": ("lockstep: %d batch threads, the SCF steps of each batch's requests advance together, one "
+                              "launch per phase for all of them" % SERVE_LOCKSTEP) if SERVE_LOCKSTEP else
+                             "slot threads: one worker thread per slot",
                    "jk": "in-core ERI store per request",
                    "eri_convention": "exact (6-31G has no d shells: identical to the reference's)",
                    "schwarz_threshold": 1e-12, "scf": SCF_DESC, "max_iterations": 300,
-                   "arrival": "all queued at once (closed loop) in the timed region; Poisson at 0.8x capacity for "
-                              "latency", "parallelism": "single GPU"},
+                   "arrival": "all queued at once (closed loop) in the timed region, on a server warmed up with "
+                              "--warmup x --batch requests; Poisson at 0.8x capacity on the same server for latency",
+                   "parallelism": "single GPU"},
         "mixed": {"molecules_per_s": cap_m, "wall_s": wall_m, "converged": len(conv_m), "within_1e-6_of_fp64":
                   len(good_m), "max_abs_dE_vs_fp64_Eh": max(dE) if dE else None,
                   "dE_reference": "FP64 deterministic mode, same requests (%d converged)" % len(conv_r),
@@ -512,6 +534,9 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
         k["write_ratio"] = k["write_bytes"] / k["write_alg"] if k["write_alg"] else None
         k["traffic_ratio"] = (k["fetch_ratio"] or 0.0) + (k["write_ratio"] or 0.0)
         k["mean_alg_bytes_per_dispatch"] = k["fetch_alg"] / max(1, k["dispatches"])
+        # measured HBM bytes per dispatch; with the store's screened rows left out, the kernel reads less than the
+        # whole store (fetch_ratio < 1), and roofline.traffic uses this mean over the profiled workload directly
+        k["mean_traffic_bytes_per_dispatch"] = (k["fetch_bytes"] + k["write_bytes"]) / max(1, k["dispatches"])
     print(json.dumps(out), flush=True)
 
 
@@ -552,7 +577,12 @@ def with_traffic(roof, fname):
     if roof and roof.get("unit") == "GB/s" and os.path.exists(path):
         with open(path) as fh:
             rec = json.load(fh).get("kernels", {}).get(roof["kernel"])
-        if rec and rec.get("traffic_ratio"):
+        if rec and rec.get("mean_traffic_bytes_per_dispatch"):
+            roof["traffic"] = rec["mean_traffic_bytes_per_dispatch"]
+            roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same workload, mean HBM bytes "
+                                      "per dispatch (%.3f x this run's streamed bytes per launch), profiles/%s"
+                                      % (rec["mean_traffic_bytes_per_dispatch"] / roof["bytes_per_launch"], fname))
+        elif rec and rec.get("traffic_ratio"):
             roof["traffic"] = rec["traffic_ratio"] * roof["bytes_per_launch"]
             roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the kernel's algorithmic bytes "
                                       "(ratio %.3f), profiles/%s" % (rec["traffic_ratio"], fname))
@@ -579,7 +609,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=100, help="serving: requests per step")
     ap.add_argument("--slots", type=int, default=32, help="serving: concurrent requests (server slots)")
-    ap.add_argument("--lockstep", type=int, default=1, help="serving: 1 lockstep batch server (default), 0 slot threads")
+    ap.add_argument("--lockstep", type=int, default=2, help="serving: batch threads of the lockstep server (default 2); "
+                    "0: one worker thread per slot")
     ap.add_argument("--no-slot-leg", action="store_true", help="serving: skip the FP64 slot-thread comparison leg")
     ap.add_argument("--workload", default="serving", help="serving (default) | decane | water | water32 | eicosyl | "
                     "incore-shapes (in-core contraction per tile shape and size)")

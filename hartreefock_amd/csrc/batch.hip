@@ -260,6 +260,13 @@ __global__ void __launch_bounds__(256) bsp2_step_kernel(BSp2 g, double* __restri
         if (fin >= 0 && fin < g.s) return;
     }
     const int n = g.n[j], nt = (n + 15) / 16, s = g.s, t = (int)blockIdx.x - g.tile_off[j];
+    // block t takes tile (tx, ty), tx <= ty, of the upper triangle (column by column); the product's element
+    // (i, j) = sum_k X(i, k) X(j, k) is the same sum of the same products as (j, i), so a block writes its tile and
+    // the mirrored one
+    int ty = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((ty + 1) * (ty + 2) / 2 <= t) ++ty;
+    while (ty * (ty + 1) / 2 > t) --ty;
+    const int tx = t - ty * (ty + 1) / 2;
     double* __restrict__ hist = hist_all + (size_t)g.idx[j] * kHistStride;
     double* __restrict__ gpart = part_all + (size_t)g.idx[j] * kPartStride;
     const double nocc = g.nocc[j];
@@ -269,8 +276,8 @@ __global__ void __launch_bounds__(256) bsp2_step_kernel(BSp2 g, double* __restri
     double* __restrict__ Xo = g.X[j][s & 1];
     double* __restrict__ X2o = g.W[j][s & 1];
     const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4, w = threadIdx.x >> 6;
-    const int tx = t % nt, ty = t / nt;
     const int i0 = tx * 16, j0 = ty * 16;
+    const bool mirror = tx != ty;
     const bool lead = t == 0 && threadIdx.x == 0;
     int kb, ke;
     hfgpu::k_strip(n, w, kb, ke);
@@ -317,7 +324,7 @@ __global__ void __launch_bounds__(256) bsp2_step_kernel(BSp2 g, double* __restri
             }
         }
         if (stop_p) {
-            // X_s is the result: write this tile of it and stop
+            // X_s is the result: write this tile of it (and the mirrored one) and stop
             if (w == 0)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -325,6 +332,10 @@ __global__ void __launch_bounds__(256) bsp2_step_kernel(BSp2 g, double* __restri
                     if (i < n && jj < n) {
                         const size_t o = i + (size_t)jj * n;
                         Xo[o] = square ? X2p[o] : 2.0 * Xp[o] - X2p[o];
+                        if (mirror) {
+                            const size_t om = jj + (size_t)i * n;
+                            Xo[om] = square ? X2p[om] : 2.0 * Xp[om] - X2p[om];
+                        }
                     }
                 }
             return;
@@ -363,6 +374,11 @@ __global__ void __launch_bounds__(256) bsp2_step_kernel(BSp2 g, double* __restri
             const double x = xval(i, jj);
             Xo[o] = x;
             X2o[o] = acc[r];
+            if (mirror) {
+                const size_t om = jj + (size_t)i * n;
+                Xo[om] = xval(jj, i);
+                X2o[om] = acc[r];
+            }
             if (i == jj) {
                 d1 += x;
                 d2 += acc[r];
@@ -405,8 +421,13 @@ __global__ void __launch_bounds__(1024) bsp2_local_kernel(BSp2 g, int max_steps,
     const int nt = (n + 15) / 16;
     int final_step = -1;
     for (int s = 0; s <= max_steps; ++s) {
-        for (int tile = w; tile < nt * nt; tile += nw) {
-            const int i0 = (tile % nt) * 16, j0 = (tile / nt) * 16;
+        // the tiles on and above the diagonal (tx <= ty, column by column); element (i, j) of the product is the same
+        // sum of the same products as (j, i), so each tile is written twice
+        for (int tile = w; tile < nt * (nt + 1) / 2; tile += nw) {
+            int ty = 0;
+            while ((ty + 1) * (ty + 2) / 2 <= tile) ++ty;
+            const int tx = tile - ty * (ty + 1) / 2;
+            const int i0 = tx * 16, j0 = ty * 16;
             v4d acc = {0.0, 0.0, 0.0, 0.0};
             for (int k0 = 0; k0 < n; k0 += 4) {
                 const int kk = k0 + lk, ia = i0 + lr, jb = j0 + lr;
@@ -417,7 +438,10 @@ __global__ void __launch_bounds__(1024) bsp2_local_kernel(BSp2 g, int max_steps,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = i0 + lk + 4 * r, jj = j0 + lr;
-                if (i < n && jj < n) X2[i + jj * n] = acc[r];
+                if (i < n && jj < n) {
+                    X2[i + jj * n] = acc[r];
+                    X2[jj + i * n] = acc[r];
+                }
             }
         }
         __syncthreads();
@@ -622,7 +646,10 @@ BSp2 sp2_desc(const std::vector<Sp2Job>& jobs, const std::vector<int>& pick, siz
         g.X[j][1] = b.X1;
         g.W[j][0] = b.W0;
         g.W[j][1] = b.W1;
-        if (tiles) t += ((b.n + 15) / 16) * ((b.n + 15) / 16);
+        if (tiles) {
+            const int nt = (b.n + 15) / 16;
+            t += nt * (nt + 1) / 2;     // the tiles on and above the diagonal
+        }
     }
     g.tile_off[g.count] = t;
     return g;

@@ -467,6 +467,11 @@ int setup_molecule(hf_ctx* ctx) {
         HF_HIP_CHECK(ctx, ctx->d_qf.alloc(std::max<size_t>(qf.size(), 1)));
         if (int rc = hf_upload(ctx, ctx->d_qf.p, qf.data(), qf.size() * sizeof(float))) return rc;
         ctx->qf_max = qmax;
+        // rows the contraction reads, by its own test (incore.hip, store_skip)
+        const double thr = ctx->opt.schwarz_threshold * (1.0 - 1e-6);
+        int64_t live = 0;
+        for (float f : qf) live += (double)f * qmax >= thr ? 1 : 0;
+        ctx->qf_live = live;
         ctx->qf_valid = true;
     }
     // task sets of the previous molecule: marked stale (rebuilt on first use), device buffers kept

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -75,7 +76,16 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // Grows to at least count elements.  A buffer that already holds memory grows by half again at least: every
+    // hipFree synchronises the whole device (stalling the other contexts of a server), so the growth steps of a
+    // context that meets larger and larger molecules are kept few.
     hipError_t alloc(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        const size_t want = p ? std::max(count, n + n / 2) : count;
+        return alloc_exact(want);
+    }
+    // Grows to exactly count elements (the in-core store, whose size the admission of a server charges).
+    hipError_t alloc_exact(size_t count) {
         if (count <= n && p) return hipSuccess;
         release();
         if (count == 0) return hipSuccess;
@@ -155,6 +165,7 @@ struct hf_ctx {
     // largest; a store element (P, Q) whose product of the two is below the threshold was screened out (zero)
     DevBuf<float> d_qf;
     double qf_max = 0;
+    int64_t qf_live = 0;                   // store rows the contraction reads (qf_P qf_max at or above the threshold)
     bool qf_valid = false;
     hipStream_t eri_stream = nullptr;      // the SCF evaluates the in-core store here, beside its own stream
     double* hpin = nullptr;                // kPinnedDoubles of pinned host memory for small read-backs

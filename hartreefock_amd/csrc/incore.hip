@@ -606,8 +606,10 @@ int incore_contract(hf_ctx* ctx, int ndens, const double* G, const float* g32, c
     }
     KernelStat& ks = ctx->incore_stat[f];
     ks.launches += 1;
-    ks.eris += (double)np * (double)np;                                  // elements streamed
-    ks.flops += (double)np * (double)np * (2.0 + 8.0 * ndens);           // J, and row + column parts of K
+    // elements streamed: the rows the screening left (all rows without skipping), whole
+    const double rows = sk.qf ? (double)ctx->qf_live : (double)np;
+    ks.eris += rows * (double)np;
+    ks.flops += rows * (double)np * (2.0 + 8.0 * ndens);                 // J, and row + column parts of K
     // deterministic mode: K accumulates in fixed point (fx_add) and is folded in after the contraction
     long long* fxK = nullptr;
     if (ctx->opt.deterministic) {

@@ -412,28 +412,31 @@ int ScfRun::init() {
     st.n = n;
     st.nn = (size_t)n * n;
     nn = st.nn;
+    // the workspace matrices are allocated for at least 256 functions, so a context that solves one molecule after
+    // another (a server's slot) does not regrow them for every larger molecule
+    const size_t nc = std::max<size_t>(nn, (size_t)kWorkspaceMinN * kWorkspaceMinN);
     // ---- Init (HartreeFock/HartreeFockAlgorithm.cpp:21-101) ----
     Enuc = hfhost::nuclear_repulsion(ctx->mol);
     nsp = uhf ? 2 : 1;
-    HF_HIP_CHECK(ctx, st.S.alloc(nn));
-    HF_HIP_CHECK(ctx, st.h.alloc(nn));
-    HF_HIP_CHECK(ctx, st.X.alloc(nn));
-    HF_HIP_CHECK(ctx, st.F0.alloc(2 * nn));
-    HF_HIP_CHECK(ctx, st.Fa.alloc(2 * nn));
-    HF_HIP_CHECK(ctx, st.J.alloc(nn));
-    HF_HIP_CHECK(ctx, st.K.alloc(2 * nn));
-    HF_HIP_CHECK(ctx, st.D.alloc(2 * nn));
-    HF_HIP_CHECK(ctx, st.Dn.alloc(2 * nn));
-    HF_HIP_CHECK(ctx, st.Dprev.alloc(2 * nn));
-    HF_HIP_CHECK(ctx, st.dD.alloc(2 * nn));
-    HF_HIP_CHECK(ctx, st.C.alloc(2 * nn));
+    HF_HIP_CHECK(ctx, st.S.alloc(nc));
+    HF_HIP_CHECK(ctx, st.h.alloc(nc));
+    HF_HIP_CHECK(ctx, st.X.alloc(nc));
+    HF_HIP_CHECK(ctx, st.F0.alloc(2 * nc));
+    HF_HIP_CHECK(ctx, st.Fa.alloc(2 * nc));
+    HF_HIP_CHECK(ctx, st.J.alloc(nc));
+    HF_HIP_CHECK(ctx, st.K.alloc(2 * nc));
+    HF_HIP_CHECK(ctx, st.D.alloc(2 * nc));
+    HF_HIP_CHECK(ctx, st.Dn.alloc(2 * nc));
+    HF_HIP_CHECK(ctx, st.Dprev.alloc(2 * nc));
+    HF_HIP_CHECK(ctx, st.dD.alloc(2 * nc));
+    HF_HIP_CHECK(ctx, st.C.alloc(2 * nc));
     HF_HIP_CHECK(ctx, st.eps.alloc(2 * n));
-    HF_HIP_CHECK(ctx, st.T1.alloc(nn));
-    HF_HIP_CHECK(ctx, st.T2.alloc(nn));
-    HF_HIP_CHECK(ctx, st.T3.alloc(nn));
+    HF_HIP_CHECK(ctx, st.T1.alloc(nc));
+    HF_HIP_CHECK(ctx, st.T2.alloc(nc));
+    HF_HIP_CHECK(ctx, st.T3.alloc(nc));
     HF_HIP_CHECK(ctx, st.sp2.alloc(8 + 2 * (kSp2MaxSteps + 1)));
-    if (n <= kSmallDense) HF_HIP_CHECK(ctx, st.sp2W.alloc(2 * nn));
-    HF_HIP_CHECK(ctx, st.err.alloc(nn));
+    if (n <= kSmallDense) HF_HIP_CHECK(ctx, st.sp2W.alloc(2 * nc));
+    HF_HIP_CHECK(ctx, st.err.alloc(nc));
     HF_HIP_CHECK(ctx, st.work.alloc(n));
     HF_HIP_CHECK(ctx, st.red.alloc(16));
     HF_HIP_CHECK(ctx, st.redpart.alloc(8 * kDotBlocks));
@@ -455,7 +458,7 @@ int ScfRun::init() {
         const double held = (double)ctx->d_G.n * 8.0 + (double)ctx->d_G32.n * 4.0;   // stores of an earlier SCF
         const double cap = p->incore_max_bytes > 0 ? p->incore_max_bytes : 0.5 * ((double)free_b + held);
         if (want <= cap) {
-            incore = ctx->d_G.alloc(len) == hipSuccess && (!mixed_tier || ctx->d_G32.alloc(len) == hipSuccess);
+            incore = ctx->d_G.alloc_exact(len) == hipSuccess && (!mixed_tier || ctx->d_G32.alloc_exact(len) == hipSuccess);
             (void)hipGetLastError();
         }
         if (incore) {
@@ -517,8 +520,8 @@ int ScfRun::init() {
                        uhf && p->add_asymmetry && n > 1 ? p->asymmetry : 0.0, st.F0.p, st.F0.p + nn);
     HF_HIP_CHECK(ctx, hipMemsetAsync(st.D.p, 0, 2 * nn * sizeof(double), s));
     for (int k = 0; k < nsp; ++k) {
-        HF_HIP_CHECK(ctx, ws.diisF[k].alloc(Diis::kKeep * nn));
-        HF_HIP_CHECK(ctx, ws.diisE[k].alloc(Diis::kKeep * nn));
+        HF_HIP_CHECK(ctx, ws.diisF[k].alloc(Diis::kKeep * nc));
+        HF_HIP_CHECK(ctx, ws.diisE[k].alloc(Diis::kKeep * nc));
         diis[k].F = ws.diisF[k].p;
         diis[k].E = ws.diisE[k].p;
         diis[k].nn = nn;
@@ -773,7 +776,7 @@ int ScfRun::finish(hf_scf_result* r, double* D_out) {
         if (!incore) {
             if (n > kIncoreMaxN) return ctx->fail(HF_E_UNSUPPORTED, "MP2 needs the in-core ERI store (n <= 512)");
             const uint64_t len = hf_pair_matrix_len(n);
-            HF_HIP_CHECK(ctx, ctx->d_G.alloc(len));
+            HF_HIP_CHECK(ctx, ctx->d_G.alloc_exact(len));
             if (int rc = hf_eri_pairs_impl(ctx, ctx->d_G.p, len, s)) return rc;
         }
         double e2 = 0;

@@ -11,6 +11,7 @@
 
 constexpr int kSp2MaxSteps = 100;     // purification steps before a density step falls back to diagonalisation
 constexpr int kSp2LocalMax = 96;      // n up to which all SP2 steps run in one workgroup (matrices in LDS)
+constexpr int kWorkspaceMinN = 256;   // SCF workspace matrices are allocated for at least this many functions
 
 // DIIS history of one spin (HartreeFock/DIIS.h:9-96): device slots for F and e, host inner products.
 struct Diis {
