@@ -439,6 +439,45 @@ def single_molecule(name, steps, warmup, cpu=True, cpu_budget=20.0, max_iteratio
     return out
 
 
+def serving_batch(a):
+    """The serving mix solved by the lockstep batch entry point (hf_scf_batch) from one host thread: --steps batches of
+    --batch molecules (after --warmup untimed batches), each batch one hf_scf_batch call on contexts kept between
+    batches, in-core stores, mixed or FP64 (--precision).  One host thread issues everything, so a kernel trace sees
+    exactly the lockstep phases, the in-core contraction and the store evaluation."""
+    import hartreefock_amd as hf
+    from hartreefock_amd import geometry as g
+    bfile = hf.basis_path("6-31g.1.nw")
+    K, W, B = a.steps, a.warmup, a.batch
+    mols = g.serving_mix((W + K) * B, seed=2024)
+    engs = [hf.Engine(precision=a.precision, fp32_threshold=1e-6, exact_transfer=1) for _ in range(B)]
+    kw = dict(SCF_KW, incore=1, max_iterations=300)
+    out_res = []
+
+    def run(chunk):
+        for e, (_, atoms) in zip(engs, chunk):
+            e.set_molecule(bfile, atoms)
+        return hf.scf_batch(engs[:len(chunk)], **kw)
+
+    for w in range(W):
+        run(mols[w * B:(w + 1) * B])
+    hip_sync()
+    t0 = time.perf_counter()
+    for k in range(K):
+        out_res += run(mols[(W + k) * B:(W + k + 1) * B])
+    hip_sync()
+    wall = time.perf_counter() - t0
+    conv = sum(1 for st, r in out_res if st == 0 and r.converged)
+    its = [r.iterations for st, r in out_res if st == 0 and r.converged]
+    store_ms = [r.eri_ms for st, r in out_res if st == 0 and r.incore]
+    print(json.dumps({"metric": "SCF molecules/s, serving mix through hf_scf_batch (one host thread)",
+                      "value": conv / wall, "unit": "molecules/s", "steps": K, "warmup": W, "batch": B,
+                      "precision": "mixed" if a.precision else "fp64", "wall_s": wall, "converged": conv,
+                      "mean_iterations": sum(its) / max(1, len(its)),
+                      "mean_store_ms": sum(store_ms) / max(1, len(store_ms))}), flush=True)
+    for e in engs:
+        e.close()
+
+
 def incore_shape_sweep():
     """In-core contraction: every tile shape (hf_options.incore_shape) at sizes across the serving range and C10H22,
     one and two densities; HIP-event time per launch and the store bytes over it (GB/s), printed as one JSON line."""
@@ -613,7 +652,9 @@ def main():
                     "0: one worker thread per slot")
     ap.add_argument("--no-slot-leg", action="store_true", help="serving: skip the FP64 slot-thread comparison leg")
     ap.add_argument("--workload", default="serving", help="serving (default) | decane | water | water32 | eicosyl | "
-                    "incore-shapes (in-core contraction per tile shape and size)")
+                    "incore-shapes (in-core contraction per tile shape and size) | serving-batch (the serving mix "
+                    "through hf_scf_batch from one host thread)")
+    ap.add_argument("--precision", type=int, default=1, help="serving-batch: 1 mixed tier (default), 0 FP64")
     ap.add_argument("--plain-requests", type=int, default=300,
                     help="serving: requests of the one-at-a-time plain FP64 direct run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU work for the baseline sample")
@@ -645,6 +686,9 @@ def main():
         sys.exit(2)
     if a.workload == "incore-shapes":
         incore_shape_sweep()
+        return
+    if a.workload == "serving-batch":
+        serving_batch(a)
         return
     if a.workload == "serving":
         global SERVE_SLOTS, SERVE_LOCKSTEP

@@ -482,6 +482,21 @@ int hf_serve_create(const hf_serve_options* o, hf_server** out) {
         s->nactive.assign(nb, 0);
         for (int w = 0; w < o->slots && ok; ++w)
             ok = hipEventCreateWithFlags(&s->slot_ev[w], hipEventDisableTiming) == hipSuccess;
+        // Each slot's in-core store is allocated now at the size a slot keeps between requests (budget / slots, at
+        // most 16 GiB; FP64 store and, in the mixed tier, its FP32 copy): growing stores while requests run means
+        // hipFree and hipMalloc of gigabytes, which synchronise the device and stall every batch.  The bytes are
+        // charged to the slots, as retained stores are; a request whose store is larger than this still gets one
+        // (the slot's store is reallocated), and idle slots release theirs when the queue head needs the memory.
+        if (ok && o->scf.incore) {
+            const double per = o->ctx.precision == HF_PREC_MIXED ? 12.0 : 8.0;
+            const size_t len = (size_t)(s->keep_store / per);
+            for (int w = 0; w < o->slots && ok && len > 0; ++w) {
+                hf_ctx* c = s->ctxs[w];
+                ok = c->d_G.alloc_exact(len) == hipSuccess &&
+                     (o->ctx.precision != HF_PREC_MIXED || c->d_G32.alloc_exact(len) == hipSuccess);
+                s->charged[w] = hf_ctx_held_bytes(c);
+            }
+        }
         if (!ok) {
             for (hipEvent_t e : s->slot_ev)
                 if (e) (void)hipEventDestroy(e);
