@@ -610,8 +610,8 @@ def trace_summary(root):
 
 
 def with_traffic(roof, fname):
-    """roofline.traffic = the measured traffic-to-algorithmic-bytes ratio of the kernel (profiles/fname) times this
-    run's algorithmic bytes per launch."""
+    """roofline.traffic = the kernel's mean HBM bytes per dispatch measured on the same workload (profiles/fname);
+    for a summary without that figure, its traffic-to-algorithmic-bytes ratio times this run's bytes per launch."""
     path = os.path.join(ROOT, "profiles", fname)
     if roof and roof.get("unit") == "GB/s" and os.path.exists(path):
         with open(path) as fh:

@@ -119,32 +119,41 @@ def test_config_scf_invariants(gpu_engine_factory, name, bfile, make, nbf, uhf):
         scale = 1.0 if uhf else 2.0
         assert np.abs(Dk @ S @ Dk - scale * Dk).max() <= 1e-6
     # Fock matrices from the GPU's direct J/K of the converged density: the DIIS commutator vanishes and the
-    # energy functional reproduces the SCF energy
-    J, K = eng.build_jk(np.array(Ds) if uhf else Ds[0])
-    Ks = K if uhf else K
-    Fs = _fock(J, Ks, h, uhf)
-    for Fk, Dk in zip(Fs, Ds):
-        c = Fk @ Dk @ S - S @ Dk @ Fk
-        assert np.abs(c).max() <= 1e-5, np.abs(c).max()
+    # energy functional reproduces the SCF energy.  With the in-core store, the same from its J/K, and the two
+    # builds against each other.  Every figure is printed before any is asserted, so a failure shows which build
+    # disagrees.
     Enuc = eng.nuclear_repulsion()
-    E = sum(0.5 * np.sum(Dk * (h + Fk)) for Dk, Fk in zip(Ds, Fs)) + Enuc
-    print("%s: E[D] %.10f reported %.10f" % (name, E, r.energy), flush=True)
+    Dx = np.array(Ds) if uhf else Ds[0]
+
+    def fock_checks(J, K):
+        Fs = _fock(J, K, h, uhf)
+        comm = max(np.abs(Fk @ Dk @ S - S @ Dk @ Fk).max() for Fk, Dk in zip(Fs, Ds))
+        return comm, sum(0.5 * np.sum(Dk * (h + Fk)) for Dk, Fk in zip(Ds, Fs)) + Enuc
+
+    J, K = eng.build_jk(Dx)
+    J, K = np.asarray(J), np.asarray(K)
+    comm, E = fock_checks(J, K)
+    print("%s: direct J/K: max|F D S - S D F| %.2e, E[D] %.10f, reported %.10f" % (name, comm, E, r.energy),
+          flush=True)
+    if incore:
+        # The two sum the same ERIs (same screened quartets) in different orders, so they differ by rounding, which
+        # scales with the terms' magnitudes rather than with their sums: the diffuse 6-31+G* density has large
+        # elements of both signs.  The scale is the J/K of |D| (the same contraction without cancellation).
+        Ji, Ki = eng.build_jk_incore(Dx)
+        Ji, Ki = np.asarray(Ji), np.asarray(Ki)
+        comm_i, E_i = fock_checks(Ji, Ki)
+        Ja, Ka = eng.build_jk(np.abs(Dx))
+        dj = np.abs(Ji - J).max() / np.abs(Ja).max()
+        dk = np.abs(Ki - K).max() / np.abs(np.asarray(Ka)).max()
+        rel = max(np.abs(Ji - J).max() / np.abs(J).max(), np.abs(Ki - K).max() / np.abs(K).max())
+        print("%s: in-core J/K: max|F D S - S D F| %.2e, E[D] %.10f; direct vs in-core: max|dJ| / max J[|D|] %.2e, "
+              "max|dK| / max K[|D|] %.2e (relative to J, K themselves %.2e; max|D| %.3f)"
+              % (name, comm_i, E_i, dj, dk, rel, max(np.abs(x).max() for x in Ds)), flush=True)
+    assert comm <= 1e-5
     assert abs(E - r.energy) <= 1e-6
     if uhf:
         assert 0.75 <= r.s2 < 0.80
     if incore:
-        # the in-core store's J/K against the direct build on the same density
-        # The two sum the same ERIs (same screened quartets) in different orders, so they differ by rounding, which
-        # scales with the terms' magnitudes rather than with their sums: the diffuse 6-31+G* density has large
-        # elements of both signs.  The scale is the J/K of |D| (the same contraction without cancellation).
-        Ji, Ki = eng.build_jk_incore(np.array(Ds) if uhf else Ds[0])
-        Ki, K = np.asarray(Ki), np.asarray(K)
-        Ja, Ka = eng.build_jk(np.abs(np.array(Ds)) if uhf else np.abs(Ds[0]))
-        dj = np.abs(Ji - J).max() / np.abs(Ja).max()
-        dk = np.abs(Ki - K).max() / np.abs(np.asarray(Ka)).max()
-        rel = max(np.abs(Ji - J).max() / np.abs(J).max(), np.abs(Ki - K).max() / np.abs(K).max())
-        print("%s: direct vs in-core J/K: max|dJ| / max J[|D|] %.2e, max|dK| / max K[|D|] %.2e (relative to J, K "
-              "themselves %.2e; max|D| %.3f)" % (name, dj, dk, rel, max(np.abs(x).max() for x in Ds)), flush=True)
         assert dj <= 1e-12 and dk <= 1e-12 and rel <= 1e-9
     if name.startswith("cfg2"):
         # the direct SCF reaches the in-core one's energy
