@@ -469,11 +469,21 @@ def serving_batch(a):
     conv = sum(1 for st, r in out_res if st == 0 and r.converged)
     its = [r.iterations for st, r in out_res if st == 0 and r.converged]
     store_ms = [r.eri_ms for st, r in out_res if st == 0 and r.incore]
+    # the in-core contraction's launches and algorithmic bytes (the store rows it streams) over every batch,
+    # warm-up included, as a counter run of this command sees them (--pmc-summary takes this line)
+    incore = {}
+    for e in engs:
+        for x in e.kernel_stats():
+            if x.la < 0:
+                k = incore.setdefault("incore_jk_kernel<%s>" % ("float" if x.fp32 else "double"),
+                                      {"launches": 0, "bytes": 0.0})
+                k["launches"] += x.launches
+                k["bytes"] += x.eris * (4.0 if x.fp32 else 8.0)
     print(json.dumps({"metric": "SCF molecules/s, serving mix through hf_scf_batch (one host thread)",
                       "value": conv / wall, "unit": "molecules/s", "steps": K, "warmup": W, "batch": B,
                       "precision": "mixed" if a.precision else "fp64", "wall_s": wall, "converged": conv,
                       "mean_iterations": sum(its) / max(1, len(its)),
-                      "mean_store_ms": sum(store_ms) / max(1, len(store_ms))}), flush=True)
+                      "mean_store_ms": sum(store_ms) / max(1, len(store_ms)), "incore_stats": incore}), flush=True)
     for e in engs:
         e.close()
 
@@ -539,43 +549,61 @@ def _counter_rows(root, counter):
     return rows
 
 
-def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, workload_json=None):
     """HBM traffic of the in-core contraction from rocprofv3 --pmc runs (FETCH_SIZE and WRITE_SIZE in KiB per
-    dispatch), scaled by what the same counters read on the plain 1 GiB device copy of --copy-calibration, over the
-    kernel's algorithmic bytes per dispatch: the store streamed once, np^2 elements of 8 (FP64) or 4 (FP32) bytes,
-    with n recovered from the dispatch's grid (blocks of NR store rows, NR the sixth template argument)."""
+    dispatch), scaled by what the same counters read on the plain 1 GiB device copy of --copy-calibration.  The
+    kernel's algorithmic bytes (the store rows it streams, 8 or 4 bytes per element) come from the profiled command's
+    own JSON line when it carries them (workload_json: the serving-batch workload's "incore_stats", launches and
+    bytes over the whole command, batched launches covering several stores); otherwise from each single-store
+    dispatch's grid (n from the blocks of NR store rows, NR the sixth template argument), as the whole store np^2."""
     import re
     gib_kib = float(1 << 20)
     scale = {}
     for counter, d in (("FETCH_SIZE", copy_fetch_dir), ("WRITE_SIZE", copy_write_dir)):
         v = [float(r["Counter_Value"]) for r in _counter_rows(d, counter) if "copyBuffer" in r["Kernel_Name"]]
         scale[counter] = gib_kib / (sum(v) / len(v))
+    alg_total = {}
+    if workload_json:
+        with open(workload_json) as fh:
+            line = [ln for ln in fh if ln.lstrip().startswith("{")][-1]
+        alg_total = json.loads(line).get("incore_stats", {})
     out = {"copy_scale": scale, "kernels": {}}
     for counter, d, key in (("FETCH_SIZE", fetch_dir, "fetch"), ("WRITE_SIZE", write_dir, "write")):
         for r in _counter_rows(d, counter):
-            m = re.search(r"incore_jk_kernel<(\d+), (\d+), (\d+), (\d+), (true|false), (\d+), (double|float)>",
+            m = re.search(r"incore_jk_(batch_)?kernel<(\d+), (\d+), (\d+), (\d+), (true|false), (\d+), (double|float)>",
                           r["Kernel_Name"])
             if not m:
                 continue
-            nr, t = int(m.group(6)), m.group(7)
-            blocks = int(r["Grid_Size"]) // 256
-            n = next(x for x in range(1, 513) if _blocks_before(x, nr) >= blocks)
-            alg = _pair_pad_base(n) ** 2 * (8 if t == "double" else 4)
-            k = out["kernels"].setdefault("incore_jk_kernel<%s>" % t, {"fetch_bytes": 0.0, "fetch_alg": 0.0,
-                                                                        "write_bytes": 0.0, "write_alg": 0.0,
-                                                                        "dispatches": 0})
+            nr, t = int(m.group(7)), m.group(8)
+            name = "incore_jk_kernel<%s>" % t
+            k = out["kernels"].setdefault(name, {"fetch_bytes": 0.0, "fetch_alg": 0.0, "write_bytes": 0.0,
+                                                 "write_alg": 0.0, "dispatches": 0, "batched_dispatches": 0})
+            if not alg_total:
+                if m.group(1):
+                    raise SystemExit("batched in-core dispatches: give the profiled command's JSON line")
+                blocks = int(r["Grid_Size"]) // 256
+                n = next(x for x in range(1, 513) if _blocks_before(x, nr) >= blocks)
+                k[key + "_alg"] += _pair_pad_base(n) ** 2 * (8 if t == "double" else 4)
             k[key + "_bytes"] += float(r["Counter_Value"]) * 1024.0 * scale[counter]
-            k[key + "_alg"] += alg
             if key == "fetch":
                 k["dispatches"] += 1
-    for k in out["kernels"].values():
+                k["batched_dispatches"] += 1 if m.group(1) else 0
+    for name, k in out["kernels"].items():
+        if alg_total:
+            a = alg_total.get(name, {})
+            if not a.get("launches") or a["launches"] > k["dispatches"]:
+                raise SystemExit("%s: %s dispatches counted, %s launches in the command's statistics"
+                                 % (name, k["dispatches"], a.get("launches")))
+            # the autotuner's timing runs (the first build of each kind of store in the process) are dispatches the
+            # statistics do not count: they are taken at the command's mean bytes per launch
+            k["tuning_dispatches"] = k["dispatches"] - a["launches"]
+            k["fetch_alg"] = k["write_alg"] = a["bytes"] * k["dispatches"] / a["launches"]
         k["fetch_ratio"] = k["fetch_bytes"] / k["fetch_alg"] if k["fetch_alg"] else None
         k["write_ratio"] = k["write_bytes"] / k["write_alg"] if k["write_alg"] else None
         k["traffic_ratio"] = (k["fetch_ratio"] or 0.0) + (k["write_ratio"] or 0.0)
         k["mean_alg_bytes_per_dispatch"] = k["fetch_alg"] / max(1, k["dispatches"])
-        # measured HBM bytes per dispatch; with the store's screened rows left out, the kernel reads less than the
-        # whole store (fetch_ratio < 1), and roofline.traffic uses this mean over the profiled workload directly
         k["mean_traffic_bytes_per_dispatch"] = (k["fetch_bytes"] + k["write_bytes"]) / max(1, k["dispatches"])
+        k["alg_bytes"] = "the store rows streamed (command statistics)" if alg_total else "the whole store (grid)"
     print(json.dumps(out), flush=True)
 
 
@@ -610,21 +638,22 @@ def trace_summary(root):
 
 
 def with_traffic(roof, fname):
-    """roofline.traffic = the kernel's mean HBM bytes per dispatch measured on the same workload (profiles/fname);
-    for a summary without that figure, its traffic-to-algorithmic-bytes ratio times this run's bytes per launch."""
+    """roofline.traffic = the kernel's measured HBM bytes over its algorithmic bytes (profiles/fname: counter runs of
+    a workload of the same kind) times this run's algorithmic bytes per launch; for a summary whose algorithmic bytes
+    are whole stores (the rows the screening zeroed not left out), its mean HBM bytes per dispatch instead."""
     path = os.path.join(ROOT, "profiles", fname)
     if roof and roof.get("unit") == "GB/s" and os.path.exists(path):
         with open(path) as fh:
             rec = json.load(fh).get("kernels", {}).get(roof["kernel"])
-        if rec and rec.get("mean_traffic_bytes_per_dispatch"):
+        if rec and rec.get("alg_bytes", "").startswith("the store rows") and rec.get("traffic_ratio"):
+            roof["traffic"] = rec["traffic_ratio"] * roof["bytes_per_launch"]
+            roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same kind of workload: %.3f x "
+                                      "the streamed store bytes, profiles/%s" % (rec["traffic_ratio"], fname))
+        elif rec and rec.get("mean_traffic_bytes_per_dispatch"):
             roof["traffic"] = rec["mean_traffic_bytes_per_dispatch"]
             roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same workload, mean HBM bytes "
                                       "per dispatch (%.3f x this run's streamed bytes per launch), profiles/%s"
                                       % (rec["mean_traffic_bytes_per_dispatch"] / roof["bytes_per_launch"], fname))
-        elif rec and rec.get("traffic_ratio"):
-            roof["traffic"] = rec["traffic_ratio"] * roof["bytes_per_launch"]
-            roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the kernel's algorithmic bytes "
-                                      "(ratio %.3f), profiles/%s" % (rec["traffic_ratio"], fname))
     return roof
 
 
@@ -663,9 +692,10 @@ def main():
     ap.add_argument("--legs", default="", help="single-molecule workloads: fp64_direct,fp64_incore,mixed_direct,"
                     "mixed_incore (default all)")
     ap.add_argument("--max-iterations", type=int, default=0)
-    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
-                    help="only summarise rocprofv3 --pmc output directories: in-core contraction HBM traffic against "
-                         "its algorithmic bytes, scaled by the copy calibration")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
+                    help="only summarise rocprofv3 --pmc output: FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR "
+                         "[WORKLOAD_JSON]: in-core contraction HBM traffic against its algorithmic bytes (from the "
+                         "profiled command's JSON line when given), scaled by the copy calibration")
     ap.add_argument("--trace-summary", metavar="TRACE_DIR",
                     help="only summarise a rocprofv3 --kernel-trace directory: per-kernel resources and durations")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -675,6 +705,8 @@ def main():
         trace_summary(a.trace_summary)
         return
     if a.pmc_summary:
+        if len(a.pmc_summary) not in (4, 5):
+            raise SystemExit("--pmc-summary takes 4 directories and optionally the workload's JSON output")
         pmc_summary(*a.pmc_summary)
         return
     if a.copy_calibration:

@@ -246,8 +246,8 @@ struct IncoreJob {
     bool own_store;
 };
 // The in-core builds of several molecules on stream st: one launch packs every weighted density and clears the
-// outputs, then each molecule's contraction (its tile shape and register layout depend on its n).  Same results
-// as hf_incore_jk_impl per molecule.
+// outputs, then one contraction launch per kind of build (store precision, densities, tile shape, column chunks)
+// covers all the molecules of that kind.  The same sums per molecule as hf_incore_jk_impl (FP64 atomics aside).
 int hf_incore_jk_batch(const std::vector<IncoreJob>& jobs, hipStream_t st);
 // Takes the device address of this device's allocation-free scratch array (once per device; hf_create).
 int hf_incore_scratch_init(hf_ctx* ctx);

@@ -32,6 +32,8 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
+#include <vector>
 
 #include "hf_internal.h"
 #include "wave_reduce.h"
@@ -138,14 +140,14 @@ struct StoreSkip {
 // rows, row k having k + 1 elements).  The J weights Dw of a chunk are loaded once for the NR rows, and
 // the rows share K_i: its row sums (sum_l B_s[k][l] D_{j_s}[l] over s) and its column part are one
 // accumulator each, so a row k of the triangle needs NDENS (NR + 1) row sums instead of 2 NR NDENS.
+// blk: the block's index among the blocks of this store (blocks_before(n, NR) of them).
 template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
-__global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t npad,
-                                                        const double* __restrict__ D, const double* __restrict__ Dw,
-                                                        double* __restrict__ J, double* __restrict__ K,
-                                                        long long* __restrict__ fxK, StoreSkip sk) {
+__device__ __forceinline__ void incore_jk_block(const T* __restrict__ G, int n, int64_t npad,
+                                                const double* __restrict__ D, const double* __restrict__ Dw,
+                                                double* __restrict__ J, double* __restrict__ K,
+                                                long long* __restrict__ fxK, const StoreSkip& sk, int64_t blk) {
     extern __shared__ double lds[];
     constexpr int W = 64 * CPL;
-    const int64_t blk = blockIdx.x;
     int i = (int)sqrt(2.0 * NR * (double)blk) - 1;
     if (i < 0) i = 0;
     while (blocks_before(i + 1, NR) <= blk) ++i;
@@ -221,8 +223,11 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
             const int64_t b = hfgpu::pair_pad_base(kk);
             const int len = k < n ? (int)(hfgpu::pair_pad_base(kk + 1) - b) : 0;
             const auto rd = __builtin_amdgcn_make_buffer_rsrc((void*)(dwb + b * 8), (short)0, len * 8, kBufferFlags);
+            // chunk m holds columns W m ..; row k has columns 0..k, so the chunks past it are zero and their loads
+            // are not issued (k is the same for the whole wavefront)
 #pragma unroll
-            for (int m = 0; m < MC; ++m) dd[r][m] = buffer_load<DV>(rd, (unsigned)(CPL * lane + W * m) * 8u);
+            for (int m = 0; m < MC; ++m)
+                dd[r][m] = m == 0 || W * m <= k ? buffer_load<DV>(rd, (unsigned)(CPL * lane + W * m) * 8u) : DV{};
 #pragma unroll
             for (int s = 0; s < NR; ++s) {
                 // an empty range for a row the store's screening zeroed: its loads return zero without a fetch
@@ -231,7 +236,9 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
                     kBufferFlags);
 #pragma unroll
                 for (int m = 0; m < MC; ++m)
-                    g[r][s][m] = buffer_load<PV>(rg, (unsigned)(CPL * lane + W * m) * (unsigned)sizeof(T));
+                    g[r][s][m] = m == 0 || W * m <= k
+                                     ? buffer_load<PV>(rg, (unsigned)(CPL * lane + W * m) * (unsigned)sizeof(T))
+                                     : PV{};
             }
         }
     };
@@ -388,6 +395,42 @@ __global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G,
     }
 }
 
+template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
+__global__ void __launch_bounds__(256) incore_jk_kernel(const T* __restrict__ G, int n, int64_t npad,
+                                                        const double* __restrict__ D, const double* __restrict__ Dw,
+                                                        double* __restrict__ J, double* __restrict__ K,
+                                                        long long* __restrict__ fxK, StoreSkip sk) {
+    incore_jk_block<NDENS, MC, CPL, ROWS, PREFETCH, NR, T>(G, n, npad, D, Dw, J, K, fxK, sk, blockIdx.x);
+}
+
+// The contractions of several stores in one launch (the molecules of a lockstep batch): job j owns blocks
+// [first[j], first[j + 1]) of the grid, each store with its own density, weights, outputs and row skips.  The
+// jobs of one launch share the kernel's shape (columns per lane, chunks, rows per group, store rows per block);
+// the dynamic LDS is sized for the largest n.
+constexpr int kIncoreBatch = 32;
+template <typename T>
+struct IncoreBatchArgs {
+    int count;
+    int64_t first[kIncoreBatch + 1];
+    int n[kIncoreBatch];
+    const T* G[kIncoreBatch];
+    const double* D[kIncoreBatch];
+    const double* Dw[kIncoreBatch];
+    double* J[kIncoreBatch];
+    double* K[kIncoreBatch];
+    StoreSkip sk[kIncoreBatch];
+};
+
+template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
+__global__ void __launch_bounds__(256) incore_jk_batch_kernel(IncoreBatchArgs<T> g) {
+    const int64_t b = blockIdx.x;
+    int j = 0;
+    while (j + 1 < g.count && b >= g.first[j + 1]) ++j;
+    const int n = g.n[j];
+    incore_jk_block<NDENS, MC, CPL, ROWS, PREFETCH, NR, T>(g.G[j], n, hfgpu::pair_pad_base(n), g.D[j], g.Dw[j], g.J[j],
+                                                           g.K[j], nullptr, g.sk[j], b - g.first[j]);
+}
+
 // Deterministic mode: K (+)= the fixed-point sums
 __global__ void incore_fx_finalize_kernel(const long long* __restrict__ fx, double* __restrict__ K, size_t len,
                                           int accumulate) {
@@ -397,34 +440,77 @@ __global__ void incore_fx_finalize_kernel(const long long* __restrict__ fx, doub
     }
 }
 
-// The kernel for MC column chunks, instantiated only where the in-core size limit (n <= kIncoreMaxN) and the
-// register bounds of shape_fits can reach it.
-template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
-hipError_t launch_incore_mc(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
-                            long long* fxK, hipStream_t st, const StoreSkip& sk) {
-    if constexpr (64 * CPL * (MC - 1) < kIncoreMaxN && (CPL == 2 || NDENS == 1 || MC <= 4) &&
-                  (NR == 1 || NDENS == 1 || 64 * CPL * (MC - 1) < 256)) {
-        const size_t lds = ((2 * NR + 2) * (size_t)NDENS * n + (size_t)NR * n + 4 * NR) * sizeof(double);
-        hipLaunchKernelGGL((incore_jk_kernel<NDENS, MC, CPL, ROWS, PREFETCH, NR, T>),
-                           dim3((unsigned)blocks_before(n, NR)), dim3(256), lds, st, G, n, npad, D, Dw, J, K, fxK, sk);
-        return hipGetLastError();
-    } else {
-        return hipErrorInvalidValue;
-    }
+// Launches of the single-store kernel and of the batched one: run<...>() launches the instance for MC column
+// chunks.  Only instances that the in-core size limit (n <= kIncoreMaxN) and the register bounds of shape_fits can
+// reach are compiled.
+template <int NDENS, int MC, int CPL, int NR>
+constexpr bool kIncoreInstance = 64 * CPL * (MC - 1) < kIncoreMaxN && (CPL == 2 || NDENS == 1 || MC <= 4) &&
+                                 (NR == 1 || NDENS == 1 || 64 * CPL * (MC - 1) < 256);
+
+inline size_t incore_lds_bytes(int ndens, int nr, int n) {
+    return ((2 * nr + 2) * (size_t)ndens * n + (size_t)nr * n + 4 * nr) * sizeof(double);
 }
 
-template <int NDENS, int CPL, int ROWS, bool PREFETCH, int NR, typename T>
-hipError_t launch_incore(const T* G, int n, int64_t npad, const double* D, const double* Dw, double* J, double* K,
-                         long long* fxK, hipStream_t st, const StoreSkip& sk) {
+template <typename T>
+struct OneStore {
+    const T* G;
+    int n;
+    int64_t npad;
+    const double* D;
+    const double* Dw;
+    double* J;
+    double* K;
+    long long* fxK;
+    hipStream_t st;
+    StoreSkip sk;
+    template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR>
+    hipError_t run() const {
+        if constexpr (kIncoreInstance<NDENS, MC, CPL, NR>) {
+            hipLaunchKernelGGL((incore_jk_kernel<NDENS, MC, CPL, ROWS, PREFETCH, NR, T>),
+                               dim3((unsigned)blocks_before(n, NR)), dim3(256), incore_lds_bytes(NDENS, NR, n), st, G, n,
+                               npad, D, Dw, J, K, fxK, sk);
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+};
+
+// g->first[] is filled by run() for the shape's NR.
+template <typename T>
+struct ManyStores {
+    IncoreBatchArgs<T>* g;
+    int nmax;
+    hipStream_t st;
+    template <int NDENS, int MC, int CPL, int ROWS, bool PREFETCH, int NR>
+    hipError_t run() const {
+        if constexpr (kIncoreInstance<NDENS, MC, CPL, NR>) {
+            int64_t b = 0;
+            for (int j = 0; j < g->count; ++j) {
+                g->first[j] = b;
+                b += blocks_before(g->n[j], NR);
+            }
+            g->first[g->count] = b;
+            hipLaunchKernelGGL((incore_jk_batch_kernel<NDENS, MC, CPL, ROWS, PREFETCH, NR, T>), dim3((unsigned)b),
+                               dim3(256), incore_lds_bytes(NDENS, NR, nmax), st, *g);
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+};
+
+template <int NDENS, int CPL, int ROWS, bool PREFETCH, int NR, class Go>
+hipError_t launch_incore(int n, const Go& go) {
     switch ((n + 64 * CPL - 1) / (64 * CPL)) {
-        case 1: return launch_incore_mc<NDENS, 1, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
-        case 2: return launch_incore_mc<NDENS, 2, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
-        case 3: return launch_incore_mc<NDENS, 3, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
-        case 4: return launch_incore_mc<NDENS, 4, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
-        case 5: return launch_incore_mc<NDENS, 5, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
-        case 6: return launch_incore_mc<NDENS, 6, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
-        case 7: return launch_incore_mc<NDENS, 7, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
-        case 8: return launch_incore_mc<NDENS, 8, CPL, ROWS, PREFETCH, NR>(G, n, npad, D, Dw, J, K, fxK, st, sk);
+        case 1: return go.template run<NDENS, 1, CPL, ROWS, PREFETCH, NR>();
+        case 2: return go.template run<NDENS, 2, CPL, ROWS, PREFETCH, NR>();
+        case 3: return go.template run<NDENS, 3, CPL, ROWS, PREFETCH, NR>();
+        case 4: return go.template run<NDENS, 4, CPL, ROWS, PREFETCH, NR>();
+        case 5: return go.template run<NDENS, 5, CPL, ROWS, PREFETCH, NR>();
+        case 6: return go.template run<NDENS, 6, CPL, ROWS, PREFETCH, NR>();
+        case 7: return go.template run<NDENS, 7, CPL, ROWS, PREFETCH, NR>();
+        case 8: return go.template run<NDENS, 8, CPL, ROWS, PREFETCH, NR>();
         default: return hipErrorInvalidValue;
     }
 }
@@ -451,19 +537,27 @@ inline bool shape_fits(int shape, int n, int ndens) {
     return true;
 }
 
-template <int NDENS, typename T>
-hipError_t launch_incore_nd(int shape, const T* G, int n, int64_t np, const double* D, const double* Dw, double* J,
-                            double* K, long long* fxK, hipStream_t st, const StoreSkip& sk) {
-    if (!shape_fits(shape, n, NDENS)) shape = 0;
-    switch (shape) {
-        case 1: return launch_incore<NDENS, 2, 2, true, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
-        case 2: return launch_incore<NDENS, 2, 1, true, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
-        case 3: return launch_incore<NDENS, 1, 2, false, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
-        case 4: return launch_incore<NDENS, 1, 2, true, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
-        case 5: return launch_incore<NDENS, 2, 1, false, 3>(G, n, np, D, Dw, J, K, fxK, st, sk);
-        case 6: return launch_incore<NDENS, 2, 1, true, 3>(G, n, np, D, Dw, J, K, fxK, st, sk);
-        case 7: return launch_incore<NDENS, 2, 2, false, 3>(G, n, np, D, Dw, J, K, fxK, st, sk);
-        default: return launch_incore<NDENS, 2, 2, false, 1>(G, n, np, D, Dw, J, K, fxK, st, sk);
+// The shape a launch for n functions and ndens densities uses (the default where the asked one does not fit).
+inline int shape_for(int shape, int n, int ndens) { return shape_fits(shape, n, ndens) ? shape : 0; }
+
+// Column chunks of the kernel instance that shape runs with for n functions.
+inline int shape_chunks(int shape, int n) {
+    const int w = 64 * kShapes[shape].cpl;
+    return (n + w - 1) / w;
+}
+
+// n: the size that selects the column chunks (the largest store of a batched launch).
+template <int NDENS, class Go>
+hipError_t launch_incore_nd(int shape, int n, const Go& go) {
+    switch (shape_for(shape, n, NDENS)) {
+        case 1: return launch_incore<NDENS, 2, 2, true, 1>(n, go);
+        case 2: return launch_incore<NDENS, 2, 1, true, 1>(n, go);
+        case 3: return launch_incore<NDENS, 1, 2, false, 1>(n, go);
+        case 4: return launch_incore<NDENS, 1, 2, true, 1>(n, go);
+        case 5: return launch_incore<NDENS, 2, 1, false, 3>(n, go);
+        case 6: return launch_incore<NDENS, 2, 1, true, 3>(n, go);
+        case 7: return launch_incore<NDENS, 2, 2, false, 3>(n, go);
+        default: return launch_incore<NDENS, 2, 2, false, 1>(n, go);
     }
 }
 
@@ -471,8 +565,14 @@ template <typename T>
 hipError_t launch_incore_shape(int shape, int ndens, const T* G, int n, int64_t np, const double* D,
                                const double* Dw, double* J, double* K, long long* fxK, hipStream_t st,
                                const StoreSkip& sk) {
-    return ndens == 1 ? launch_incore_nd<1>(shape, G, n, np, D, Dw, J, K, fxK, st, sk)
-                      : launch_incore_nd<2>(shape, G, n, np, D, Dw, J, K, fxK, st, sk);
+    const OneStore<T> go{G, n, np, D, Dw, J, K, fxK, st, sk};
+    return ndens == 1 ? launch_incore_nd<1>(shape, n, go) : launch_incore_nd<2>(shape, n, go);
+}
+
+template <typename T>
+hipError_t launch_incore_batch(int shape, int ndens, IncoreBatchArgs<T>& g, int nmax, hipStream_t st) {
+    const ManyStores<T> go{&g, nmax, st};
+    return ndens == 1 ? launch_incore_nd<1>(shape, nmax, go) : launch_incore_nd<2>(shape, nmax, go);
 }
 
 // Autotuner: the tile shape is picked per (64-column chunks ceil(n/64), densities, store
@@ -573,6 +673,38 @@ void hf_release_incore(hf_ctx* ctx, size_t keep_bytes) {
 
 namespace {
 
+// Kernel statistics of the contraction of one store: the elements streamed (the rows the screening left, all rows
+// without skipping) and their flops; launch: the launch is counted here too (a batched launch counts once).
+void incore_account(hf_ctx* ctx, int f, const StoreSkip& sk, int64_t np, int ndens, bool launch) {
+    KernelStat& ks = ctx->incore_stat[f];
+    if (launch) ks.launches += 1;
+    const double rows = sk.qf ? (double)ctx->qf_live : (double)np;
+    ks.eris += rows * (double)np;
+    ks.flops += rows * (double)np * (2.0 + 8.0 * ndens);                 // J, and row + column parts of K
+}
+
+// profile = 1: HIP events around a contraction launch on st; its time is read at the next launch's begin (or the
+// context's next statistics flush) and added to incore_stat[f].
+int incore_events_begin(hf_ctx* ctx, hipStream_t st) {
+    if (ctx->incore_pending >= 0) {
+        HF_HIP_CHECK(ctx, hipEventSynchronize(ctx->incore_ev[1]));
+        float ms = 0;
+        HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->incore_ev[0], ctx->incore_ev[1]));
+        ctx->incore_stat[ctx->incore_pending].ms += ms;
+        ctx->incore_pending = -1;
+    }
+    for (hipEvent_t& ev : ctx->incore_ev)
+        if (!ev) HF_HIP_CHECK(ctx, hipEventCreate(&ev));
+    HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[0], st));
+    return HF_OK;
+}
+
+int incore_events_end(hf_ctx* ctx, hipStream_t st, int f) {
+    HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[1], st));
+    ctx->incore_pending = f;
+    return HF_OK;
+}
+
 // The contraction of one in-core J/K build, after its packed weighted density is in Dw (np doubles) and, unless
 // accumulate, J and K are zero.  no_alloc: the tile shape is the one already tuned (or the default), and no events
 // are created or recorded, so nothing is allocated.
@@ -592,24 +724,9 @@ int incore_contract(hf_ctx* ctx, int ndens, const double* G, const float* g32, c
     // statistics, and with profile = 1 events around the contraction (read at the next flush)
     const int f = g32 ? 1 : 0;
     const bool prof = ctx->opt.profile != 0 && !no_alloc;
-    if (prof) {
-        if (ctx->incore_pending >= 0) {
-            HF_HIP_CHECK(ctx, hipEventSynchronize(ctx->incore_ev[1]));
-            float ms = 0;
-            HF_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->incore_ev[0], ctx->incore_ev[1]));
-            ctx->incore_stat[ctx->incore_pending].ms += ms;
-            ctx->incore_pending = -1;
-        }
-        for (hipEvent_t& ev : ctx->incore_ev)
-            if (!ev) HF_HIP_CHECK(ctx, hipEventCreate(&ev));
-        HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[0], st));
-    }
-    KernelStat& ks = ctx->incore_stat[f];
-    ks.launches += 1;
-    // elements streamed: the rows the screening left (all rows without skipping), whole
-    const double rows = sk.qf ? (double)ctx->qf_live : (double)np;
-    ks.eris += rows * (double)np;
-    ks.flops += rows * (double)np * (2.0 + 8.0 * ndens);                 // J, and row + column parts of K
+    if (prof)
+        if (int rc = incore_events_begin(ctx, st)) return rc;
+    incore_account(ctx, f, sk, np, ndens, true);
     // deterministic mode: K accumulates in fixed point (fx_add) and is folded in after the contraction
     long long* fxK = nullptr;
     if (ctx->opt.deterministic) {
@@ -630,10 +747,8 @@ int incore_contract(hf_ctx* ctx, int ndens, const double* G, const float* g32, c
     if (fxK)
         hipLaunchKernelGGL(incore_fx_finalize_kernel, dim3((unsigned)std::min<size_t>((ndens * nn + 255) / 256, 4096)),
                            dim3(256), 0, st, fxK, d_K, ndens * nn, accumulate ? 1 : 0);
-    if (prof) {
-        HF_HIP_CHECK(ctx, hipEventRecord(ctx->incore_ev[1], st));
-        ctx->incore_pending = f;
-    }
+    if (prof)
+        if (int rc = incore_events_end(ctx, st, f)) return rc;
     return HF_OK;
 }
 
@@ -730,6 +845,49 @@ extern "C" int hf_build_jk_incore_device_noalloc(hf_ctx* ctx, int ndens, const d
                          Dw, true, store_skip(ctx, false));
 }
 
+namespace {
+
+// One launch of the batched contraction over jobs[grp] (same store precision T, densities and tile shape, at most
+// kIncoreBatch jobs), after the set-up launch of hf_incore_jk_batch.  Statistics go to each job's context; the
+// launch (and, with profile = 1, its time) to the first job's.
+template <typename T>
+int incore_contract_many(const std::vector<IncoreJob>& jobs, const std::vector<size_t>& grp, int shape,
+                         hipStream_t st) {
+    hf_ctx* lead = jobs[grp[0]].ctx;
+    IncoreBatchArgs<T> g{};
+    g.count = (int)grp.size();
+    int nmax = 1;
+    const int ndens = jobs[grp[0]].ndens;
+    const int f = std::is_same<T, float>::value ? 1 : 0;
+    for (int j = 0; j < g.count; ++j) {
+        const IncoreJob& jb = jobs[grp[j]];
+        hf_ctx* ctx = jb.ctx;
+        const int n = ctx->mol.nbf;
+        g.n[j] = n;
+        if constexpr (std::is_same<T, float>::value)
+            g.G[j] = jb.g32;
+        else
+            g.G[j] = jb.G;
+        g.D[j] = jb.D;
+        g.Dw[j] = ctx->d_Dw.p;
+        g.J[j] = jb.J;
+        g.K[j] = jb.K;
+        g.sk[j] = store_skip(ctx, jb.own_store);
+        nmax = std::max(nmax, n);
+        incore_account(ctx, f, g.sk[j], hfgpu::pair_pad_base(n), ndens, j == 0);
+    }
+    const bool prof = lead->opt.profile != 0;
+    if (prof)
+        if (int rc = incore_events_begin(lead, st)) return rc;
+    const hipError_t e = launch_incore_batch(shape, ndens, g, nmax, st);
+    if (e != hipSuccess) return lead->fail(HF_E_HIP, std::string("batched in-core J/K: ") + hipGetErrorString(e));
+    if (prof)
+        if (int rc = incore_events_end(lead, st, f)) return rc;
+    return HF_OK;
+}
+
+}  // namespace
+
 int hf_incore_jk_batch(const std::vector<IncoreJob>& jobs, hipStream_t st) {
     if (jobs.empty()) return HF_OK;
     for (const IncoreJob& q : jobs) {
@@ -757,9 +915,56 @@ int hf_incore_jk_batch(const std::vector<IncoreJob>& jobs, hipStream_t st) {
         hipLaunchKernelGGL(incore_prep_batch_kernel, dim3(nmax, g.count), dim3(256), 0, st, g);
     }
     HF_HIP_CHECK(jobs[0].ctx, hipGetLastError());
-    for (const IncoreJob& q : jobs)
-        if (int rc = incore_contract(q.ctx, q.ndens, q.G, q.g32, q.D, q.J, q.K, st, q.accumulate, q.ctx->d_Dw.p, false,
-                                     store_skip(q.ctx, q.own_store)))
+    // Contractions.  Jobs of one kind — store precision, densities, tile shape and column chunks — run as one
+    // launch of the batched kernel; a job alone in its kind, and every job in deterministic mode (whose K goes
+    // to its own fixed-point accumulator), runs its own launch.
+    struct Item {
+        size_t q;
+        int key, shape;
+    };
+    std::vector<Item> items;
+    std::vector<size_t> single;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+        const IncoreJob& jb = jobs[q];
+        hf_ctx* ctx = jb.ctx;
+        const int forced = ctx->opt.incore_shape;
+        if (ctx->opt.deterministic || forced >= kIncoreShapes) {
+            single.push_back(q);
+            continue;
+        }
+        const int n = ctx->mol.nbf;
+        const int64_t np = hfgpu::pair_pad_base(n);
+        const StoreSkip sk = store_skip(ctx, jb.own_store);
+        int shape = forced >= 0 ? forced
+                    : jb.g32    ? tuned_shape(ctx, jb.ndens, jb.g32, n, np, jb.D, ctx->d_Dw.p, st, sk)
+                                : tuned_shape(ctx, jb.ndens, jb.G, n, np, jb.D, ctx->d_Dw.p, st, sk);
+        shape = shape_for(shape, n, jb.ndens);
+        const int key = (((jb.g32 ? 1 : 0) * 2 + jb.ndens - 1) * 16 + shape) * 16 + shape_chunks(shape, n);
+        items.push_back({q, key, shape});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.key < b.key; });
+    for (size_t i0 = 0; i0 < items.size();) {
+        size_t i1 = i0 + 1;
+        while (i1 < items.size() && items[i1].key == items[i0].key) ++i1;
+        if (i1 - i0 == 1) {
+            single.push_back(items[i0].q);
+        } else {
+            for (size_t c0 = i0; c0 < i1; c0 += kIncoreBatch) {
+                const size_t c1 = std::min(i1, c0 + kIncoreBatch);
+                std::vector<size_t> grp;
+                for (size_t i = c0; i < c1; ++i) grp.push_back(items[i].q);
+                const int rc = jobs[grp[0]].g32 ? incore_contract_many<float>(jobs, grp, items[c0].shape, st)
+                                                : incore_contract_many<double>(jobs, grp, items[c0].shape, st);
+                if (rc) return rc;
+            }
+        }
+        i0 = i1;
+    }
+    for (size_t q : single) {
+        const IncoreJob& jb = jobs[q];
+        if (int rc = incore_contract(jb.ctx, jb.ndens, jb.G, jb.g32, jb.D, jb.J, jb.K, st, jb.accumulate,
+                                     jb.ctx->d_Dw.p, false, store_skip(jb.ctx, jb.own_store)))
             return rc;
+    }
     return HF_OK;
 }

@@ -33,7 +33,9 @@ def test_poisson_slice_results_match_alone(gpu_engine_factory, prec, lockstep):
     lockstep server of 32 slots (every admitted request stepped together) or to 8 slot threads: every request
     comes back, and each converged energy equals the molecule's SCF run alone on one context (1e-9 Eh FP64;
     the mixed tier's request against its own solo run in the same tier, 1e-9 Eh, and against the FP64 solo
-    energy, 1e-6 Eh)."""
+    energy, 1e-6 Eh).  FP64 atomics add in no fixed order, so a request whose |dE| crosses the 1e-8 Eh
+    convergence test within rounding may stop one iteration earlier or later than its solo run: such a pair
+    (different iteration counts) is held to 1e-7 Eh instead, and at most 2 of the 200 may be one."""
     import hartreefock_amd as hf
     print("case poisson precision=%d lockstep=%d" % (prec, lockstep), flush=True)
     bfile = basis("6-31g.1.nw")
@@ -57,7 +59,7 @@ def test_poisson_slice_results_match_alone(gpu_engine_factory, prec, lockstep):
     assert sorted(r.tag for r in got.values()) == list(range(len(mols)))
     solo = hf.Engine(**opts)
     solo64 = hf.Engine(exact_transfer=1)
-    bad = 0
+    bad = boundary = 0
     for r in got.values():
         name, atoms = mols[r.tag]
         a = solo.set_molecule(bfile, atoms).scf(check=False, incore=1, **SCF)
@@ -67,12 +69,19 @@ def test_poisson_slice_results_match_alone(gpu_engine_factory, prec, lockstep):
             bad += 1
             continue
         assert r.status == 0
-        assert abs(r.energy - a.energy) <= 1e-9, (name, r.energy, a.energy)
+        if r.iterations != a.iterations:
+            print("%s: %d iterations served, %d alone, |dE| %.2e" % (name, r.iterations, a.iterations,
+                                                                     abs(r.energy - a.energy)), flush=True)
+            boundary += 1
+            assert abs(r.energy - a.energy) <= 1e-7, (name, r.energy, a.energy)
+        else:
+            assert abs(r.energy - a.energy) <= 1e-9, (name, r.energy, a.energy, r.iterations)
         if prec:
             e64 = solo64.set_molecule(bfile, atoms).scf(incore=1, **SCF).energy
             assert abs(r.energy - e64) <= 1e-6, (name, r.energy, e64)
-    print("not converged within 300 iterations (alone and served): %d" % bad, flush=True)
-    assert bad <= 4
+    print("not converged within 300 iterations (alone and served): %d; stopped at another iteration: %d"
+          % (bad, boundary), flush=True)
+    assert bad <= 4 and boundary <= 2
 
 
 def test_request_above_budget_is_rejected(gpu_engine_factory):
