@@ -670,6 +670,18 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
     } else if constexpr (MODE == MODE_JK32) {
         eri_jk_block<C, float>(tl, sh, pp, sink.boys32, sink);
     } else {
+        // The Boys rows of this class (F_L..F_{L+7} and e^{-T_k}) staged in LDS, as the J/K kernels do: the table
+        // row a primitive quartet selects is then an LDS load, not an L2 round trip, in the loop's dependent
+        // chain (T, row, polynomial).  Every thread stages before any leaves (padding tasks, the grid's tail).
+        constexpr bool kLdsBoys = boys_in_lds<C>();
+        extern __shared__ double btab[];
+        if constexpr (kLdsBoys) {
+            for (int i = threadIdx.x; i < BOYS_NROW * kBoysLdsStride; i += blockDim.x) {
+                const int r = i / kBoysLdsStride, c = i - r * kBoysLdsStride;
+                btab[i] = boys[r * BOYS_STRIDE + (c < 8 ? C::L + c : BOYS_NCOL)];
+            }
+            __syncthreads();
+        }
         int p, q;
         if (MODE == MODE_DIAG) {
             const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -685,7 +697,10 @@ __global__ void __launch_bounds__(256) eri_class_kernel(TaskList tl, ShellsDev s
         const int A = bp.sa, B = bp.sb, Cs = kp.sa, Ds = kp.sb;
         const int a0 = bp.a0, b0 = bp.b0, c0 = kp.a0, d0 = kp.b0;
         double out[C::NOUT];
-        quartet_values<C>(sh, pp, boys, bp, kp, out);
+        if constexpr (kLdsBoys)
+            quartet_values<C, double, true>(sh, pp, (const double*)btab, bp, kp, out);
+        else
+            quartet_values<C>(sh, pp, boys, bp, kp, out);
         if constexpr (MODE == MODE_PACKED) {
             sink_packed<C>(out, A, B, Cs, Ds, a0, b0, c0, d0, tl.same_class && p == q, sink);
         } else {
@@ -726,6 +741,8 @@ inline hipError_t launch_eri(const TaskList& tl, const ShellsDev& sh, const Prim
         s.lds_slots = sink.fx ? 0 : (int)std::min<size_t>(4, cap / slot_bytes);
         lds = s.lds_slots * slot_bytes;
         if (boys_in_lds<C>()) lds += kBoysLdsBytes;    // (the FP32 slice needs less)
+    } else if (boys_in_lds<EriClass<LA, LB, LC, LD, EXACT>>()) {
+        lds = kBoysLdsBytes;                           // MODE_PACKED, MODE_DIAG: the Boys rows in LDS
     }
     hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, EXACT, MODE>), dim3((unsigned)blocks), dim3(256), lds, st, tl,
                        sh, pp, boys, s);
