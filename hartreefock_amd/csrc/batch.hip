@@ -394,10 +394,17 @@ __global__ void __launch_bounds__(256) bsp2_step_kernel(BSp2 g, double* __restri
     }
 }
 
-// SP2 for n <= kSp2LocalMax, one workgroup per job: X and X^2 stay in LDS for every step, so the whole
-// purification is one launch with no host round trip.  Same steps and stopping rule as bsp2_step_kernel;
-// 16 wavefronts share the 16 x 16 tiles of each product on the matrix cores.  The projector goes to
-// X[j][0]; res[job] = the final step (-1 if the rule was not met within max_steps).
+// SP2 for n <= kSp2LocalMax, one workgroup per job: X stays in LDS for every step, so the whole purification is
+// one launch with no host round trip.  Same steps and stopping rule as bsp2_step_kernel; the 16 wavefronts share
+// the 16 x 16 tiles on and above the diagonal of each product X X on the matrix cores and keep their tiles of X^2
+// in registers: once the step's traces decide between X^2 and 2X - X^2, each wavefront writes the new X over its own
+// tiles and their mirror images (element (i, j) of X X is the same sum of the same products as (j, i)), so X^2
+// never goes to memory and only X (n^2 doubles) needs the LDS.  The projector goes to X[j][0]; res[job] =
+// the final step (-1 if the rule was not met within max_steps).
+constexpr int kSp2LocalWaves = 16;
+constexpr int kSp2LocalTiles = ((kSp2LocalMax + 15) / 16 * ((kSp2LocalMax + 15) / 16 + 1) / 2 + kSp2LocalWaves - 1) /
+                               kSp2LocalWaves;   // tiles per wavefront, at most
+
 __global__ void __launch_bounds__(1024) bsp2_local_kernel(BSp2 g, int max_steps, double* __restrict__ res) {
     extern __shared__ double lds[];
     const int job = blockIdx.x;
@@ -406,67 +413,87 @@ __global__ void __launch_bounds__(1024) bsp2_local_kernel(BSp2 g, int max_steps,
     const double* __restrict__ Fp = g.F[job];
     double* __restrict__ Xout = g.X[job][0];
     double* X = lds;              // n x n, column-major (symmetric)
-    double* X2 = lds + n * n;
-    __shared__ double red[2][32];
-    __shared__ double e_hist[3];
+    __shared__ double red[2][kSp2LocalWaves];
+    __shared__ double e_hist[2];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int l = tid & 63, lr = l & 15, lk = l >> 4, w = tid >> 6, nw = nthr >> 6;
+    const int l = tid & 63, lr = l & 15, lk = l >> 4, w = tid >> 6;
     double lo, hi;
-    hfgpu::gershgorin_block(Fp, n, lds, lo, hi);      // (X and X^2 are not set up yet)
+    hfgpu::gershgorin_block(Fp, n, lds, lo, hi);      // (X is not set up yet)
     if (hi - lo < 1e-12) hi = lo + 1.0;
     const double inv = 1.0 / (hi - lo);
-    for (int o = tid; o < n * n; o += nthr) X[o] = ((o / n == o % n ? hi : 0.0) - Fp[o]) * inv;
-    if (tid == 0) e_hist[0] = e_hist[1] = e_hist[2] = 1e300;   // e_{s-1}, e_{s-2}
     __syncthreads();
-    const int nt = (n + 15) / 16;
+    for (int o = tid; o < n * n; o += nthr) X[o] = ((o / n == o % n ? hi : 0.0) - Fp[o]) * inv;
+    if (tid == 0) e_hist[0] = e_hist[1] = 1e300;   // e_{s-1}, e_{s-2}
+    __syncthreads();
+    const int nt = (n + 15) / 16, ntiles = nt * (nt + 1) / 2;
     int final_step = -1;
     for (int s = 0; s <= max_steps; ++s) {
-        // the tiles on and above the diagonal (tx <= ty, column by column); element (i, j) of the product is the same
-        // sum of the same products as (j, i), so each tile is written twice
-        for (int tile = w; tile < nt * (nt + 1) / 2; tile += nw) {
+        // this wavefront's tiles of X X (tx <= ty, column by column), kept in registers
+        v4d acc[kSp2LocalTiles];
+        double d2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < kSp2LocalTiles; ++q) {
+            acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+            const int tile = w + q * kSp2LocalWaves;
+            if (tile >= ntiles) continue;
             int ty = 0;
             while ((ty + 1) * (ty + 2) / 2 <= tile) ++ty;
             const int tx = tile - ty * (ty + 1) / 2;
             const int i0 = tx * 16, j0 = ty * 16;
-            v4d acc = {0.0, 0.0, 0.0, 0.0};
+            v4d a4 = {0.0, 0.0, 0.0, 0.0};
             for (int k0 = 0; k0 < n; k0 += 4) {
                 const int kk = k0 + lk, ia = i0 + lr, jb = j0 + lr;
                 const double a = kk < n && ia < n ? X[ia + kk * n] : 0.0;
                 const double b = kk < n && jb < n ? X[jb + kk * n] : 0.0;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, a4, 0, 0, 0);
             }
+            acc[q] = a4;
+            if (tx == ty)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = i0 + lk + 4 * r, jj = j0 + lr;
-                if (i < n && jj < n) {
-                    X2[i + jj * n] = acc[r];
-                    X2[jj + i * n] = acc[r];
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + lk + 4 * r, jj = j0 + lr;
+                    if (i == jj && i < n) d2 += a4[r];
                 }
-            }
         }
-        __syncthreads();
-        double t1 = 0, t2 = 0;
-        for (int i = tid; i < n; i += nthr) {
-            t1 += X[i + i * n];
-            t2 += X2[i + i * n];
-        }
+        double d1 = 0.0;
+        for (int i = tid; i < n; i += nthr) d1 += X[i + i * n];
         {
-            double V[2] = {t1, t2};
-            hfgpu::Halving<2, 32>::run(V, l);  // lanes 0..31: the t1 total, lanes 32..63: the t2 total
+            double V[2] = {d1, d2};
+            hfgpu::Halving<2, 32>::run(V, l);  // lanes 0..31: the d1 total, lanes 32..63: the d2 total
             if (l == 0) red[0][w] = V[0];
             if (l == 32) red[1][w] = V[0];
         }
-        __syncthreads();
-        t1 = 0;
-        t2 = 0;
-        for (int q = 0; q < nw; ++q) {
+        __syncthreads();                        // every wavefront's reads of X for this step are done
+        double t1 = 0, t2 = 0;
+        for (int q = 0; q < kSp2LocalWaves; ++q) {
             t1 += red[0][q];
             t2 += red[1][q];
         }
         const double e = t1 - t2;
         const bool square = fabs(t2 - nocc) <= fabs(2.0 * t1 - t2 - nocc);
         const bool stop = e < 1e-4 && e >= e_hist[1];
-        for (int o = tid; o < n * n; o += nthr) X[o] = square ? X2[o] : 2.0 * X[o] - X2[o];
+        // X <- X^2 or 2X - X^2 over this wavefront's tiles and their mirror images (X symmetric: the old mirrored
+        // element is the old element).  In a diagonal tile both (i, j) and (j, i) are the tile's own: only the lane
+        // with i <= j updates the pair, so no lane reads an element another lane has already overwritten.
+#pragma unroll
+        for (int q = 0; q < kSp2LocalTiles; ++q) {
+            const int tile = w + q * kSp2LocalWaves;
+            if (tile >= ntiles) continue;
+            int ty = 0;
+            while ((ty + 1) * (ty + 2) / 2 <= tile) ++ty;
+            const int tx = tile - ty * (ty + 1) / 2;
+            const int i0 = tx * 16, j0 = ty * 16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + lk + 4 * r, jj = j0 + lr;
+                if (i < n && jj < n && (tx != ty || i <= jj)) {
+                    const double x = X[i + jj * n];
+                    const double xn = square ? acc[q][r] : 2.0 * x - acc[q][r];
+                    X[i + jj * n] = xn;
+                    X[jj + i * n] = xn;
+                }
+            }
+        }
         __syncthreads();
         if (tid == 0) {
             e_hist[1] = e_hist[0];
@@ -773,7 +800,7 @@ bool sp2_local_ready() {
     static std::once_flag once;
     static bool ok = false;
     std::call_once(once, [] {
-        const size_t lds = 2 * (size_t)kSp2LocalMax * kSp2LocalMax * sizeof(double);
+        const size_t lds = (size_t)kSp2LocalMax * kSp2LocalMax * sizeof(double);
         ok = hipFuncSetAttribute((const void*)bsp2_local_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds) == hipSuccess;
         (void)hipGetLastError();
@@ -1071,7 +1098,7 @@ int phase_density(const std::vector<ScfRun*>& ms, int k, hipStream_t st, BatchSc
             BSp2 g = sp2_desc(local, all, j0, 0, false);
             size_t mx = 0;
             for (int j = 0; j < g.count; ++j) mx = std::max(mx, (size_t)g.n[j] * g.n[j]);
-            const size_t lds = std::max<size_t>(2 * mx, 1024) * sizeof(double);   // >= the bounds' workspace
+            const size_t lds = std::max<size_t>(mx, 1024) * sizeof(double);   // >= the bounds' workspace
             hipLaunchKernelGGL(bsp2_local_kernel, dim3(g.count), dim3(1024), lds, ls, g, kSp2MaxSteps, bs.res.p);
         }
         const hipError_t e = hipGetLastError();

@@ -10,7 +10,7 @@
 #include "hf_internal.h"
 
 constexpr int kSp2MaxSteps = 100;     // purification steps before a density step falls back to diagonalisation
-constexpr int kSp2LocalMax = 96;      // n up to which all SP2 steps run in one workgroup (matrices in LDS)
+constexpr int kSp2LocalMax = 96;      // n up to which all SP2 steps run in one workgroup (X in LDS: n^2 doubles)
 constexpr int kWorkspaceMinN = 256;   // SCF workspace matrices are allocated for at least this many functions
 
 // DIIS history of one spin (HartreeFock/DIIS.h:9-96): device slots for F and e, host inner products.

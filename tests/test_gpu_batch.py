@@ -26,6 +26,7 @@ MIX = [
     ("ch3_631gs_uhf", "6-31g_st_.1.nw", CH3, True),
     ("water4_631gss", "6-31g_st__st_.nw", g.water_cluster(4, seed=7), False),    # 100 functions: stepped SP2
     ("pentane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(5, seed=31), False),     # 99 functions
+    ("octane_631gs", "6-31g_st_.1.nw", g.alkane_zigzag(8, seed=13), False),      # 156 functions
 ]
 
 

@@ -293,6 +293,7 @@ def test_jk_at_scale_matches_packed_contraction(gpu_engine_factory, name, bfile,
 DENSITY_CASES = [
     ("water2_631gs_local_sp2", "6-31g_st_.1.nw", g.water_cluster(2, seed=41), False),   # n <= 96: one workgroup
     ("pentane_631gs_fused_sp2", "6-31g_st_.1.nw", g.alkane_zigzag(5, seed=42), False),  # n = 99: one kernel per step
+    ("octane_631gs_fused_sp2", "6-31g_st_.1.nw", g.alkane_zigzag(8, seed=42), False),   # n = 156: one kernel per step
     ("ch3_631gs_uhf", "6-31g_st_.1.nw", [(6, (0.0, 0.0, 0.0)), (1, (2.04, 0.0, 0.0)), (1, (-1.02, 1.767, 0.0)),
                                          (1, (-1.02, -1.767, 0.0))], True),
 ]
