@@ -469,6 +469,7 @@ def serving_batch(a):
     conv = sum(1 for st, r in out_res if st == 0 and r.converged)
     its = [r.iterations for st, r in out_res if st == 0 and r.converged]
     store_ms = [r.eri_ms for st, r in out_res if st == 0 and r.incore]
+    f32 = [r.fp32_builds for st, r in out_res if st == 0 and r.converged]
     # the in-core contraction's launches and algorithmic bytes (the store rows it streams) over every batch,
     # warm-up included, as a counter run of this command sees them (--pmc-summary takes this line)
     incore = {}
@@ -483,6 +484,7 @@ def serving_batch(a):
                       "value": conv / wall, "unit": "molecules/s", "steps": K, "warmup": W, "batch": B,
                       "precision": "mixed" if a.precision else "fp64", "wall_s": wall, "converged": conv,
                       "mean_iterations": sum(its) / max(1, len(its)),
+                      "mean_fp32_builds": sum(f32) / max(1, len(f32)),
                       "mean_store_ms": sum(store_ms) / max(1, len(store_ms)), "incore_stats": incore}), flush=True)
     for e in engs:
         e.close()

@@ -544,7 +544,9 @@ int ScfRun::init() {
     rebuild_every = p->full_rebuild_every > 0 ? p->full_rebuild_every : 8;
     // mixed-precision tier: quartets below the context's fp32_threshold in FP32 until fp64_phase
     mixed = ctx->opt.precision == HF_PREC_MIXED;
-    final_fp64 = p->final_fp64_iterations > 0 ? p->final_fp64_iterations : 3;
+    // in-core, every FP64 build is a full build from the FP64 store (see jk_plan), so one such step makes the Fock
+    // matrix exactly FP64; direct FP32 quartets leave rounding in the DIIS history for a few steps
+    final_fp64 = p->final_fp64_iterations > 0 ? p->final_fp64_iterations : incore_mixed ? 1 : 3;
     fp64_phase = !mixed || external;      // a caller's own steps (hf_scf_step) are all FP64
     useDIIS = p->use_diis != 0;
     eps_h.assign(2 * (size_t)n, 0.0);
@@ -566,7 +568,11 @@ void ScfRun::jk_plan(JkPlan& pl) {
     ScfState& ss = ctx->scf_ws->st;
     pl = JkPlan{};
     pl.q0 = quartet_count();
-    const bool full = !incremental || force_full || full_builds < first_full || since_full >= rebuild_every;
+    bool full = !incremental || force_full || full_builds < first_full || since_full >= rebuild_every;
+    // In-core mixed: an increment pays only from the FP32 store (an FP64 increment streams the same store as a full
+    // build and adds its rounding to the previous J/K), so a step that will not use it -- the FP64 phase, or a
+    // density change above the FP32 limit -- builds in full.
+    if (incore_mixed && !full && (fp64_phase || !(dmax_next >= 0 && dmax_next <= kIncoreFp32MaxDelta))) full = true;
     bool used_fp32 = !fp64_phase;
     if (incore && store_pending) {   // the store (own stream) must be complete
         pl.wait_store = true;
@@ -887,7 +893,7 @@ void hf_scf_defaults(hf_scf_params* p) {
     p->density_method = 0;
     p->incremental_fock = 0;
     p->full_rebuild_every = 8;
-    p->final_fp64_iterations = 3;
+    p->final_fp64_iterations = 0;
     p->incore = 0;
     p->incore_max_bytes = 0;
     p->mp2 = 0;

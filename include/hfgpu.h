@@ -205,7 +205,8 @@ typedef struct hf_scf_params {
                                       with a full build.  0 (default): full J/K every step. */
     int full_rebuild_every;        /* incremental steps between full builds (default 8) */
     int final_fp64_iterations;     /* mixed precision (context option): all-FP64 steps required at the
-                                      end before convergence is accepted (default 3) */
+                                      end before convergence is accepted (default 0: 3 with direct J/K,
+                                      1 with the in-core store, whose FP64 steps are full FP64 builds) */
     int incore;                    /* 1: evaluate the ERIs once per SCF into HBM (hf_eri_pairs_device)
                                       and build J/K from the store when it fits; 0 (default): direct */
     double incore_max_bytes;       /* largest store the SCF allocates (0: half the free device memory;
