@@ -609,16 +609,20 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, workload_j
     print(json.dumps(out), flush=True)
 
 
-def trace_summary(root):
+def trace_summary(root, table=False):
     """Per-kernel resources and durations from a rocprofv3 --kernel-trace directory (kernel_trace.csv): calls, mean
     duration, VGPRs (arch + accumulation), SGPRs, LDS and scratch per workgroup/lane, workgroup size, and the
-    wavefronts per SIMD that the VGPR count allows (512 VGPRs per SIMD lane, granule 8, at most 8)."""
+    wavefronts per SIMD that the VGPR count allows (512 VGPRs per SIMD lane, granule 8, at most 8); also the trace's
+    span and the time the device ran any kernel (the union of the kernels' intervals).  table: a text table of time
+    by kernel instead of the JSON line."""
     import csv
     import glob
     agg = {}
+    iv = []
     for f in sorted(glob.glob(os.path.join(root, "**", "*kernel_trace.csv"), recursive=True)):
         with open(f) as fh:
             for r in csv.DictReader(fh):
+                iv.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
                 name = r["Kernel_Name"]
                 k = agg.setdefault(name, {"calls": 0, "ns": 0.0, "vgpr": int(r.get("VGPR_Count", 0) or 0),
                                           "agpr": int(r.get("Accum_VGPR_Count", 0) or 0),
@@ -636,7 +640,31 @@ def trace_summary(root):
                          agpr=k["agpr"], sgpr=k["sgpr"], lds_bytes=k["lds"], scratch_bytes_per_lane=k["scratch"],
                          workgroup=k["wg"], max_grid=k["grid_max"], waves_per_simd=min(8, 512 // regs)))
     rows.sort(key=lambda x: -x["calls"] * x["mean_us"])
-    print(json.dumps({"kernels": rows}), flush=True)
+    iv.sort()
+    busy, span = 0, 0
+    if iv:
+        cs, ce = iv[0]
+        for a, b in iv[1:]:
+            if a > ce:
+                busy += ce - cs
+                cs, ce = a, b
+            else:
+                ce = max(ce, b)
+        busy += ce - cs
+        span = max(b for _, b in iv) - iv[0][0]
+    if not table:
+        print(json.dumps({"span_ms": span / 1e6, "busy_ms": busy / 1e6, "kernels": rows}), flush=True)
+        return
+    total = sum(x["calls"] * x["mean_us"] for x in rows) / 1e3
+    print("# kernel%s calls    total_ms   mean_us  share" % (" " * 62))
+    print("kernels %d, total %.1f ms" % (sum(x["calls"] for x in rows), total))
+    for x in rows[:40]:
+        name = x["kernel"].replace("(anonymous namespace)::", "").replace("void ", "").replace("hfgpu::", "")
+        name = name.split("(")[0] if not name.startswith("(") else name
+        ms = x["calls"] * x["mean_us"] / 1e3
+        print("%-70s %7d %9.2f ms %8.1f us %5.1f%%" % (name[:70], x["calls"], ms, x["mean_us"], 100.0 * ms / total))
+    print("span %.1f ms, device busy (union of kernels) %.1f ms = %.0f%%" % (span / 1e6, busy / 1e6,
+                                                                             100.0 * busy / max(span, 1)))
 
 
 def with_traffic(roof, fname):
@@ -698,13 +726,15 @@ def main():
                     help="only summarise rocprofv3 --pmc output: FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR "
                          "[WORKLOAD_JSON]: in-core contraction HBM traffic against its algorithmic bytes (from the "
                          "profiled command's JSON line when given), scaled by the copy calibration")
+    ap.add_argument("--trace-table", metavar="TRACE_DIR",
+                    help="a rocprofv3 --kernel-trace directory: text table of kernel time by kernel, span and busy time")
     ap.add_argument("--trace-summary", metavar="TRACE_DIR",
                     help="only summarise a rocprofv3 --kernel-trace directory: per-kernel resources and durations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
-    if a.trace_summary:
-        trace_summary(a.trace_summary)
+    if a.trace_summary or a.trace_table:
+        trace_summary(a.trace_summary or a.trace_table, table=bool(a.trace_table))
         return
     if a.pmc_summary:
         if len(a.pmc_summary) not in (4, 5):
