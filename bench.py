@@ -609,6 +609,37 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, workload_j
     print(json.dumps(out), flush=True)
 
 
+def pmc_table(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
+    """HBM bytes per dispatch of every kernel from rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch), each scaled by what the counter reads on the plain 1 GiB device copy of --copy-calibration; a text table
+    of the median over each kernel's dispatches (robust to a minority of atypical launches of the same kernel, such as
+    the fixed-point accumulation of a deterministic-mode SCF in the same command) and the mean, largest write first."""
+    gib_kib = float(1 << 20)
+    scale = {}
+    for counter, d in (("FETCH_SIZE", copy_fetch_dir), ("WRITE_SIZE", copy_write_dir)):
+        v = [float(r["Counter_Value"]) for r in _counter_rows(d, counter) if "copyBuffer" in r["Kernel_Name"]]
+        scale[counter] = gib_kib / (sum(v) / len(v))
+    agg = {}
+    for counter, d, key in (("FETCH_SIZE", fetch_dir, 0), ("WRITE_SIZE", write_dir, 1)):
+        for r in _counter_rows(d, counter):
+            name = r["Kernel_Name"].replace("(anonymous namespace)::", "").replace("void ", "").replace("hfgpu::", "")
+            name = name.split("(")[0] if not name.startswith("(") else name
+            agg.setdefault(name, ([], []))[key].append(float(r["Counter_Value"]) * 1024.0 * scale[counter])
+    print("# copy scale: FETCH_SIZE %.4f, WRITE_SIZE %.4f; bytes per dispatch" % (scale["FETCH_SIZE"],
+                                                                                  scale["WRITE_SIZE"]))
+    print("%-56s %8s %11s %11s %11s %11s" % ("kernel", "dispatch", "fetch med", "write med", "fetch mean",
+                                             "write mean"))
+
+    def med(v):
+        v = sorted(v)
+        return 0.5 * (v[(len(v) - 1) // 2] + v[len(v) // 2])
+
+    rows = sorted(((n, f, w) for n, (f, w) in agg.items() if f and w), key=lambda x: -med(x[2]))
+    for name, f, w in rows:
+        print("%-56s %8d %11.4g %11.4g %11.4g %11.4g" % (name[:56], len(f), med(f), med(w), sum(f) / len(f),
+                                                           sum(w) / len(w)))
+
+
 def trace_summary(root, table=False):
     """Per-kernel resources and durations from a rocprofv3 --kernel-trace directory (kernel_trace.csv): calls, mean
     duration, VGPRs (arch + accumulation), SGPRs, LDS and scratch per workgroup/lane, workgroup size, and the
@@ -726,6 +757,9 @@ def main():
                     help="only summarise rocprofv3 --pmc output: FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR "
                          "[WORKLOAD_JSON]: in-core contraction HBM traffic against its algorithmic bytes (from the "
                          "profiled command's JSON line when given), scaled by the copy calibration")
+    ap.add_argument("--pmc-table", nargs=4, metavar="DIR",
+                    help="FETCH_SIZE, WRITE_SIZE, copy FETCH_SIZE and copy WRITE_SIZE rocprofv3 --pmc directories: "
+                         "HBM bytes per dispatch of every kernel, as a text table")
     ap.add_argument("--trace-table", metavar="TRACE_DIR",
                     help="a rocprofv3 --kernel-trace directory: text table of kernel time by kernel, span and busy time")
     ap.add_argument("--trace-summary", metavar="TRACE_DIR",
@@ -733,6 +767,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
+    if a.pmc_table:
+        pmc_table(*a.pmc_table)
+        return
     if a.trace_summary or a.trace_table:
         trace_summary(a.trace_summary or a.trace_table, table=bool(a.trace_table))
         return
