@@ -122,6 +122,7 @@ def cpu_baseline_single(bfile, atoms, iterations, budget_s):
 # ---------------------------------------------------------------------------------------------- serving
 SERVE_SLOTS = 32       # bench.py --slots N changes it
 SERVE_LOCKSTEP = 2     # batch threads of the lockstep server (bench.py --lockstep k); 0: one worker thread per slot
+SERVE_PREP = None      # preparation threads of the lockstep server (bench.py --prep-threads p; None: the server's 8)
 SLOT_MODE_SLOTS = 8    # slots of the slot-thread comparison leg (its measured saturation point is 4-8)
 
 
@@ -166,7 +167,8 @@ def serve(hf, bfile, reqs, prec, rate=None, slots=None, incore=1, tag="", profil
     srv = hf.Server(slots=slots or SERVE_SLOTS, options=dict(precision=prec, fp32_threshold=1e-6, exact_transfer=1,
                                                              profile=profile, profile_class=profile_class,
                                                              deterministic=deterministic),
-                    scf=dict(SCF_KW, incore=incore, max_iterations=max_iterations), lockstep=ls if ls else False)
+                    scf=dict(SCF_KW, incore=incore, max_iterations=max_iterations), lockstep=ls if ls else False,
+                    prep_threads=SERVE_PREP)
     try:
         if warm:
             _pass(srv, bfile, warm, None, seed, tag + " (warm-up)")
@@ -740,6 +742,8 @@ def main():
     ap.add_argument("--slots", type=int, default=32, help="serving: concurrent requests (server slots)")
     ap.add_argument("--lockstep", type=int, default=2, help="serving: batch threads of the lockstep server (default 2); "
                     "0: one worker thread per slot")
+    ap.add_argument("--prep-threads", type=int, default=None,
+                    help="serving: preparation threads of the lockstep server (default: the server's 8)")
     ap.add_argument("--no-slot-leg", action="store_true", help="serving: skip the FP64 slot-thread comparison leg")
     ap.add_argument("--workload", default="serving", help="serving (default) | decane | water | water32 | eicosyl | "
                     "incore-shapes (in-core contraction per tile shape and size) | serving-batch (the serving mix "
@@ -792,7 +796,8 @@ def main():
         serving_batch(a)
         return
     if a.workload == "serving":
-        global SERVE_SLOTS, SERVE_LOCKSTEP
+        global SERVE_SLOTS, SERVE_LOCKSTEP, SERVE_PREP
+        SERVE_PREP = a.prep_threads
         SERVE_SLOTS = a.slots
         SERVE_LOCKSTEP = a.lockstep
         serving_bench(a)
